@@ -1,0 +1,255 @@
+"""ctypes binding of libchimes_gpu.so (C ABI declared in include/chimes_gpu.h).
+
+Used by the tests and bench.py; the C++ command-line drivers link the library
+directly.  The HIP entry points fail loudly (OSError / RuntimeError) when the
+library was not built -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchimes_gpu.so")
+
+CG_FP64, CG_FP32 = 0, 1
+STATUS = {0: "CG_OK", 1: "CG_ERR_ARG", 2: "CG_ERR_INPUT", 3: "CG_ERR_UNSUPPORTED",
+          4: "CG_ERR_DEVICE", 5: "CG_ERR_MEMORY", 6: "CG_ERR_NUMERIC", 7: "CG_ERR_EOF"}
+
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+class ClusterSet(C.Structure):
+    _fields_ = [("nbody", C.c_int), ("npairs", C.c_int), ("order", C.c_int), ("ntypes", C.c_int),
+                ("total_allowed", C.c_int), ("ncols", C.c_int), ("max_cutoff", C.c_double),
+                ("excluded", _ip), ("atom_type", _ip), ("col_offset", _ip), ("n_unique", _ip),
+                ("perm_offset", _ip), ("perm_powers", _ip), ("perm_col", _ip), ("perm_equiv", _ip),
+                ("perm_count", _ip), ("s_min", _dp), ("s_max", _dp), ("x_avg", _dp), ("x_diff", _dp),
+                ("pair_type", _ip), ("id_size", C.c_int), ("id_map", _ip), ("id_pair_map", _ip),
+                ("id_atom_perm", _ip)]
+
+
+class Model(C.Structure):
+    _fields_ = [("ntypes", C.c_int), ("type_names", C.c_char_p), ("npairs", C.c_int),
+                ("pair_t1", _ip), ("pair_t2", _ip), ("int_pair_map", _ip),
+                ("pair_smin", _dp), ("pair_smax", _dp), ("pair_lambda", _dp), ("pair_xavg", _dp),
+                ("pair_xdiff", _dp), ("transform", C.c_int), ("order2b", C.c_int),
+                ("deriv_const", C.c_double), ("cheby_low", C.c_int), ("cheby_high", C.c_int),
+                ("fcut_type", C.c_int), ("fcut_offset", C.c_double), ("fix_type", C.c_int),
+                ("smooth_dist", C.c_double), ("nlayers", C.c_int), ("small_mode", C.c_int),
+                ("fit_stress", C.c_int), ("fit_energy", C.c_int), ("nstress", C.c_int),
+                ("nener", C.c_int), ("n2b_cols", C.c_int), ("ncols_short", C.c_int),
+                ("n_offset_cols", C.c_int), ("ncols", C.c_int), ("max_cutoff_2b", C.c_double),
+                ("c3", ClusterSet), ("c4", ClusterSet)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("xyz", _dp), ("type", _ip), ("cell", C.c_double * 9),
+                ("forces", _dp), ("stress", C.c_double * 6), ("energy", C.c_double),
+                ("index", C.c_int)]
+
+
+class FitResult(C.Structure):
+    _fields_ = [("rms", C.c_double), ("max_abs_x", C.c_double), ("nvars", C.c_int),
+                ("eps_abs", C.c_double), ("bic", C.c_double), ("ms_gram", C.c_double),
+                ("ms_solve", C.c_double), ("ms_refine", C.c_double)]
+
+
+_lib = None
+
+
+def lib(path=None):
+    """Load libchimes_gpu.so (raises OSError if it is missing)."""
+    global _lib
+    if _lib is None or path:
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise OSError("libchimes_gpu.so is not built (%s); run __graft_entry__.build()" % p)
+        L = C.CDLL(p)
+        L.cg_last_error.restype = C.c_char_p
+        L.cg_last_error.argtypes = [C.c_void_p]
+        L.cg_setup_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.cg_setup_free.argtypes = [C.c_void_p]
+        L.cg_setup_model.restype = C.POINTER(Model)
+        L.cg_setup_model.argtypes = [C.c_void_p]
+        L.cg_setup_nframes.argtypes = [C.c_void_p]
+        L.cg_model_frame_rows.restype = C.c_int64
+        L.cg_model_frame_rows.argtypes = [C.POINTER(Model), C.c_int, C.c_int]
+        L.cg_traj_open.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.cg_traj_next.argtypes = [C.c_void_p, C.POINTER(Frame)]
+        L.cg_traj_close.argtypes = [C.c_void_p]
+        L.cg_write_open.argtypes = [C.c_void_p, C.c_char_p]
+        L.cg_write_frame.argtypes = [C.c_void_p, C.POINTER(Frame), _dp, C.c_int]
+        L.cg_write_close.argtypes = [C.c_void_p]
+        L.cg_write_header.argtypes = [C.c_void_p, C.c_char_p]
+        if hasattr(L, "cg_create"):
+            L.cg_create.argtypes = [C.POINTER(Model), C.c_int, C.POINTER(C.c_void_p)]
+            L.cg_destroy.argtypes = [C.c_void_p]
+            L.cg_assemble.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int,
+                                      C.POINTER(C.c_void_p), _dp, C.POINTER(C.c_int64)]
+            L.cg_copy_A.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _dp]
+            L.cg_last_timing.argtypes = [C.c_void_p, _dp, _dp, _dp]
+            L.cg_fit.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)]
+            L.cg_matvec.argtypes = [C.c_void_p, _dp, _dp]
+            L.cg_write_params.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_int, C.c_int64,
+                                          C.POINTER(FitResult), _dp, _dp]
+            L.cg_force.argtypes = [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def check(st, ctx=None):
+    if st != 0:
+        msg = lib().cg_last_error(ctx)
+        raise RuntimeError("%s: %s" % (STATUS.get(st, st), msg.decode() if msg else ""))
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+class PyFrame:
+    """A frame whose arrays are owned by numpy (safe to keep)."""
+
+    def __init__(self, natoms, xyz, types, cell, forces=None, stress=None, energy=0.0, index=0):
+        self.natoms = int(natoms)
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        self.cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+        self.forces = (np.zeros(3 * self.natoms) if forces is None
+                       else np.ascontiguousarray(forces, dtype=np.float64).reshape(-1))
+        self.stress = np.zeros(6) if stress is None else np.asarray(stress, dtype=np.float64)
+        self.energy = float(energy)
+        self.index = int(index)
+
+    def c(self):
+        f = Frame()
+        f.natoms = self.natoms
+        f.xyz = dptr(self.xyz)
+        f.type = iptr(self.types)
+        for i in range(9):
+            f.cell[i] = self.cell[i]
+        f.forces = dptr(self.forces)
+        for i in range(6):
+            f.stress[i] = self.stress[i]
+        f.energy = self.energy
+        f.index = self.index
+        return f
+
+
+class Setup:
+    """Parsed fm_setup.in (host C++ parser behind the C ABI)."""
+
+    def __init__(self, path):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.cg_setup_read(path.encode(), C.byref(h)))
+        self.h = h
+        self.path = path
+        self.model_ptr = self.L.cg_setup_model(h)
+        self.model = self.model_ptr.contents
+
+    def __del__(self):
+        try:
+            self.L.cg_setup_free(self.h)
+        except Exception:
+            pass
+
+    @property
+    def nframes(self):
+        return self.L.cg_setup_nframes(self.h)
+
+    def frame_rows(self, natoms, index):
+        return self.L.cg_model_frame_rows(self.model_ptr, natoms, index)
+
+    def frames(self, workdir=None, limit=None):
+        """Read frames of the trajectory into PyFrame objects."""
+        t = C.c_void_p()
+        check(self.L.cg_traj_open(self.h, workdir.encode() if workdir else None, C.byref(t)))
+        out = []
+        try:
+            n = self.nframes if limit is None else min(limit, self.nframes)
+            for _ in range(n):
+                f = Frame()
+                check(self.L.cg_traj_next(t, C.byref(f)))
+                N = f.natoms
+                out.append(PyFrame(N, np.ctypeslib.as_array(f.xyz, (3 * N,)).copy(),
+                                   np.ctypeslib.as_array(f.type, (N,)).copy(), list(f.cell),
+                                   np.ctypeslib.as_array(f.forces, (3 * N,)).copy(),
+                                   list(f.stress), f.energy, f.index))
+        finally:
+            self.L.cg_traj_close(t)
+        return out
+
+    def write_header(self, outdir):
+        check(self.L.cg_write_header(self.h, outdir.encode()))
+
+
+class Device:
+    """A device context (cg_create / cg_destroy)."""
+
+    def __init__(self, setup, device=0):
+        self.L = lib()
+        if not hasattr(self.L, "cg_create"):
+            raise OSError("libchimes_gpu.so was built without the HIP entry points")
+        self.setup = setup
+        h = C.c_void_p()
+        check(self.L.cg_create(setup.model_ptr, device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.cg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble(self, frames, prec=CG_FP64, copy=True):
+        cf = (Frame * len(frames))(*[f.c() for f in frames])
+        rows = sum(self.setup.frame_rows(f.natoms, f.index) for f in frames)
+        b = np.zeros(rows)
+        Ad = C.c_void_p()
+        nrow = C.c_int64()
+        check(self.L.cg_assemble(self.h, cf, len(frames), prec, C.byref(Ad), dptr(b), C.byref(nrow)), self.h)
+        assert nrow.value == rows
+        if not copy:
+            return None, b
+        A = np.zeros((rows, self.setup.model.ncols))
+        check(self.L.cg_copy_A(self.h, 0, rows, dptr(A)), self.h)
+        return A, b
+
+    def timing(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.cg_last_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), self.h)
+        return a.value, b.value, c.value
+
+    def fit(self, b, eps=1e-5, refine=2):
+        x = np.zeros(self.setup.model.ncols)
+        res = FitResult()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        check(self.L.cg_fit(self.h, dptr(b), eps, refine, dptr(x), C.byref(res)), self.h)
+        return x, res
+
+    def matvec(self, x, rows):
+        y = np.zeros(rows)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(self.L.cg_matvec(self.h, dptr(x), dptr(y)), self.h)
+        return y
+
+    def force(self, x, frame):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        F = np.zeros(3 * frame.natoms)
+        E = C.c_double()
+        V = np.zeros(9)
+        f = frame.c()
+        check(self.L.cg_force(self.h, dptr(x), C.byref(f), dptr(F), C.byref(E), dptr(V)), self.h)
+        return F, E.value, V

@@ -1,0 +1,390 @@
+// Host I/O of the drop-in boundary: trajectory reader (.xyzf), A/b/b-labeled/
+// natoms/dim text writers, params.header and ff_groups.map writers, and the
+// C ABI wrappers around the setup parser.
+//
+// Formats follow FRAME::READ_XYZF (src/ClassDefs.C:2411-2683), A_MAT::PRINT_FRAME
+// and CLEANUP_FILES (src/A_Matrix.C:210-760), print_param_header / print_map_file
+// (src/chimes_lsq.C:562-716), FCUT::print_header (src/Fcut.C:169-178) and
+// CLUSTER[_LIST]::print_header / print_special (src/Cluster.C:423-508,1636-1697).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "setup.hpp"
+
+using namespace cg;
+
+namespace cg {
+thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+}  // namespace cg
+
+namespace {
+cg_status fail(const Error &e) { g_err = e.what(); return e.code; }
+cg_status fail(const std::exception &e) { g_err = e.what(); return CG_ERR_INPUT; }
+
+constexpr double kForceConv = 627.50960803 * 1.889725989;  // Hartree/Bohr -> kcal/mol/A
+constexpr double kGPa = 6.9476955;                          // src/functions.h:45
+
+std::vector<std::string> split_ws(const std::string &line) {
+  std::vector<std::string> out;
+  std::istringstream is(line);
+  std::string t;
+  while (is >> t) out.push_back(t);
+  return out;
+}
+}  // namespace
+
+struct cg_setup {
+  Setup *s;
+};
+
+struct cg_traj {
+  const Setup *s;
+  std::string workdir;
+  size_t file_idx = 0;
+  int frame = 0;
+  int offset = 0;
+  std::ifstream in;
+  std::vector<double> xyz, forces;
+  std::vector<int> type;
+};
+
+extern "C" {
+
+const char *cg_last_error(const void *) { return g_err.c_str(); }
+
+cg_status cg_setup_read(const char *path, cg_setup **out) {
+  if (!path || !out) { g_err = "null argument"; return CG_ERR_ARG; }
+  try {
+    *out = new cg_setup{read_setup(path)};
+    return CG_OK;
+  } catch (const Error &e) { return fail(e); } catch (const std::exception &e) { return fail(e); }
+}
+
+void cg_setup_free(cg_setup *s) {
+  if (!s) return;
+  delete s->s;
+  delete s;
+}
+
+const cg_model *cg_setup_model(const cg_setup *s) { return s ? &s->s->model : nullptr; }
+int cg_setup_nframes(const cg_setup *s) { return s ? s->s->nframes : 0; }
+
+int64_t cg_model_frame_rows(const cg_model *m, int natoms, int fi) {
+  int64_t r = 3LL * natoms;
+  if (m->fit_stress != CG_STRESS_NONE && (m->nstress < 0 || fi < m->nstress))
+    r += m->fit_stress == CG_STRESS_ALL ? 9 : 3;
+  if (m->fit_energy && (m->nener < 0 || fi < m->nener)) r += 3;
+  return r;
+}
+
+cg_status cg_traj_open(const cg_setup *cs, const char *workdir, cg_traj **out) {
+  if (!cs || !out) { g_err = "null argument"; return CG_ERR_ARG; }
+  auto t = new cg_traj;
+  t->s = cs->s;
+  t->workdir = workdir ? workdir : cs->s->dir;
+  if (t->s->infiles.empty()) { delete t; g_err = "no trajectory file"; return CG_ERR_INPUT; }
+  t->in.open(t->workdir + "/" + t->s->infiles[0]);
+  if (!t->in) { g_err = "cannot open trajectory " + t->s->infiles[0]; delete t; return CG_ERR_INPUT; }
+  t->offset = t->s->infile_frames.empty() ? t->s->nframes : t->s->infile_frames[0];
+  *out = t;
+  return CG_OK;
+}
+
+void cg_traj_close(cg_traj *t) { delete t; }
+
+// FRAME::READ_XYZF.  Cell vectors are returned as given; cell-derived
+// quantities are recomputed by the consumers exactly as BOX::UPDATE_CELL does.
+cg_status cg_traj_next(cg_traj *t, cg_frame *f) {
+  const Setup &s = *t->s;
+  if (t->frame >= s.nframes) { g_err = "end of trajectory"; return CG_ERR_EOF; }
+  try {
+    if (t->frame + 1 > t->offset) {
+      ++t->file_idx;
+      if (t->file_idx >= s.infiles.size()) throw Error(CG_ERR_INPUT, "ran out of trajectory files");
+      t->in.close();
+      t->in.clear();
+      t->in.open(t->workdir + "/" + s.infiles[t->file_idx]);
+      if (!t->in) throw Error(CG_ERR_INPUT, "cannot open trajectory " + s.infiles[t->file_idx]);
+      t->offset += s.infile_frames[t->file_idx];
+    }
+    const int fi = t->frame;
+    int natoms = 0;
+    if (!(t->in >> natoms)) throw Error(CG_ERR_INPUT, "Error: Reading frame " + std::to_string(fi) + ": missing atom count");
+    std::string header;
+    std::getline(t->in, header);
+    std::getline(t->in, header);
+    auto tk = split_ws(header);
+    if (tk.size() < 3) throw Error(CG_ERR_INPUT, "Error:  Reading frame " + std::to_string(fi) + "\n        Missing box length components.\n" + header);
+    bool ortho = tk[0] != "NON_ORTHO";
+    std::memset(f, 0, sizeof(*f));
+    if (ortho) {
+      f->cell[0] = std::stod(tk[0]); f->cell[4] = std::stod(tk[1]); f->cell[8] = std::stod(tk[2]);
+    } else {
+      if (tk.size() < 10) throw Error(CG_ERR_INPUT, "Error: Reading frame " + std::to_string(fi) + ": NON_ORTHO needs 9 cell values");
+      for (int i = 0; i < 9; ++i) f->cell[i] = std::stod(tk[1 + i]);
+    }
+    if (s.nstress < 0 || fi < s.nstress) {
+      if (s.fit_stress == CG_STRESS_DIAG) {
+        size_t b = ortho ? 3 : 10;
+        if (tk.size() < b + 3) throw Error(CG_ERR_INPUT, "Error:  Reading frame " + std::to_string(fi) + "\n        Missing diagonal stress tensor components.\n" + header);
+        for (int i = 0; i < 3; ++i) f->stress[i] = std::stod(tk[b + i]);
+      } else if (s.fit_stress == CG_STRESS_ALL) {
+        size_t b = ortho ? 3 : 10;
+        if (tk.size() < b + 6) throw Error(CG_ERR_INPUT, "Error:  Reading frame " + std::to_string(fi) + "\n        Missing full stress tensor components.\n" + header);
+        for (int i = 0; i < 6; ++i) f->stress[i] = std::stod(tk[b + i]);
+      }
+    }
+    if ((s.nener < 0 || fi < s.nener) && s.fit_ener) f->energy = std::stod(tk.back());
+    t->xyz.resize(3 * natoms);
+    t->forces.resize(3 * natoms);
+    t->type.resize(natoms);
+    for (int j = 0; j < natoms; ++j) {
+      std::string nm;
+      t->in >> nm;
+      int ty = type_of(s, nm);
+      if (ty < 0) throw Error(CG_ERR_INPUT, "Did not recognize atom type " + nm);
+      t->type[j] = ty;
+      for (int c = 0; c < 3; ++c) t->in >> t->xyz[3 * j + c];
+      for (int c = 0; c < 3; ++c) {
+        t->in >> t->forces[3 * j + c];
+        t->forces[3 * j + c] *= kForceConv;
+      }
+      if (!t->in) throw Error(CG_ERR_INPUT, "Error: Reading frame " + std::to_string(fi) + ": truncated atom line");
+    }
+    f->natoms = natoms;
+    f->xyz = t->xyz.data();
+    f->type = t->type.data();
+    f->forces = t->forces.data();
+    f->index = fi;
+    ++t->frame;
+    return CG_OK;
+  } catch (const Error &e) { return fail(e); } catch (const std::exception &e) { return fail(e); }
+}
+
+// ------------------------------------------------------------ A/b text files
+cg_status cg_write_open(const cg_setup *cs, const char *outdir) {
+  Setup &s = *cs->s;
+  s.outdir = outdir ? outdir : ".";
+  s.fA.open(s.outdir + "/A.txt");
+  s.fb.open(s.outdir + "/b.txt");
+  s.fbl.open(s.outdir + "/b-labeled.txt");
+  s.fna.open(s.outdir + "/natoms.txt");
+  if (!s.fA || !s.fb || !s.fbl || !s.fna) { g_err = "Could not open output files in " + s.outdir; return CG_ERR_INPUT; }
+  s.fA.precision(16); s.fA << std::scientific;
+  s.fb.precision(16); s.fb << std::scientific;
+  s.data_count = 0;
+  return CG_OK;
+}
+
+namespace {
+void ones(const Setup &s, std::ostream &o, bool energy_row, const std::vector<int> &count) {
+  if (!s.fit_ener) return;
+  for (int i = 0; i < s.ntypes; ++i) {
+    if (!energy_row) o << " " << "0.0";
+    else o << " " << count[i];
+  }
+}
+}  // namespace
+
+cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *rows, int nrows) {
+  Setup &s = *cs->s;
+  const cg_model &m = s.model;
+  const int nc = m.ncols, ns = m.ncols_short;
+  int64_t expect = cg_model_frame_rows(&m, f->natoms, f->index);
+  if (nrows != expect) { g_err = "row count mismatch in cg_write_frame"; return CG_ERR_ARG; }
+  // which input file the frame came from (for the b-labeled flags)
+  size_t file = 0;
+  int check = 0;
+  for (size_t i = 0; i < s.infile_frames.size(); ++i) {
+    check += s.infile_frames[i];
+    if (f->index < check) break;
+    ++file;
+  }
+  file = std::min(file, s.force_flags.size() - 1);
+  std::vector<int> count(s.ntypes, 0);
+  for (int i = 0; i < f->natoms; ++i) count[f->type[i]]++;
+  int total = 0;
+  for (int c : count) total += c;
+  int r = 0;
+  auto row_out = [&](const char *sep, bool energy_row) {
+    const double *a = rows + (int64_t)r * nc;
+    for (int n = 0; n < ns; ++n) s.fA << a[n] << sep;
+    ones(s, s.fA, energy_row, count);
+    s.fna << total << "\n";
+    s.fA << "\n";
+    ++r;
+  };
+  for (int a = 0; a < f->natoms; ++a) {
+    for (int c = 0; c < 3; ++c) row_out("   ", false);
+    for (int c = 0; c < 3; ++c) s.fb << f->forces[3 * a + c] << "\n";
+    s.data_count += 3;
+    const std::string &nm = s.type_names[f->type[a]];
+    for (int c = 0; c < 3; ++c) s.fbl << s.force_flags[file] << nm << " " << f->forces[3 * a + c] << "\n";
+  }
+  bool st = s.fit_stress != CG_STRESS_NONE && (s.nstress < 0 || f->index < s.nstress);
+  if (st && s.fit_stress == CG_STRESS_DIAG) {
+    for (int c = 0; c < 3; ++c) row_out(" ", false);
+    const char *lab[3] = {"s_xx ", "s_yy ", "s_zz "};
+    for (int c = 0; c < 3; ++c) s.fb << f->stress[c] / kGPa << "\n";
+    s.data_count += 3;
+    for (int c = 0; c < 3; ++c) s.fbl << s.stress_flags[file] << lab[c] << f->stress[c] / kGPa << "\n";
+  } else if (st && s.fit_stress == CG_STRESS_ALL) {
+    for (int c = 0; c < 9; ++c) row_out(" ", false);
+    // xx xy xz / yx yy yz / zx zy zz from (xx yy zz xy xz yz)
+    const int idx[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    const char *lab[9] = {"s_xx ", "s_xy ", "s_xz ", "s_yx ", "s_yy ", "s_yz ", "s_zx ", "s_zy ", "s_zz "};
+    for (int c = 0; c < 9; ++c) s.fb << f->stress[idx[c]] / kGPa << "\n";
+    s.data_count += 9;
+    for (int c = 0; c < 9; ++c) s.fbl << s.stress_flags[file] << lab[c] << f->stress[idx[c]] / kGPa << "\n";
+  }
+  if (s.fit_ener && (s.nener < 0 || f->index < s.nener)) {
+    for (int c = 0; c < 3; ++c) row_out(" ", true);
+    for (int c = 0; c < 3; ++c) {
+      s.fb << f->energy << "\n";
+      s.fbl << s.energy_flags[file] << "+1 " << f->energy << "\n";
+    }
+    s.data_count += 3;
+  }
+  if (!s.fA.good() || !s.fb.good()) { g_err = "Error in A/b file"; return CG_ERR_INPUT; }
+  return CG_OK;
+}
+
+cg_status cg_write_close(const cg_setup *cs) {
+  Setup &s = *cs->s;
+  s.fA.close(); s.fb.close(); s.fbl.close(); s.fna.close();
+  std::ofstream dim(s.outdir + "/dim.txt");
+  if (!dim) { g_err = "Could not open dim.txt"; return CG_ERR_INPUT; }
+  dim << s.ncols << " " << s.data_count << std::endl;
+  return CG_OK;
+}
+
+cg_status cg_write_header(const cg_setup *cs, const char *outdir) {
+  try {
+    std::string d = outdir ? outdir : ".";
+    write_header(*cs->s, d);
+    write_map(*cs->s, d);
+    return CG_OK;
+  } catch (const Error &e) { return fail(e); } catch (const std::exception &e) { return fail(e); }
+}
+
+}  // extern "C"
+
+namespace cg {
+
+static const char *trans_name(int t) {
+  return t == CG_MORSE ? "MORSE" : t == CG_INVRSE_R ? "INVRSE_R" : "NONE";
+}
+
+static void print_special_family(std::ostream &h, const ClusterFamily &F) {
+  if (!F.used || F.types.empty()) return;
+  for (int mode = 0; mode < 2; ++mode) {  // S_MINIM then S_MAXIM
+    int found = 0;
+    for (auto &c : F.types)
+      if ((mode == 0 ? c.special_min : c.special_max) && !c.excluded) ++found;
+    if (!found) continue;
+    h << std::endl << "SPECIAL " << F.natoms << "B " << (mode == 0 ? "S_MINIM" : "S_MAXIM") << ": SPECIFIC " << found << std::endl;
+    for (size_t i = 0; i < F.types.size(); ++i) {
+      const auto &c = F.types[i];
+      if (!(mode == 0 ? c.special_min : c.special_max)) continue;
+      if (c.excluded) continue;
+      auto it = F.name_rev.find((int)i);
+      std::string rev = it == F.name_rev.end() ? "" : it->second;
+      if (mode == 0) h << " " << rev << " ";
+      else h << rev << " ";
+      for (auto &p : c.pair_names) h << p << " ";
+      h << std::fixed << std::setprecision(5);
+      for (double v : (mode == 0 ? c.smin : c.smax)) h << v << " ";
+      h << std::endl;
+    }
+  }
+}
+
+static void print_family(std::ostream &h, const ClusterFamily &F, int natoms, int order) {
+  std::string tup = natoms == 3 ? "TRIPLETS" : "QUADRUPLETS";
+  h << std::endl << "ATOM PAIR " << tup << ": ";
+  if (order == 0 || F.types.empty()) { h << 0 << std::endl; return; }
+  h << F.types.size() << std::endl << std::endl;
+  for (const auto &c : F.types) {
+    h << " INDEX: " << c.index << " ATOMS: ";
+    for (int m = 0; m < c.natoms; ++m) {
+      h << c.atom_names[m];
+      if (m < c.npairs - 1) h << " ";
+    }
+    h << std::endl << " PAIRS: ";
+    for (int m = 0; m < c.npairs; ++m) {
+      h << c.pair_names[m];
+      if (m < c.npairs - 1) h << " ";
+    }
+    if (c.excluded) {
+      h << " EXCLUDED:\n";
+    } else {
+      h << "  " << c.unique_powers.size() << " parameters, " << c.allowed.size() << " total parameters " << std::endl;
+      h << "     index  |  powers  |  equiv index  |  param index  " << std::endl;
+      h << "   ----------------------------------------------------" << std::endl;
+      for (size_t j = 0; j < c.allowed.size(); ++j) {
+        h << "      " << std::setw(6) << std::fixed << std::left << j << " ";
+        h << " ";
+        for (int m = 0; m < c.npairs; ++m) h << std::setw(2) << std::fixed << std::left << c.allowed[j][m] << " ";
+        h << "       " << std::setw(8) << c.equiv_idx[j] << " ";
+        h << "       " << std::setw(8) << c.param_idx[j] << std::endl;
+      }
+    }
+    h << std::endl;
+  }
+}
+
+void write_header(const Setup &s, const std::string &dir) {
+  std::ofstream h(dir + "/params.header");
+  if (!h) throw Error(CG_ERR_INPUT, "cannot write params.header");
+  h << "USECOUL: false" << std::endl;
+  h << "FITCOUL: false" << std::endl;
+  h << "USE3BCH: " << (s.o3 > 0 ? "true" : "false") << std::endl;
+  h << "USE4BCH: " << (s.o4 > 0 ? "true" : "false") << std::endl;
+  h << "EXCLD1B: false" << std::endl;
+  h << "EXCLD2B: false" << std::endl;
+  h << std::endl << "PAIRTYP: " << "CHEBYSHEV" << " ";
+  h << " " << s.o2 << " " << s.o3 << " " << s.o4 << " " << double(s.cheby_low) << " " << double(s.cheby_high) << std::endl;
+  h << std::endl << "ATOM TYPES: " << s.ntypes << std::endl << std::endl;
+  h << "# TYPEIDX #\t# ATM_TYP #\t# ATMCHRG #\t# ATMMASS #" << std::endl;
+  for (int i = 0; i < s.ntypes; ++i)
+    h << i << "\t\t" << s.pairs[i].a1 << "\t\t" << s.pairs[i].chg1 << "\t\t" << s.pairs[i].mass1 << std::endl;
+  h << std::endl << "ATOM PAIRS: " << s.pairs.size() << std::endl << std::endl;
+  h << "# PAIRIDX #\t# ATM_TY1 #\t# ATM_TY1 #\t# S_MINIM #\t# S_MAXIM #\t# CHBDIST #\t# MORSE_LAMBDA #" << std::endl;
+  for (const auto &p : s.pairs) {
+    h << "\t" << std::setw(16) << std::left << p.idx << std::setw(16) << std::left << p.a1
+      << std::setw(16) << std::left << p.a2 << std::setw(16) << std::left << p.smin
+      << std::setw(16) << std::left << p.smax << std::setw(16) << std::left << trans_name(s.transform);
+    if (s.transform == CG_MORSE) h << std::setw(16) << std::left << p.lambda << std::endl;
+    else h << std::endl;
+  }
+  h << std::endl << "FCUT TYPE: " << (s.fcut_type == CG_FCUT_CUBIC ? "CUBIC" : "TERSOFF");
+  if (s.fcut_type == CG_FCUT_TERSOFF) h << " " << s.fcut_offset;
+  h << std::endl;
+  print_special_family(h, s.trips);
+  print_special_family(h, s.quads);
+  print_family(h, s.trips, 3, s.o3);
+  print_family(h, s.quads, 4, s.o4);
+  h << std::endl;
+}
+
+void write_map(const Setup &s, const std::string &dir) {
+  std::ofstream o(dir + "/ff_groups.map");
+  if (!o) throw Error(CG_ERR_INPUT, "cannot write ff_groups.map");
+  o << "PAIRMAPS: " << s.pair_map.size() << std::endl;
+  for (auto &kv : s.pair_map) o << kv.second << " " << kv.first << std::endl;
+  for (const ClusterFamily *F : {&s.trips, &s.quads}) {
+    if (F->name_map.empty()) continue;
+    o << std::endl << (F->natoms == 3 ? "TRIPMAPS: " : "QUADMAPS: ") << F->name_map.size() << std::endl;
+    for (auto &kv : F->name_map) o << kv.second << " " << kv.first << std::endl;
+  }
+}
+
+}  // namespace cg

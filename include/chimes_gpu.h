@@ -1,0 +1,214 @@
+/*
+ * chimes_gpu.h -- C ABI of the MI355X (gfx950) ChIMES design-matrix / fit /
+ * force-evaluation path.
+ *
+ * Everything here is plain C: POD structs, pointers and sizes, integer status
+ * codes.  No exceptions or process exits cross this boundary; the last error
+ * message of a context (or of the thread, for calls without one) is returned
+ * by cg_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the ChIMES-lsq tree):
+ *   cg_setup_read         INPUT::PARSE_INFILE_LSQ + build_clusters
+ *                         (src/input.C:172-225, src/chimes_lsq.C:533-560)
+ *   cg_traj_*             FRAME::READ_XYZF (src/ClassDefs.C:2411-2683)
+ *   cg_assemble           ZCalc_Deriv -> Cheby::Deriv_2B/3B/4B for a batch of
+ *                         frames (src/functions.C:755-804, src/Cheby.C:709-1729)
+ *                         plus A_MAT row layout (src/A_Matrix.C:210-567)
+ *   cg_write_*            A_MAT::PRINT_FRAME / CLEANUP_FILES, print_param_header,
+ *                         print_map_file (src/A_Matrix.C:210-760,
+ *                         src/chimes_lsq.C:562-716)
+ *   cg_fit                chimes_lsq.py SVD path (src/chimes_lsq.py:169-214,284-306)
+ *   cg_write_params       chimes_lsq.py params.txt printer (src/chimes_lsq.py:315-672)
+ *   cg_force              ZCalc -> Cheby::Force_all/Force_3B/Force_4B
+ *                         (src/functions.C:815-891, src/Cheby.C:1731-2675)
+ *   cg_serve_*            no reference counterpart (the reference evaluates one
+ *                         configuration per MD step); a request queue over cg_force.
+ */
+#ifndef CHIMES_GPU_H
+#define CHIMES_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef int cg_status;
+enum {
+  CG_OK = 0,
+  CG_ERR_ARG = 1,        /* bad argument / inconsistent sizes            */
+  CG_ERR_INPUT = 2,      /* malformed fm_setup.in / trajectory           */
+  CG_ERR_UNSUPPORTED = 3,/* option outside this path (FITCOUL, ...)      */
+  CG_ERR_DEVICE = 4,     /* HIP runtime error                            */
+  CG_ERR_MEMORY = 5,     /* allocation / budget                          */
+  CG_ERR_NUMERIC = 6,    /* solver failure                               */
+  CG_ERR_EOF = 7         /* trajectory exhausted                         */
+};
+
+/* Last error message for a context (ctx may be NULL: thread-local message). */
+const char *cg_last_error(const void *ctx);
+
+/* ----------------------------------------------------------- model (POD) */
+#define CG_MAX_ATOM_TYPES 10   /* src/functions.h:34 (INT_MAP key base)      */
+
+enum { CG_MORSE = 0, CG_INVRSE_R = 1, CG_NONE = 2 };            /* Cheby_trans */
+enum { CG_FCUT_CUBIC = 0, CG_FCUT_TERSOFF = 1 };                /* FCUT_TYPE   */
+enum { CG_FIX_ZERO_DERIV = 0, CG_FIX_CONSTANT_DERIV = 1, CG_FIX_SMOOTH = 2 };
+enum { CG_STRESS_NONE = 0, CG_STRESS_DIAG = 1, CG_STRESS_ALL = 2 };
+
+/* One n-body cluster family (3-body triplets or 4-body quadruplets).
+ * Cluster types are in the reference's CLUSTER_LIST::VEC order; canonical
+ * atom order inside a type is the sorted-name order of CLUSTER::ATOM_NAMES,
+ * canonical pair m enumerates atom pairs (i<j) of that order. */
+typedef struct cg_cluster_set {
+  int nbody;          /* 3 or 4; 0 when the family is unused             */
+  int npairs;         /* 3 or 6                                          */
+  int order;          /* powers run 0..order-1 (PAIRTYP token)           */
+  int ntypes;         /* cluster types, excluded ones included           */
+  int total_allowed;  /* sum of N_ALLOWED_POWERS                         */
+  int ncols;          /* sum of N_TRUE_ALLOWED_POWERS                    */
+  double max_cutoff;  /* CLUSTER_LIST::MAX_CUTOFF                        */
+  const int *excluded;      /* [ntypes]                                  */
+  const int *atom_type;     /* [ntypes*nbody] atom-type index per slot   */
+  const int *col_offset;    /* [ntypes] absolute first column            */
+  const int *n_unique;      /* [ntypes] N_TRUE_ALLOWED_POWERS            */
+  const int *perm_offset;   /* [ntypes+1] first allowed set of a type    */
+  const int *perm_powers;   /* [total_allowed*npairs] ALLOWED_POWERS     */
+  const int *perm_col;      /* [total_allowed] PARAM_INDICES (type-rel.) */
+  const int *perm_equiv;    /* [total_allowed] EQUIV_INDICES             */
+  const int *perm_count;    /* [total_allowed] POWER_COUNT               */
+  const double *s_min;      /* [ntypes*npairs] per canonical pair        */
+  const double *s_max;
+  const double *x_avg;
+  const double *x_diff;
+  const int *pair_type;     /* [ntypes*npairs] 2-body pair-type index    */
+  int id_size;              /* CG_MAX_ATOM_TYPES^nbody                   */
+  const int *id_map;        /* [id_size] INT_MAP (-1: excluded/absent)   */
+  const int *id_pair_map;   /* [id_size*npairs] PAIR_INDICES             */
+  const int *id_atom_perm;  /* [id_size*nbody] canonical slot -> position
+                               of that atom in the enumerated tuple      */
+} cg_cluster_set;
+
+typedef struct cg_model {
+  int ntypes;                 /* NATMTYP                                   */
+  const char *type_names;     /* ntypes * 8 bytes, NUL padded              */
+  int npairs;                 /* 2-body pair types                         */
+  const int *pair_t1;         /* [npairs] atom type of ATM1                */
+  const int *pair_t2;
+  const int *int_pair_map;    /* [ntypes*ntypes] INT_PAIR_MAP              */
+  const double *pair_smin;    /* [npairs] S_MINIM                          */
+  const double *pair_smax;    /* [npairs] S_MAXIM                          */
+  const double *pair_lambda;  /* [npairs] MORSE_LAMBDA                     */
+  const double *pair_xavg;    /* [npairs] X_AVG                            */
+  const double *pair_xdiff;   /* [npairs] X_DIFF                           */
+  int transform;              /* CG_MORSE / CG_INVRSE_R / CG_NONE          */
+  int order2b;                /* SNUM: 2-body columns per pair type        */
+  double deriv_const;         /* (CHEBY_RANGE_HIGH-CHEBY_RANGE_LOW)/2      */
+  int cheby_low, cheby_high;  /* PAIRTYP range tokens                      */
+  int fcut_type;              /* CG_FCUT_*                                 */
+  double fcut_offset;
+  int fix_type;               /* CG_FIX_*                                  */
+  double smooth_dist;
+  int nlayers;
+  int small_mode;             /* 1: USENEIG ... SMALL requested            */
+  int fit_stress;             /* CG_STRESS_*                               */
+  int fit_energy;
+  int nstress, nener;         /* -1: every frame                           */
+  int n2b_cols;               /* TOT_SNUM                                  */
+  int ncols_short;            /* TOT_SHORT_RANGE                           */
+  int n_offset_cols;          /* energy-offset columns (0 or ntypes)       */
+  int ncols;                  /* columns of A (TOT_ALL_PARAMS)             */
+  double max_cutoff_2b;       /* NEIGHBORS::MAX_CUTOFF                     */
+  cg_cluster_set c3, c4;
+} cg_model;
+
+/* -------------------------------------------------------------- frames */
+typedef struct cg_frame {
+  int natoms;
+  const double *xyz;      /* [natoms*3] Angstrom, as read                  */
+  const int *type;        /* [natoms] atom-type index                      */
+  double cell[9];         /* a, b, c cell vectors (row-wise)               */
+  const double *forces;   /* [natoms*3] kcal/mol/A (converted) or NULL     */
+  double stress[6];       /* xx yy zz xy xz yz, GPa as read                */
+  double energy;          /* kcal/mol                                      */
+  int index;              /* frame index in the trajectory                 */
+} cg_frame;
+
+/* --------------------------------------------------- host-side setup */
+typedef struct cg_setup cg_setup;
+cg_status cg_setup_read(const char *fm_setup_path, cg_setup **out);
+void cg_setup_free(cg_setup *s);
+const cg_model *cg_setup_model(const cg_setup *s);
+int cg_setup_nframes(const cg_setup *s);
+/* rows of A produced by a frame (3N + stress rows + energy rows) */
+int64_t cg_model_frame_rows(const cg_model *m, int natoms, int frame_index);
+
+/* Trajectory reader: frames are owned by the reader until the next call. */
+typedef struct cg_traj cg_traj;
+cg_status cg_traj_open(const cg_setup *s, const char *workdir, cg_traj **out);
+cg_status cg_traj_next(cg_traj *t, cg_frame *f);
+void cg_traj_close(cg_traj *t);
+
+/* Text writers (byte layout of the reference's files).  rows: row-major
+ * [nrows*ncols] for the frame in reference row order. */
+cg_status cg_write_open(const cg_setup *s, const char *outdir);
+cg_status cg_write_frame(const cg_setup *s, const cg_frame *f, const double *rows, int nrows);
+cg_status cg_write_close(const cg_setup *s);
+cg_status cg_write_header(const cg_setup *s, const char *outdir); /* params.header + ff_groups.map */
+
+/* ------------------------------------------------------- device context */
+typedef struct cg_ctx cg_ctx;
+enum { CG_FP64 = 0, CG_FP32 = 1 };
+
+cg_status cg_create(const cg_model *m, int device, cg_ctx **out);
+void cg_destroy(cg_ctx *ctx);
+
+/* Assemble the A rows of nframes frames (reference row order, frames in
+ * order).  Output lives on the device: *A_dev points to a row-major
+ * [rows x ncols] array of double (CG_FP64) or float (CG_FP32) owned by the
+ * context and valid until the next cg_assemble / cg_destroy.  b is written
+ * to the host array b_host (may be NULL) as doubles.  rows_out receives the
+ * total row count. */
+cg_status cg_assemble(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec,
+                      const void **A_dev, double *b_host, int64_t *rows_out);
+/* Copy the last assembled A (rows [r0, r0+nrows)) to host doubles. */
+cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host);
+/* Kernel timing of the last cg_assemble (ms, measured with HIP events on the
+ * context stream): neighbour+enumeration, assembly kernels, total. */
+cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, double *ms_total);
+
+/* Fit: solve A x ~ b for the last assembled A (on device).
+ *   mode CG_FP64 : FP64 Gram on MFMA, FP64 eigensolve, FP64 refinement.
+ *   mode CG_FP32 : A in FP32, split-BF16x3 Gram on MFMA (FP32 accumulate),
+ *                  FP64 eigensolve + FP64 residual iterative refinement.
+ * eps: relative singular-value cut (chimes_lsq.py --eps), refine: iterations.
+ * Outputs: x[ncols], rms (over all rows), nvars kept, sigma_max. */
+typedef struct cg_fit_result {
+  double rms;
+  double max_abs_x;
+  int nvars;
+  double eps_abs;       /* eps * sigma_max                                 */
+  double bic;
+  double ms_gram, ms_solve, ms_refine;
+} cg_fit_result;
+cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters,
+                 double *x, cg_fit_result *res);
+/* y = A x for the last assembled A (host output). */
+cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y);
+cg_status cg_write_params(const cg_setup *s, const char *outdir, const double *x,
+                          int ncols, int64_t nrows, const cg_fit_result *res,
+                          const double *y, const double *b);
+
+/* ------------------------------------------------------ force evaluation */
+/* Force/energy/stress of one configuration with coefficients x (the fitted
+ * vector in A column order).  F: [natoms*3] kcal/mol/A, E kcal/mol,
+ * virial9: sum over interactions of -f r (kcal/mol), no 1/V. */
+cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f,
+                   double *F, double *E, double *virial9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

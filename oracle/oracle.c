@@ -1,0 +1,590 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the ChIMES-lsq design-matrix and
+ * force-evaluation arithmetic.  TEST INFRASTRUCTURE ONLY: it is the checker
+ * for the HIP path (tests/, __graft_entry__.smoke(), bench.py's cpu_baseline
+ * leg).  Nothing in the shipped library links or calls it.
+ *
+ * It follows the reference loop for loop (paths relative to the ChIMES-lsq tree):
+ *   cell geometry      BOX::UPDATE_CELL/UPDATE_INVER_CELL/LAMMPSIFY/UPDATE_EXTENT/
+ *                      UPDATE_VOLUME           src/ClassDefs.C:1740-2021,2194-2211
+ *   wrap + ghosts      BOX::WRAP_ATOM, LAYER_ATOM, FRAME::build_layers
+ *                                              src/ClassDefs.C:2053-2163,2685-2752
+ *   pair vector        BOX::GET_DISTANCE (no MIC with ghosts) src/ClassDefs.C:2321-2358
+ *   neighbour lists    NEIGHBORS::DO_UPDATE_BIG / DO_UPDATE_SMALL (same sets; the
+ *                      bins only accelerate)   src/ClassDefs.C:137-393
+ *   transform/polys    Cheby::transform, set_polys, set_polys_out_of_range,
+ *                      cheby_var_deriv        src/Cheby.C:58-165,207-443
+ *   cutoff             FCUT::get_fcut, PROCEED src/Fcut.C:17-72,125-140
+ *   assembly           Cheby::Deriv_2B/3B/4B  src/Cheby.C:709-1729
+ *   row layout         A_MAT::PRINT_FRAME      src/A_Matrix.C:210-567
+ *   force evaluation   Cheby::Force_all/Force_3B/Force_4B src/Cheby.C:1731-2675
+ *
+ * Parity pins for each function are listed in tests/test_oracle.py.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/chimes_gpu.h"
+
+#define PI_REF 3.14159265359 /* src/functions.h:47 */
+
+typedef struct {
+  double H[9], Hi[9];           /* HMAT, INVR_HMAT (row-major as reference) */
+  double ax, ay, az, bx, by, bz, cx, cy, cz;
+  double la, lb, lc, alpha, beta, gamma;
+  double lx, ly, lz, xy, xz, yz;
+  double xlo, ylo, zlo, xhi, yhi, zhi, ex, ey, ez, vol;
+  int ortho;
+} box_t;
+
+static double vmag(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+static double vangle(double x1, double y1, double z1, double x2, double y2, double z2) {
+  return acos((x1 * x2 + y1 * y2 + z1 * z2) / (vmag(x1, y1, z1) * vmag(x2, y2, z2)));
+}
+
+static int box_setup(const double *cell, box_t *b) {
+  b->ax = cell[0]; b->ay = cell[1]; b->az = cell[2];
+  b->bx = cell[3]; b->by = cell[4]; b->bz = cell[5];
+  b->cx = cell[6]; b->cy = cell[7]; b->cz = cell[8];
+  double *H = b->H;
+  H[0] = b->ax; H[1] = b->bx; H[2] = b->cx;
+  H[3] = b->ay; H[4] = b->by; H[5] = b->cy;
+  H[6] = b->az; H[7] = b->bz; H[8] = b->cz;
+  b->ortho = (fabs(b->ay) + fabs(b->az) + fabs(b->bx) + fabs(b->bz) + fabs(b->cx) + fabs(b->cy) > 1.0e-10) ? 0 : 1;
+  double det = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+               H[2] * (H[3] * H[7] - H[4] * H[6]);
+  if (det == 0) return -1;
+  double T[9];
+  T[0] = (H[4] * H[8] - H[5] * H[7]);
+  T[1] = -1 * (H[3] * H[8] - H[5] * H[6]);
+  T[2] = (H[3] * H[7] - H[4] * H[6]);
+  T[3] = -1 * (H[1] * H[8] - H[2] * H[7]);
+  T[4] = (H[0] * H[8] - H[2] * H[6]);
+  T[5] = -1 * (H[0] * H[7] - H[1] * H[6]);
+  T[6] = (H[1] * H[5] - H[2] * H[4]);
+  T[7] = -1 * (H[0] * H[5] - H[2] * H[3]);
+  T[8] = (H[0] * H[4] - H[1] * H[3]);
+  double *I = b->Hi;
+  I[0] = T[0]; I[1] = T[3]; I[2] = T[6];
+  I[3] = T[1]; I[4] = T[4]; I[5] = T[7];
+  I[6] = T[2]; I[7] = T[5]; I[8] = T[8];
+  for (int i = 0; i < 9; ++i) I[i] /= det;
+  b->la = vmag(b->ax, b->ay, b->az);
+  b->lb = vmag(b->bx, b->by, b->bz);
+  b->lc = vmag(b->cx, b->cy, b->cz);
+  b->alpha = vangle(b->bx, b->by, b->bz, b->cx, b->cy, b->cz);
+  b->beta = vangle(b->cx, b->cy, b->cz, b->ax, b->ay, b->az);
+  b->gamma = vangle(b->ax, b->ay, b->az, b->bx, b->by, b->bz);
+  /* LAMMPSIFY */
+  b->lx = b->la;
+  b->xy = b->lb * cos(b->gamma); if (fabs(b->xy) < 1e-12) b->xy = 0.0;
+  b->xz = b->lc * cos(b->beta); if (fabs(b->xz) < 1e-12) b->xz = 0.0;
+  b->ly = sqrt(b->lb * b->lb - b->xy * b->xy);
+  b->yz = (b->lb * b->lc * cos(b->alpha) - b->xy * b->xz) / b->ly; if (fabs(b->yz) < 1e-12) b->yz = 0.0;
+  b->lz = sqrt(b->lc * b->lc - b->xz * b->xz - b->yz * b->yz);
+  /* UPDATE_EXTENT */
+  b->xlo = b->ylo = b->zlo = 0.0;
+  if (b->xy < b->xlo) b->xlo = b->xy;
+  if (b->xz < b->xlo) b->xlo = b->xz;
+  if (b->xy + b->xz < b->xlo) b->xlo = b->xy + b->xz;
+  if (b->yz < b->ylo) b->ylo = b->yz;
+  double t = 0.0;
+  b->xhi = b->ax;
+  if (b->xy > t) t = b->xy;
+  if (b->xz > t) t = b->xz;
+  if (b->xy + b->xz > t) t = b->xy + b->xz;
+  b->xhi += t;
+  b->yhi = b->by; if (b->yz > 0.0) b->yhi += b->yz;
+  b->zhi = b->cz;
+  b->ex = b->xhi - b->xlo; b->ey = b->yhi - b->ylo; b->ez = b->zhi - b->zlo;
+  /* UPDATE_VOLUME */
+  if (b->ortho) b->vol = b->ax * b->by * b->cz;
+  else {
+    double v = 1;
+    v += 2 * cos(b->alpha) * cos(b->beta) * cos(b->gamma);
+    v -= cos(b->alpha) * cos(b->alpha);
+    v -= cos(b->beta) * cos(b->beta);
+    v -= cos(b->gamma) * cos(b->gamma);
+    b->vol = b->la * b->lb * b->lc * sqrt(v);
+  }
+  return 0;
+}
+
+static void mulv(const double *M, const double *x, double *y) {
+  y[0] = M[0] * x[0] + M[1] * x[1] + M[2] * x[2];
+  y[1] = M[3] * x[0] + M[4] * x[1] + M[5] * x[2];
+  y[2] = M[6] * x[0] + M[7] * x[1] + M[8] * x[2];
+}
+
+static void wrap_atom(const box_t *b, const double *u, double *w) {
+  if (b->ortho) {
+    double ix = floor(u[0] / b->lx), iy = floor(u[1] / b->ly), iz = floor(u[2] / b->lz);
+    w[0] = u[0] - ix * b->lx; w[1] = u[1] - iy * b->ly; w[2] = u[2] - iz * b->lz;
+  } else {
+    double f[3];
+    mulv(b->Hi, u, f);
+    f[0] -= floor(f[0]); f[1] -= floor(f[1]); f[2] -= floor(f[2]);
+    mulv(b->H, f, w);
+  }
+}
+
+static void layer_atom(const box_t *b, const double *r, const int *n, double *o) {
+  if (b->ortho) {
+    o[0] = r[0] + n[0] * b->ax; o[1] = r[1] + n[1] * b->by; o[2] = r[2] + n[2] * b->cz;
+  } else {
+    double f[3];
+    mulv(b->Hi, r, f);
+    f[0] += n[0]; f[1] += n[1]; f[2] += n[2];
+    mulv(b->H, f, o);
+  }
+}
+
+static void get_dist_vec(const box_t *b, const double *p1, const double *p2, double *rab) {
+  double s1[3], s2[3], d[3];
+  mulv(b->Hi, p1, s1);
+  mulv(b->Hi, p2, s2);
+  d[0] = s2[0] - s1[0]; d[1] = s2[1] - s1[1]; d[2] = s2[2] - s1[2];
+  mulv(b->H, d, rab);
+}
+
+/* ---------------------------------------------------- polynomials, cutoff */
+static void transform(const cg_model *m, double r, double xdiff, double xavg, double lambda, double *x, double *e) {
+  switch (m->transform) {
+    case CG_MORSE: *e = exp(-r / lambda); *x = (*e - xavg) / xdiff; break;
+    case CG_INVRSE_R: *e = 0.0; *x = (1.0 / r - xavg) / xdiff; break;
+    default: *e = 0.0; *x = (r - xavg) / xdiff; break;
+  }
+}
+
+static double dxdr_of(const cg_model *m, double xdiff, double r, double lambda, double e) {
+  switch (m->transform) {
+    case CG_MORSE: return (-e / lambda) / xdiff;
+    case CG_INVRSE_R: return -1.0 / (r * r * xdiff);
+    default: return 1.0 / xdiff;
+  }
+}
+
+static void polys_in_range(const cg_model *m, double *Tn, double *Tnd, double r, double xdiff, double xavg,
+                           double lambda, int snum) {
+  double x, e;
+  transform(m, r, xdiff, xavg, lambda, &x, &e);
+  Tn[0] = 1.0; Tn[1] = x;
+  Tnd[0] = 1.0; Tnd[1] = 2.0 * x;
+  for (int i = 2; i <= snum; ++i) {
+    Tn[i] = 2.0 * x * Tn[i - 1] - Tn[i - 2];
+    Tnd[i] = 2.0 * x * Tnd[i - 1] - Tnd[i - 2];
+  }
+  double dxdr = m->deriv_const * dxdr_of(m, xdiff, r, lambda, e);
+  for (int i = snum; i >= 1; --i) Tnd[i] = i * dxdr * Tnd[i - 1];
+  Tnd[0] = 0.0;
+}
+
+/* Cheby::set_polys / set_polys_out_of_range */
+static void set_polys(const cg_model *m, int ptype, double *Tn, double *Tnd, double r, double xdiff, double xavg,
+                      int snum, double smin) {
+  double lambda = m->pair_lambda[ptype];
+  if (r >= smin) { polys_in_range(m, Tn, Tnd, r, xdiff, xavg, lambda, snum); return; }
+  if (m->fix_type == CG_FIX_ZERO_DERIV) {
+    double x, e;
+    transform(m, smin, xdiff, xavg, lambda, &x, &e);
+    Tn[0] = 1.0; Tn[1] = x;
+    Tnd[0] = 0.0; Tnd[1] = 0.0;
+    for (int i = 2; i <= snum; ++i) { Tn[i] = 2.0 * x * Tn[i - 1] - Tn[i - 2]; Tnd[i] = 0.0; }
+  } else if (m->fix_type == CG_FIX_CONSTANT_DERIV) {
+    polys_in_range(m, Tn, Tnd, smin, xdiff, xavg, lambda, snum);
+    for (int i = 0; i <= snum; ++i) Tn[i] += Tnd[i] * (r - smin);
+  } else {
+    polys_in_range(m, Tn, Tnd, smin, xdiff, xavg, lambda, snum);
+    double d = m->smooth_dist;
+    double f = exp((r - smin) / d);
+    for (int i = 0; i <= snum; ++i) {
+      Tn[i] += d * (f - 1.0) * Tnd[i];
+      Tnd[i] *= f;
+    }
+  }
+}
+
+static void get_fcut(const cg_model *m, double *fc, double *fcd, double r, double rmax) {
+  if (m->fcut_type == CG_FCUT_CUBIC) {
+    double f0 = 1.0 - r / rmax;
+    *fc = pow(f0, 3);
+    *fcd = pow(f0, 2);
+    *fcd *= -1.0 * 3 / rmax;
+  } else {
+    double th = rmax - m->fcut_offset * rmax;
+    if (r < th) { *fc = 1.0; *fcd = 0.0; }
+    else if (r > rmax) { *fc = 0.0; *fcd = 0.0; }
+    else {
+      double f0 = (r - th) / (rmax - th) * PI_REF + PI_REF / 2.0;
+      double f0d = PI_REF / (rmax - th);
+      *fc = 0.5 + 0.5 * sin(f0);
+      *fcd = 0.5 * cos(f0) * f0d;
+    }
+  }
+}
+
+static int proceed(double r, double rmax) { return r < rmax; }
+
+/* ----------------------------------------------------------- frame setup */
+typedef struct {
+  box_t box;
+  int n, nall;
+  double *all;     /* [nall*3] ALL_COORDS */
+  int *parent;     /* [nall] */
+  int *type;       /* [nall] */
+  int small;
+  double ps[5];    /* PERM_SCALE */
+  int *l2, *l3, *l4; /* flattened lists */
+  int *o2, *o3, *o4; /* offsets [n+1] */
+} sys_t;
+
+static void sys_free(sys_t *s) {
+  free(s->all); free(s->parent); free(s->type);
+  free(s->l2); free(s->l3); free(s->l4); free(s->o2); free(s->o3); free(s->o4);
+}
+
+static double dist(const sys_t *s, int a1, int a2, double *rab) {
+  get_dist_vec(&s->box, s->all + 3 * a1, s->all + 3 * a2, rab);
+  return sqrt(rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2]);
+}
+
+/* small_state: in/out sticky SMALL flag (NEIGHBORS::UPDATE_WITH_BIG). */
+static int sys_build(const cg_model *m, const cg_frame *f, int *small_state, sys_t *s) {
+  memset(s, 0, sizeof(*s));
+  if (box_setup(f->cell, &s->box)) return -1;
+  int L = m->nlayers;
+  int ncell = (2 * L + 1) * (2 * L + 1) * (2 * L + 1);
+  s->n = f->natoms;
+  s->nall = f->natoms * ncell;
+  s->all = (double *)malloc(sizeof(double) * 3 * s->nall);
+  s->parent = (int *)malloc(sizeof(int) * s->nall);
+  s->type = (int *)malloc(sizeof(int) * s->nall);
+  for (int a = 0; a < s->n; ++a) {
+    wrap_atom(&s->box, f->xyz + 3 * a, s->all + 3 * a);
+    s->parent[a] = a;
+    s->type[a] = f->type[a];
+  }
+  int idx = s->n;
+  for (int n1 = -L; n1 <= L; ++n1)
+    for (int n2 = -L; n2 <= L; ++n2)
+      for (int n3 = -L; n3 <= L; ++n3) {
+        if (n1 == 0 && n2 == 0 && n3 == 0) continue;
+        int nn[3] = {n1, n2, n3};
+        for (int a = 0; a < s->n; ++a) {
+          layer_atom(&s->box, s->all + 3 * a, nn, s->all + 3 * idx);
+          s->parent[idx] = a;
+          s->type[idx] = f->type[a];
+          ++idx;
+        }
+      }
+  double mc2 = m->max_cutoff_2b, mc3 = mc2, mc4 = mc2;
+  if (m->c3.nbody) mc3 = m->c3.max_cutoff;
+  if (m->c4.nbody) mc4 = m->c4.max_cutoff;
+  double mx = mc2;
+  if (mc3 > mx) mx = mc3;
+  if (mc4 > mx) mx = mc4;
+  if (m->small_mode || s->box.ex <= mx || s->box.ey <= mx || s->box.ez <= mx) *small_state = 1;
+  s->small = *small_state;
+  s->ps[0] = s->ps[1] = 1.0;
+  for (int j = 2; j < 5; ++j) s->ps[j] = s->small ? s->ps[j - 1] / j : 1.0;
+  if (!s->small) for (int j = 0; j < 5; ++j) s->ps[j] = 1.0;
+  const double pad = 0.3;
+  /* lists in ghost-index order */
+  int cap = 64;
+  s->l2 = (int *)malloc(sizeof(int) * cap); s->l3 = (int *)malloc(sizeof(int) * cap); s->l4 = (int *)malloc(sizeof(int) * cap);
+  int c2 = 0, c3 = 0, c4 = 0, k2 = cap, k3 = cap, k4 = cap;
+  s->o2 = (int *)malloc(sizeof(int) * (s->n + 1));
+  s->o3 = (int *)malloc(sizeof(int) * (s->n + 1));
+  s->o4 = (int *)malloc(sizeof(int) * (s->n + 1));
+  for (int a1 = 0; a1 < s->n; ++a1) {
+    s->o2[a1] = c2; s->o3[a1] = c3; s->o4[a1] = c4;
+    for (int a2 = 0; a2 < s->nall; ++a2) {
+      if (a2 == a1) continue;
+      if (!s->small && a1 > s->parent[a2]) continue;
+      double rab[3];
+      double r = dist(s, a1, a2, rab);
+#define PUSH(l, c, k) do { if (c == k) { k *= 2; l = (int *)realloc(l, sizeof(int) * k); } l[c++] = a2; } while (0)
+      if (r < mc2 + pad) PUSH(s->l2, c2, k2);
+      if (r < mc3 + pad) PUSH(s->l3, c3, k3);
+      if (r < mc4 + pad) PUSH(s->l4, c4, k4);
+#undef PUSH
+    }
+  }
+  s->o2[s->n] = c2; s->o3[s->n] = c3; s->o4[s->n] = c4;
+  return 0;
+}
+
+/* ------------------------------------------------------------- assembly */
+/* Accumulators for one frame: FORCES[N][P] xyz, STRESSES[P] (6), ENERGIES[P]. */
+typedef struct {
+  int P;
+  double *F;   /* [N*P*3] */
+  double *S;   /* [P*6]  xx yy zz xy xz yz */
+  double *E;   /* [P] */
+} acc_t;
+
+static void add_pair_terms(const cg_model *m, acc_t *A, int a, int b, int col, double g, const double *rab, double r) {
+  for (int c = 0; c < 3; ++c) {
+    double d = g * rab[c] / r;
+    A->F[((size_t)a * A->P + col) * 3 + c] += d;
+    A->F[((size_t)b * A->P + col) * 3 + c] -= d;
+  }
+  if (m->fit_stress == CG_STRESS_DIAG) {
+    A->S[col * 6 + 0] -= g * rab[0] * rab[0] / r;
+    A->S[col * 6 + 1] -= g * rab[1] * rab[1] / r;
+    A->S[col * 6 + 2] -= g * rab[2] * rab[2] / r;
+  } else if (m->fit_stress == CG_STRESS_ALL) {
+    A->S[col * 6 + 0] -= g * rab[0] * rab[0] / r;
+    A->S[col * 6 + 3] -= g * rab[0] * rab[1] / r;
+    A->S[col * 6 + 4] -= g * rab[0] * rab[2] / r;
+    A->S[col * 6 + 1] -= g * rab[1] * rab[1] / r;
+    A->S[col * 6 + 5] -= g * rab[1] * rab[2] / r;
+    A->S[col * 6 + 2] -= g * rab[2] * rab[2] / r;
+  }
+}
+
+static int pair_index(const cg_model *m, const sys_t *s, int a1, int a2) {
+  return m->int_pair_map[s->type[s->parent[a1]] * m->ntypes + s->type[s->parent[a2]]];
+}
+
+static void deriv_2b(const cg_model *m, const sys_t *s, acc_t *A, int fit_stress, int fit_ener) {
+  int snum = m->order2b;
+  double Tn[64], Tnd[64];
+  cg_model mm = *m;
+  mm.fit_stress = fit_stress;
+  for (int a1 = 0; a1 < s->n; ++a1)
+    for (int k = s->o2[a1]; k < s->o2[a1 + 1]; ++k) {
+      int a2 = s->l2[k];
+      int p = pair_index(m, s, a1, a2);
+      int vstart = p * snum;
+      double rab[3];
+      double r = dist(s, a1, a2, rab);
+      if (!(r > m->pair_smin[p] && r < m->pair_smax[p])) continue;
+      set_polys(m, p, Tn, Tnd, r, m->pair_xdiff[p], m->pair_xavg[p], snum, m->pair_smin[p]);
+      double fc, fcd;
+      get_fcut(m, &fc, &fcd, r, m->pair_smax[p]);
+      int b = s->parent[a2];
+      for (int i = 0; i < snum; ++i) {
+        double g = s->ps[2] * (fc * Tnd[i + 1] + fcd * Tn[i + 1]);
+        add_pair_terms(&mm, A, a1, b, vstart + i, g, rab, r);
+        if (fit_ener) A->E[vstart + i] += s->ps[2] * fc * Tn[i + 1];
+      }
+    }
+}
+
+static int make_id(const int *t, int n) {
+  int sum = 0, w = 1;
+  for (int i = 0; i < n; ++i) { sum += w * (t[i] + 1); w *= CG_MAX_ATOM_TYPES; }
+  return sum;
+}
+
+/* Deriv_3B (nb=3) and Deriv_4B (nb=4) share one enumeration. */
+static void deriv_nb(const cg_model *m, const cg_cluster_set *cs, const sys_t *s, acc_t *A, int fit_stress,
+                     int fit_ener) {
+  const int nb = cs->nbody, np = cs->npairs;
+  const int *lst = nb == 3 ? s->l3 : s->l4;
+  const int *off = nb == 3 ? s->o3 : s->o4;
+  const double perm_scale = s->ps[nb];
+  double Tn[6][32], Tnd[6][32];
+  int at[4];
+  cg_model mm = *m;
+  mm.fit_stress = fit_stress;
+  for (int a1 = 0; a1 < s->n; ++a1) {
+    int b = off[a1], e = off[a1 + 1];
+    for (int i2 = b; i2 < e; ++i2)
+      for (int i3 = b; i3 < e; ++i3) {
+        int a2 = lst[i2], a3 = lst[i3];
+        if (a3 == a2) continue;
+        if (perm_scale == 1.0 && s->parent[a2] > s->parent[a3]) continue;
+        for (int i4 = (nb == 4 ? b : 0); i4 < (nb == 4 ? e : 1); ++i4) {
+          int a4 = nb == 4 ? lst[i4] : -1;
+          if (nb == 4) {
+            if (a2 == a4 || a3 == a4) continue;
+            if (perm_scale == 1.0 && s->parent[a3] > s->parent[a4]) continue;
+          }
+          int atoms[4] = {a1, a2, a3, a4};
+          for (int q = 0; q < nb; ++q) at[q] = s->type[atoms[q]];
+          int id = make_id(at, nb);
+          int ct = cs->id_map[id];
+          if (ct < 0) continue;
+          const int *pmap = cs->id_pair_map + (size_t)id * np;
+          double rab[6][3], r[6];
+          int pa[6], pb[6];
+          int q = 0;
+          for (int i = 0; i < nb; ++i)
+            for (int j = i + 1; j < nb; ++j) {
+              r[q] = dist(s, atoms[i], atoms[j], rab[q]);
+              pa[q] = i; pb[q] = j;
+              ++q;
+            }
+          int ok = 1;
+          double smin[6], smax[6], xavg[6], xdiff[6];
+          for (int f = 0; f < np; ++f) {
+            smax[f] = cs->s_max[ct * np + pmap[f]];
+            smin[f] = cs->s_min[ct * np + pmap[f]];
+            xavg[f] = cs->x_avg[ct * np + pmap[f]];
+            xdiff[f] = cs->x_diff[ct * np + pmap[f]];
+            if (!proceed(r[f], smax[f])) { ok = 0; break; }
+          }
+          if (!ok) continue;
+          double fc[6], fcd[6];
+          for (int f = 0; f < np; ++f) {
+            int pt = pair_index(m, s, atoms[pa[f]], atoms[pb[f]]);
+            set_polys(m, pt, Tn[f], Tnd[f], r[f], xdiff[f], xavg[f], cs->order, smin[f]);
+          }
+          for (int f = 0; f < np; ++f) get_fcut(m, &fc[f], &fcd[f], r[f], smax[f]);
+          int vstart = cs->col_offset[ct];
+          int parent[4];
+          for (int i = 0; i < nb; ++i) parent[i] = i == 0 ? a1 : s->parent[atoms[i]];
+          for (int i = cs->perm_offset[ct]; i < cs->perm_offset[ct + 1]; ++i) {
+            int col = vstart + cs->perm_col[i];
+            int pw[6];
+            for (int f = 0; f < np; ++f) pw[f] = cs->perm_powers[(size_t)i * np + pmap[f]];
+            double deriv[6], val[6];
+            for (int f = 0; f < np; ++f) {
+              deriv[f] = perm_scale * (fc[f] * Tnd[f][pw[f]] + fcd[f] * Tn[f][pw[f]]);
+              val[f] = fc[f] * Tn[f][pw[f]];
+            }
+            for (int f = 0; f < np; ++f) {
+              double g = deriv[f];
+              for (int h = 0; h < np; ++h) if (h != f) g *= val[h];
+              add_pair_terms(&mm, A, parent[pa[f]], parent[pb[f]], col, g, rab[f], r[f]);
+            }
+            if (fit_ener) {
+              double en = perm_scale;
+              for (int f = 0; f < np; ++f) en *= val[f];
+              A->E[col] += en;
+            }
+          }
+        }
+      }
+  }
+}
+
+/* Rows of one frame in reference order; returns number of rows written.
+ * out: [rows x ncols] row-major.  small_state is the sticky SMALL flag. */
+int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, double *out) {
+  sys_t s;
+  if (sys_build(m, f, small_state, &s)) return -1;
+  const int P = m->ncols_short, nc = m->ncols;
+  int fit_stress = (m->fit_stress != CG_STRESS_NONE && (m->nstress < 0 || f->index < m->nstress)) ? m->fit_stress : 0;
+  int fit_ener = m->fit_energy && (m->nener < 0 || f->index < m->nener);
+  acc_t A;
+  A.P = P;
+  A.F = (double *)calloc((size_t)s.n * P * 3, sizeof(double));
+  A.S = (double *)calloc((size_t)P * 6, sizeof(double));
+  A.E = (double *)calloc((size_t)P, sizeof(double));
+  double inv_vol = 1.0 / s.box.vol;
+  deriv_2b(m, &s, &A, fit_stress, fit_ener);
+  for (int i = 0; i < m->n2b_cols; ++i) for (int c = 0; c < 6; ++c) A.S[i * 6 + c] *= inv_vol;
+  if (m->c3.nbody) {
+    deriv_nb(m, &m->c3, &s, &A, fit_stress, fit_ener);
+    for (int i = 0; i < m->c3.ncols; ++i) for (int c = 0; c < 6; ++c) A.S[(m->n2b_cols + i) * 6 + c] *= inv_vol;
+  }
+  if (m->c4.nbody) {
+    deriv_nb(m, &m->c4, &s, &A, fit_stress, fit_ener);
+    int b = m->n2b_cols + m->c3.ncols;
+    for (int i = 0; i < m->c4.ncols; ++i) for (int c = 0; c < 6; ++c) A.S[(b + i) * 6 + c] *= inv_vol;
+  }
+  int counts[CG_MAX_ATOM_TYPES] = {0};
+  for (int a = 0; a < s.n; ++a) counts[f->type[a]]++;
+  int row = 0;
+  for (int a = 0; a < s.n; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double *o = out + (size_t)row * nc;
+      for (int p = 0; p < P; ++p) o[p] = A.F[((size_t)a * P + p) * 3 + c];
+      for (int t = 0; t < m->n_offset_cols; ++t) o[P + t] = 0.0;
+      ++row;
+    }
+  if (fit_stress) {
+    static const int diag[3] = {0, 1, 2};
+    static const int all[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    int nr = fit_stress == CG_STRESS_ALL ? 9 : 3;
+    const int *sel = fit_stress == CG_STRESS_ALL ? all : diag;
+    for (int k = 0; k < nr; ++k) {
+      double *o = out + (size_t)row * nc;
+      for (int p = 0; p < P; ++p) o[p] = A.S[p * 6 + sel[k]];
+      for (int t = 0; t < m->n_offset_cols; ++t) o[P + t] = 0.0;
+      ++row;
+    }
+  }
+  if (fit_ener) {
+    for (int k = 0; k < 3; ++k) {
+      double *o = out + (size_t)row * nc;
+      for (int p = 0; p < P; ++p) o[p] = A.E[p];
+      for (int t = 0; t < m->n_offset_cols; ++t) o[P + t] = counts[t];
+      ++row;
+    }
+  }
+  free(A.F); free(A.S); free(A.E);
+  sys_free(&s);
+  return row;
+}
+
+/* Number of (2-body pairs, 3-body, 4-body clusters) the frame contributes;
+ * used by tests to cross-check enumeration counts. */
+int orc_count_clusters(const cg_model *m, const cg_frame *f, int *small_state, long long *counts3) {
+  sys_t s;
+  if (sys_build(m, f, small_state, &s)) return -1;
+  counts3[0] = counts3[1] = counts3[2] = 0;
+  for (int a1 = 0; a1 < s.n; ++a1)
+    for (int k = s.o2[a1]; k < s.o2[a1 + 1]; ++k) {
+      int a2 = s.l2[k];
+      int p = pair_index(m, &s, a1, a2);
+      double rab[3];
+      double r = dist(&s, a1, a2, rab);
+      if (r > m->pair_smin[p] && r < m->pair_smax[p]) counts3[0]++;
+    }
+  for (int fam = 0; fam < 2; ++fam) {
+    const cg_cluster_set *cs = fam == 0 ? &m->c3 : &m->c4;
+    if (!cs->nbody) continue;
+    int nb = cs->nbody, np = cs->npairs;
+    const int *lst = nb == 3 ? s.l3 : s.l4;
+    const int *off = nb == 3 ? s.o3 : s.o4;
+    double ps = s.ps[nb];
+    for (int a1 = 0; a1 < s.n; ++a1) {
+      int b = off[a1], e = off[a1 + 1];
+      for (int i2 = b; i2 < e; ++i2)
+        for (int i3 = b; i3 < e; ++i3) {
+          int a2 = lst[i2], a3 = lst[i3];
+          if (a3 == a2) continue;
+          if (ps == 1.0 && s.parent[a2] > s.parent[a3]) continue;
+          for (int i4 = (nb == 4 ? b : 0); i4 < (nb == 4 ? e : 1); ++i4) {
+            int a4 = nb == 4 ? lst[i4] : -1;
+            if (nb == 4 && (a2 == a4 || a3 == a4)) continue;
+            if (nb == 4 && ps == 1.0 && s.parent[a3] > s.parent[a4]) continue;
+            int atoms[4] = {a1, a2, a3, a4}, at[4];
+            for (int q = 0; q < nb; ++q) at[q] = s.type[atoms[q]];
+            int id = make_id(at, nb);
+            int ct = cs->id_map[id];
+            if (ct < 0) continue;
+            const int *pmap = cs->id_pair_map + (size_t)id * np;
+            int q = 0, ok = 1;
+            for (int i = 0; i < nb && ok; ++i)
+              for (int j = i + 1; j < nb; ++j) {
+                double rab[3];
+                double r = dist(&s, atoms[i], atoms[j], rab);
+                if (!proceed(r, cs->s_max[ct * np + pmap[q]])) { ok = 0; break; }
+                ++q;
+              }
+            if (ok) counts3[1 + fam]++;
+          }
+        }
+    }
+  }
+  sys_free(&s);
+  return 0;
+}
+
+/* Primitive pieces exposed for known-answer tests. */
+void orc_polys(const cg_model *m, int ptype, double r, double xdiff, double xavg, int snum, double smin,
+               double *Tn, double *Tnd) {
+  set_polys(m, ptype, Tn, Tnd, r, xdiff, xavg, snum, smin);
+}
+void orc_fcut(const cg_model *m, double r, double rmax, double *fc, double *fcd) { get_fcut(m, fc, fcd, r, rmax); }
+double orc_volume(const double *cell) {
+  box_t b;
+  if (box_setup(cell, &b)) return 0;
+  return b.vol;
+}
