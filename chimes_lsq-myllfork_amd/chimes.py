@@ -51,7 +51,7 @@ class Frame(C.Structure):
 
 
 class FitResult(C.Structure):
-    _fields_ = [("rms", C.c_double), ("max_abs_x", C.c_double), ("nvars", C.c_int),
+    _fields_ = [("eps", C.c_double), ("rms", C.c_double), ("max_abs_x", C.c_double), ("nvars", C.c_int),
                 ("eps_abs", C.c_double), ("bic", C.c_double), ("ms_gram", C.c_double),
                 ("ms_solve", C.c_double), ("ms_refine", C.c_double)]
 
@@ -83,6 +83,9 @@ def lib(path=None):
         L.cg_write_frame.argtypes = [C.c_void_p, C.POINTER(Frame), _dp, C.c_int]
         L.cg_write_close.argtypes = [C.c_void_p]
         L.cg_write_header.argtypes = [C.c_void_p, C.c_char_p]
+        def opt(name, argtypes):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = argtypes
         if hasattr(L, "cg_create"):
             L.cg_create.argtypes = [C.POINTER(Model), C.c_int, C.POINTER(C.c_void_p)]
             L.cg_destroy.argtypes = [C.c_void_p]
@@ -90,11 +93,11 @@ def lib(path=None):
                                       C.POINTER(C.c_void_p), _dp, C.POINTER(C.c_int64)]
             L.cg_copy_A.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _dp]
             L.cg_last_timing.argtypes = [C.c_void_p, _dp, _dp, _dp]
-            L.cg_fit.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)]
-            L.cg_matvec.argtypes = [C.c_void_p, _dp, _dp]
-            L.cg_write_params.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_int, C.c_int64,
-                                          C.POINTER(FitResult), _dp, _dp]
-            L.cg_force.argtypes = [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp]
+        opt("cg_fit", [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)])
+        opt("cg_matvec", [C.c_void_p, _dp, _dp])
+        opt("cg_write_params", [C.c_void_p, C.c_char_p, _dp, C.c_int, C.c_int64,
+                                C.POINTER(FitResult), _dp, _dp])
+        opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         _lib = L
     return _lib
 

@@ -1,0 +1,1292 @@
+// gfx950 design-matrix assembly: neighbour build, cluster enumeration and the
+// 2/3/4-body A-row kernels of the ChIMES least-squares path.
+//
+// Reference behaviour reproduced (paths relative to the ChIMES-lsq tree):
+//   neighbour sets      NEIGHBORS::DO_UPDATE_BIG/SMALL  src/ClassDefs.C:137-393
+//   2B rows             Cheby::Deriv_2B                 src/Cheby.C:709-883
+//   3B rows             Cheby::Deriv_3B                 src/Cheby.C:885-1277
+//   4B rows             Cheby::Deriv_4B                 src/Cheby.C:1279-1729
+//   row layout          A_MAT::PRINT_FRAME              src/A_Matrix.C:210-567
+//
+// Design: every cluster is re-ordered into the canonical atom order of its
+// cluster type when it is enumerated, so the power tables of a type are the
+// same for every cluster of that type.  One workgroup owns (frame, cluster
+// type, tile of columns); its lanes walk the clusters of that (frame, type)
+// bucket, keep per-pair Chebyshev tables in vector registers (indexed by the
+// wave-uniform power of the current allowed power set), and accumulate the
+// force / stress / energy rows of the tile in LDS.  The tile is written to A
+// once, coalesced along its columns.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "setup.hpp"
+
+namespace cg {
+
+
+// ------------------------------------------------------------ host geometry
+// Cell setup of BOX::UPDATE_CELL (src/ClassDefs.C:1940-1971 and callees).
+static void host_cell(const double *cell, DevFrame &f) {
+  double ax = cell[0], ay = cell[1], az = cell[2], bx = cell[3], by = cell[4], bz = cell[5], cx = cell[6],
+         cy = cell[7], cz = cell[8];
+  double H[9] = {ax, bx, cx, ay, by, cy, az, bz, cz};
+  f.ortho = (std::fabs(ay) + std::fabs(az) + std::fabs(bx) + std::fabs(bz) + std::fabs(cx) + std::fabs(cy) > 1.0e-10) ? 0 : 1;
+  double det = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) + H[2] * (H[3] * H[7] - H[4] * H[6]);
+  if (det == 0) throw Error(CG_ERR_INPUT, "ERROR: H-matrix determinant of zero computed in INVERT_HMAT");
+  double T[9] = {(H[4] * H[8] - H[5] * H[7]), -(H[3] * H[8] - H[5] * H[6]), (H[3] * H[7] - H[4] * H[6]),
+                 -(H[1] * H[8] - H[2] * H[7]), (H[0] * H[8] - H[2] * H[6]), -(H[0] * H[7] - H[1] * H[6]),
+                 (H[1] * H[5] - H[2] * H[4]), -(H[0] * H[5] - H[2] * H[3]), (H[0] * H[4] - H[1] * H[3])};
+  double I[9] = {T[0], T[3], T[6], T[1], T[4], T[7], T[2], T[5], T[8]};
+  for (int i = 0; i < 9; ++i) { f.H[i] = H[i]; f.Hi[i] = I[i] / det; }
+  auto mag = [](double x, double y, double z) { return std::sqrt(x * x + y * y + z * z); };
+  auto ang = [&](double x1, double y1, double z1, double x2, double y2, double z2) {
+    return std::acos((x1 * x2 + y1 * y2 + z1 * z2) / (mag(x1, y1, z1) * mag(x2, y2, z2)));
+  };
+  double la = mag(ax, ay, az), lb = mag(bx, by, bz), lc = mag(cx, cy, cz);
+  double al = ang(bx, by, bz, cx, cy, cz), be = ang(cx, cy, cz, ax, ay, az), ga = ang(ax, ay, az, bx, by, bz);
+  double xy = lb * std::cos(ga); if (std::fabs(xy) < 1e-12) xy = 0.0;
+  double xz = lc * std::cos(be); if (std::fabs(xz) < 1e-12) xz = 0.0;
+  double ly = std::sqrt(lb * lb - xy * xy);
+  double yz = (lb * lc * std::cos(al) - xy * xz) / ly; if (std::fabs(yz) < 1e-12) yz = 0.0;
+  double lz = std::sqrt(lc * lc - xz * xz - yz * yz);
+  f.lx = la; f.ly = ly; f.lz = lz;
+  {  // BOX::UPDATE_EXTENT
+    double xlo = 0, ylo = 0, t = 0;
+    if (xy < xlo) xlo = xy;
+    if (xz < xlo) xlo = xz;
+    if (xy + xz < xlo) xlo = xy + xz;
+    if (yz < ylo) ylo = yz;
+    double xhi = ax;
+    if (xy > t) t = xy;
+    if (xz > t) t = xz;
+    if (xy + xz > t) t = xy + xz;
+    xhi += t;
+    double yhi = by;
+    if (yz > 0.0) yhi += yz;
+    f.min_extent = std::min(xhi - xlo, std::min(yhi - ylo, cz));
+  }
+  double vol;
+  if (f.ortho) vol = ax * by * cz;
+  else {
+    double v = 1 + 2 * std::cos(al) * std::cos(be) * std::cos(ga) - std::cos(al) * std::cos(al) -
+               std::cos(be) * std::cos(be) - std::cos(ga) * std::cos(ga);
+    vol = la * lb * lc * std::sqrt(v);
+  }
+  f.inv_vol = 1.0 / vol;
+  // perpendicular widths |det| / |b x c| etc.
+  auto cross = [](const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  const double *A = cell, *B = cell + 3, *C = cell + 6;
+  double w[3];
+  cross(B, C, w); f.width[0] = std::fabs(det) / mag(w[0], w[1], w[2]);
+  cross(C, A, w); f.width[1] = std::fabs(det) / mag(w[0], w[1], w[2]);
+  cross(A, B, w); f.width[2] = std::fabs(det) / mag(w[0], w[1], w[2]);
+}
+
+// ------------------------------------------------------------ device helpers
+__device__ __forceinline__ int ref_atom(uint32_t r) { return int(r & 0xFFFFFu); }
+__device__ __forceinline__ int ref_sx(uint32_t r) { return int((r >> 20) & 7u) - 4; }
+__device__ __forceinline__ int ref_sy(uint32_t r) { return int((r >> 23) & 7u) - 4; }
+__device__ __forceinline__ int ref_sz(uint32_t r) { return int((r >> 26) & 7u) - 4; }
+__device__ __forceinline__ uint32_t make_ref(int a, int sx, int sy, int sz) {
+  return uint32_t(a) | (uint32_t(sx + 4) << 20) | (uint32_t(sy + 4) << 23) | (uint32_t(sz + 4) << 26);
+}
+
+// Cartesian image position: wrapped coordinate + H * shift.
+__device__ __forceinline__ void image_pos(const DevFrame &f, const double *wrap, uint32_t r, double *p) {
+  int a = ref_atom(r);
+  double sx = ref_sx(r), sy = ref_sy(r), sz = ref_sz(r);
+  const double *w = wrap + 3 * (f.atom_off + a);
+  p[0] = w[0] + f.H[0] * sx + f.H[1] * sy + f.H[2] * sz;
+  p[1] = w[1] + f.H[3] * sx + f.H[4] * sy + f.H[5] * sz;
+  p[2] = w[2] + f.H[6] * sx + f.H[7] * sy + f.H[8] * sz;
+}
+
+// BOX::WRAP_ATOM (src/ClassDefs.C:2053-2110).
+__global__ void k_wrap(const DevFrame *frames, const int *atom_frame, const double *xyz, double *wrap, int natoms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const DevFrame &f = frames[atom_frame[i]];
+  const double *u = xyz + 3 * i;
+  double *w = wrap + 3 * i;
+  if (f.ortho) {
+    double ix = floor(u[0] / f.lx), iy = floor(u[1] / f.ly), iz = floor(u[2] / f.lz);
+    w[0] = u[0] - ix * f.lx; w[1] = u[1] - iy * f.ly; w[2] = u[2] - iz * f.lz;
+  } else {
+    double s[3];
+    for (int r = 0; r < 3; ++r) s[r] = f.Hi[3 * r] * u[0] + f.Hi[3 * r + 1] * u[1] + f.Hi[3 * r + 2] * u[2];
+    for (int r = 0; r < 3; ++r) s[r] -= floor(s[r]);
+    for (int r = 0; r < 3; ++r) w[r] = f.H[3 * r] * s[0] + f.H[3 * r + 1] * s[1] + f.H[3 * r + 2] * s[2];
+  }
+}
+
+// Neighbour images of atom i within rc: every (j, shift) with shift in
+// [-L, L]^3 (the reference's ghost layers), restricted to shifts that can lie
+// within rc along each axis.  Big cells keep j >= i (half list with the
+// reference's a1 <= PARENT[a2] rule); small cells keep every image.
+template <bool FILL>
+__global__ void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap, int natoms, double rc,
+                        int L, int *count, const int *offset, uint32_t *list, double *rlist) {
+  int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= natoms) return;
+  const DevFrame &f = frames[atom_frame[gi]];
+  const int i = gi - f.atom_off;
+  const double *wi = wrap + 3 * gi;
+  double fi[3];
+  for (int r = 0; r < 3; ++r) fi[r] = f.Hi[3 * r] * wi[0] + f.Hi[3 * r + 1] * wi[1] + f.Hi[3 * r + 2] * wi[2];
+  const double rc2 = rc * rc;
+  double reach[3];
+  for (int r = 0; r < 3; ++r) reach[r] = rc / f.width[r] + 1e-9;
+  int n = 0;
+  int base = FILL ? offset[gi] : 0;
+  for (int j = f.small ? 0 : i; j < f.natoms; ++j) {
+    const double *wj = wrap + 3 * (f.atom_off + j);
+    double fj[3], d[3];
+    for (int r = 0; r < 3; ++r) fj[r] = f.Hi[3 * r] * wj[0] + f.Hi[3 * r + 1] * wj[1] + f.Hi[3 * r + 2] * wj[2];
+    for (int r = 0; r < 3; ++r) d[r] = fj[r] - fi[r];
+    int lo[3], hi[3];
+    for (int r = 0; r < 3; ++r) {
+      lo[r] = max(-L, (int)ceil(-d[r] - reach[r]));
+      hi[r] = min(L, (int)floor(-d[r] + reach[r]));
+    }
+    for (int sx = lo[0]; sx <= hi[0]; ++sx)
+      for (int sy = lo[1]; sy <= hi[1]; ++sy)
+        for (int sz = lo[2]; sz <= hi[2]; ++sz) {
+          if (j == i && sx == 0 && sy == 0 && sz == 0) continue;
+          double v[3];
+          for (int r = 0; r < 3; ++r)
+            v[r] = wj[r] - wi[r] + f.H[3 * r] * sx + f.H[3 * r + 1] * sy + f.H[3 * r + 2] * sz;
+          double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+          if (r2 < rc2) {
+            if (FILL) {
+              list[base + n] = make_ref(j, sx, sy, sz);
+              rlist[base + n] = sqrt(r2);
+            }
+            ++n;
+          }
+        }
+  }
+  if (!FILL) count[gi] = n;
+}
+
+struct EnumArgs {
+  const DevFrame *frames;
+  const int *atom_frame;
+  const int *type;      // per atom (batch)
+  const double *wrap;
+  const int *noff;      // neighbour offsets [natoms+1]
+  const uint32_t *nlist;
+  const double *rlist;
+  int natoms, ntypes_atom;
+  double cutoff;
+  int *count;           // [natoms]
+  const int *coff;      // [natoms+1]
+  uint32_t *refs;       // [ncl * nb]
+  int *ctype;           // [ncl]
+  int *cframe;          // [ncl]
+  // family tables
+  const int *id_map, *id_atom_perm, *id_pair_map;
+  const double *smin, *smax;
+  const int *int_pair_map;
+  int md;               // force-evaluation pair set
+};
+
+__device__ __forceinline__ double pair_dist(const DevFrame &f, const double *wrap, uint32_t a, uint32_t b) {
+  double pa[3], pb[3];
+  image_pos(f, wrap, a, pa);
+  image_pos(f, wrap, b, pb);
+  double dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// Enumerate the clusters anchored at each real atom (the reference's
+// Deriv_3B/4B loop structure), keep those whose pairs all pass PROCEED, and
+// store them in the canonical atom order of their cluster type.
+template <int NB, bool FILL>
+__global__ void k_enum(EnumArgs a) {
+  int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= a.natoms) return;
+  const int fidx = a.atom_frame[gi];
+  const DevFrame &f = a.frames[fidx];
+  const int i = gi - f.atom_off;
+  const int b = a.noff[gi], e = a.noff[gi + 1];
+  const uint32_t r1 = make_ref(i, 0, 0, 0);
+  const int t1 = a.type[gi];
+  int n = 0;
+  int out = FILL ? a.coff[gi] : 0;
+  const bool ordered = !f.small;
+  constexpr int NP = NB * (NB - 1) / 2;
+  if (NB == 2) {
+    for (int k = b; k < e; ++k) {
+      double r = a.rlist[k];
+      if (r >= a.cutoff) continue;
+      uint32_t r2 = a.nlist[k];
+      int t2 = a.type[f.atom_off + ref_atom(r2)];
+      int p = a.int_pair_map[t1 * a.ntypes_atom + t2];
+      if (!(r < a.smax[p] && (a.md || r > a.smin[p]))) continue;
+      if (FILL) {
+        a.refs[2 * (out + n)] = r1;
+        a.refs[2 * (out + n) + 1] = r2;
+        a.ctype[out + n] = p;
+        a.cframe[out + n] = fidx;
+      }
+      ++n;
+    }
+  } else {
+    for (int k2 = b; k2 < e; ++k2) {
+      if (a.rlist[k2] >= a.cutoff) continue;
+      uint32_t r2 = a.nlist[k2];
+      int p2 = ref_atom(r2);
+      for (int k3 = b; k3 < e; ++k3) {
+        if (k3 == k2 || a.rlist[k3] >= a.cutoff) continue;
+        uint32_t r3 = a.nlist[k3];
+        int p3 = ref_atom(r3);
+        if (ordered && p2 > p3) continue;
+        for (int k4 = (NB == 4 ? b : 0); k4 < (NB == 4 ? e : 1); ++k4) {
+          uint32_t r4 = 0;
+          if (NB == 4) {
+            if (k4 == k2 || k4 == k3 || a.rlist[k4] >= a.cutoff) continue;
+            r4 = a.nlist[k4];
+            if (ordered && p3 > ref_atom(r4)) continue;
+          }
+          uint32_t refs[4] = {r1, r2, r3, r4};
+          int types[4];
+          int id = 0, w = 1;
+          for (int q = 0; q < NB; ++q) {
+            types[q] = a.type[f.atom_off + ref_atom(refs[q])];
+            id += w * (types[q] + 1);
+            w *= CG_MAX_ATOM_TYPES;
+          }
+          int ct = a.id_map[id];
+          if (ct < 0) continue;
+          // PROCEED on every pair with the canonical pair's S_MAXIM
+          bool ok = true;
+          int pf = 0;
+          for (int x = 0; x < NB && ok; ++x)
+            for (int y = x + 1; y < NB; ++y) {
+              double r;
+              if (x == 0) {
+                int kk = y == 1 ? k2 : (y == 2 ? k3 : k4);
+                r = a.rlist[kk];
+              } else {
+                r = pair_dist(f, a.wrap, refs[x], refs[y]);
+              }
+              int cp = a.id_pair_map[id * NP + pf];
+              if (!(r < a.smax[ct * NP + cp])) { ok = false; break; }
+              ++pf;
+            }
+          if (!ok) continue;
+          if (FILL) {
+            const int *perm = a.id_atom_perm + id * NB;
+            for (int s = 0; s < NB; ++s) a.refs[NB * (out + n) + s] = refs[perm[s]];
+            a.ctype[out + n] = ct;
+            a.cframe[out + n] = fidx;
+          }
+          ++n;
+        }
+      }
+    }
+  }
+  if (!FILL) a.count[gi] = n;
+}
+
+__global__ void k_bucket_rank(const int *ctype, const int *cframe, int ncl, int ntypes, int *bcount, int *rank) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  rank[c] = atomicAdd(&bcount[cframe[c] * ntypes + ctype[c]], 1);
+}
+
+template <int NB>
+__global__ void k_bucket_scatter(const uint32_t *refs, const int *ctype, const int *cframe, const int *rank,
+                                 const int *boff, int ntypes, int ncl, uint32_t *out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  int d = boff[cframe[c] * ntypes + ctype[c]] + rank[c];
+  for (int s = 0; s < NB; ++s) out[NB * d + s] = refs[NB * c + s];
+}
+
+// ------------------------------------------------------------ assembly kernel
+struct Task {
+  int frame;
+  int type;
+  int col0;    // first column of the tile, relative to the type
+  int ncols;   // tile width
+  int c0, c1;  // cluster range (bucketed arrays)
+};
+
+struct AsmArgs {
+  const Task *tasks;
+  const DevFrame *frames;
+  const double *wrap;
+  const uint32_t *refs;     // bucketed cluster refs [ncl*NB]
+  const uint32_t *perm;     // packed powers
+  const int *colptr;        // [family ncols + 1]
+  const int *col_rel;       // [ntypes] family-relative first column of a type
+  const int *col_abs;       // [ntypes] absolute first column of a type
+  const double *smin, *smax, *xavg, *xdiff, *lambda;  // [ntypes*NP]
+  void *A;
+  long long ld;
+  int transform, fix_type, fcut_type, stress_mode;
+  double deriv_const, smooth_dist, fcut_offset;
+  int nb_scale;             // index into perm_scale (2, 3, 4)
+};
+
+template <class T, int N> using vecN = T __attribute__((ext_vector_type(N)));
+
+// Chebyshev tables of one pair: F[i] = fc*T_{i+BASE}(x), D[i] = d/dr of that,
+// with set_polys / set_polys_out_of_range / FCUT::get_fcut semantics.
+template <class T, int VW, int BASE>
+__device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double smin, double smax, double xavg,
+                                           double xdiff, double lambda, vecN<T, VW> &F, vecN<T, VW> &D) {
+  double re = r < smin ? smin : r;
+  double x, ex = 0.0, dxdr;
+  if (a.transform == CG_MORSE) {
+    ex = exp(-re / lambda);
+    x = (ex - xavg) / xdiff;
+    dxdr = (-ex / lambda) / xdiff;
+  } else if (a.transform == CG_INVRSE_R) {
+    x = (1.0 / re - xavg) / xdiff;
+    dxdr = -1.0 / (re * re * xdiff);
+  } else {
+    x = (re - xavg) / xdiff;
+    dxdr = 1.0 / xdiff;
+  }
+  dxdr *= a.deriv_const;
+  double damp = 1.0, shift = 0.0, dscale = 1.0;
+  bool zero_d = false;
+  if (r < smin) {
+    if (a.fix_type == CG_FIX_ZERO_DERIV) zero_d = true;
+    else if (a.fix_type == CG_FIX_CONSTANT_DERIV) shift = r - smin;
+    else {
+      damp = exp((r - smin) / a.smooth_dist);
+      shift = a.smooth_dist * (damp - 1.0);
+      dscale = damp;
+    }
+  }
+  double fc, fcd;
+  if (a.fcut_type == CG_FCUT_CUBIC) {
+    double f0 = 1.0 - r / smax;
+    fc = f0 * f0 * f0;
+    fcd = f0 * f0 * (-3.0 / smax);
+  } else {
+    double th = smax - a.fcut_offset * smax;
+    if (r < th) { fc = 1.0; fcd = 0.0; }
+    else if (r > smax) { fc = 0.0; fcd = 0.0; }
+    else {
+      double f0 = (r - th) / (smax - th) * 3.14159265359 + 3.14159265359 / 2.0;
+      fc = 0.5 + 0.5 * sin(f0);
+      fcd = 0.5 * cos(f0) * (3.14159265359 / (smax - th));
+    }
+  }
+  // T_n by recurrence; dT_n/dr = n * U_{n-1}(x) * dx/dr with U by recurrence
+  constexpr int NT = VW + BASE;
+  double tn[NT + 1], un[NT + 1];
+  tn[0] = 1.0; tn[1] = x;
+  un[0] = 1.0; un[1] = 2.0 * x;
+#pragma unroll
+  for (int n = 2; n <= NT; ++n) {
+    tn[n] = 2.0 * x * tn[n - 1] - tn[n - 2];
+    un[n] = 2.0 * x * un[n - 1] - un[n - 2];
+  }
+#pragma unroll
+  for (int n = BASE; n < NT; ++n) {
+    double tdn = n == 0 ? 0.0 : n * dxdr * un[n - 1];
+    if (zero_d) tdn = 0.0;
+    const double tv = tn[n] + shift * tdn;
+    const double tdv = tdn * dscale;
+    F[n - BASE] = (T)(fc * tv);
+    D[n - BASE] = (T)(fc * tdv + fcd * tv);
+  }
+}
+
+template <class T> __device__ __forceinline__ void lds_add(T *p, T v) { unsafeAtomicAdd(p, v); }
+
+template <int NB, int VW, int BASE, class T, int BS>
+__global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
+  constexpr int NP = NB * (NB - 1) / 2;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *acc = reinterpret_cast<T *>(smem_raw);
+  const Task t = a.tasks[blockIdx.x];
+  const DevFrame &f = a.frames[t.frame];
+  const int nf = 3 * f.natoms;
+  const int W = t.ncols;
+  const int nacc = (nf + 56) * W;
+  for (int i = threadIdx.x; i < nacc; i += BS) acc[i] = (T)0;
+  __syncthreads();
+  const int ty = t.type;
+  const int colbase = a.col_rel[ty] + t.col0;
+  const T ps = (T)f.perm_scale[a.nb_scale];
+  const bool full_stress = a.stress_mode == CG_STRESS_ALL;
+  const bool any_stress = a.stress_mode != CG_STRESS_NONE;
+  const int rep = threadIdx.x & 7;
+  T *se = acc + (size_t)nf * W;
+  for (int base = t.c0; base < t.c1; base += BS) {
+    const int ci = base + threadIdx.x;
+    const bool valid = ci < t.c1;
+    vecN<T, VW> F[NP], D[NP];
+    T rh[NP][3], rl[NP];  // unit pair vectors and lengths
+    int atom[NB];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) { F[k] = (T)0; D[k] = (T)0; rh[k][0] = rh[k][1] = rh[k][2] = (T)0; rl[k] = (T)1; }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) atom[s] = 0;
+    if (valid) {
+      double p[NB][3];
+#pragma unroll
+      for (int s = 0; s < NB; ++s) {
+        uint32_t r = a.refs[(size_t)NB * ci + s];
+        atom[s] = ref_atom(r);
+        image_pos(f, a.wrap, r, p[s]);
+      }
+      int k = 0;
+#pragma unroll
+      for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = x + 1; y < NB; ++y) {
+          double v0 = p[y][0] - p[x][0], v1 = p[y][1] - p[x][1], v2 = p[y][2] - p[x][2];
+          double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+          rh[k][0] = (T)(v0 / r); rh[k][1] = (T)(v1 / r); rh[k][2] = (T)(v2 / r); rl[k] = (T)r;
+          const int q = ty * NP + k;
+          pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], F[k], D[k]);
+          ++k;
+        }
+    }
+    for (int cc = 0; cc < W; ++cc) {
+      const int col = colbase + cc;
+      const int pb = a.colptr[col], pe = a.colptr[col + 1];
+      T e = (T)0, g[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) g[k] = (T)0;
+      uint32_t wnext = pb < pe ? a.perm[pb] : 0u;
+      for (int q = pb; q < pe; ++q) {
+        const uint32_t w = wnext;
+        if (q + 1 < pe) wnext = a.perm[q + 1];
+        T fv[NP], dv[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const unsigned idx = (w >> (4 * k)) & 15u;
+          fv[k] = F[k][idx];
+          dv[k] = D[k][idx];
+        }
+        T pre[NP + 1], suf[NP + 1];
+        pre[0] = (T)1;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) pre[k + 1] = pre[k] * fv[k];
+        suf[NP] = (T)1;
+#pragma unroll
+        for (int k = NP - 1; k >= 0; --k) suf[k] = suf[k + 1] * fv[k];
+        e += pre[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) g[k] += dv[k] * (pre[k] * suf[k + 1]);
+      }
+      if (valid) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) g[k] *= ps;
+        // force on each canonical atom: sum over its pairs of +-g * rhat
+#pragma unroll
+        for (int s = 0; s < NB; ++s) {
+          T fx0 = 0, fx1 = 0, fx2 = 0;
+          int k = 0;
+#pragma unroll
+          for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int y = x + 1; y < NB; ++y) {
+              if (x == s) { fx0 += g[k] * rh[k][0]; fx1 += g[k] * rh[k][1]; fx2 += g[k] * rh[k][2]; }
+              if (y == s) { fx0 -= g[k] * rh[k][0]; fx1 -= g[k] * rh[k][1]; fx2 -= g[k] * rh[k][2]; }
+              ++k;
+            }
+          T *dst = acc + (3 * atom[s]) * W + cc;
+          lds_add(dst, fx0);
+          lds_add(dst + W, fx1);
+          lds_add(dst + 2 * W, fx2);
+        }
+        if (any_stress) {
+          const int nsc = full_stress ? 6 : 3;
+#pragma unroll
+          for (int comp = 0; comp < 6; ++comp) {
+            if (comp >= nsc) break;
+            const int ia = comp < 3 ? comp : (comp == 5 ? 1 : 0);
+            const int ib = comp < 3 ? comp : (comp == 3 ? 1 : 2);
+            T sv = 0;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) sv -= g[k] * rl[k] * rh[k][ia] * rh[k][ib];
+            lds_add(&se[(comp * 8 + rep) * W + cc], sv);
+          }
+        }
+        lds_add(&se[(6 * 8 + rep) * W + cc], ps * e);
+      }
+    }
+  }
+  __syncthreads();
+  // write the tile: force rows, stress rows (x 1/V), energy rows (x3)
+  T *Aout = reinterpret_cast<T *>(a.A);
+  const long long r0 = f.row_off;
+  const int cabs = a.col_abs[ty] + t.col0;
+  for (int i = threadIdx.x; i < nf * W; i += BS) {
+    const int r = i / W, cc = i - r * W;
+    Aout[(r0 + r) * a.ld + cabs + cc] = acc[i];
+  }
+  const int nsr = f.stress_rows, ner = f.ener_rows;
+  const int sel9[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+  for (int i = threadIdx.x; i < (nsr + ner) * W; i += BS) {
+    const int k = i / W, cc = i - k * W;
+    int comp;
+    T scale;
+    if (k < nsr) { comp = nsr == 9 ? sel9[k] : k; scale = (T)f.inv_vol; }
+    else { comp = 6; scale = (T)1; }
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += se[(comp * 8 + q) * W + cc];
+    Aout[(r0 + nf + k) * a.ld + cabs + cc] = s * scale;
+  }
+}
+
+// Force / energy / virial of clusters with fitted coefficients
+// (Cheby::Force_all 2-body part, Force_3B, Force_4B; src/Cheby.C:1731-2675).
+// Lanes walk the clusters of one (frame, type) bucket; each lane contracts its
+// cluster's power products with the coefficients of every allowed power set
+// of the type, then scatters the pair forces to its atoms.  Energy and virial
+// are reduced over the workgroup before one atomic add per frame.
+struct ForceArgs {
+  const Task *tasks;
+  const DevFrame *frames;
+  const double *wrap;
+  const uint32_t *refs;
+  const uint32_t *perm;
+  const int *colptr, *col_rel, *col_abs, *ncol_type;
+  const double *smin, *smax, *xavg, *xdiff, *lambda;
+  const double *x;       // coefficients in A column order
+  double *F;             // [batch atoms * 3]
+  double *EV;            // [nframes * 10]: energy, virial xx xy xz yx yy yz zx zy zz
+  int transform, fix_type, fcut_type;
+  double deriv_const, smooth_dist, fcut_offset;
+  double penalty_scale, penalty_dist;
+  int nb_scale;
+};
+
+template <int NB, int VW, int BASE, int BS>
+__global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
+  constexpr int NP = NB * (NB - 1) / 2;
+  __shared__ double red[10][BS / 64];
+  const Task t = a.tasks[blockIdx.x];
+  const DevFrame &f = a.frames[t.frame];
+  const int ty = t.type;
+  const double ps = f.perm_scale[a.nb_scale];
+  const int cb = a.col_rel[ty], ce = cb + a.ncol_type[ty];
+  const int cabs = a.col_abs[ty];
+  AsmArgs aa{};
+  aa.transform = a.transform; aa.fix_type = a.fix_type; aa.fcut_type = a.fcut_type;
+  aa.deriv_const = a.deriv_const; aa.smooth_dist = a.smooth_dist; aa.fcut_offset = a.fcut_offset;
+  double ev[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) ev[q] = 0.0;
+  for (int base = t.c0; base < t.c1; base += BS) {
+    const int ci = base + threadIdx.x;
+    const bool valid = ci < t.c1;
+    vecN<double, VW> Fv[NP], Dv[NP];
+    double rh[NP][3], rl[NP];
+    int atom[NB];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) { Fv[k] = 0.0; Dv[k] = 0.0; rh[k][0] = rh[k][1] = rh[k][2] = 0.0; rl[k] = 1.0; }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) atom[q] = 0;
+    if (valid) {
+      double p[NB][3];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        uint32_t r = a.refs[(size_t)NB * ci + q];
+        atom[q] = ref_atom(r);
+        image_pos(f, a.wrap, r, p[q]);
+      }
+      int k = 0;
+#pragma unroll
+      for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = x + 1; y < NB; ++y) {
+          double v0 = p[y][0] - p[x][0], v1 = p[y][1] - p[x][1], v2 = p[y][2] - p[x][2];
+          double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+          rh[k][0] = v0 / r; rh[k][1] = v1 / r; rh[k][2] = v2 / r; rl[k] = r;
+          const int q = ty * NP + k;
+          pair_table<double, VW, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], Fv[k], Dv[k]);
+          ++k;
+        }
+    }
+    double e = 0.0, g[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) g[k] = 0.0;
+    for (int col = cb; col < ce; ++col) {
+      const double coef = a.x[cabs + (col - cb)];
+      const int pb = a.colptr[col], pe = a.colptr[col + 1];
+      double ec = 0.0, gc[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) gc[k] = 0.0;
+      for (int q = pb; q < pe; ++q) {
+        const uint32_t w = a.perm[q];
+        double fv[NP], dv[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const unsigned idx = (w >> (4 * k)) & 15u;
+          fv[k] = Fv[k][idx];
+          dv[k] = Dv[k][idx];
+        }
+        double pre[NP + 1], suf[NP + 1];
+        pre[0] = 1.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) pre[k + 1] = pre[k] * fv[k];
+        suf[NP] = 1.0;
+#pragma unroll
+        for (int k = NP - 1; k >= 0; --k) suf[k] = suf[k + 1] * fv[k];
+        ec += pre[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) gc[k] += dv[k] * (pre[k] * suf[k + 1]);
+      }
+      e += coef * ec;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) g[k] += coef * gc[k];
+    }
+    if (valid) {
+      e *= ps;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) g[k] *= ps;
+      if (NB == 2 && a.penalty_scale > 0.0) {
+        const double sm = a.smin[ty];
+        if (rl[0] - a.penalty_dist < sm) {
+          const double rp = sm + a.penalty_dist - rl[0];
+          g[0] += -3.0 * rp * rp * a.penalty_scale;
+          e += rp * rp * rp * a.penalty_scale;
+        }
+      }
+      int k = 0;
+#pragma unroll
+      for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = x + 1; y < NB; ++y) {
+          double *Fa = a.F + 3 * (size_t)(f.atom_off + atom[x]);
+          double *Fb = a.F + 3 * (size_t)(f.atom_off + atom[y]);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double d = g[k] * rh[k][c];
+            atomicAdd(Fa + c, d);
+            atomicAdd(Fb + c, -d);
+          }
+#pragma unroll
+          for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) ev[1 + 3 * u + v] -= g[k] * rl[k] * rh[k][u] * rh[k][v];
+          ++k;
+        }
+      ev[0] += e;
+    }
+  }
+  // workgroup reduction of energy + virial
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    double v = ev[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[q][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    double v = 0.0;
+    for (int w = 0; w < BS / 64; ++w) v += red[threadIdx.x][w];
+    atomicAdd(&a.EV[t.frame * 10 + threadIdx.x], v);
+  }
+}
+
+// Energy-offset columns: atom counts on energy rows (A_MAT::add_col_of_ones).
+template <class T>
+__global__ void k_offsets(const DevFrame *frames, const int *type, int nframes, int ntypes, int col0, T *A,
+                          long long ld) {
+  int fi = blockIdx.x;
+  if (fi >= nframes) return;
+  const DevFrame &f = frames[fi];
+  __shared__ int cnt[CG_MAX_ATOM_TYPES];
+  if (threadIdx.x < CG_MAX_ATOM_TYPES) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < f.natoms; i += blockDim.x) atomicAdd(&cnt[type[f.atom_off + i]], 1);
+  __syncthreads();
+  if (f.ener_rows == 0) return;
+  long long r = f.row_off + 3LL * f.natoms + f.stress_rows;
+  for (int k = threadIdx.x; k < 3 * ntypes; k += blockDim.x) {
+    int rr = k / ntypes, t = k % ntypes;
+    A[(r + rr) * ld + col0 + t] = (T)cnt[t];
+  }
+}
+
+// ------------------------------------------------------------ host side
+static void upload(Ctx &c, void *dst, const void *src, size_t bytes) {
+  CG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+}
+
+template <class T> static T *dalloc_copy(const std::vector<T> &v) {
+  T *p = nullptr;
+  size_t n = std::max<size_t>(v.size(), 1);
+  CG_HIP(hipMalloc(&p, n * sizeof(T)));
+  if (!v.empty()) CG_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+static void build_family(Ctx &c, int which) {
+  const cg_model &m = c.m;
+  DevFamily &F = c.fam[which];
+  std::vector<int> colptr, col_rel, col_abs;
+  std::vector<uint32_t> perm;
+  std::vector<double> smin, smax, xavg, xdiff, lam;
+  std::vector<int> idm, idap, idpm;
+  if (which == 0) {
+    F.np = 1; F.nb = 2; F.ntypes = m.npairs; F.ncols = m.n2b_cols; F.max_pow = m.order2b;
+    F.cutoff = m.max_cutoff_2b;
+    for (int p = 0; p < m.npairs; ++p) {
+      col_rel.push_back(p * m.order2b);
+      col_abs.push_back(p * m.order2b);
+      F.h_col_offset.push_back(p * m.order2b);
+      F.h_ncols.push_back(m.order2b);
+      F.h_excluded.push_back(0);
+      for (int i = 0; i < m.order2b; ++i) { colptr.push_back((int)perm.size()); perm.push_back((uint32_t)i); }
+      smin.push_back(m.pair_smin[p]); smax.push_back(m.pair_smax[p]);
+      xavg.push_back(m.pair_xavg[p]); xdiff.push_back(m.pair_xdiff[p]); lam.push_back(m.pair_lambda[p]);
+    }
+    colptr.push_back((int)perm.size());
+  } else {
+    const cg_cluster_set &cs = which == 1 ? m.c3 : m.c4;
+    F.np = cs.npairs; F.nb = cs.nbody; F.ntypes = cs.ntypes; F.ncols = cs.ncols; F.max_pow = cs.order;
+    F.cutoff = cs.max_cutoff;
+    int fam_base = cs.ntypes ? cs.col_offset[0] : 0;
+    for (int t = 0; t < cs.ntypes; ++t) {
+      col_rel.push_back(cs.col_offset[t] - fam_base);
+      col_abs.push_back(cs.col_offset[t]);
+      F.h_col_offset.push_back(cs.col_offset[t]);
+      F.h_ncols.push_back(cs.n_unique[t]);
+      F.h_excluded.push_back(cs.excluded[t]);
+      int prev = -1;
+      for (int i = cs.perm_offset[t]; i < cs.perm_offset[t + 1]; ++i) {
+        int col = cs.perm_col[i];
+        if (col != prev) {
+          if (col != prev + 1) throw Error(CG_ERR_INPUT, "allowed powers are not grouped by column");
+          colptr.push_back((int)perm.size());
+          prev = col;
+        }
+        uint32_t w = 0;
+        for (int k = 0; k < cs.npairs; ++k) {
+          int p = cs.perm_powers[(size_t)i * cs.npairs + k];
+          if (p < 0 || p > 15) throw Error(CG_ERR_UNSUPPORTED, "power > 15 is not supported");
+          w |= uint32_t(p) << (4 * k);
+        }
+        perm.push_back(w);
+      }
+      if (prev + 1 != cs.n_unique[t]) throw Error(CG_ERR_INPUT, "column count mismatch");
+      for (int k = 0; k < cs.npairs; ++k) {
+        smin.push_back(cs.s_min[t * cs.npairs + k]); smax.push_back(cs.s_max[t * cs.npairs + k]);
+        xavg.push_back(cs.x_avg[t * cs.npairs + k]); xdiff.push_back(cs.x_diff[t * cs.npairs + k]);
+        lam.push_back(m.pair_lambda[cs.pair_type[t * cs.npairs + k]]);
+      }
+    }
+    colptr.push_back((int)perm.size());
+    idm.assign(cs.id_map, cs.id_map + cs.id_size);
+    idap.assign(cs.id_atom_perm, cs.id_atom_perm + (size_t)cs.id_size * cs.nbody);
+    idpm.assign(cs.id_pair_map, cs.id_pair_map + (size_t)cs.id_size * cs.npairs);
+    F.id_map = dalloc_copy(idm);
+    F.id_atom_perm = dalloc_copy(idap);
+    F.id_pair_map = dalloc_copy(idpm);
+  }
+  F.colptr = dalloc_copy(colptr);
+  F.perm = dalloc_copy(perm);
+  F.smin = dalloc_copy(smin); F.smax = dalloc_copy(smax);
+  F.xavg = dalloc_copy(xavg); F.xdiff = dalloc_copy(xdiff); F.lambda = dalloc_copy(lam);
+  std::vector<int> both(col_rel);
+  both.insert(both.end(), col_abs.begin(), col_abs.end());
+  F.col_offset = dalloc_copy(both);  // [0,ntypes): relative, [ntypes,2*ntypes): absolute
+  F.ncol_type = dalloc_copy(F.h_ncols);
+}
+
+// Which kernel instance handles a family.
+template <class T>
+static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, const Task *d_tasks,
+                          const DevFrame *d_frames, const uint32_t *refs, size_t lds_bytes, void *A, long long ld) {
+  const DevFamily &F = c.fam[which];
+  AsmArgs a{};
+  a.tasks = d_tasks;
+  a.frames = d_frames;
+  a.wrap = c.b_wrap.as<double>(0);
+  a.refs = refs;
+  a.perm = F.perm;
+  a.colptr = F.colptr;
+  a.col_rel = F.col_offset;
+  a.col_abs = F.col_offset + F.ntypes;
+  a.smin = F.smin; a.smax = F.smax; a.xavg = F.xavg; a.xdiff = F.xdiff; a.lambda = F.lambda;
+  a.A = A;
+  a.ld = ld;
+  a.transform = c.m.transform; a.fix_type = c.m.fix_type; a.fcut_type = c.m.fcut_type;
+  a.stress_mode = c.m.fit_stress;
+  a.deriv_const = c.m.deriv_const; a.smooth_dist = c.m.smooth_dist; a.fcut_offset = c.m.fcut_offset;
+  a.nb_scale = F.nb;
+  dim3 grid((unsigned)tasks.size()), block(512);
+  (void)block;
+  if (tasks.empty()) return;
+#define CG_LAUNCH(NBv, VWv, BASEv)                                                                          \
+  do {                                                                                                      \
+    constexpr int BSv = 2 * (NBv * (NBv - 1) / 2) * VWv * (int)sizeof(T) / 4 > 150 ? 256 : 512;           \
+    auto kern = k_assemble<NBv, VWv, BASEv, T, BSv>;                                                        \
+    block = dim3(BSv);                                                             \
+    CG_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes)); \
+    hipLaunchKernelGGL(kern, grid, block, lds_bytes, c.stream, a);                                          \
+    CG_HIP(hipGetLastError());                                                                              \
+  } while (0)
+  if (which == 0) {
+    if (F.max_pow <= 16) CG_LAUNCH(2, 16, 1);
+    else throw Error(CG_ERR_UNSUPPORTED, "2-body order above 16 is not supported");
+  } else if (which == 1) {
+    if (F.max_pow <= 4) CG_LAUNCH(3, 4, 0);
+    else if (F.max_pow <= 8) CG_LAUNCH(3, 8, 0);
+    else if (F.max_pow <= 12) CG_LAUNCH(3, 12, 0);
+    else if (F.max_pow <= 16) CG_LAUNCH(3, 16, 0);
+    else throw Error(CG_ERR_UNSUPPORTED, "3-body order above 16 is not supported");
+  } else {
+    if (F.max_pow <= 4) CG_LAUNCH(4, 4, 0);
+    else if (F.max_pow <= 5) CG_LAUNCH(4, 5, 0);
+    else if (F.max_pow <= 8) CG_LAUNCH(4, 8, 0);
+    else throw Error(CG_ERR_UNSUPPORTED, "4-body order above 8 is not supported");
+  }
+#undef CG_LAUNCH
+}
+
+static size_t exclusive_scan(Ctx &c, const int *in, int *out, int n) {
+  size_t tmp = 0;
+  CG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n + 1, c.stream));
+  void *t = c.b_scan.get(tmp);
+  if (!t) throw Error(CG_ERR_MEMORY, "scan scratch");
+  CG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n + 1, c.stream));
+  return tmp;
+}
+
+static int read_int(Ctx &c, const int *d) {
+  int v = 0;
+  CG_HIP(hipMemcpyAsync(&v, d, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  return v;
+}
+
+struct FamOut {
+  int ncl = 0;
+  uint32_t *refs = nullptr;
+  std::vector<int> boff;
+};
+
+struct Batch {
+  std::vector<DevFrame> hf;
+  int natoms = 0;
+  long long rows = 0;
+  DevFrame *d_frames = nullptr;
+  int *d_type = nullptr;
+  FamOut fo[3];
+};
+
+// Upload a batch of frames, wrap atoms, build neighbour lists and enumerate
+// + bucket the clusters of every family.  md != 0 selects the force-evaluation
+// pair set (every pair below S_MAXIM, Cheby::Force_all) instead of the
+// least-squares one (S_MINIM < r < S_MAXIM, Cheby::Deriv_2B).
+static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B) {
+  std::vector<DevFrame> &hf = B.hf;
+  const cg_model &m = c.m;
+  if (nframes <= 0) throw Error(CG_ERR_ARG, "no frames");
+  // ---- host frame table
+  hf.assign(nframes, DevFrame{});
+  std::vector<int> atom_frame;
+  std::vector<double> xyz;
+  std::vector<int> type;
+  long long rows = 0;
+  int natoms = 0;
+  double mx = m.max_cutoff_2b;
+  if (m.c3.nbody) mx = std::max(mx, m.c3.max_cutoff);
+  if (m.c4.nbody) mx = std::max(mx, m.c4.max_cutoff);
+  for (int i = 0; i < nframes; ++i) {
+    const cg_frame &fr = frames[i];
+    if (fr.natoms <= 0 || fr.natoms >= (1 << 20)) throw Error(CG_ERR_ARG, "bad atom count");
+    DevFrame &d = hf[i];
+    host_cell(fr.cell, d);
+    d.natoms = fr.natoms;
+    d.atom_off = natoms;
+    d.row_off = rows;
+    d.stress_rows = (m.fit_stress != CG_STRESS_NONE && (m.nstress < 0 || fr.index < m.nstress))
+                        ? (m.fit_stress == CG_STRESS_ALL ? 9 : 3) : 0;
+    d.ener_rows = (m.fit_energy && (m.nener < 0 || fr.index < m.nener)) ? 3 : 0;
+    if (m.small_mode || d.min_extent <= mx) c.sticky_small = 1;
+    d.small = c.sticky_small;
+    d.perm_scale[0] = d.perm_scale[1] = 1.0;
+    for (int j = 2; j < 5; ++j) d.perm_scale[j] = d.small ? d.perm_scale[j - 1] / j : 1.0;
+    rows += 3LL * fr.natoms + d.stress_rows + d.ener_rows;
+    for (int a = 0; a < fr.natoms; ++a) {
+      atom_frame.push_back(i);
+      for (int k = 0; k < 3; ++k) xyz.push_back(fr.xyz[3 * a + k]);
+      if (fr.type[a] < 0 || fr.type[a] >= m.ntypes) throw Error(CG_ERR_ARG, "bad atom type");
+      type.push_back(fr.type[a]);
+    }
+    natoms += fr.natoms;
+  }
+  DevFrame *d_frames = c.b_frames.as<DevFrame>(nframes);
+  int *d_af = c.b_type.as<int>(2 * (size_t)natoms);
+  int *d_type = d_af + natoms;
+  double *d_xyz = c.b_xyz.as<double>(3 * (size_t)natoms);
+  double *d_wrap = c.b_wrap.as<double>(3 * (size_t)natoms);
+  if (!d_frames || !d_af || !d_xyz || !d_wrap) throw Error(CG_ERR_MEMORY, "frame buffers");
+  upload(c, d_frames, hf.data(), hf.size() * sizeof(DevFrame));
+  upload(c, d_af, atom_frame.data(), natoms * sizeof(int));
+  upload(c, d_type, type.data(), natoms * sizeof(int));
+  upload(c, d_xyz, xyz.data(), xyz.size() * sizeof(double));
+  const int TB = 256;
+  hipLaunchKernelGGL(k_wrap, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_xyz, d_wrap, natoms);
+  // ---- neighbour lists (one list at the largest family cutoff)
+  int *d_ncount = c.b_ncount.as<int>(natoms + 1);
+  int *d_noff = c.b_noff.as<int>(natoms + 1);
+  CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
+  hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
+                     natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr, (double *)nullptr);
+  exclusive_scan(c, d_ncount, d_noff, natoms);
+  int nnb = read_int(c, d_noff + natoms);
+  uint32_t *d_nlist = c.b_nlist.as<uint32_t>(std::max(nnb, 1));
+  double *d_rlist = c.b_tmp3.as<double>(std::max(nnb, 1));
+  if (!d_nlist || !d_rlist) throw Error(CG_ERR_MEMORY, "neighbour list");
+  hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
+                     natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_rlist);
+  // ---- per-family cluster enumeration and bucketing
+  FamOut *fo = B.fo;
+  std::vector<DBuf *> fbufs = {&c.b_clist, &c.b_tmp, &c.b_tmp2};
+  for (int w = 0; w < 3; ++w) {
+    const DevFamily &F = c.fam[w];
+    if (F.ntypes == 0 || F.ncols == 0) continue;
+    EnumArgs ea{};
+    ea.frames = d_frames; ea.atom_frame = d_af; ea.type = d_type; ea.wrap = d_wrap; ea.noff = d_noff;
+    ea.nlist = d_nlist; ea.rlist = d_rlist; ea.natoms = natoms; ea.ntypes_atom = m.ntypes; ea.cutoff = F.cutoff;
+    ea.count = c.b_ccount.as<int>(natoms + 1);
+    int *coff = c.b_coff.as<int>(natoms + 1);
+    ea.coff = coff;
+    ea.id_map = F.id_map; ea.id_atom_perm = F.id_atom_perm; ea.id_pair_map = F.id_pair_map;
+    ea.smin = F.smin; ea.smax = F.smax; ea.int_pair_map = c.d_int_pair_map;
+    ea.md = md;
+    CG_HIP(hipMemsetAsync(ea.count, 0, (natoms + 1) * sizeof(int), c.stream));
+    dim3 g((natoms + 63) / 64), bl(64);
+    if (w == 0) hipLaunchKernelGGL((k_enum<2, false>), g, bl, 0, c.stream, ea);
+    else if (w == 1) hipLaunchKernelGGL((k_enum<3, false>), g, bl, 0, c.stream, ea);
+    else hipLaunchKernelGGL((k_enum<4, false>), g, bl, 0, c.stream, ea);
+    exclusive_scan(c, ea.count, coff, natoms);
+    int ncl = read_int(c, coff + natoms);
+    fo[w].ncl = ncl;
+    const int nb = F.nb;
+    size_t n1 = std::max(ncl, 1);
+    // layout in one buffer: refs [n1*nb] | ctype [n1] | cframe [n1] | rank [n1] | bucketed refs [n1*nb]
+    uint32_t *blk = fbufs[w]->as<uint32_t>(n1 * (2 * nb + 3));
+    if (!blk) throw Error(CG_ERR_MEMORY, "cluster list");
+    ea.refs = blk;
+    ea.ctype = reinterpret_cast<int *>(blk + n1 * nb);
+    ea.cframe = ea.ctype + n1;
+    int *rank = ea.cframe + n1;
+    uint32_t *bref = reinterpret_cast<uint32_t *>(rank + n1);
+    if (ncl > 0) {
+      if (w == 0) hipLaunchKernelGGL((k_enum<2, true>), g, bl, 0, c.stream, ea);
+      else if (w == 1) hipLaunchKernelGGL((k_enum<3, true>), g, bl, 0, c.stream, ea);
+      else hipLaunchKernelGGL((k_enum<4, true>), g, bl, 0, c.stream, ea);
+    }
+    // bucket by (frame, type)
+    const int nbk = nframes * F.ntypes;
+    int *bcount = c.b_bcount.as<int>(2 * (size_t)nbk + 1);
+    int *boffd = bcount + nbk;
+    CG_HIP(hipMemsetAsync(bcount, 0, nbk * sizeof(int), c.stream));
+    if (ncl > 0)
+      hipLaunchKernelGGL(k_bucket_rank, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.ctype, ea.cframe, ncl,
+                         F.ntypes, bcount, rank);
+    std::vector<int> hc(nbk);
+    CG_HIP(hipMemcpyAsync(hc.data(), bcount, nbk * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CG_HIP(hipStreamSynchronize(c.stream));
+    fo[w].boff.assign(nbk + 1, 0);
+    for (int i = 0; i < nbk; ++i) fo[w].boff[i + 1] = fo[w].boff[i] + hc[i];
+    upload(c, boffd, fo[w].boff.data(), nbk * sizeof(int));
+    if (ncl > 0) {
+      if (nb == 2) hipLaunchKernelGGL(k_bucket_scatter<2>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.refs, ea.ctype, ea.cframe, rank, boffd, F.ntypes, ncl, bref);
+      else if (nb == 3) hipLaunchKernelGGL(k_bucket_scatter<3>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.refs, ea.ctype, ea.cframe, rank, boffd, F.ntypes, ncl, bref);
+      else hipLaunchKernelGGL(k_bucket_scatter<4>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.refs, ea.ctype, ea.cframe, rank, boffd, F.ntypes, ncl, bref);
+    }
+    fo[w].refs = bref;
+    CG_HIP(hipGetLastError());
+  }
+  CG_HIP(hipEventRecord(c.ev[1], c.stream));
+  B.natoms = natoms;
+  B.rows = rows;
+  B.d_frames = d_frames;
+  B.d_type = d_type;
+}
+
+void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_host) {
+  const cg_model &m = c.m;
+  Batch B;
+  CG_HIP(hipEventRecord(c.ev[0], c.stream));
+  prepare(c, frames, nframes, 0, B);
+  std::vector<DevFrame> &hf = B.hf;
+  const long long rows = B.rows;
+  DevFrame *d_frames = B.d_frames;
+  int *d_type = B.d_type;
+  FamOut *fo = B.fo;
+  // ---- b (A_MAT::PRINT_FRAME b.txt order)
+  c.h_b.assign(rows, 0.0);
+  {
+    long long r = 0;
+    const int sel9[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    for (int i = 0; i < nframes; ++i) {
+      const cg_frame &fr = frames[i];
+      for (int k = 0; k < 3 * fr.natoms; ++k) c.h_b[r++] = fr.forces ? fr.forces[k] : 0.0;
+      for (int k = 0; k < hf[i].stress_rows; ++k)
+        c.h_b[r++] = fr.stress[hf[i].stress_rows == 9 ? sel9[k] : k] / 6.9476955;
+      for (int k = 0; k < hf[i].ener_rows; ++k) c.h_b[r++] = fr.energy;
+    }
+    if (b_host) std::memcpy(b_host, c.h_b.data(), rows * sizeof(double));
+  }
+  // ---- A allocation and tasks
+  const size_t esz = prec == CG_FP32 ? sizeof(float) : sizeof(double);
+  const long long ld = m.ncols;
+  void *A = c.b_A.get((size_t)rows * ld * esz);
+  if (!A) throw Error(CG_ERR_MEMORY, "A matrix does not fit in device memory");
+  CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
+  int maxn = 0;
+  for (auto &d : hf) maxn = std::max(maxn, d.natoms);
+  const size_t lds_budget = 150 * 1024;
+  for (int w = 0; w < 3; ++w) {
+    const DevFamily &F = c.fam[w];
+    if (F.ntypes == 0 || F.ncols == 0 || fo[w].ncl == 0) continue;
+    std::vector<Task> tasks;
+    for (int fi = 0; fi < nframes; ++fi) {
+      const size_t rowsf = 3 * (size_t)hf[fi].natoms + 56;
+      int wmax = (int)(lds_budget / (rowsf * esz));
+      if (wmax < 1) throw Error(CG_ERR_UNSUPPORTED, "frame too large for the LDS tile (" + std::to_string(hf[fi].natoms) + " atoms)");
+      wmax = std::min(wmax, 64);
+      for (int t = 0; t < F.ntypes; ++t) {
+        int c0 = fo[w].boff[fi * F.ntypes + t], c1 = fo[w].boff[fi * F.ntypes + t + 1];
+        if (c1 <= c0) continue;
+        int nc = F.h_ncols[t];
+        int ntile = (nc + wmax - 1) / wmax;
+        int wt = (nc + ntile - 1) / ntile;
+        for (int col = 0; col < nc; col += wt) tasks.push_back(Task{fi, t, col, std::min(wt, nc - col), c0, c1});
+      }
+    }
+    // heavy tasks first
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &x, const Task &y) {
+      return (long long)(x.c1 - x.c0) * x.ncols > (long long)(y.c1 - y.c0) * y.ncols;
+    });
+    Task *d_tasks = c.b_tasks.as<Task>(tasks.size());
+    if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
+    upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(Task));
+    size_t lds = 0;
+    for (auto &tk : tasks) lds = std::max(lds, (3 * (size_t)hf[tk.frame].natoms + 56) * tk.ncols * esz);
+    if (prec == CG_FP32) launch_family<float>(c, w, tasks, d_tasks, d_frames, fo[w].refs, lds, A, ld);
+    else launch_family<double>(c, w, tasks, d_tasks, d_frames, fo[w].refs, lds, A, ld);
+    // the task buffer is reused by the next family: wait so its upload cannot race
+    CG_HIP(hipStreamSynchronize(c.stream));
+  }
+  if (m.n_offset_cols > 0) {
+    if (prec == CG_FP32)
+      hipLaunchKernelGGL(k_offsets<float>, dim3(nframes), dim3(256), 0, c.stream, d_frames, d_type, nframes, m.ntypes, m.ncols_short, (float *)A, ld);
+    else
+      hipLaunchKernelGGL(k_offsets<double>, dim3(nframes), dim3(256), 0, c.stream, d_frames, d_type, nframes, m.ntypes, m.ncols_short, (double *)A, ld);
+  }
+  CG_HIP(hipEventRecord(c.ev[2], c.stream));
+  CG_HIP(hipEventSynchronize(c.ev[2]));
+  CG_HIP(hipGetLastError());
+  CG_HIP(hipEventElapsedTime(&c.ms_enum, c.ev[0], c.ev[1]));
+  CG_HIP(hipEventElapsedTime(&c.ms_asm, c.ev[1], c.ev[2]));
+  c.ms_total = c.ms_enum + c.ms_asm;
+  c.A = A;
+  c.A_prec = prec;
+  c.A_rows = rows;
+  c.A_ld = (int)ld;
+}
+
+
+void force(Ctx &c, const double *x_host, const cg_frame *frame, double *F, double *E, double *vir) {
+  const cg_model &m = c.m;
+  Batch B;
+  const int saved_small = c.sticky_small;
+  c.sticky_small = 0;  // each configuration stands alone
+  prepare(c, frame, 1, 1, B);
+  c.sticky_small = saved_small;
+  const int natoms = B.natoms;
+  double *d_x = c.b_tmp.as<double>(m.ncols + 3 * (size_t)natoms + 16);
+  if (!d_x) throw Error(CG_ERR_MEMORY, "force buffers");
+  double *d_F = d_x + m.ncols;
+  double *d_EV = d_F + 3 * (size_t)natoms;
+  CG_HIP(hipMemcpyAsync(d_x, x_host, m.ncols * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CG_HIP(hipMemsetAsync(d_F, 0, (3 * (size_t)natoms + 16) * sizeof(double), c.stream));
+  for (int w = 0; w < 3; ++w) {
+    const DevFamily &Fm = c.fam[w];
+    if (Fm.ntypes == 0 || Fm.ncols == 0 || B.fo[w].ncl == 0) continue;
+    std::vector<Task> tasks;
+    for (int t = 0; t < Fm.ntypes; ++t) {
+      int c0 = B.fo[w].boff[t], c1 = B.fo[w].boff[t + 1];
+      const int chunk = 4096;
+      for (int k = c0; k < c1; k += chunk) tasks.push_back(Task{0, t, 0, 0, k, std::min(c1, k + chunk)});
+    }
+    if (tasks.empty()) continue;
+    Task *d_tasks = c.b_tasks.as<Task>(tasks.size());
+    if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
+    CG_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, c.stream));
+    ForceArgs a{};
+    a.tasks = d_tasks; a.frames = B.d_frames; a.wrap = c.b_wrap.as<double>(0); a.refs = B.fo[w].refs;
+    a.perm = Fm.perm; a.colptr = Fm.colptr; a.col_rel = Fm.col_offset; a.col_abs = Fm.col_offset + Fm.ntypes;
+    a.ncol_type = Fm.ncol_type;
+    a.smin = Fm.smin; a.smax = Fm.smax; a.xavg = Fm.xavg; a.xdiff = Fm.xdiff; a.lambda = Fm.lambda;
+    a.x = d_x; a.F = d_F; a.EV = d_EV;
+    a.transform = m.transform; a.fix_type = m.fix_type; a.fcut_type = m.fcut_type;
+    a.deriv_const = m.deriv_const; a.smooth_dist = m.smooth_dist; a.fcut_offset = m.fcut_offset;
+    a.penalty_scale = w == 0 ? c.penalty_scale : 0.0;
+    a.penalty_dist = c.penalty_dist;
+    a.nb_scale = Fm.nb;
+    dim3 grid((unsigned)tasks.size());
+#define CG_FLAUNCH(NBv, VWv, BASEv, BSv) hipLaunchKernelGGL((k_force<NBv, VWv, BASEv, BSv>), grid, dim3(BSv), 0, c.stream, a)
+    if (w == 0) {
+      if (Fm.max_pow <= 16) CG_FLAUNCH(2, 16, 1, 256);
+      else throw Error(CG_ERR_UNSUPPORTED, "2-body order above 16 is not supported");
+    } else if (w == 1) {
+      if (Fm.max_pow <= 4) CG_FLAUNCH(3, 4, 0, 256);
+      else if (Fm.max_pow <= 8) CG_FLAUNCH(3, 8, 0, 256);
+      else if (Fm.max_pow <= 12) CG_FLAUNCH(3, 12, 0, 256);
+      else if (Fm.max_pow <= 16) CG_FLAUNCH(3, 16, 0, 256);
+      else throw Error(CG_ERR_UNSUPPORTED, "3-body order above 16 is not supported");
+    } else {
+      if (Fm.max_pow <= 4) CG_FLAUNCH(4, 4, 0, 256);
+      else if (Fm.max_pow <= 5) CG_FLAUNCH(4, 5, 0, 256);
+      else if (Fm.max_pow <= 8) CG_FLAUNCH(4, 8, 0, 256);
+      else throw Error(CG_ERR_UNSUPPORTED, "4-body order above 8 is not supported");
+    }
+#undef CG_FLAUNCH
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipStreamSynchronize(c.stream));
+  }
+  double ev[10];
+  CG_HIP(hipMemcpyAsync(F, d_F, 3 * (size_t)natoms * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(ev, d_EV, 10 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  if (E) *E = ev[0];
+  if (vir) for (int q = 0; q < 9; ++q) vir[q] = ev[1 + q];
+}
+
+void ctx_init(Ctx &c, const cg_model *m, int device) {
+  c.model = m;
+  c.m = *m;
+  c.device = device;
+  CG_HIP(hipSetDevice(device));
+  CG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (auto &e : c.ev) CG_HIP(hipEventCreate(&e));
+  std::vector<int> ipm(m->int_pair_map, m->int_pair_map + m->ntypes * m->ntypes);
+  c.d_int_pair_map = dalloc_copy(ipm);
+  build_family(c, 0);
+  if (m->c3.nbody && m->c3.ncols > 0) build_family(c, 1);
+  if (m->c4.nbody && m->c4.ncols > 0) build_family(c, 2);
+}
+
+void ctx_free(Ctx &c) {
+  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_ccount,
+                  &c.b_coff, &c.b_clist, &c.b_ctype, &c.b_crank, &c.b_bucket, &c.b_bcount, &c.b_tasks, &c.b_A,
+                  &c.b_scan, &c.b_x, &c.b_tmp, &c.b_tmp2, &c.b_tmp3})
+    b->release();
+  for (auto &F : c.fam) {
+    for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
+                    (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.id_map, (void *)F.id_atom_perm,
+                    (void *)F.id_pair_map})
+      if (p) (void)hipFree(p);
+  }
+  if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);
+  for (auto &e : c.ev) if (e) (void)hipEventDestroy(e);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+}  // namespace cg
+
+// ------------------------------------------------------------ C ABI
+
+extern "C" {
+
+cg_status cg_create(const cg_model *m, int device, cg_ctx **out) {
+  if (!m || !out) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  auto *x = new cg_ctx;
+  try {
+    cg::ctx_init(x->c, m, device);
+    *out = x;
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    cg::ctx_free(x->c);
+    delete x;
+    return e.code;
+  }
+}
+
+void cg_destroy(cg_ctx *ctx) {
+  if (!ctx) return;
+  cg::ctx_free(ctx->c);
+  delete ctx;
+}
+
+cg_status cg_assemble(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, const void **A_dev,
+                      double *b_host, int64_t *rows_out) {
+  if (!ctx || !frames) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  if (prec != CG_FP64 && prec != CG_FP32) { cg::set_error("bad precision"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::assemble(ctx->c, frames, nframes, prec, b_host);
+    if (A_dev) *A_dev = ctx->c.A;
+    if (rows_out) *rows_out = ctx->c.A_rows;
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host) {
+  if (!ctx || !host || !ctx->c.A) { cg::set_error("nothing assembled"); return CG_ERR_ARG; }
+  auto &c = ctx->c;
+  if (r0 < 0 || r0 + nrows > c.A_rows) { cg::set_error("row range"); return CG_ERR_ARG; }
+  try {
+    size_t n = (size_t)nrows * c.A_ld;
+    if (c.A_prec == CG_FP64) {
+      CG_HIP(hipMemcpy(host, (double *)c.A + r0 * c.A_ld, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> tmp(n);
+      CG_HIP(hipMemcpy(tmp.data(), (float *)c.A + r0 * c.A_ld, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+    }
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f, double *F, double *E, double *virial9) {
+  if (!ctx || !x || !f || !F) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::force(ctx->c, x, f, F, E, virial9);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_asm, double *ms_total) {
+  if (!ctx) return CG_ERR_ARG;
+  if (ms_enum) *ms_enum = ctx->c.ms_enum;
+  if (ms_asm) *ms_asm = ctx->c.ms_asm;
+  if (ms_total) *ms_total = ctx->c.ms_total;
+  return CG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// chimes_lsq_gpu: drop-in for `chimes_lsq fm_setup.in` (src/chimes_lsq.C:108-424)
+// with the design matrix assembled on the GPU, optionally followed by the fit
+// that `chimes_lsq.py` performs (src/chimes_lsq.py:169-672).
+//
+//   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--refine K]
+//                  [--batch FRAMES] [--no-text]
+//
+// Files written in the working directory: A.txt b.txt b-labeled.txt natoms.txt
+// dim.txt params.header ff_groups.map; with --fit also params.txt force.txt.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/chimes_gpu.h"
+
+namespace {
+struct OwnedFrame {
+  cg_frame f;
+  std::vector<double> xyz, forces;
+  std::vector<int> type;
+};
+
+int die(const char *what, cg_status st, const void *ctx = nullptr) {
+  std::fprintf(stderr, "Error (%s): status %d: %s\n", what, st, cg_last_error(ctx));
+  return 1;
+}
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--refine K] "
+                         "[--batch FRAMES] [--no-text]\n");
+    return 1;
+  }
+  const char *infile = argv[1];
+  bool do_fit = false, text = true;
+  double eps = 1.0e-5;
+  int prec = CG_FP64, refine = 3, batch = 0;
+  for (int i = 2; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "--fit") do_fit = true;
+    else if (a == "--no-text") text = false;
+    else if (a == "--eps" && i + 1 < argc) eps = std::atof(argv[++i]);
+    else if (a == "--refine" && i + 1 < argc) refine = std::atoi(argv[++i]);
+    else if (a == "--batch" && i + 1 < argc) batch = std::atoi(argv[++i]);
+    else if (a == "--prec" && i + 1 < argc) {
+      std::string p = argv[++i];
+      prec = p == "fp32" ? CG_FP32 : CG_FP64;
+    } else {
+      std::fprintf(stderr, "unknown option %s\n", a.c_str());
+      return 1;
+    }
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  cg_setup *setup = nullptr;
+  cg_status st = cg_setup_read(infile, &setup);
+  if (st) return die("reading input", st);
+  const cg_model *m = cg_setup_model(setup);
+  std::printf("Reading input file... %s\n", infile);
+  std::printf("The number of two-body non-coulomb parameters is: %d\n", m->n2b_cols);
+  if (m->c3.nbody) std::printf("The number of three-body Chebyshev parameters is: %d\n", m->c3.ncols);
+  if (m->c4.nbody) std::printf("The number of four-body  Chebyshev parameters is: %d\n", m->c4.ncols);
+  // read every frame
+  std::string dir = ".";
+  cg_traj *traj = nullptr;
+  if ((st = cg_traj_open(setup, dir.c_str(), &traj))) return die("opening trajectory", st);
+  const int nframes = cg_setup_nframes(setup);
+  std::vector<OwnedFrame> frames(nframes);
+  for (int i = 0; i < nframes; ++i) {
+    cg_frame f;
+    if ((st = cg_traj_next(traj, &f))) return die("reading trajectory", st);
+    OwnedFrame &o = frames[i];
+    o.xyz.assign(f.xyz, f.xyz + 3 * f.natoms);
+    o.forces.assign(f.forces, f.forces + 3 * f.natoms);
+    o.type.assign(f.type, f.type + f.natoms);
+    o.f = f;
+    o.f.xyz = o.xyz.data();
+    o.f.forces = o.forces.data();
+    o.f.type = o.type.data();
+  }
+  cg_traj_close(traj);
+  cg_ctx *ctx = nullptr;
+  if ((st = cg_create(m, 0, &ctx))) return die("creating device context", st);
+  if (text && (st = cg_write_open(setup, dir.c_str()))) return die("opening A/b files", st);
+  std::vector<cg_frame> cf(nframes);
+  for (int i = 0; i < nframes; ++i) cf[i] = frames[i].f;
+  if (do_fit || batch <= 0) batch = nframes;
+  double ms_asm = 0;
+  int64_t total_rows = 0;
+  std::vector<double> b_all;
+  for (int f0 = 0; f0 < nframes; f0 += batch) {
+    int nb = std::min(batch, nframes - f0);
+    int64_t rows = 0;
+    for (int i = 0; i < nb; ++i) rows += cg_model_frame_rows(m, cf[f0 + i].natoms, cf[f0 + i].index);
+    std::vector<double> b(rows);
+    const void *A = nullptr;
+    if ((st = cg_assemble(ctx, cf.data() + f0, nb, prec, &A, b.data(), &rows))) return die("assembling", st, ctx);
+    double te, ta, tt;
+    cg_last_timing(ctx, &te, &ta, &tt);
+    ms_asm += tt;
+    b_all.insert(b_all.end(), b.begin(), b.end());
+    if (text) {
+      std::vector<double> host((size_t)rows * m->ncols);
+      if ((st = cg_copy_A(ctx, 0, rows, host.data()))) return die("copying A", st, ctx);
+      int64_t r = 0;
+      for (int i = 0; i < nb; ++i) {
+        int nr = (int)cg_model_frame_rows(m, cf[f0 + i].natoms, cf[f0 + i].index);
+        if ((st = cg_write_frame(setup, &cf[f0 + i], host.data() + r * m->ncols, nr))) return die("writing A", st);
+        r += nr;
+      }
+    }
+    total_rows += rows;
+  }
+  if (text && (st = cg_write_close(setup))) return die("closing A/b files", st);
+  if ((st = cg_write_header(setup, dir.c_str()))) return die("writing params.header", st);
+  std::printf("...matrix population complete: %lld rows x %d columns, %.3f ms on the GPU (%.3e rows/s)\n",
+              (long long)total_rows, m->ncols, ms_asm, total_rows / (ms_asm * 1e-3));
+  if (do_fit) {
+    std::vector<double> x(m->ncols), y(total_rows);
+    cg_fit_result res{};
+    if ((st = cg_fit(ctx, b_all.data(), eps, refine, x.data(), &res))) return die("fitting", st, ctx);
+    if ((st = cg_matvec(ctx, x.data(), y.data()))) return die("computing A x", st, ctx);
+    if ((st = cg_write_params(setup, dir.c_str(), x.data(), m->ncols, total_rows, &res, y.data(), b_all.data())))
+      return die("writing params.txt", st);
+    std::printf("fit: RMS %.6e, %d of %d variables kept, gram %.3f ms, solve %.3f ms, refine %.3f ms\n", res.rms,
+                res.nvars, m->ncols, res.ms_gram, res.ms_solve, res.ms_refine);
+  }
+  cg_destroy(ctx);
+  cg_setup_free(setup);
+  std::chrono::duration<double> dt = std::chrono::steady_clock::now() - t0;
+  std::printf("total wall time %.3f s\n", dt.count());
+  return 0;
+}

@@ -1,0 +1,107 @@
+// Device-side context shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/chimes_gpu.h"
+
+namespace cg {
+
+void set_error(const std::string &m);
+
+#define CG_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::cg::Error(CG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Grow-only device buffer.
+struct DBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  void *get(size_t bytes) {
+    if (bytes > cap) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      size_t n = bytes + bytes / 4 + 256;
+      if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; cap = 0; return nullptr; }
+      cap = n;
+    }
+    return p;
+  }
+  template <class T> T *as(size_t count) { return static_cast<T *>(get(count * sizeof(T))); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+// Per-frame geometry handed to the kernels (computed on the host exactly as
+// BOX::UPDATE_CELL does, src/ClassDefs.C:1940-2211).
+struct DevFrame {
+  int natoms;
+  int atom_off;        // first atom of the frame in the batch arrays
+  long long row_off;   // first row of the frame in A
+  int stress_rows;     // 0, 3 or 9
+  int ener_rows;       // 0 or 3
+  int small;           // unordered neighbour lists (NEIGHBORS::DO_UPDATE_SMALL)
+  int ortho;
+  double H[9];         // HMAT (row-major: H[r*3+c])
+  double Hi[9];        // INVR_HMAT
+  double lx, ly, lz;   // LAMMPS lengths used by the orthorhombic wrap
+  double width[3];     // perpendicular cell widths
+  double min_extent;   // smallest LAMMPS extent (small-cell switch)
+  double inv_vol;
+  double perm_scale[5];
+};
+
+// Device copy of one cluster family (3B or 4B) or the 2B pair family.
+struct DevFamily {
+  int np = 0;            // pairs per cluster (1, 3, 6)
+  int nb = 0;            // atoms per cluster (2, 3, 4)
+  int ntypes = 0;        // cluster (or pair) types
+  int ncols = 0;         // columns of the family
+  int max_pow = 0;       // table width needed (largest power index + 1)
+  double cutoff = 0;     // list cutoff of the family
+  std::vector<int> h_col_offset;   // absolute first column per type
+  std::vector<int> h_ncols;        // columns per type
+  std::vector<int> h_excluded;
+  // device tables
+  int *col_offset = nullptr;  // [2*ntypes] family-relative then absolute first column
+  int *ncol_type = nullptr;   // [ntypes] columns per type
+  int *colptr = nullptr;      // [ncols+1] first perm of each family column (family-relative)
+  uint32_t *perm = nullptr;   // [nperm] packed powers, 4 bits per canonical pair
+  double *smin = nullptr, *smax = nullptr, *xavg = nullptr, *xdiff = nullptr, *lambda = nullptr; // [ntypes*np]
+  int *id_map = nullptr;      // [10^nb]
+  int *id_atom_perm = nullptr;// [10^nb * nb]
+  int *id_pair_map = nullptr; // [10^nb * np]
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  const cg_model *model = nullptr;
+  cg_model m{};               // copy of the POD header
+  DevFamily fam[3];           // 0: 2B pairs, 1: 3B, 2: 4B
+  int *d_int_pair_map = nullptr;
+  int sticky_small = 0;       // NEIGHBORS::UPDATE_WITH_BIG is sticky across frames
+  double penalty_scale = 1.0e4;  // PAIR CHEBYSHEV PENALTY SCALING default (chimes_md)
+  double penalty_dist = 0.01;    // PAIR CHEBYSHEV PENALTY DIST default
+  // batch buffers
+  DBuf b_frames, b_xyz, b_type, b_wrap, b_ncount, b_noff, b_nlist, b_ccount, b_coff, b_clist,
+      b_ctype, b_crank, b_bucket, b_bcount, b_tasks, b_A, b_scan, b_x, b_tmp, b_tmp2, b_tmp3;
+  void *A = nullptr;          // last assembled A (device)
+  int A_prec = CG_FP64;
+  int64_t A_rows = 0;
+  int A_ld = 0;
+  std::vector<double> h_b;    // last b
+  float ms_enum = 0, ms_asm = 0, ms_total = 0;
+  hipEvent_t ev[4] = {};
+};
+
+}  // namespace cg
+
+struct cg_ctx {
+  cg::Ctx c;
+};

@@ -1,0 +1,406 @@
+// Least-squares fit of the assembled design matrix on gfx950.
+//
+// Replaces the SVD path of chimes_lsq.py (src/chimes_lsq.py:169-214,284-306):
+//   x = V diag(1/sigma_i for sigma_i > eps*sigma_max) U^T b.
+// Here the same truncated solution is obtained from the Gram matrix
+// G = A^T A (eigen-decomposition G = V diag(lambda) V^T, lambda = sigma^2,
+// kept while lambda_i > eps^2 lambda_max) followed by FP64 iterative
+// refinement on the residual r = b - A x, which converges to the truncated
+// least-squares solution in the kept subspace.
+//
+//   CG_FP64 : G on FP64 MFMA (v_mfma_f64_16x16x4f64), A in FP64.
+//   CG_FP32 : A in FP32; G as split-BF16x3 (hi*hi + hi*lo + lo*hi) on
+//             v_mfma_f32_16x16x32_bf16 with FP32 accumulation per K-slab and
+//             FP64 accumulation across slabs; residual and A^T r in FP64.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "setup.hpp"
+
+namespace cg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int GT = 64;      // Gram output tile (GT x GT per workgroup)
+constexpr int GK = 16;      // rows staged per LDS step (FP64 path)
+constexpr int GK32 = 32;    // rows staged per LDS step (BF16 path)
+
+// Upper-triangular tile pair index -> (ti, tj) with ti <= tj.
+__device__ __forceinline__ void tile_of(int p, int nt, int &ti, int &tj) {
+  int i = 0;
+  while (p >= nt - i) { p -= nt - i; ++i; }
+  ti = i;
+  tj = i + p;
+}
+
+// Partial Gram of rows [k0, k1) for one 64x64 tile (FP64 MFMA).
+// Output layout of v_mfma_f64_16x16x4f64: lane l, register v holds
+// D[4*(l/16)+v][l%16]; operand A lane l supplies A[l%16][l/16], operand B
+// supplies B[l/16][l%16].
+__global__ __launch_bounds__(256) void k_gram_f64(const double *__restrict__ A, long long ld, long long m, int n,
+                                                  long long kchunk, int nt, double *__restrict__ part) {
+  __shared__ double sI[GK][GT + 1];
+  __shared__ double sJ[GK][GT + 1];
+  int ti, tj;
+  tile_of(blockIdx.x, nt, ti, tj);
+  const int i0 = ti * GT, j0 = tj * GT;
+  const long long k0 = (long long)blockIdx.y * kchunk;
+  const long long k1 = min(m, k0 + kchunk);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;  // wave's 32x32 sub-tile
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (d4)0.0;
+  for (long long k = k0; k < k1; k += GK) {
+    // stage GK rows of the I and J column panels
+    for (int e = threadIdx.x; e < GK * GT; e += 256) {
+      int r = e / GT, c = e % GT;
+      long long row = k + r;
+      double vi = 0.0, vj = 0.0;
+      if (row < k1) {
+        if (i0 + c < n) vi = A[row * ld + i0 + c];
+        if (j0 + c < n) vj = A[row * ld + j0 + c];
+      }
+      sI[r][c] = vi;
+      sJ[r][c] = vj;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 4) {
+      const int kr = kk + lane / 16;
+      double av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = sI[kr][wr + a * 16 + lane % 16];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = sJ[kr][wc + b * 16 + lane % 16];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double *P = part + (size_t)blockIdx.y * n * n;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int gi = i0 + wr + a * 16 + 4 * (lane / 16) + v;
+        int gj = j0 + wc + b * 16 + lane % 16;
+        if (gi < n && gj < n) P[(size_t)gi * n + gj] = acc[a][b][v];
+      }
+}
+
+// Split-BF16x3 partial Gram (A in FP32).  Operand A of
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][8*(l/16) .. +7]; operand B
+// supplies B[8*(l/16) .. +7][l%16]; output as the FP64 variant.
+__global__ __launch_bounds__(256) void k_gram_bf16x3(const float *__restrict__ A, long long ld, long long m, int n,
+                                                     long long kchunk, int nt, double *__restrict__ part) {
+  __shared__ float sI[GK32][GT + 1];
+  __shared__ float sJ[GK32][GT + 1];
+  int ti, tj;
+  tile_of(blockIdx.x, nt, ti, tj);
+  const int i0 = ti * GT, j0 = tj * GT;
+  const long long k0 = (long long)blockIdx.y * kchunk;
+  const long long k1 = min(m, k0 + kchunk);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;
+  f4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (f4)0.0f;
+  for (long long k = k0; k < k1; k += GK32) {
+    for (int e = threadIdx.x; e < GK32 * GT; e += 256) {
+      int r = e / GT, c = e % GT;
+      long long row = k + r;
+      float vi = 0.f, vj = 0.f;
+      if (row < k1) {
+        if (i0 + c < n) vi = A[row * ld + i0 + c];
+        if (j0 + c < n) vj = A[row * ld + j0 + c];
+      }
+      sI[r][c] = vi;
+      sJ[r][c] = vj;
+    }
+    __syncthreads();
+    const int kb = 8 * (lane / 16);
+    bf8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float v = sI[kb + q][wr + a * 16 + lane % 16];
+        __bf16 h = (__bf16)v;
+        ah[a][q] = h;
+        al[a][q] = (__bf16)(v - (float)h);
+      }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float v = sJ[kb + q][wc + b * 16 + lane % 16];
+        __bf16 h = (__bf16)v;
+        bh[b][q] = h;
+        bl[b][q] = (__bf16)(v - (float)h);
+      }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  double *P = part + (size_t)blockIdx.y * n * n;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int gi = i0 + wr + a * 16 + 4 * (lane / 16) + v;
+        int gj = j0 + wc + b * 16 + lane % 16;
+        if (gi < n && gj < n) P[(size_t)gi * n + gj] = (double)acc[a][b][v];
+      }
+}
+
+// Sum the split-K partials of the upper triangle and mirror to a full
+// symmetric matrix.
+__global__ void k_gram_reduce(const double *part, int S, int n, double *G) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  int i = (int)(idx / n), j = (int)(idx % n);
+  int a = min(i, j), b = max(i, j);
+  int ta = a / GT, tb = b / GT;
+  // partial of tile (ta, tb) holds row index from the I panel (ta) and column from J (tb)
+  (void)ta; (void)tb;
+  double s = 0.0;
+  for (int q = 0; q < S; ++q) s += part[(size_t)q * n * n + (size_t)a * n + b];
+  G[(size_t)i * n + j] = s;
+}
+
+// y = b - A x   (FP64 accumulation; one wave per row)
+template <class T>
+__global__ __launch_bounds__(256) void k_residual(const T *__restrict__ A, long long ld, long long m, int n,
+                                                  const double *__restrict__ x, const double *__restrict__ b,
+                                                  double *__restrict__ r) {
+  long long row = (long long)blockIdx.x * 4 + threadIdx.x / 64;
+  int lane = threadIdx.x % 64;
+  if (row >= m) return;
+  const T *a = A + row * ld;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += (double)a[j] * x[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) r[row] = (b ? b[row] : 0.0) - s;
+}
+
+// g = A^T v over a slab of rows; one thread per column, slabs summed with atomics.
+template <class T>
+__global__ __launch_bounds__(256) void k_atv(const T *__restrict__ A, long long ld, long long m, int n,
+                                             const double *__restrict__ v, long long slab, double *__restrict__ g) {
+  int j = blockIdx.x * 256 + threadIdx.x;
+  long long r0 = (long long)blockIdx.y * slab, r1 = min(m, r0 + slab);
+  if (j >= n) return;
+  double s = 0.0;
+  for (long long r = r0; r < r1; ++r) s += (double)A[r * ld + j] * v[r];
+  atomicAdd(&g[j], s);
+}
+
+__global__ void k_scale(double *v, const double *w, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= w[i];
+}
+__global__ void k_axpy(double *x, const double *d, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+struct FitState {
+  rocblas_handle h = nullptr;
+};
+
+static FitState &fit_state(Ctx &c) {
+  static thread_local FitState fs;
+  if (!fs.h) {
+    if (rocblas_create_handle(&fs.h) != rocblas_status_success) throw Error(CG_ERR_DEVICE, "rocblas_create_handle");
+  }
+  rocblas_set_stream(fs.h, c.stream);
+  return fs;
+}
+
+template <class T>
+static void atv(Ctx &c, const T *A, long long m, int n, const double *v, double *g) {
+  CG_HIP(hipMemsetAsync(g, 0, n * sizeof(double), c.stream));
+  long long slab = 2048;
+  long long nslab = (m + slab - 1) / slab;
+  hipLaunchKernelGGL(k_atv<T>, dim3((n + 255) / 256, (unsigned)nslab), dim3(256), 0, c.stream, A, (long long)c.A_ld,
+                     m, n, v, slab, g);
+}
+
+template <class T>
+static void residual(Ctx &c, const T *A, long long m, int n, const double *x, const double *b, double *r) {
+  hipLaunchKernelGGL(k_residual<T>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, c.stream, A, (long long)c.A_ld, m, n,
+                     x, b, r);
+}
+
+void fit(Ctx &c, const double *b_host, double eps, int refine, double *x_host, cg_fit_result *res) {
+  if (!c.A) throw Error(CG_ERR_ARG, "nothing assembled");
+  const long long m = c.A_rows;
+  const int n = c.A_ld;
+  FitState &fs = fit_state(c);
+  hipEvent_t e0, e1, e2, e3;
+  CG_HIP(hipEventCreate(&e0)); CG_HIP(hipEventCreate(&e1)); CG_HIP(hipEventCreate(&e2)); CG_HIP(hipEventCreate(&e3));
+  // buffers: G [n*n], part [S*n*n], b [m], r [m], x, c, t, dx, w [n]
+  const int nt = (n + GT - 1) / GT;
+  const int npair = nt * (nt + 1) / 2;
+  long long target = std::max<long long>(1, (long long)(1024 + npair - 1) / npair);  // >= ~1024 workgroups
+  long long S = std::min<long long>(target, std::max<long long>(1, m / 256));
+  S = std::min<long long>(S, 64);
+  long long kchunk = (m + S - 1) / S;
+  kchunk = (kchunk + GK32 - 1) / GK32 * GK32;
+  S = (m + kchunk - 1) / kchunk;
+  double *G = c.b_x.as<double>((size_t)n * n + (size_t)S * n * n + 2 * (size_t)m + 8 * (size_t)n + 64);
+  if (!G) throw Error(CG_ERR_MEMORY, "fit buffers");
+  double *part = G + (size_t)n * n;
+  double *d_b = part + (size_t)S * n * n;
+  double *d_r = d_b + m;
+  double *d_x = d_r + m;
+  double *d_c = d_x + n, *d_t = d_c + n, *d_dx = d_t + n, *d_w = d_dx + n, *d_W = d_w + n, *d_E = d_W + n;
+  CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CG_HIP(hipEventRecord(e0, c.stream));
+  if (c.A_prec == CG_FP64)
+    hipLaunchKernelGGL(k_gram_f64, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const double *)c.A,
+                       (long long)n, m, n, kchunk, nt, part);
+  else
+    hipLaunchKernelGGL(k_gram_bf16x3, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const float *)c.A,
+                       (long long)n, m, n, kchunk, nt, part);
+  CG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)(((long long)n * n + 255) / 256)), dim3(256), 0, c.stream, part,
+                     (int)S, n, G);
+  CG_HIP(hipEventRecord(e1, c.stream));
+  // eigen-decomposition G = V diag(W) V^T (column-major V in G; G is symmetric)
+  rocblas_int *d_info = reinterpret_cast<rocblas_int *>(d_E + n);
+  if (rocsolver_dsyevd(fs.h, rocblas_evect_original, rocblas_fill_upper, n, G, n, d_W, d_E, d_info) !=
+      rocblas_status_success)
+    throw Error(CG_ERR_NUMERIC, "rocsolver_dsyevd failed");
+  std::vector<double> W(n);
+  int info = 0;
+  CG_HIP(hipMemcpyAsync(W.data(), d_W, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  if (info != 0) throw Error(CG_ERR_NUMERIC, "eigensolver did not converge (info=" + std::to_string(info) + ")");
+  double lmax = 0;
+  for (double v : W) lmax = std::max(lmax, std::fabs(v));
+  const double thr = eps * eps * lmax;
+  std::vector<double> winv(n, 0.0);
+  int nvars = 0;
+  for (int i = 0; i < n; ++i)
+    if (W[i] > thr) { winv[i] = 1.0 / W[i]; ++nvars; }
+  CG_HIP(hipMemcpyAsync(d_w, winv.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  // x = V W+ V^T (A^T b)
+  auto project = [&](const double *in, double *out) {
+    const double one = 1.0, zero = 0.0;
+    if (rocblas_dgemv(fs.h, rocblas_operation_transpose, n, n, &one, G, n, in, 1, &zero, d_t, 1) != rocblas_status_success)
+      throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
+    hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_t, d_w, n);
+    if (rocblas_dgemv(fs.h, rocblas_operation_none, n, n, &one, G, n, d_t, 1, &zero, out, 1) != rocblas_status_success)
+      throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
+  };
+  if (c.A_prec == CG_FP64) atv(c, (const double *)c.A, m, n, d_b, d_c);
+  else atv(c, (const float *)c.A, m, n, d_b, d_c);
+  project(d_c, d_x);
+  CG_HIP(hipEventRecord(e2, c.stream));
+  for (int it = 0; it < refine; ++it) {
+    if (c.A_prec == CG_FP64) {
+      residual(c, (const double *)c.A, m, n, d_x, d_b, d_r);
+      atv(c, (const double *)c.A, m, n, d_r, d_c);
+    } else {
+      residual(c, (const float *)c.A, m, n, d_x, d_b, d_r);
+      atv(c, (const float *)c.A, m, n, d_r, d_c);
+    }
+    project(d_c, d_dx);
+    hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_x, d_dx, n);
+  }
+  if (c.A_prec == CG_FP64) residual(c, (const double *)c.A, m, n, d_x, d_b, d_r);
+  else residual(c, (const float *)c.A, m, n, d_x, d_b, d_r);
+  CG_HIP(hipEventRecord(e3, c.stream));
+  std::vector<double> r(m);
+  CG_HIP(hipMemcpyAsync(x_host, d_x, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(r.data(), d_r, m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  double Z = 0;
+  for (double v : r) Z += v * v;
+  double mx = 0;
+  for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(x_host[i]));
+  float tg = 0, ts = 0, tr = 0;
+  CG_HIP(hipEventElapsedTime(&tg, e0, e1));
+  CG_HIP(hipEventElapsedTime(&ts, e1, e2));
+  CG_HIP(hipEventElapsedTime(&tr, e2, e3));
+  for (auto e : {e0, e1, e2, e3}) (void)hipEventDestroy(e);
+  res->eps = eps;
+  res->rms = std::sqrt(Z / (double)m);
+  res->max_abs_x = mx;
+  res->nvars = nvars;
+  res->eps_abs = eps * std::sqrt(lmax);
+  res->bic = (double)m * std::log(Z / (double)m) + (double)nvars * std::log((double)m);
+  res->ms_gram = tg;
+  res->ms_solve = ts;
+  res->ms_refine = tr;
+}
+
+void matvec(Ctx &c, const double *x_host, double *y_host) {
+  const long long m = c.A_rows;
+  const int n = c.A_ld;
+  double *d = c.b_tmp.as<double>(m + n);
+  if (!d) throw Error(CG_ERR_MEMORY, "matvec");
+  double *dx = d + m;
+  CG_HIP(hipMemcpyAsync(dx, x_host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (c.A_prec == CG_FP64) residual(c, (const double *)c.A, m, n, dx, nullptr, d);
+  else residual(c, (const float *)c.A, m, n, dx, nullptr, d);
+  CG_HIP(hipMemcpyAsync(y_host, d, m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  for (long long i = 0; i < m; ++i) y_host[i] = -y_host[i];
+}
+
+}  // namespace cg
+
+
+extern "C" {
+
+cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, double *x, cg_fit_result *res) {
+  if (!ctx || !b_host || !x || !res) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::fit(ctx->c, b_host, eps, refine_iters, x, res);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y) {
+  if (!ctx || !x || !y || !ctx->c.A) { cg::set_error("nothing assembled"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::matvec(ctx->c, x, y);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@
 // (src/chimes_lsq.C:562-716), FCUT::print_header (src/Fcut.C:169-178) and
 // CLUSTER[_LIST]::print_header / print_special (src/Cluster.C:423-508,1636-1697).
 #include <cmath>
+#include <ctime>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -341,9 +342,8 @@ static void print_family(std::ostream &h, const ClusterFamily &F, int natoms, in
   }
 }
 
-void write_header(const Setup &s, const std::string &dir) {
-  std::ofstream h(dir + "/params.header");
-  if (!h) throw Error(CG_ERR_INPUT, "cannot write params.header");
+std::string header_text(const Setup &s) {
+  std::ostringstream h;
   h << "USECOUL: false" << std::endl;
   h << "FITCOUL: false" << std::endl;
   h << "USE3BCH: " << (s.o3 > 0 ? "true" : "false") << std::endl;
@@ -373,11 +373,17 @@ void write_header(const Setup &s, const std::string &dir) {
   print_family(h, s.trips, 3, s.o3);
   print_family(h, s.quads, 4, s.o4);
   h << std::endl;
+  return h.str();
 }
 
-void write_map(const Setup &s, const std::string &dir) {
-  std::ofstream o(dir + "/ff_groups.map");
-  if (!o) throw Error(CG_ERR_INPUT, "cannot write ff_groups.map");
+void write_header(const Setup &s, const std::string &dir) {
+  std::ofstream h(dir + "/params.header");
+  if (!h) throw Error(CG_ERR_INPUT, "cannot write params.header");
+  h << header_text(s);
+}
+
+std::string map_text(const Setup &s) {
+  std::ostringstream o;
   o << "PAIRMAPS: " << s.pair_map.size() << std::endl;
   for (auto &kv : s.pair_map) o << kv.second << " " << kv.first << std::endl;
   for (const ClusterFamily *F : {&s.trips, &s.quads}) {
@@ -385,6 +391,199 @@ void write_map(const Setup &s, const std::string &dir) {
     o << std::endl << (F->natoms == 3 ? "TRIPMAPS: " : "QUADMAPS: ") << F->name_map.size() << std::endl;
     for (auto &kv : F->name_map) o << kv.second << " " << kv.first << std::endl;
   }
+  return o.str();
+}
+
+void write_map(const Setup &s, const std::string &dir) {
+  std::ofstream o(dir + "/ff_groups.map");
+  if (!o) throw Error(CG_ERR_INPUT, "cannot write ff_groups.map");
+  o << map_text(s);
+}
+
+
+// params.txt as printed by chimes_lsq.py (src/chimes_lsq.py:155-162,303-309,315-672):
+// the fit summary, the header up to the cluster blocks, the 2-body coefficients,
+// the cluster tables with a parameter column, the maps and the energy offsets.
+// The cluster blocks are located by the same fixed line offsets the script uses.
+static std::vector<std::string> split_lines(const std::string &t) {
+  std::vector<std::string> v;
+  size_t p = 0;
+  while (p < t.size()) {
+    size_t q = t.find('\n', p);
+    if (q == std::string::npos) q = t.size();
+    v.push_back(t.substr(p, q - p));
+    p = q + 1;
+  }
+  return v;
+}
+
+static std::string rstrip(const std::string &x) {
+  size_t e = x.find_last_not_of(" \t\r\n");
+  return e == std::string::npos ? "" : x.substr(0, e + 1);
+}
+
+std::string params_text(const Setup &s, const double *x, int nx, long long nlines, const cg_fit_result &r,
+                        const std::string &date) {
+  std::ostringstream o;
+  char buf[512];
+  auto P = [&](const std::string &t) { o << t << "\n"; };
+  auto fx = [&](const char *fmt, double v) { std::snprintf(buf, sizeof buf, fmt, v); return std::string(buf); };
+  P("! Date  " + date);
+  P("!");
+  P("! Number of variables            =  " + std::to_string(nx));
+  P("! Number of equations            =  " + std::to_string(nlines));
+  P("! svd algorithm used");
+  P(fx("! eps (= args.eps*dmax)          =  %11.4e", r.eps_abs));
+  P(fx("! SVD regularization factor      = %11.4e", r.eps));
+  P(fx("! RMS force error                = %11.4e", r.rms));
+  P(fx("! max abs variable               = %11.4e", r.max_abs_x));
+  P("! number of fitting vars         =  " + std::to_string(r.nvars));
+  P(fx("! Bayesian Information Criterion = %11.4e", r.bic));
+  P("!");
+  std::vector<std::string> hf = split_lines(header_text(s));
+  auto tok = [](const std::string &l) { return split_ws(l); };
+  int trips_line = 0, quads_line = 0;
+  long total_trips = 0, total_quads = 0;
+  for (size_t i = 0; i < hf.size(); ++i) {
+    P(hf[i]);
+    auto t = tok(hf[i]);
+    bool brk = false;
+    if (t.size() > 3 && t[2] == "TRIPLETS:") {
+      total_trips = std::stol(t[3]);
+      trips_line = (int)i;
+      for (size_t j = i; j < hf.size(); ++j) {
+        auto u = tok(hf[j]);
+        if (u.size() > 3 && u[2] == "QUADRUPLETS:") {
+          P(hf[j]);
+          total_quads = std::stol(u[3]);
+          quads_line = (int)j;
+          brk = true;
+          break;
+        }
+      }
+    }
+    if (brk) break;
+  }
+  std::string pot = tok(hf[7])[1];
+  P("");
+  P("PAIR " + pot + " PARAMS \n");
+  int snum2 = 0;
+  {
+    auto t = tok(hf[7]);
+    if (t.size() >= 4) snum2 = std::stoi(t[2]);
+  }
+  int ntypes = std::stoi(tok(hf[9])[2]);
+  int pairs_line = 9 + 2 + ntypes + 2;
+  int total_pairs = std::stoi(tok(hf[pairs_line])[2]);
+  int effective_pair = 0;
+  for (int i = 0; i < total_pairs; ++i) {
+    auto t = tok(hf[pairs_line + 2 + i + 1]);
+    P("PAIRTYPE PARAMS: " + std::to_string(i) + " " + t[1] + " " + t[2] + "\n");
+    for (int j = 0; j < snum2; ++j) {
+      std::snprintf(buf, sizeof buf, "%3d %21.13e", j, x[effective_pair * snum2 + j]);
+      P(buf);
+    }
+    ++effective_pair;
+    P(" ");
+  }
+  int counted_trip = 0, counted_quad = 0;
+  if (total_trips > 0) {
+    P("TRIPLET " + pot + " PARAMS \n");
+    int add = 0, par = 0;
+    for (long t = 0; t < total_trips; ++t) {
+      P("TRIPLETTYPE PARAMS:");
+      P("  " + rstrip(hf[trips_line + 2 + add]));
+      auto p1 = tok(hf[trips_line + 3 + add]);
+      if (p1[4] == "EXCLUDED:") {
+        P("   PAIRS: " + p1[1] + " " + p1[2] + " " + p1[3] + " EXCLUDED:");
+        add += 1;
+      } else {
+        std::string uniq = p1[4], totl = p1[6];
+        P("   PAIRS: " + p1[1] + " " + p1[2] + " " + p1[3] + " UNIQUE: " + uniq + " TOTAL: " + totl);
+        P("     index  |  powers  |  equiv index  |  param index  |       parameter       ");
+        P("   ----------------------------------------------------------------------------");
+        add += 3;
+        for (int i = 0; i < std::stoi(totl); ++i) {
+          add += 1;
+          const std::string &L = hf[trips_line + 2 + add];
+          auto ls = tok(L);
+          std::snprintf(buf, sizeof buf, "%s %21.13e", L.c_str(), x[effective_pair * snum2 + par + std::stoi(ls[5])]);
+          P(buf);
+        }
+        par += std::stoi(uniq);
+        counted_trip += std::stoi(uniq);
+      }
+      P("");
+      add += 2;
+    }
+  }
+  if (total_quads > 0) {
+    P("QUADRUPLET " + pot + " PARAMS \n");
+    int add = 0, par = 0;
+    for (long t = 0; t < total_quads; ++t) {
+      P("QUADRUPLETYPE PARAMS: ");
+      P("  " + rstrip(hf[quads_line + 2 + add]));
+      auto p1 = tok(hf[quads_line + 3 + add]);
+      std::string v = p1[1] + " " + p1[2] + " " + p1[3] + " " + p1[4] + " " + p1[5] + " " + p1[6];
+      if (p1[7] == "EXCLUDED:") {
+        P("   PAIRS: " + v + " EXCLUDED: ");
+        add += 1;
+      } else {
+        std::string uniq = p1[7], totl = p1[9];
+        P("   PAIRS: " + v + " UNIQUE: " + uniq + " TOTAL: " + totl);
+        P("     index  |  powers  |  equiv index  |  param index  |       parameter       ");
+        P("   ----------------------------------------------------------------------------");
+        add += 3;
+        for (int i = 0; i < std::stoi(totl); ++i) {
+          add += 1;
+          const std::string &L = hf[quads_line + 2 + add];
+          auto ls = tok(L);
+          std::snprintf(buf, sizeof buf, "%s %21.13e", L.c_str(),
+                        x[effective_pair * snum2 + counted_trip + par + std::stoi(ls[8])]);
+          P(buf);
+        }
+        par += std::stoi(uniq);
+        counted_quad += std::stoi(uniq);
+      }
+      P("");
+      add += 2;
+    }
+  }
+  P("");
+  for (auto &l : split_lines(map_text(s))) P(l);
+  P("");
+  int total = effective_pair * snum2 + counted_trip + counted_quad;
+  if (total != nx && nx != total + ntypes) throw Error(CG_ERR_ARG, "Error in counting parameters");
+  if (nx == total + ntypes) {
+    P("NO ENERGY OFFSETS:  " + std::to_string(ntypes));
+    for (int i = 0; i < ntypes; ++i) {
+      std::snprintf(buf, sizeof buf, "ENERGY OFFSET %d %21.13e", i + 1, x[total + i]);
+      P(buf);
+    }
+  }
+  P("ENDFILE");
+  return o.str();
 }
 
 }  // namespace cg
+
+extern "C" cg_status cg_write_params(const cg_setup *cs, const char *outdir, const double *x, int ncols,
+                                     int64_t nrows, const cg_fit_result *res, const double *y, const double *b) {
+  try {
+    std::string d = outdir ? outdir : ".";
+    std::time_t now = std::time(nullptr);
+    char date[32];
+    std::strftime(date, sizeof date, "%Y-%m-%d", std::localtime(&now));
+    std::ofstream p(d + "/params.txt");
+    if (!p) throw Error(CG_ERR_INPUT, "cannot write params.txt");
+    p << params_text(*cs->s, x, ncols, nrows, *res, date);
+    if (y) {
+      std::FILE *f = std::fopen((d + "/force.txt").c_str(), "w");
+      if (!f) throw Error(CG_ERR_INPUT, "cannot write force.txt");
+      for (int64_t i = 0; i < nrows; ++i) std::fprintf(f, "%13.6e\n", y[i]);
+      std::fclose(f);
+    }
+    (void)b;
+    return CG_OK;
+  } catch (const Error &e) { return fail(e); } catch (const std::exception &e) { return fail(e); }
+}

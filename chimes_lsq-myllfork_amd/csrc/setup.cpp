@@ -451,7 +451,8 @@ Setup *read_setup(const std::string &path) {
     // NATMTYP / TYPEIDX
     if ((k = find_kw("NATMTYP")) < 0) throw Error(CG_ERR_INPUT, "missing # NATMTYP #");
     s->ntypes = to_int(L.at(k + 1, 0));
-    if (s->ntypes < 1 || s->ntypes > CG_MAX_ATOM_TYPES) throw Error(CG_ERR_INPUT, "bad NATMTYP");
+    if (s->ntypes < 1 || s->ntypes > CG_MAX_ATOM_TYPES - 1)
+      throw Error(CG_ERR_UNSUPPORTED, "NATMTYP must be 1..9 (cluster keys are base-10 digits)");
     const int nt = s->ntypes;
     s->pairs.resize(nt * (nt + 1) / 2);
     if ((k = find_kw("TYPEIDX")) < 0) throw Error(CG_ERR_INPUT, "missing # TYPEIDX #");

@@ -186,6 +186,7 @@ cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, doub
  * eps: relative singular-value cut (chimes_lsq.py --eps), refine: iterations.
  * Outputs: x[ncols], rms (over all rows), nvars kept, sigma_max. */
 typedef struct cg_fit_result {
+  double eps;           /* relative cut used (input)                       */
   double rms;
   double max_abs_x;
   int nvars;
@@ -197,7 +198,7 @@ cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters
                  double *x, cg_fit_result *res);
 /* y = A x for the last assembled A (host output). */
 cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y);
-cg_status cg_write_params(const cg_setup *s, const char *outdir, const double *x,
+cg_status cg_write_params(const cg_setup *s, /* params.txt + force.txt */ const char *outdir, const double *x,
                           int ncols, int64_t nrows, const cg_fit_result *res,
                           const double *y, const double *b);
 

@@ -348,7 +348,7 @@ static int pair_index(const cg_model *m, const sys_t *s, int a1, int a2) {
   return m->int_pair_map[s->type[s->parent[a1]] * m->ntypes + s->type[s->parent[a2]]];
 }
 
-static void deriv_2b(const cg_model *m, const sys_t *s, acc_t *A, int fit_stress, int fit_ener) {
+static void deriv_2b(const cg_model *m, const sys_t *s, acc_t *A, int fit_stress, int fit_ener, int md) {
   int snum = m->order2b;
   double Tn[64], Tnd[64];
   cg_model mm = *m;
@@ -360,7 +360,8 @@ static void deriv_2b(const cg_model *m, const sys_t *s, acc_t *A, int fit_stress
       int vstart = p * snum;
       double rab[3];
       double r = dist(s, a1, a2, rab);
-      if (!(r > m->pair_smin[p] && r < m->pair_smax[p])) continue;
+      /* least squares: S_MINIM < r < S_MAXIM (Deriv_2B); force evaluation: r < S_MAXIM (Force_all) */
+      if (!(r < m->pair_smax[p] && (md || r > m->pair_smin[p]))) continue;
       set_polys(m, p, Tn, Tnd, r, m->pair_xdiff[p], m->pair_xavg[p], snum, m->pair_smin[p]);
       double fc, fcd;
       get_fcut(m, &fc, &fcd, r, m->pair_smax[p]);
@@ -476,7 +477,7 @@ int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, d
   A.S = (double *)calloc((size_t)P * 6, sizeof(double));
   A.E = (double *)calloc((size_t)P, sizeof(double));
   double inv_vol = 1.0 / s.box.vol;
-  deriv_2b(m, &s, &A, fit_stress, fit_ener);
+  deriv_2b(m, &s, &A, fit_stress, fit_ener, 0);
   for (int i = 0; i < m->n2b_cols; ++i) for (int c = 0; c < 6; ++c) A.S[i * 6 + c] *= inv_vol;
   if (m->c3.nbody) {
     deriv_nb(m, &m->c3, &s, &A, fit_stress, fit_ener);
@@ -587,4 +588,65 @@ double orc_volume(const double *cell) {
   box_t b;
   if (box_setup(cell, &b)) return 0;
   return b.vol;
+}
+
+/* Force evaluation of one configuration with coefficients x (A column order):
+ * Cheby::Force_all (2-body incl. r < S_MINIM and the penalty
+ * A_p (s_min + d_p - r)^3), Force_3B, Force_4B.  F [3N] kcal/mol/A, E kcal/mol,
+ * vir9 = -sum g r_a r_b / r (no 1/V).  penalty_scale/dist: chimes_md defaults
+ * 1e4 / 0.01 (src/chimes_md.C:1747-1748). */
+int orc_force(const cg_model *m, const cg_frame *f, const double *x, double penalty_scale, double penalty_dist,
+              double *F, double *E, double *vir9) {
+  int small = 0;
+  sys_t s;
+  if (sys_build(m, f, &small, &s)) return -1;
+  const int P = m->ncols_short;
+  acc_t A;
+  A.P = P;
+  A.F = (double *)calloc((size_t)s.n * P * 3, sizeof(double));
+  A.S = (double *)calloc((size_t)P * 6, sizeof(double));
+  A.E = (double *)calloc((size_t)P, sizeof(double));
+  deriv_2b(m, &s, &A, CG_STRESS_ALL, 1, 1);
+  if (m->c3.nbody) deriv_nb(m, &m->c3, &s, &A, CG_STRESS_ALL, 1);
+  if (m->c4.nbody) deriv_nb(m, &m->c4, &s, &A, CG_STRESS_ALL, 1);
+  for (int a = 0; a < s.n; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double v = 0;
+      for (int p = 0; p < P; ++p) v += A.F[((size_t)a * P + p) * 3 + c] * x[p];
+      F[3 * a + c] = v;
+    }
+  double e = 0, sv[6] = {0, 0, 0, 0, 0, 0};
+  for (int p = 0; p < P; ++p) {
+    e += A.E[p] * x[p];
+    for (int c = 0; c < 6; ++c) sv[c] += A.S[p * 6 + c] * x[p];
+  }
+  /* penalty (not linear in x) */
+  for (int a1 = 0; a1 < s.n; ++a1)
+    for (int k = s.o2[a1]; k < s.o2[a1 + 1]; ++k) {
+      int a2 = s.l2[k];
+      int p = pair_index(m, &s, a1, a2);
+      double rab[3];
+      double r = dist(&s, a1, a2, rab);
+      if (!(r < m->pair_smax[p])) continue;
+      if (r - penalty_dist < m->pair_smin[p]) {
+        double rp = m->pair_smin[p] + penalty_dist - r;
+        double coeff = -3.0 * rp * rp * penalty_scale;
+        int b = s.parent[a2];
+        for (int c = 0; c < 3; ++c) {
+          F[3 * a1 + c] += coeff * rab[c] / r;
+          F[3 * b + c] -= coeff * rab[c] / r;
+        }
+        e += rp * rp * rp * penalty_scale;
+        sv[0] -= coeff * rab[0] * rab[0] / r; sv[1] -= coeff * rab[1] * rab[1] / r; sv[2] -= coeff * rab[2] * rab[2] / r;
+        sv[3] -= coeff * rab[0] * rab[1] / r; sv[4] -= coeff * rab[0] * rab[2] / r; sv[5] -= coeff * rab[1] * rab[2] / r;
+      }
+    }
+  *E = e;
+  /* xx xy xz / yx yy yz / zx zy zz */
+  vir9[0] = sv[0]; vir9[1] = sv[3]; vir9[2] = sv[4];
+  vir9[3] = sv[3]; vir9[4] = sv[1]; vir9[5] = sv[5];
+  vir9[6] = sv[4]; vir9[7] = sv[5]; vir9[8] = sv[2];
+  free(A.F); free(A.S); free(A.E);
+  sys_free(&s);
+  return 0;
 }

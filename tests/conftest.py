@@ -1,0 +1,113 @@
+"""Shared fixtures: the ctypes binding, the CPU oracle and the golden data."""
+import ctypes
+import importlib.util
+import json
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "chimes_lsq-myllfork_amd")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def ch():
+    return _load("chimes", os.path.join(PKG, "chimes.py"))
+
+
+@pytest.fixture(scope="session")
+def synth():
+    return _load("synth", os.path.join(PKG, "synth.py"))
+
+
+class Oracle:
+    """ctypes wrapper of oracle/liboracle.so (test infrastructure only)."""
+
+    def __init__(self, ch):
+        self.ch = ch
+        L = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_assemble_frame.argtypes = [ctypes.POINTER(ch.Model), ctypes.POINTER(ch.Frame),
+                                         ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_count_clusters.argtypes = [ctypes.POINTER(ch.Model), ctypes.POINTER(ch.Frame),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
+        L.orc_polys.argtypes = [ctypes.POINTER(ch.Model), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_double, dp, dp]
+        L.orc_fcut.argtypes = [ctypes.POINTER(ch.Model), ctypes.c_double, ctypes.c_double, dp, dp]
+        L.orc_volume.argtypes = [dp]
+        L.orc_volume.restype = ctypes.c_double
+        L.orc_force.argtypes = [ctypes.POINTER(ch.Model), ctypes.POINTER(ch.Frame), dp, ctypes.c_double,
+                                ctypes.c_double, dp, dp, dp]
+        self.L = L
+
+    def assemble(self, setup, frames):
+        small = ctypes.c_int(0)
+        out = []
+        for f in frames:
+            o = np.zeros((setup.frame_rows(f.natoms, f.index), setup.model.ncols))
+            cf = f.c()
+            n = self.L.orc_assemble_frame(setup.model_ptr, ctypes.byref(cf), ctypes.byref(small),
+                                          o.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            assert n == o.shape[0]
+            out.append(o)
+        return np.vstack(out)
+
+    def force(self, setup, frame, x, penalty_scale=1e4, penalty_dist=0.01):
+        F = np.zeros(3 * frame.natoms)
+        E = ctypes.c_double()
+        V = np.zeros(9)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        cf = frame.c()
+        dp = ctypes.POINTER(ctypes.c_double)
+        rc = self.L.orc_force(setup.model_ptr, ctypes.byref(cf), x.ctypes.data_as(dp), penalty_scale, penalty_dist,
+                              F.ctypes.data_as(dp), ctypes.byref(E), V.ctypes.data_as(dp))
+        assert rc == 0
+        return F, E.value, V
+
+
+@pytest.fixture(scope="session")
+def oracle(ch):
+    return Oracle(ch)
+
+
+def colscaled_error(A, R, floor=1e-3):
+    """max |A-R| / max(|R|, floor * max_i |R[:, j]|) (column-scaled relative error)."""
+    A = np.asarray(A)
+    R = np.asarray(R)
+    colmax = np.abs(R).max(axis=0)
+    den = np.maximum(np.abs(R), floor * np.maximum(colmax, 1e-300))
+    err = np.abs(A - R) / den
+    err[(A == 0) & (R == 0)] = 0.0
+    return float(err.max()) if err.size else 0.0
+
+
+def golden_dir(name):
+    return os.path.join(GOLDEN, "lsq", name)
+
+
+def headers():
+    return json.load(open(os.path.join(GOLDEN, "headers.json")))
+
+
+def synth_setup(ch, synth, tmp_path, gen):
+    """Write a generated fm_setup.in and return (Setup, [PyFrame])."""
+    text, frames, names = gen
+    p = os.path.join(str(tmp_path), "fm_setup.in")
+    open(p, "w").write(text)
+    s = ch.Setup(p)
+    fr = [ch.PyFrame(f["natoms"], f["xyz"], f["types"], f["cell"], f["forces"], f["stress"], f["energy"], f["index"])
+          for f in frames]
+    return s, fr

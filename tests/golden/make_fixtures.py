@@ -1,0 +1,121 @@
+"""Extract the golden fixtures used by the tests from the ChIMES-lsq test suite.
+
+Run once in a tree that has the reference mounted (default /root/reference);
+the outputs under tests/golden/ are data only (inputs and the reference's
+expected outputs), trimmed to the frames the tests need:
+
+  headers.json             per LSQ test: fm_setup.in text (+ MULTI list), and
+                           sha256/length of the golden params.header and
+                           ff_groups.map (byte-exact pins of the parser and the
+                           cluster power-set tables).
+  lsq/nonorth2/            full input (10 triclinic frames), golden A (8640x12),
+                           b, force.txt and the fitted params.txt.
+  lsq/test_4atoms.2/       last frame (2+3+4-body, C/H, exclusions) and its
+                           12 golden A rows (12x1308).
+  lsq/fitstrs/             last frame (96 atoms, diagonal stress rows) and the
+                           176 golden A rows that survive in A.txt.ab.
+"""
+import hashlib
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+LSQ = os.path.join(REF, "test_suite-lsq")
+
+HEADER_TESTS = ["nonorth2", "test_4atoms", "test_4atoms.2", "fitstrs", "fitstrs3b", "h2o-invr", "special3b",
+                "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2", "stress-and-ener-2b3",
+                "stress-and-ener-4b", "stress-and-ener-4b2", "two-traj-files", "one-file-three-trajs"]
+
+
+def sha(path):
+    data = open(path, "rb").read()
+    return hashlib.sha256(data).hexdigest(), len(data)
+
+
+def frames_of(path):
+    """Split an .xyzf file into frame text blocks."""
+    lines = open(path).read().split("\n")
+    out, i = [], 0
+    while i < len(lines) and lines[i].strip():
+        n = int(lines[i].split()[0])
+        out.append("\n".join(lines[i:i + n + 2]) + "\n")
+        i += n + 2
+    return out
+
+
+def tail_rows(path):
+    """Rows of a split A.txt.* piece; the first line is a partial row (byte split)."""
+    lines = [l for l in open(path).read().split("\n")[1:] if l.strip()]
+    return np.array([[float(x) for x in l.split()] for l in lines])
+
+
+def set_key(text, key, value):
+    return re.sub(r"(# %s #[^\n]*\n)[^\n]*\n" % key, lambda m: m.group(1) + "\t%s\n" % value, text, count=1)
+
+
+def main():
+    headers = {}
+    for t in HEADER_TESTS:
+        d = os.path.join(LSQ, t)
+        if not os.path.exists(os.path.join(d, "correct_output", "params.header")):
+            continue
+        ent = {"fm_setup": open(os.path.join(d, "fm_setup.in")).read()}
+        if os.path.exists(os.path.join(d, "traj_list.dat")):
+            ent["traj_list"] = open(os.path.join(d, "traj_list.dat")).read()
+        for fn in ["params.header", "ff_groups.map"]:
+            h, n = sha(os.path.join(d, "correct_output", fn))
+            ent[fn] = {"sha256": h, "bytes": n}
+        tsp = os.path.join(d, "correct_output", "test_suite_params.txt")
+        if os.path.exists(tsp):
+            # fitted coefficients of the reference run + its params.txt (Date line
+            # dropped) as a digest: pins the params.txt printer for this header
+            ent["x"] = [float(l.split()[1]) for l in open(tsp) if l.strip()]
+            lines = open(os.path.join(d, "correct_output", "params.txt")).read().split("\n")
+            body = "\n".join(lines[1:]).encode()
+            ent["params_txt"] = {"sha256": hashlib.sha256(body).hexdigest(), "bytes": len(body),
+                                 "summary": [l for l in lines[1:13] if l.startswith("!")]}
+        dim = os.path.join(d, "correct_output", "dim.txt")
+        if os.path.exists(dim):
+            ent["dim"] = open(dim).read().split()
+        headers[t] = ent
+    json.dump(headers, open(os.path.join(HERE, "headers.json"), "w"), indent=1, sort_keys=True)
+
+    # nonorth2: full
+    out = os.path.join(HERE, "lsq", "nonorth2")
+    os.makedirs(out, exist_ok=True)
+    src = os.path.join(LSQ, "nonorth2")
+    open(os.path.join(out, "fm_setup.in"), "w").write(open(os.path.join(src, "fm_setup.in")).read())
+    open(os.path.join(out, "input.xyzf"), "w").write(open(os.path.join(src, "input.xyzf")).read())
+    A = np.loadtxt(os.path.join(src, "correct_output", "A.txt.aa"))
+    b = np.loadtxt(os.path.join(src, "correct_output", "b.txt"))
+    y = np.loadtxt(os.path.join(src, "correct_output", "force.txt"))
+    np.savez_compressed(os.path.join(out, "golden.npz"), A=A, b=b, force=y, row0=0)
+    for fn in ["params.txt", "test_suite_params.txt", "dim.txt"]:
+        open(os.path.join(out, fn), "w").write(open(os.path.join(src, "correct_output", fn)).read())
+
+    # last frame of test_4atoms.2 and fitstrs
+    for t, traj, apiece, frame_rows in [("test_4atoms.2", "dump2.xyzf", "A.txt.ab", 12),
+                                        ("fitstrs", "run_5.OUTCAR.xyzf", "A.txt.ab", 291)]:
+        out = os.path.join(HERE, "lsq", t)
+        os.makedirs(out, exist_ok=True)
+        src = os.path.join(LSQ, t)
+        fr = frames_of(os.path.join(src, traj))
+        text = open(os.path.join(src, "fm_setup.in")).read()
+        # the run uses the first NFRAMES frames of the file; the golden rows
+        # belong to frame NFRAMES-1
+        nframes = int(re.search(r"# NFRAMES #[^\n]*\n\s*(\d+)", text).group(1))
+        open(os.path.join(out, "traj.xyzf"), "w").write(fr[nframes - 1])
+        text = set_key(set_key(text, "NFRAMES", "1"), "TRJFILE", "traj.xyzf")
+        open(os.path.join(out, "fm_setup.in"), "w").write(text)
+        rows = tail_rows(os.path.join(src, "correct_output", apiece))
+        np.savez_compressed(os.path.join(out, "golden.npz"), A=rows, row0=frame_rows - rows.shape[0])
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
