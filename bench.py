@@ -1,0 +1,210 @@
+"""Benchmark of the ChIMES design-matrix / fit / force-evaluation path on one GPU.
+
+Workload (one "step"): assembly of the full design matrix A (force, stress and
+energy rows) of 100 frames of 512-atom carbon at 2.2 g/cc with 2+3+4-body
+Chebyshev orders 16/9/5 (A = 154200 rows x 1038 columns, FP64), synthetic
+seeded inputs.  `value` is A rows per second for that whole batch, timed around
+`cg_assemble` (frames handed over, neighbour build, cluster enumeration,
+2/3/4-body assembly, A resident in HBM afterwards).
+
+Also reported: the FP32 assembly rate, the fit time of the FP64 path and of the
+FP32 + split-BF16x3 Gram + FP64 refinement path with their force-RMS agreement,
+force-evaluation latency p50/p99 of 1k-atom C/H/N/O configurations (orders
+12/6/3), the 4-body kernel's roofline position, and the CPU oracle timed on
+the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames F] [--no-cpu]
+"""
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "chimes_lsq-myllfork_amd")
+METRIC = "A-matrix rows/s + fit time at FP64-oracle tol vs plain FP64; force-eval p50/p99"
+PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(ch, frames):
+    return [ch.PyFrame(f["natoms"], f["xyz"], f["types"], f["cell"], f["forces"], f["stress"], f["energy"],
+                       f["index"]) for f in frames]
+
+
+def setup_from(ch, text):
+    d = tempfile.mkdtemp(prefix="chimes_bench_")
+    p = os.path.join(d, "fm_setup.in")
+    open(p, "w").write(text)
+    return ch.Setup(p)
+
+
+def cpu_baseline(ch, setup, frames, seconds_hint=25.0):
+    """Time the oracle (CPU FP64 restatement) on the host cores."""
+    orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    orc.orc_assemble_many.restype = ctypes.c_longlong
+    orc.orc_assemble_many.argtypes = [ctypes.POINTER(ch.Model), ctypes.POINTER(ch.Frame), ctypes.c_int, ctypes.c_int]
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    cores = max(1, min(cores, 16, len(frames)))
+    nf = cores  # one frame per thread: ~23 s of work per frame per core
+    cf = (ch.Frame * nf)(*[f.c() for f in frames[:nf]])
+    t0 = time.perf_counter()
+    rows = orc.orc_assemble_many(setup.model_ptr, cf, nf, cores)
+    dt = time.perf_counter() - t0
+    return {"value": rows / dt, "unit": "A rows/s", "cores": cores, "kind": "port",
+            "sample": "%d of the benchmark's 512-atom carbon frames (%d rows), one frame per OpenMP thread, "
+                      "%.1f s" % (nf, rows, dt)}
+
+
+def serving_latency(ch, synth, n_req=40, natoms=1000, seed=105):
+    """Force/energy/stress latency of 1k-atom C/H/N/O configurations."""
+    text, frames, _ = synth.chno(nframes=n_req, nmin=natoms, nmax=natoms, seed=seed, target_density=0.1)
+    s = setup_from(ch, text)
+    fr = make_frames(ch, frames)
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1e-3, s.model.ncols)
+    dev = ch.Device(s)
+    dev.force(x, fr[0])
+    lat = []
+    t0 = time.perf_counter()
+    for f in fr:
+        t1 = time.perf_counter()
+        dev.force(x, f)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    tot = time.perf_counter() - t0
+    dev.close()
+    lat = np.array(lat)
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "atoms_per_s": natoms * n_req / tot, "requests": n_req, "natoms": natoms,
+            "orders": "12/6/3", "ncols": int(s.model.ncols)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fit", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the design-matrix path of this build is single-device "
+              "(multi-GPU frame sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        sys.exit(2)
+    ch = _load("chimes", os.path.join(PKG, "chimes.py"))
+    synth = _load("synth", os.path.join(PKG, "synth.py"))
+    log("generating %d carbon frames" % args.frames)
+    text, frames, _ = synth.carbon(nframes=args.frames, natoms=512, seed=102)
+    setup = setup_from(ch, text)
+    fr = make_frames(ch, frames)
+    rows = sum(setup.frame_rows(f.natoms, f.index) for f in fr)
+    dev = ch.Device(setup)
+    import torch  # device synchronisation around the timed region (HIP runtime)
+
+    def step(prec):
+        dev.assemble(fr, prec=prec, copy=False)
+
+    for _ in range(args.warmup):
+        step(ch.CG_FP64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k4 = []
+    asm = []
+    for _ in range(args.steps):
+        step(ch.CG_FP64)
+        ms, ncl = dev.detail()
+        k4.append(ms[2])
+        asm.append(dev.timing())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    value = rows / (dt / args.steps)
+    ms4 = float(np.mean(k4))
+    # algorithmic work of the 4-body kernel: 4 flops per pair per allowed power set
+    # (product, prefix/suffix force coefficient, accumulation), 6 pairs per cluster
+    nallowed = setup.model.c4.total_allowed
+    flops4 = 24.0 * ncl[2] * nallowed
+    achieved = flops4 / (ms4 * 1e-3) / 1e12
+    log("FP64: %.3f ms/step, %.4e rows/s, 4-body kernel %.3f ms (%d quads), %.2f TFLOP/s" % (
+        ms_step, value, ms4, ncl[2], achieved))
+    # FP32 assembly
+    for _ in range(args.warmup):
+        step(ch.CG_FP32)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(ch.CG_FP32)
+    torch.cuda.synchronize()
+    value32 = rows / ((time.perf_counter() - t0) / args.steps)
+    log("FP32: %.4e rows/s" % value32)
+    extra = {"fp32_rows_per_s": value32, "kernel_ms": {"enum": asm[-1][0], "assembly": asm[-1][1]},
+             "family_ms": dict(zip(["2B", "3B", "4B"], dev.detail()[0])),
+             "clusters": dict(zip(["pairs", "triplets", "quadruplets"], ncl))}
+    if not args.no_fit:
+        b = np.zeros(rows)
+        _, b = dev.assemble(fr, prec=ch.CG_FP64, copy=False)
+        t0 = time.perf_counter()
+        x64, r64 = dev.fit(b, eps=1e-5, refine=3)
+        t_fit64 = time.perf_counter() - t0
+        t_asm64 = rows / value
+        dev.assemble(fr, prec=ch.CG_FP32, copy=False)
+        t0 = time.perf_counter()
+        x32, r32 = dev.fit(b, eps=1e-5, refine=4)
+        t_fit32 = time.perf_counter() - t0
+        t_asm32 = rows / value32
+        extra["fit"] = {
+            "fp64": {"fit_s": t_fit64, "gram_ms": r64.ms_gram, "solve_ms": r64.ms_solve, "refine_ms": r64.ms_refine,
+                     "rms": r64.rms, "nvars": r64.nvars, "assembly_plus_fit_rows_per_s": rows / (t_asm64 + t_fit64)},
+            "fp32_bf16x3_refined": {"fit_s": t_fit32, "gram_ms": r32.ms_gram, "solve_ms": r32.ms_solve,
+                                    "refine_ms": r32.ms_refine, "rms": r32.rms, "nvars": r32.nvars,
+                                    "rms_rel_err_vs_fp64": abs(r32.rms - r64.rms) / r64.rms,
+                                    "assembly_plus_fit_rows_per_s": rows / (t_asm32 + t_fit32)},
+        }
+        log("fit FP64 %.3f s rms %.6e | FP32+refine %.3f s rms %.6e" % (t_fit64, r64.rms, t_fit32, r32.rms))
+    dev.close()
+    if not args.no_serve:
+        extra["serving"] = serving_latency(ch, synth)
+        log("serving p50 %.3f ms p99 %.3f ms" % (extra["serving"]["p50_ms"], extra["serving"]["p99_ms"]))
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU oracle on the host")
+        cpu = cpu_baseline(ch, setup, fr)
+        log("CPU: %.1f rows/s on %d cores" % (cpu["value"], cpu["cores"]))
+    out = {
+        "metric": METRIC, "value": value, "unit": "A rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded carbon, uniform random positions, "
+        "1.1 A minimum separation; Gaussian forces/stress/energy)",
+        "config": {"workload": "carbon 2+3+4-body design-matrix assembly, %d frames x 512 atoms, 2.2 g/cc, "
+                               "orders 16/9/5, cutoffs 4.5/4.0/3.5 A, %d rows x %d cols" % (
+                                   args.frames, rows, setup.model.ncols),
+                   "frames": args.frames, "natoms": 512, "rows": rows, "cols": int(setup.model.ncols)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_assemble<4,5,0,double> (4-body rows)", "kernel_ms": ms4,
+                     "algorithmic_flops": flops4},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

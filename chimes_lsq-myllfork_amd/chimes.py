@@ -93,6 +93,7 @@ def lib(path=None):
                                       C.POINTER(C.c_void_p), _dp, C.POINTER(C.c_int64)]
             L.cg_copy_A.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _dp]
             L.cg_last_timing.argtypes = [C.c_void_p, _dp, _dp, _dp]
+            L.cg_last_detail.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int64)]
         opt("cg_fit", [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)])
         opt("cg_matvec", [C.c_void_p, _dp, _dp])
         opt("cg_write_params", [C.c_void_p, C.c_char_p, _dp, C.c_int, C.c_int64,
@@ -234,6 +235,12 @@ class Device:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(self.L.cg_last_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), self.h)
         return a.value, b.value, c.value
+
+    def detail(self):
+        ms = (C.c_double * 3)()
+        n = (C.c_int64 * 3)()
+        check(self.L.cg_last_detail(self.h, ms, n), self.h)
+        return list(ms), list(n)
 
     def fit(self, b, eps=1e-5, refine=2):
         x = np.zeros(self.setup.model.ncols)

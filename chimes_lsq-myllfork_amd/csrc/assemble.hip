@@ -1058,6 +1058,8 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
   int maxn = 0;
   for (auto &d : hf) maxn = std::max(maxn, d.natoms);
   const size_t lds_budget = 150 * 1024;
+  bool fam_ran[3] = {false, false, false};
+  for (int w = 0; w < 3; ++w) c.ncl[w] = fo[w].ncl;
   for (int w = 0; w < 3; ++w) {
     const DevFamily &F = c.fam[w];
     if (F.ntypes == 0 || F.ncols == 0 || fo[w].ncl == 0) continue;
@@ -1085,8 +1087,11 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(Task));
     size_t lds = 0;
     for (auto &tk : tasks) lds = std::max(lds, (3 * (size_t)hf[tk.frame].natoms + 56) * tk.ncols * esz);
+    CG_HIP(hipEventRecord(c.fev[w][0], c.stream));
     if (prec == CG_FP32) launch_family<float>(c, w, tasks, d_tasks, d_frames, fo[w].refs, lds, A, ld);
     else launch_family<double>(c, w, tasks, d_tasks, d_frames, fo[w].refs, lds, A, ld);
+    CG_HIP(hipEventRecord(c.fev[w][1], c.stream));
+    fam_ran[w] = true;
     // the task buffer is reused by the next family: wait so its upload cannot race
     CG_HIP(hipStreamSynchronize(c.stream));
   }
@@ -1102,6 +1107,10 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
   CG_HIP(hipEventElapsedTime(&c.ms_enum, c.ev[0], c.ev[1]));
   CG_HIP(hipEventElapsedTime(&c.ms_asm, c.ev[1], c.ev[2]));
   c.ms_total = c.ms_enum + c.ms_asm;
+  for (int w = 0; w < 3; ++w) {
+    c.ms_fam[w] = 0.f;
+    if (fam_ran[w]) CG_HIP(hipEventElapsedTime(&c.ms_fam[w], c.fev[w][0], c.fev[w][1]));
+  }
   c.A = A;
   c.A_prec = prec;
   c.A_rows = rows;
@@ -1183,6 +1192,7 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
   CG_HIP(hipSetDevice(device));
   CG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (auto &e : c.ev) CG_HIP(hipEventCreate(&e));
+  for (auto &fe : c.fev) for (auto &e : fe) CG_HIP(hipEventCreate(&e));
   std::vector<int> ipm(m->int_pair_map, m->int_pair_map + m->ntypes * m->ntypes);
   c.d_int_pair_map = dalloc_copy(ipm);
   build_family(c, 0);
@@ -1203,6 +1213,7 @@ void ctx_free(Ctx &c) {
   }
   if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);
   for (auto &e : c.ev) if (e) (void)hipEventDestroy(e);
+  for (auto &fe : c.fev) for (auto &e : fe) if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -1279,6 +1290,15 @@ cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f, double *F, d
     cg::set_error(e.what());
     return e.code;
   }
+}
+
+cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3) {
+  if (!ctx) return CG_ERR_ARG;
+  for (int w = 0; w < 3; ++w) {
+    if (ms_family3) ms_family3[w] = ctx->c.ms_fam[w];
+    if (clusters3) clusters3[w] = ctx->c.ncl[w];
+  }
+  return CG_OK;
 }
 
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_asm, double *ms_total) {

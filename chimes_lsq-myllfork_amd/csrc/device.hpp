@@ -98,6 +98,9 @@ struct Ctx {
   std::vector<double> h_b;    // last b
   float ms_enum = 0, ms_asm = 0, ms_total = 0;
   hipEvent_t ev[4] = {};
+  hipEvent_t fev[3][2] = {};   // around each family's assembly launch
+  float ms_fam[3] = {0, 0, 0};
+  long long ncl[3] = {0, 0, 0};  // clusters per family in the last batch
 };
 
 }  // namespace cg

@@ -179,6 +179,11 @@ cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host);
  * context stream): neighbour+enumeration, assembly kernels, total. */
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, double *ms_total);
 
+/* Per-family detail of the last cg_assemble: kernel time of the 2-body,
+ * 3-body and 4-body assembly launches (ms, HIP events on the context stream)
+ * and the number of pairs / triplets / quadruplets assembled. */
+cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3);
+
 /* Fit: solve A x ~ b for the last assembled A (on device).
  *   mode CG_FP64 : FP64 Gram on MFMA, FP64 eigensolve, FP64 refinement.
  *   mode CG_FP32 : A in FP32, split-BF16x3 Gram on MFMA (FP32 accumulate),

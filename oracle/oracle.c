@@ -650,3 +650,21 @@ int orc_force(const cg_model *m, const cg_frame *f, const double *x, double pena
   sys_free(&s);
   return 0;
 }
+
+/* Assemble n frames with nthreads OpenMP threads (one frame per thread at a
+ * time, the analogue of the reference's MPI frame split) and discard the rows;
+ * used to time the CPU baseline.  Returns the number of rows produced. */
+long long orc_assemble_many(const cg_model *m, const cg_frame *frames, int n, int nthreads) {
+  long long rows = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : rows)
+  for (int i = 0; i < n; ++i) {
+    int small = 0;
+    const cg_frame *f = &frames[i];
+    long long r = 3LL * f->natoms + 9 + 3;
+    double *out = (double *)malloc(sizeof(double) * (size_t)r * m->ncols);
+    int got = orc_assemble_frame(m, f, &small, out);
+    free(out);
+    rows += got > 0 ? got : 0;
+  }
+  return rows;
+}

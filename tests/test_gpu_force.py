@@ -1,0 +1,47 @@
+"""GPU force evaluation (serving kernel) vs the oracle's Force_all/3B/4B."""
+import numpy as np
+import pytest
+
+from conftest import synth_setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", ["chno", "carbon"])
+def test_force_matches_oracle(ch, synth, oracle, tmp_path, case):
+    if case == "chno":
+        gen = synth.chno(nframes=1, nmin=150, nmax=150, seed=41)
+    else:
+        gen = synth.carbon(nframes=1, natoms=128, seed=42)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=s.model.ncols)
+    dev = ch.Device(s)
+    F, E, V = dev.force(x, fr[0])
+    Fo, Eo, Vo = oracle.force(s, fr[0], x)
+    assert np.allclose(F, Fo, rtol=1e-10, atol=1e-10 * np.abs(Fo).max())
+    assert E == pytest.approx(Eo, rel=1e-10, abs=1e-10 * abs(Eo))
+    assert np.allclose(V, Vo, rtol=1e-10, atol=1e-10 * np.abs(Vo).max())
+    # F = A x on the force rows (no pair below s_min + 0.01 in these inputs)
+    A, b = dev.assemble(fr)
+    assert np.allclose(F, A[: 3 * fr[0].natoms] @ x, rtol=1e-9, atol=1e-9 * np.abs(F).max())
+    dev.close()
+
+
+def test_force_penalty_below_smin(ch, synth, oracle, tmp_path):
+    """A pair closer than s_min + 0.01 adds the penalty A_p (s_min + d_p - r)^3."""
+    text = synth.fm_setup(["C"], [12.0], 8, 4, 3, 1.2, 4.0, 1.4, smax3=3.5, smax4=3.0)
+    p = tmp_path / "fm_setup.in"
+    open(p, "w").write(text)
+    s = ch.Setup(str(p))
+    rng = np.random.default_rng(4)
+    pos = synth.random_sequential(rng, 30, np.diag([9.0, 9.0, 9.0]), 1.3)
+    pos[1] = pos[0] + np.array([1.15, 0.0, 0.0])  # below s_min
+    f = ch.PyFrame(30, pos, np.zeros(30, dtype=np.int32), np.diag([9.0, 9.0, 9.0]))
+    x = rng.normal(size=s.model.ncols)
+    dev = ch.Device(s)
+    F, E, V = dev.force(x, f)
+    Fo, Eo, Vo = oracle.force(s, f, x)
+    assert np.allclose(F, Fo, rtol=1e-10, atol=1e-10 * np.abs(Fo).max())
+    assert E == pytest.approx(Eo, rel=1e-10)
+    dev.close()
