@@ -117,14 +117,13 @@ def main():
     fr = make_frames(ch, frames)
     rows = sum(setup.frame_rows(f.natoms, f.index) for f in fr)
     dev = ch.Device(setup)
-    import torch  # device synchronisation around the timed region (HIP runtime)
 
     def step(prec):
         dev.assemble(fr, prec=prec, copy=False)
 
     for _ in range(args.warmup):
         step(ch.CG_FP64)
-    torch.cuda.synchronize()
+    dev.synchronize()
     t0 = time.perf_counter()
     k4 = []
     asm = []
@@ -133,7 +132,7 @@ def main():
         ms, ncl = dev.detail()
         k4.append(ms[2])
         asm.append(dev.timing())
-    torch.cuda.synchronize()
+    dev.synchronize()
     dt = time.perf_counter() - t0
     ms_step = dt / args.steps * 1e3
     value = rows / (dt / args.steps)
@@ -151,7 +150,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(ch.CG_FP32)
-    torch.cuda.synchronize()
+    dev.synchronize()
     value32 = rows / ((time.perf_counter() - t0) / args.steps)
     log("FP32: %.4e rows/s" % value32)
     extra = {"fp32_rows_per_s": value32, "kernel_ms": {"enum": asm[-1][0], "assembly": asm[-1][1]},

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libchimes_gpu.so")
 
 CG_FP64, CG_FP32 = 0, 1
+CG_GRAM_FP64, CG_GRAM_BF16X3 = 0, 1
 STATUS = {0: "CG_OK", 1: "CG_ERR_ARG", 2: "CG_ERR_INPUT", 3: "CG_ERR_UNSUPPORTED",
           4: "CG_ERR_DEVICE", 5: "CG_ERR_MEMORY", 6: "CG_ERR_NUMERIC", 7: "CG_ERR_EOF"}
 
@@ -53,7 +54,7 @@ class Frame(C.Structure):
 class FitResult(C.Structure):
     _fields_ = [("eps", C.c_double), ("rms", C.c_double), ("max_abs_x", C.c_double), ("nvars", C.c_int),
                 ("eps_abs", C.c_double), ("bic", C.c_double), ("ms_gram", C.c_double),
-                ("ms_solve", C.c_double), ("ms_refine", C.c_double)]
+                ("ms_solve", C.c_double), ("ms_refine", C.c_double), ("iters", C.c_int)]
 
 
 _lib = None
@@ -94,7 +95,9 @@ def lib(path=None):
             L.cg_copy_A.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _dp]
             L.cg_last_timing.argtypes = [C.c_void_p, _dp, _dp, _dp]
             L.cg_last_detail.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int64)]
-        opt("cg_fit", [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)])
+            L.cg_synchronize.argtypes = [C.c_void_p]
+        opt("cg_fit", [C.c_void_p, _dp, C.c_double, C.c_int, C.c_int, _dp, C.POINTER(FitResult)])
+        opt("cg_gram", [C.c_void_p, C.c_int, _dp, _dp, _dp])
         opt("cg_matvec", [C.c_void_p, _dp, _dp])
         opt("cg_write_params", [C.c_void_p, C.c_char_p, _dp, C.c_int, C.c_int64,
                                 C.POINTER(FitResult), _dp, _dp])
@@ -236,18 +239,33 @@ class Device:
         check(self.L.cg_last_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), self.h)
         return a.value, b.value, c.value
 
+    def synchronize(self):
+        check(self.L.cg_synchronize(self.h), self.h)
+
     def detail(self):
         ms = (C.c_double * 3)()
         n = (C.c_int64 * 3)()
         check(self.L.cg_last_detail(self.h, ms, n), self.h)
         return list(ms), list(n)
 
-    def fit(self, b, eps=1e-5, refine=2):
+    def fit(self, b, eps=1e-5, refine=2, gram=CG_GRAM_FP64):
         x = np.zeros(self.setup.model.ncols)
         res = FitResult()
         b = np.ascontiguousarray(b, dtype=np.float64)
-        check(self.L.cg_fit(self.h, dptr(b), eps, refine, dptr(x), C.byref(res)), self.h)
+        check(self.L.cg_fit(self.h, dptr(b), eps, refine, gram, dptr(x), C.byref(res)), self.h)
         return x, res
+
+    def gram(self, b=None, mode=CG_GRAM_FP64):
+        """(G, A^T b) of the last assembled A (A^T b is None without b)."""
+        n = self.setup.model.ncols
+        G = np.zeros((n, n))
+        atb = None
+        if b is not None:
+            b = np.ascontiguousarray(b, dtype=np.float64)
+            atb = np.zeros(n)
+        check(self.L.cg_gram(self.h, mode, dptr(b) if b is not None else None, dptr(G),
+                             dptr(atb) if atb is not None else None), self.h)
+        return G, atb
 
     def matvec(self, x, rows):
         y = np.zeros(rows)

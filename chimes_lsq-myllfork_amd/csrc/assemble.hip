@@ -1301,6 +1301,18 @@ cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3) {
   return CG_OK;
 }
 
+cg_status cg_synchronize(cg_ctx *ctx) {
+  if (!ctx) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    CG_HIP(hipStreamSynchronize(ctx->c.stream));
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_asm, double *ms_total) {
   if (!ctx) return CG_ERR_ARG;
   if (ms_enum) *ms_enum = ctx->c.ms_enum;

@@ -2,8 +2,8 @@
 // with the design matrix assembled on the GPU, optionally followed by the fit
 // that `chimes_lsq.py` performs (src/chimes_lsq.py:169-672).
 //
-//   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--refine K]
-//                  [--batch FRAMES] [--no-text]
+//   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--gram fp64|bf16x3]
+//                  [--refine K] [--batch FRAMES] [--no-text]
 //
 // Files written in the working directory: A.txt b.txt b-labeled.txt natoms.txt
 // dim.txt params.header ff_groups.map; with --fit also params.txt force.txt.
@@ -31,14 +31,14 @@ int die(const char *what, cg_status st, const void *ctx = nullptr) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--refine K] "
+    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--gram fp64|bf16x3] [--refine K] "
                          "[--batch FRAMES] [--no-text]\n");
     return 1;
   }
   const char *infile = argv[1];
   bool do_fit = false, text = true;
   double eps = 1.0e-5;
-  int prec = CG_FP64, refine = 3, batch = 0;
+  int prec = CG_FP64, refine = 3, batch = 0, gram = CG_GRAM_FP64;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "--fit") do_fit = true;
@@ -49,6 +49,9 @@ int main(int argc, char **argv) {
     else if (a == "--prec" && i + 1 < argc) {
       std::string p = argv[++i];
       prec = p == "fp32" ? CG_FP32 : CG_FP64;
+    } else if (a == "--gram" && i + 1 < argc) {
+      std::string p = argv[++i];
+      gram = p == "bf16x3" ? CG_GRAM_BF16X3 : CG_GRAM_FP64;
     } else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 1;
@@ -121,7 +124,7 @@ int main(int argc, char **argv) {
   if (do_fit) {
     std::vector<double> x(m->ncols), y(total_rows);
     cg_fit_result res{};
-    if ((st = cg_fit(ctx, b_all.data(), eps, refine, x.data(), &res))) return die("fitting", st, ctx);
+    if ((st = cg_fit(ctx, b_all.data(), eps, refine, gram, x.data(), &res))) return die("fitting", st, ctx);
     if ((st = cg_matvec(ctx, x.data(), y.data()))) return die("computing A x", st, ctx);
     if ((st = cg_write_params(setup, dir.c_str(), x.data(), m->ncols, total_rows, &res, y.data(), b_all.data())))
       return die("writing params.txt", st);

@@ -8,10 +8,12 @@
 // refinement on the residual r = b - A x, which converges to the truncated
 // least-squares solution in the kept subspace.
 //
-//   CG_FP64 : G on FP64 MFMA (v_mfma_f64_16x16x4f64), A in FP64.
-//   CG_FP32 : A in FP32; G as split-BF16x3 (hi*hi + hi*lo + lo*hi) on
-//             v_mfma_f32_16x16x32_bf16 with FP32 accumulation per K-slab and
-//             FP64 accumulation across slabs; residual and A^T r in FP64.
+//   CG_GRAM_FP64   : G on FP64 MFMA (v_mfma_f64_16x16x4f64) from an FP64 or
+//                    FP32 A (FP32 entries widened on load: exact products).
+//   CG_GRAM_BF16X3 : FP32 A; G as split-BF16x3 (hi*hi + hi*lo + lo*hi) on
+//                    v_mfma_f32_16x16x32_bf16 with FP32 accumulation per
+//                    K-slab and FP64 accumulation across slabs.
+// Residual and A^T r are always FP64.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -44,10 +46,13 @@ __device__ __forceinline__ void tile_of(int p, int nt, int &ti, int &tj) {
 }
 
 // Partial Gram of rows [k0, k1) for one 64x64 tile (FP64 MFMA).
-// Output layout of v_mfma_f64_16x16x4f64: lane l, register v holds
-// D[4*(l/16)+v][l%16]; operand A lane l supplies A[l%16][l/16], operand B
-// supplies B[l/16][l%16].
-__global__ __launch_bounds__(256) void k_gram_f64(const double *__restrict__ A, long long ld, long long m, int n,
+// v_mfma_f64_16x16x4f64 layouts: operand A lane l supplies A[l%16][l/16],
+// operand B supplies B[l/16][l%16]; the result register v of lane l holds
+// D[4*v + l/16][l%16] (one row per register, rows interleaved across the
+// four 16-lane groups -- unlike the FP32-output 16x16 instructions, whose
+// register v holds row 4*(l/16) + v).
+template <class T>
+__global__ __launch_bounds__(256) void k_gram_f64(const T *__restrict__ A, long long ld, long long m, int n,
                                                   long long kchunk, int nt, double *__restrict__ part) {
   __shared__ double sI[GK][GT + 1];
   __shared__ double sJ[GK][GT + 1];
@@ -68,8 +73,8 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double *__restrict__ A, 
       long long row = k + r;
       double vi = 0.0, vj = 0.0;
       if (row < k1) {
-        if (i0 + c < n) vi = A[row * ld + i0 + c];
-        if (j0 + c < n) vj = A[row * ld + j0 + c];
+        if (i0 + c < n) vi = (double)A[row * ld + i0 + c];
+        if (j0 + c < n) vj = (double)A[row * ld + j0 + c];
       }
       sI[r][c] = vi;
       sJ[r][c] = vj;
@@ -97,7 +102,7 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double *__restrict__ A, 
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int gi = i0 + wr + a * 16 + 4 * (lane / 16) + v;
+        int gi = i0 + wr + a * 16 + 4 * v + lane / 16;
         int gj = j0 + wc + b * 16 + lane % 16;
         if (gi < n && gj < n) P[(size_t)gi * n + gj] = acc[a][b][v];
       }
@@ -105,7 +110,8 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double *__restrict__ A, 
 
 // Split-BF16x3 partial Gram (A in FP32).  Operand A of
 // v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][8*(l/16) .. +7]; operand B
-// supplies B[8*(l/16) .. +7][l%16]; output as the FP64 variant.
+// supplies B[8*(l/16) .. +7][l%16]; result register v of lane l holds
+// D[4*(l/16) + v][l%16].
 __global__ __launch_bounds__(256) void k_gram_bf16x3(const float *__restrict__ A, long long ld, long long m, int n,
                                                      long long kchunk, int nt, double *__restrict__ part) {
   __shared__ float sI[GK32][GT + 1];
@@ -222,9 +228,9 @@ __global__ void k_scale(double *v, const double *w, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] *= w[i];
 }
-__global__ void k_axpy(double *x, const double *d, int n) {
+__global__ void k_axpy(double *x, const double *d, int n, double alpha) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += d[i];
+  if (i < n) x[i] += alpha * d[i];
 }
 
 struct FitState {
@@ -255,22 +261,82 @@ static void residual(Ctx &c, const T *A, long long m, int n, const double *x, co
                      x, b, r);
 }
 
-void fit(Ctx &c, const double *b_host, double eps, int refine, double *x_host, cg_fit_result *res) {
+// G = A^T A into Gd [n*n] (row-major, symmetric); part: S partial slabs.
+static void gram(Ctx &c, int mode, double *Gd, double *part, long long S, long long kchunk) {
+  const long long m = c.A_rows;
+  const int n = c.A_ld;
+  const int nt = (n + GT - 1) / GT;
+  const int npair = nt * (nt + 1) / 2;
+  if (mode == CG_GRAM_BF16X3) {
+    if (c.A_prec != CG_FP32) throw Error(CG_ERR_ARG, "the split-BF16x3 Gram needs an FP32 A");
+    hipLaunchKernelGGL(k_gram_bf16x3, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const float *)c.A,
+                       (long long)n, m, n, kchunk, nt, part);
+  } else if (c.A_prec == CG_FP64) {
+    hipLaunchKernelGGL(k_gram_f64<double>, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const double *)c.A,
+                       (long long)n, m, n, kchunk, nt, part);
+  } else {
+    hipLaunchKernelGGL(k_gram_f64<float>, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const float *)c.A,
+                       (long long)n, m, n, kchunk, nt, part);
+  }
+  CG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)(((long long)n * n + 255) / 256)), dim3(256), 0, c.stream, part,
+                     (int)S, n, Gd);
+  CG_HIP(hipGetLastError());
+}
+
+// Split-K plan of the Gram launch: S row slabs of kchunk rows (multiple of
+// the staging depth), enough workgroups to fill the chip.
+static void gram_plan(long long m, int n, long long &S, long long &kchunk) {
+  const int nt = (n + GT - 1) / GT;
+  const int npair = nt * (nt + 1) / 2;
+  long long target = std::max<long long>(1, (long long)(1024 + npair - 1) / npair);
+  S = std::min<long long>(target, std::max<long long>(1, m / 256));
+  S = std::min<long long>(S, 64);
+  kchunk = (m + S - 1) / S;
+  kchunk = (kchunk + GK32 - 1) / GK32 * GK32;
+  S = std::max<long long>(1, (m + kchunk - 1) / kchunk);
+}
+
+static void check_mode(int mode) {
+  if (mode != CG_GRAM_FP64 && mode != CG_GRAM_BF16X3) throw Error(CG_ERR_ARG, "unknown Gram mode");
+}
+
+void gram_host(Ctx &c, int mode, const double *b_host, double *G_host, double *Atb_host) {
   if (!c.A) throw Error(CG_ERR_ARG, "nothing assembled");
+  check_mode(mode);
+  const long long m = c.A_rows;
+  const int n = c.A_ld;
+  long long S, kchunk;
+  gram_plan(m, n, S, kchunk);
+  double *G = c.b_x.as<double>((size_t)n * n + (size_t)S * n * n + (size_t)m + (size_t)n + 64);
+  if (!G) throw Error(CG_ERR_MEMORY, "Gram buffers");
+  double *part = G + (size_t)n * n;
+  double *d_b = part + (size_t)S * n * n;
+  double *d_c = d_b + m;
+  gram(c, mode, G, part, S, kchunk);
+  if (G_host) CG_HIP(hipMemcpyAsync(G_host, G, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  if (Atb_host) {
+    if (!b_host) throw Error(CG_ERR_ARG, "A^T b needs b");
+    CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (c.A_prec == CG_FP64) atv(c, (const double *)c.A, m, n, d_b, d_c);
+    else atv(c, (const float *)c.A, m, n, d_b, d_c);
+    CG_HIP(hipMemcpyAsync(Atb_host, d_c, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  }
+  CG_HIP(hipStreamSynchronize(c.stream));
+}
+
+void fit(Ctx &c, const double *b_host, double eps, int refine, int mode, double *x_host, cg_fit_result *res) {
+  if (!c.A) throw Error(CG_ERR_ARG, "nothing assembled");
+  check_mode(mode);
+  if (!(eps > 0.0)) throw Error(CG_ERR_ARG, "eps must be positive");
   const long long m = c.A_rows;
   const int n = c.A_ld;
   FitState &fs = fit_state(c);
   hipEvent_t e0, e1, e2, e3;
   CG_HIP(hipEventCreate(&e0)); CG_HIP(hipEventCreate(&e1)); CG_HIP(hipEventCreate(&e2)); CG_HIP(hipEventCreate(&e3));
-  // buffers: G [n*n], part [S*n*n], b [m], r [m], x, c, t, dx, w [n]
-  const int nt = (n + GT - 1) / GT;
-  const int npair = nt * (nt + 1) / 2;
-  long long target = std::max<long long>(1, (long long)(1024 + npair - 1) / npair);  // >= ~1024 workgroups
-  long long S = std::min<long long>(target, std::max<long long>(1, m / 256));
-  S = std::min<long long>(S, 64);
-  long long kchunk = (m + S - 1) / S;
-  kchunk = (kchunk + GK32 - 1) / GK32 * GK32;
-  S = (m + kchunk - 1) / kchunk;
+  // buffers: G [n*n], part [S*n*n], b [m], r [m], x, c, t, dx, w, W, E [n], info
+  long long S, kchunk;
+  gram_plan(m, n, S, kchunk);
   double *G = c.b_x.as<double>((size_t)n * n + (size_t)S * n * n + 2 * (size_t)m + 8 * (size_t)n + 64);
   if (!G) throw Error(CG_ERR_MEMORY, "fit buffers");
   double *part = G + (size_t)n * n;
@@ -280,17 +346,9 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, double *x_host, c
   double *d_c = d_x + n, *d_t = d_c + n, *d_dx = d_t + n, *d_w = d_dx + n, *d_W = d_w + n, *d_E = d_W + n;
   CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
   CG_HIP(hipEventRecord(e0, c.stream));
-  if (c.A_prec == CG_FP64)
-    hipLaunchKernelGGL(k_gram_f64, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const double *)c.A,
-                       (long long)n, m, n, kchunk, nt, part);
-  else
-    hipLaunchKernelGGL(k_gram_bf16x3, dim3(npair, (unsigned)S), dim3(256), 0, c.stream, (const float *)c.A,
-                       (long long)n, m, n, kchunk, nt, part);
-  CG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)(((long long)n * n + 255) / 256)), dim3(256), 0, c.stream, part,
-                     (int)S, n, G);
+  gram(c, mode, G, part, S, kchunk);
   CG_HIP(hipEventRecord(e1, c.stream));
-  // eigen-decomposition G = V diag(W) V^T (column-major V in G; G is symmetric)
+  // eigen-decomposition G = V diag(W) V^T (V column-major in G; G is symmetric)
   rocblas_int *d_info = reinterpret_cast<rocblas_int *>(d_E + n);
   if (rocsolver_dsyevd(fs.h, rocblas_evect_original, rocblas_fill_upper, n, G, n, d_W, d_E, d_info) !=
       rocblas_status_success)
@@ -309,7 +367,7 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, double *x_host, c
   for (int i = 0; i < n; ++i)
     if (W[i] > thr) { winv[i] = 1.0 / W[i]; ++nvars; }
   CG_HIP(hipMemcpyAsync(d_w, winv.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  // x = V W+ V^T (A^T b)
+  // P v = V W+ V^T v
   auto project = [&](const double *in, double *out) {
     const double one = 1.0, zero = 0.0;
     if (rocblas_dgemv(fs.h, rocblas_operation_transpose, n, n, &one, G, n, in, 1, &zero, d_t, 1) != rocblas_status_success)
@@ -318,30 +376,52 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, double *x_host, c
     if (rocblas_dgemv(fs.h, rocblas_operation_none, n, n, &one, G, n, d_t, 1, &zero, out, 1) != rocblas_status_success)
       throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
   };
+  auto resid = [&](const double *x) {
+    if (c.A_prec == CG_FP64) residual(c, (const double *)c.A, m, n, x, d_b, d_r);
+    else residual(c, (const float *)c.A, m, n, x, d_b, d_r);
+  };
+  auto grad = [&]() {
+    if (c.A_prec == CG_FP64) atv(c, (const double *)c.A, m, n, d_r, d_c);
+    else atv(c, (const float *)c.A, m, n, d_r, d_c);
+  };
+  // |r|^2 on the device (one value back per iteration)
+  double *d_z = d_E + n + 2;
+  auto rnorm = [&]() {
+    double z = 0;
+    if (rocblas_set_pointer_mode(fs.h, rocblas_pointer_mode_device) != rocblas_status_success ||
+        rocblas_ddot(fs.h, (rocblas_int)m, d_r, 1, d_r, 1, d_z) != rocblas_status_success)
+      throw Error(CG_ERR_DEVICE, "rocblas_ddot");
+    rocblas_set_pointer_mode(fs.h, rocblas_pointer_mode_host);
+    CG_HIP(hipMemcpyAsync(&z, d_z, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CG_HIP(hipStreamSynchronize(c.stream));
+    return z;
+  };
   if (c.A_prec == CG_FP64) atv(c, (const double *)c.A, m, n, d_b, d_c);
   else atv(c, (const float *)c.A, m, n, d_b, d_c);
   project(d_c, d_x);
   CG_HIP(hipEventRecord(e2, c.stream));
+  // refinement: x += P A^T (b - A x), stopped when |r| no longer decreases
+  // (kept x is the best iterate; with an inexact G the update can grow)
+  resid(d_x);
+  double Z = rnorm();
+  int iters = 0;
   for (int it = 0; it < refine; ++it) {
-    if (c.A_prec == CG_FP64) {
-      residual(c, (const double *)c.A, m, n, d_x, d_b, d_r);
-      atv(c, (const double *)c.A, m, n, d_r, d_c);
-    } else {
-      residual(c, (const float *)c.A, m, n, d_x, d_b, d_r);
-      atv(c, (const float *)c.A, m, n, d_r, d_c);
-    }
+    grad();
     project(d_c, d_dx);
-    hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_x, d_dx, n);
+    hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_x, d_dx, n, 1.0);
+    resid(d_x);
+    const double Zn = rnorm();
+    if (!(Zn < Z)) {  // undo the step and stop
+      hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_x, d_dx, n, -1.0);
+      resid(d_x);
+      break;
+    }
+    Z = Zn;
+    ++iters;
   }
-  if (c.A_prec == CG_FP64) residual(c, (const double *)c.A, m, n, d_x, d_b, d_r);
-  else residual(c, (const float *)c.A, m, n, d_x, d_b, d_r);
   CG_HIP(hipEventRecord(e3, c.stream));
-  std::vector<double> r(m);
   CG_HIP(hipMemcpyAsync(x_host, d_x, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  CG_HIP(hipMemcpyAsync(r.data(), d_r, m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipStreamSynchronize(c.stream));
-  double Z = 0;
-  for (double v : r) Z += v * v;
   double mx = 0;
   for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(x_host[i]));
   float tg = 0, ts = 0, tr = 0;
@@ -358,6 +438,7 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, double *x_host, c
   res->ms_gram = tg;
   res->ms_solve = ts;
   res->ms_refine = tr;
+  res->iters = iters;
 }
 
 void matvec(Ctx &c, const double *x_host, double *y_host) {
@@ -379,11 +460,24 @@ void matvec(Ctx &c, const double *x_host, double *y_host) {
 
 extern "C" {
 
-cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, double *x, cg_fit_result *res) {
+cg_status cg_gram(cg_ctx *ctx, int gram_mode, const double *b_host, double *G, double *Atb) {
+  if (!ctx || (!G && !Atb)) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::gram_host(ctx->c, gram_mode, b_host, G, Atb);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, int gram_mode, double *x,
+                 cg_fit_result *res) {
   if (!ctx || !b_host || !x || !res) { cg::set_error("null argument"); return CG_ERR_ARG; }
   try {
     CG_HIP(hipSetDevice(ctx->c.device));
-    cg::fit(ctx->c, b_host, eps, refine_iters, x, res);
+    cg::fit(ctx->c, b_host, eps, refine_iters, gram_mode, x, res);
     return CG_OK;
   } catch (const cg::Error &e) {
     cg::set_error(e.what());

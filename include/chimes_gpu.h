@@ -184,12 +184,35 @@ cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, doub
  * and the number of pairs / triplets / quadruplets assembled. */
 cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3);
 
-/* Fit: solve A x ~ b for the last assembled A (on device).
- *   mode CG_FP64 : FP64 Gram on MFMA, FP64 eigensolve, FP64 refinement.
- *   mode CG_FP32 : A in FP32, split-BF16x3 Gram on MFMA (FP32 accumulate),
- *                  FP64 eigensolve + FP64 residual iterative refinement.
+/* Wait for all work queued on the context's stream (no reference
+ * counterpart: the reference is synchronous). */
+cg_status cg_synchronize(cg_ctx *ctx);
+
+/* Gram product G = A^T A of the last assembled A (whichever precision A
+ * was assembled in).
+ *   CG_GRAM_FP64   : FP64 MFMA (v_mfma_f64_16x16x4f64).  With an FP32 A the
+ *                    products of the FP32 entries are exact in FP64.
+ *   CG_GRAM_BF16X3 : FP32 A only; each entry split into hi + lo bfloat16,
+ *                    hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
+ *                    FP32 accumulation per row slab, FP64 across slabs.
+ *                    Relative error ~1e-6 of max|G|: too coarse to resolve
+ *                    singular values below ~1e-3 sigma_max. */
+enum { CG_GRAM_FP64 = 0, CG_GRAM_BF16X3 = 1 };
+
+/* No reference counterpart as an entry point (chimes_lsq.py forms no Gram
+ * matrix); exposed so callers can run their own solvers on G and A^T b.
+ * G: [ncols*ncols] row-major (symmetric), Atb: [ncols] or NULL (needs b). */
+cg_status cg_gram(cg_ctx *ctx, int gram_mode, const double *b_host, double *G, double *Atb);
+
+/* Fit: solve A x ~ b for the last assembled A (on device), the truncated
+ * least-squares estimator of chimes_lsq.py (src/chimes_lsq.py:169-214):
+ * x = V diag(1/sigma_i for sigma_i > eps*sigma_max) U^T b, obtained from
+ * the eigen-decomposition of G = A^T A (FP64, rocSOLVER) with lambda_i kept
+ * while lambda_i > eps^2 lambda_max, followed by FP64 iterative refinement
+ * x += V Lambda^+ V^T A^T (b - A x) with the residual in FP64.  Refinement
+ * stops early when the residual norm stops decreasing (res->iters).
  * eps: relative singular-value cut (chimes_lsq.py --eps), refine: iterations.
- * Outputs: x[ncols], rms (over all rows), nvars kept, sigma_max. */
+ * Outputs: x[ncols], rms (over all rows), nvars kept, eps*sigma_max. */
 typedef struct cg_fit_result {
   double eps;           /* relative cut used (input)                       */
   double rms;
@@ -198,8 +221,9 @@ typedef struct cg_fit_result {
   double eps_abs;       /* eps * sigma_max                                 */
   double bic;
   double ms_gram, ms_solve, ms_refine;
+  int iters;            /* refinement iterations applied                   */
 } cg_fit_result;
-cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters,
+cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, int gram_mode,
                  double *x, cg_fit_result *res);
 /* y = A x for the last assembled A (host output). */
 cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y);

@@ -37,7 +37,12 @@ def test_nonorth2_fit_matches_reference(ch):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_carbon_fit_vs_thin_svd(ch, synth, oracle, tmp_path, prec):
-    s, fr = synth_setup(ch, synth, tmp_path, synth.carbon(nframes=3, natoms=128, seed=31))
+    """Force RMS of the GPU fit vs the thin-SVD estimator on the oracle's FP64 A:
+    1e-8 relative for the FP64 path, 1e-6 for FP32 A + FP64 Gram + refinement.
+    8 frames x 128 atoms: 3120 rows x 1038 columns (rows well above columns, as
+    in a production fit; with rows ~ columns the FP32 rounding of A alone moves
+    the RMS by ~1e-6)."""
+    s, fr = synth_setup(ch, synth, tmp_path, synth.carbon(nframes=8, natoms=128, seed=31))
     dev = ch.Device(s)
     A, b = dev.assemble(fr, prec=prec)
     x, res = dev.fit(b, eps=1e-5, refine=4)
@@ -47,4 +52,45 @@ def test_carbon_fit_vs_thin_svd(ch, synth, oracle, tmp_path, prec):
         assert res.rms == pytest.approx(rmso, rel=1e-8)
     else:
         assert res.rms == pytest.approx(rmso, rel=1e-6)
+    # the reported RMS is that of the returned x
+    r = Ao @ x - b
+    assert np.sqrt(r @ r / len(b)) == pytest.approx(rmso, rel=1e-6 if prec else 1e-8)
+    assert abs(res.nvars - nk) <= (0 if prec == 0 else 2)
+    dev.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gram_matches_numpy(ch, synth, tmp_path, prec):
+    """G = A^T A and A^T b of the device A against numpy on the copied-back A
+    (FP64 MFMA; FP32 entries are widened, so the products are exact)."""
+    s, fr = synth_setup(ch, synth, tmp_path, synth.water(nframes=2))
+    dev = ch.Device(s)
+    A, b = dev.assemble(fr, prec=prec)
+    G, atb = dev.gram(b)
+    Gr = A.T @ A
+    scale = np.sqrt(np.outer(np.diag(Gr), np.diag(Gr))) + 1e-300
+    assert np.max(np.abs(G - Gr) / scale) < 1e-12
+    assert np.max(np.abs(G - G.T)) == 0.0
+    ar = A.T @ b
+    assert np.max(np.abs(atb - ar) / (np.sqrt(np.diag(Gr)) * np.linalg.norm(b) + 1e-300)) < 1e-12
+    if prec == 1:
+        Gb, _ = dev.gram(None, mode=ch.CG_GRAM_BF16X3)
+        err = np.max(np.abs(Gb - Gr) / scale)
+        assert err < 1e-4, err        # split-BF16x3: ~2^-16 per product, FP32 slab sums
+    dev.close()
+
+
+def test_bf16x3_fit_guarded(ch, synth, tmp_path):
+    """The split-BF16x3 Gram resolves only the large singular values; the
+    refinement must never return an iterate worse than its starting point."""
+    s, fr = synth_setup(ch, synth, tmp_path, synth.carbon(nframes=3, natoms=128, seed=31))
+    dev = ch.Device(s)
+    dev.assemble(fr, prec=ch.CG_FP32, copy=False)
+    _, b = dev.assemble(fr, prec=ch.CG_FP32, copy=False)
+    x0, r0 = dev.fit(b, eps=1e-5, refine=0, gram=ch.CG_GRAM_BF16X3)
+    x1, r1 = dev.fit(b, eps=1e-5, refine=6, gram=ch.CG_GRAM_BF16X3)
+    assert np.all(np.isfinite(x1)) and r1.rms <= r0.rms
+    with pytest.raises(RuntimeError):
+        dev.assemble(fr, prec=ch.CG_FP64, copy=False)
+        dev.fit(b, gram=ch.CG_GRAM_BF16X3)
     dev.close()
