@@ -132,10 +132,12 @@ __global__ void k_wrap(const DevFrame *frames, const int *atom_frame, const doub
 // Neighbour images of atom i within rc: every (j, shift) with shift in
 // [-L, L]^3 (the reference's ghost layers), restricted to shifts that can lie
 // within rc along each axis.  Big cells keep j >= i (half list with the
-// reference's a1 <= PARENT[a2] rule); small cells keep every image.
+// reference's a1 <= PARENT[a2] rule); small cells keep every image.  Entries
+// come out sorted by parent atom j.  FILL also stores, per entry, the
+// relative vector image(j) - i with its length, and the owning atom.
 template <bool FILL>
 __global__ void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap, int natoms, double rc,
-                        int L, int *count, const int *offset, uint32_t *list, double *rlist) {
+                        int L, int *count, const int *offset, uint32_t *list, double4 *nvec, int *nown) {
   int gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= natoms) return;
   const DevFrame &f = frames[atom_frame[gi]];
@@ -169,7 +171,8 @@ __global__ void k_neigh(const DevFrame *frames, const int *atom_frame, const dou
           if (r2 < rc2) {
             if (FILL) {
               list[base + n] = make_ref(j, sx, sy, sz);
-              rlist[base + n] = sqrt(r2);
+              nvec[base + n] = make_double4(v[0], v[1], v[2], sqrt(r2));
+              nown[base + n] = gi;
             }
             ++n;
           }
@@ -182,17 +185,16 @@ struct EnumArgs {
   const DevFrame *frames;
   const int *atom_frame;
   const int *type;      // per atom (batch)
-  const double *wrap;
   const int *noff;      // neighbour offsets [natoms+1]
   const uint32_t *nlist;
-  const double *rlist;
-  int natoms, ntypes_atom;
+  const double4 *nvec;  // relative vector + length per entry
+  const int *nown;      // owning atom per entry
+  int nnb, ntypes_atom, nkeys_type;
   double cutoff;
-  int *count;           // [natoms]
-  const int *coff;      // [natoms+1]
+  int *count;           // [nnb]
+  const int *coff;      // [nnb+1]
   uint32_t *refs;       // [ncl * nb]
-  int *ctype;           // [ncl]
-  int *cframe;          // [ncl]
+  int *key;             // [ncl] frame * types + type
   // family tables
   const int *id_map, *id_atom_perm, *id_pair_map;
   const double *smin, *smax;
@@ -200,118 +202,160 @@ struct EnumArgs {
   int md;               // force-evaluation pair set
 };
 
-__device__ __forceinline__ double pair_dist(const DevFrame &f, const double *wrap, uint32_t a, uint32_t b) {
-  double pa[3], pb[3];
-  image_pos(f, wrap, a, pa);
-  image_pos(f, wrap, b, pb);
-  double dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
+__device__ __forceinline__ double dist3(const double4 &u, const double4 &v) {
+  double dx = v.x - u.x, dy = v.y - u.y, dz = v.z - u.z;
   return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// Enumerate the clusters anchored at each real atom (the reference's
-// Deriv_3B/4B loop structure), keep those whose pairs all pass PROCEED, and
-// store them in the canonical atom order of their cluster type.
-template <int NB, bool FILL>
-__global__ void k_enum(EnumArgs a) {
-  int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= a.natoms) return;
-  const int fidx = a.atom_frame[gi];
-  const DevFrame &f = a.frames[fidx];
-  const int i = gi - f.atom_off;
-  const int b = a.noff[gi], e = a.noff[gi + 1];
-  const uint32_t r1 = make_ref(i, 0, 0, 0);
-  const int t1 = a.type[gi];
-  int n = 0;
-  int out = FILL ? a.coff[gi] : 0;
-  const bool ordered = !f.small;
+// Cluster-type lookup (make_id_int + INT_MAP) and PROCEED on every pair with
+// its canonical pair's S_MAXIM (src/Cheby.C:1015-1054, 1387-1500).  d[] holds
+// the pair distances in enumeration order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
+template <int NB>
+__device__ __forceinline__ int cluster_type(const EnumArgs &a, const int *types, const double *d, int &id) {
   constexpr int NP = NB * (NB - 1) / 2;
-  if (NB == 2) {
-    for (int k = b; k < e; ++k) {
-      double r = a.rlist[k];
-      if (r >= a.cutoff) continue;
-      uint32_t r2 = a.nlist[k];
-      int t2 = a.type[f.atom_off + ref_atom(r2)];
-      int p = a.int_pair_map[t1 * a.ntypes_atom + t2];
-      if (!(r < a.smax[p] && (a.md || r > a.smin[p]))) continue;
-      if (FILL) {
-        a.refs[2 * (out + n)] = r1;
-        a.refs[2 * (out + n) + 1] = r2;
-        a.ctype[out + n] = p;
-        a.cframe[out + n] = fidx;
+  id = 0;
+  int w = 1;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    id += w * (types[q] + 1);
+    w *= CG_MAX_ATOM_TYPES;
+  }
+  const int ct = a.id_map[id];
+  if (ct < 0) return -1;
+#pragma unroll
+  for (int pf = 0; pf < NP; ++pf) {
+    const int cp = a.id_pair_map[id * NP + pf];
+    if (!(d[pf] < a.smax[ct * NP + cp])) return -1;
+  }
+  return ct;
+}
+
+// First entry of atom gi's list in [b, k] that has the same parent as entry k
+// (lists are sorted by parent), so [start, e) holds every entry whose parent
+// is not below parent(k).
+__device__ __forceinline__ int group_start(const uint32_t *nlist, int b, int k) {
+  const int p = ref_atom(nlist[k]);
+  while (k > b && ref_atom(nlist[k - 1]) == p) --k;
+  return k;
+}
+
+// Enumerate the clusters anchored at each real atom (the reference's
+// Deriv_2B/3B/4B loop structure: a1 real, a2 <= a3 <= a4 by parent in big
+// cells, all orderings in small cells) and keep those whose pairs all pass
+// PROCEED.  One thread per neighbour entry (the atom's second member); the
+// inner members are pruned by the family cutoff before the type lookup.
+// FILL stores each cluster in the canonical atom order of its type, with its
+// (frame, type) key; clusters come out ordered by (atom, entry).
+template <int NB, bool FILL>
+__global__ __launch_bounds__(256) void k_enum(EnumArgs a) {
+  const int k2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k2 >= a.nnb) return;
+  const double4 v2 = a.nvec[k2];
+  int n = 0;
+  if (v2.w < a.cutoff) {
+    const int gi = a.nown[k2];
+    const int fidx = a.atom_frame[gi];
+    const DevFrame &f = a.frames[fidx];
+    const int i = gi - f.atom_off;
+    const uint32_t r1 = make_ref(i, 0, 0, 0);
+    const uint32_t r2 = a.nlist[k2];
+    int types[4];
+    types[0] = a.type[gi];
+    types[1] = a.type[f.atom_off + ref_atom(r2)];
+    const int out = FILL ? a.coff[k2] : 0;
+    const int kbase = fidx * a.nkeys_type;
+    if (NB == 2) {
+      const int p = a.int_pair_map[types[0] * a.ntypes_atom + types[1]];
+      const double r = v2.w;
+      if (r < a.smax[p] && (a.md || r > a.smin[p])) {
+        if (FILL) {
+          a.refs[2 * out] = r1;
+          a.refs[2 * out + 1] = r2;
+          a.key[out] = kbase + p;
+        }
+        n = 1;
       }
-      ++n;
-    }
-  } else {
-    for (int k2 = b; k2 < e; ++k2) {
-      if (a.rlist[k2] >= a.cutoff) continue;
-      uint32_t r2 = a.nlist[k2];
-      int p2 = ref_atom(r2);
-      for (int k3 = b; k3 < e; ++k3) {
-        if (k3 == k2 || a.rlist[k3] >= a.cutoff) continue;
-        uint32_t r3 = a.nlist[k3];
-        int p3 = ref_atom(r3);
-        if (ordered && p2 > p3) continue;
-        for (int k4 = (NB == 4 ? b : 0); k4 < (NB == 4 ? e : 1); ++k4) {
-          uint32_t r4 = 0;
-          if (NB == 4) {
-            if (k4 == k2 || k4 == k3 || a.rlist[k4] >= a.cutoff) continue;
-            r4 = a.nlist[k4];
-            if (ordered && p3 > ref_atom(r4)) continue;
-          }
-          uint32_t refs[4] = {r1, r2, r3, r4};
-          int types[4];
-          int id = 0, w = 1;
-          for (int q = 0; q < NB; ++q) {
-            types[q] = a.type[f.atom_off + ref_atom(refs[q])];
-            id += w * (types[q] + 1);
-            w *= CG_MAX_ATOM_TYPES;
-          }
-          int ct = a.id_map[id];
+    } else {
+      const int b = a.noff[gi], e = a.noff[gi + 1];
+      const bool ordered = !f.small;
+      const double cut = a.cutoff;
+      const int s3 = ordered ? group_start(a.nlist, b, k2) : b;
+      for (int k3 = s3; k3 < e; ++k3) {
+        if (k3 == k2) continue;
+        const double4 v3 = a.nvec[k3];
+        if (v3.w >= cut) continue;
+        const double r23 = dist3(v2, v3);
+        if (r23 >= cut) continue;
+        const uint32_t r3 = a.nlist[k3];
+        types[2] = a.type[f.atom_off + ref_atom(r3)];
+        if (NB == 3) {
+          const double d[3] = {v2.w, v3.w, r23};
+          int id;
+          const int ct = cluster_type<3>(a, types, d, id);
           if (ct < 0) continue;
-          // PROCEED on every pair with the canonical pair's S_MAXIM
-          bool ok = true;
-          int pf = 0;
-          for (int x = 0; x < NB && ok; ++x)
-            for (int y = x + 1; y < NB; ++y) {
-              double r;
-              if (x == 0) {
-                int kk = y == 1 ? k2 : (y == 2 ? k3 : k4);
-                r = a.rlist[kk];
-              } else {
-                r = pair_dist(f, a.wrap, refs[x], refs[y]);
-              }
-              int cp = a.id_pair_map[id * NP + pf];
-              if (!(r < a.smax[ct * NP + cp])) { ok = false; break; }
-              ++pf;
-            }
-          if (!ok) continue;
           if (FILL) {
-            const int *perm = a.id_atom_perm + id * NB;
-            for (int s = 0; s < NB; ++s) a.refs[NB * (out + n) + s] = refs[perm[s]];
-            a.ctype[out + n] = ct;
-            a.cframe[out + n] = fidx;
+            const uint32_t rr[3] = {r1, r2, r3};
+            const int *perm = a.id_atom_perm + id * 3;
+            for (int q = 0; q < 3; ++q) a.refs[3 * (out + n) + q] = rr[perm[q]];
+            a.key[out + n] = kbase + ct;
           }
           ++n;
+        } else {
+          const int s4 = ordered ? group_start(a.nlist, b, k3) : b;
+          for (int k4 = s4; k4 < e; ++k4) {
+            if (k4 == k2 || k4 == k3) continue;
+            const double4 v4 = a.nvec[k4];
+            if (v4.w >= cut) continue;
+            const double r24 = dist3(v2, v4);
+            if (r24 >= cut) continue;
+            const double r34 = dist3(v3, v4);
+            if (r34 >= cut) continue;
+            const uint32_t r4 = a.nlist[k4];
+            types[3] = a.type[f.atom_off + ref_atom(r4)];
+            const double d[6] = {v2.w, v3.w, v4.w, r23, r24, r34};
+            int id;
+            const int ct = cluster_type<4>(a, types, d, id);
+            if (ct < 0) continue;
+            if (FILL) {
+              const uint32_t rr[4] = {r1, r2, r3, r4};
+              const int *perm = a.id_atom_perm + id * 4;
+              for (int q = 0; q < 4; ++q) a.refs[4 * (out + n) + q] = rr[perm[q]];
+              a.key[out + n] = kbase + ct;
+            }
+            ++n;
+          }
         }
       }
     }
   }
-  if (!FILL) a.count[gi] = n;
+  if (!FILL) a.count[k2] = n;
 }
 
-__global__ void k_bucket_rank(const int *ctype, const int *cframe, int ncl, int ntypes, int *bcount, int *rank) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;
-  rank[c] = atomicAdd(&bcount[cframe[c] * ntypes + ctype[c]], 1);
+__global__ void k_iota(int *v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
 }
 
 template <int NB>
-__global__ void k_bucket_scatter(const uint32_t *refs, const int *ctype, const int *cframe, const int *rank,
-                                 const int *boff, int ntypes, int ncl, uint32_t *out) {
+__global__ void k_gather_refs(const uint32_t *refs, const int *idx, int ncl, uint32_t *out) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncl) return;
-  int d = boff[cframe[c] * ntypes + ctype[c]] + rank[c];
-  for (int s = 0; s < NB; ++s) out[NB * d + s] = refs[NB * c + s];
+  const int s = idx[c];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) out[NB * c + q] = refs[NB * s + q];
+}
+
+// boff[q] = first cluster whose key is >= q (keys sorted), q in [0, nkeys].
+__global__ void k_bucket_bounds(const int *key, int ncl, int nkeys, int *boff) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > nkeys) return;
+  int lo = 0, hi = ncl;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (key[mid] < q) lo = mid + 1;
+    else hi = mid;
+  }
+  boff[q] = lo;
 }
 
 // ------------------------------------------------------------ assembly kernel
@@ -948,76 +992,112 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   // ---- neighbour lists (one list at the largest family cutoff)
   int *d_ncount = c.b_ncount.as<int>(natoms + 1);
   int *d_noff = c.b_noff.as<int>(natoms + 1);
+  if (!d_ncount || !d_noff) throw Error(CG_ERR_MEMORY, "neighbour counts");
   CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
   hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
-                     natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr, (double *)nullptr);
+                     natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr,
+                     (int *)nullptr);
   exclusive_scan(c, d_ncount, d_noff, natoms);
-  int nnb = read_int(c, d_noff + natoms);
-  uint32_t *d_nlist = c.b_nlist.as<uint32_t>(std::max(nnb, 1));
-  double *d_rlist = c.b_tmp3.as<double>(std::max(nnb, 1));
-  if (!d_nlist || !d_rlist) throw Error(CG_ERR_MEMORY, "neighbour list");
+  const int nnb = read_int(c, d_noff + natoms);                               // host sync 1
+  const size_t nn1 = std::max(nnb, 1);
+  uint32_t *d_nlist = c.b_nlist.as<uint32_t>(nn1);
+  double4 *d_nvec = c.b_nvec.as<double4>(nn1);
+  int *d_nown = c.b_nown.as<int>(nn1);
+  if (!d_nlist || !d_nvec || !d_nown) throw Error(CG_ERR_MEMORY, "neighbour list");
   hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
-                     natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_rlist);
-  // ---- per-family cluster enumeration and bucketing
+                     natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_nvec, d_nown);
+  // ---- per-family cluster counts (one thread per neighbour entry)
   FamOut *fo = B.fo;
-  std::vector<DBuf *> fbufs = {&c.b_clist, &c.b_tmp, &c.b_tmp2};
+  int *d_ccount = c.b_ccount.as<int>(3 * (nn1 + 1));
+  int *d_coff = c.b_coff.as<int>(3 * (nn1 + 1));
+  if (!d_ccount || !d_coff) throw Error(CG_ERR_MEMORY, "cluster counts");
+  EnumArgs ea[3] = {};
+  bool active[3] = {false, false, false};
+  const dim3 eg((unsigned)((nnb + TB - 1) / TB)), eb(TB);
   for (int w = 0; w < 3; ++w) {
     const DevFamily &F = c.fam[w];
-    if (F.ntypes == 0 || F.ncols == 0) continue;
-    EnumArgs ea{};
-    ea.frames = d_frames; ea.atom_frame = d_af; ea.type = d_type; ea.wrap = d_wrap; ea.noff = d_noff;
-    ea.nlist = d_nlist; ea.rlist = d_rlist; ea.natoms = natoms; ea.ntypes_atom = m.ntypes; ea.cutoff = F.cutoff;
-    ea.count = c.b_ccount.as<int>(natoms + 1);
-    int *coff = c.b_coff.as<int>(natoms + 1);
-    ea.coff = coff;
-    ea.id_map = F.id_map; ea.id_atom_perm = F.id_atom_perm; ea.id_pair_map = F.id_pair_map;
-    ea.smin = F.smin; ea.smax = F.smax; ea.int_pair_map = c.d_int_pair_map;
-    ea.md = md;
-    CG_HIP(hipMemsetAsync(ea.count, 0, (natoms + 1) * sizeof(int), c.stream));
-    dim3 g((natoms + 63) / 64), bl(64);
-    if (w == 0) hipLaunchKernelGGL((k_enum<2, false>), g, bl, 0, c.stream, ea);
-    else if (w == 1) hipLaunchKernelGGL((k_enum<3, false>), g, bl, 0, c.stream, ea);
-    else hipLaunchKernelGGL((k_enum<4, false>), g, bl, 0, c.stream, ea);
-    exclusive_scan(c, ea.count, coff, natoms);
-    int ncl = read_int(c, coff + natoms);
-    fo[w].ncl = ncl;
-    const int nb = F.nb;
-    size_t n1 = std::max(ncl, 1);
-    // layout in one buffer: refs [n1*nb] | ctype [n1] | cframe [n1] | rank [n1] | bucketed refs [n1*nb]
-    uint32_t *blk = fbufs[w]->as<uint32_t>(n1 * (2 * nb + 3));
-    if (!blk) throw Error(CG_ERR_MEMORY, "cluster list");
-    ea.refs = blk;
-    ea.ctype = reinterpret_cast<int *>(blk + n1 * nb);
-    ea.cframe = ea.ctype + n1;
-    int *rank = ea.cframe + n1;
-    uint32_t *bref = reinterpret_cast<uint32_t *>(rank + n1);
-    if (ncl > 0) {
-      if (w == 0) hipLaunchKernelGGL((k_enum<2, true>), g, bl, 0, c.stream, ea);
-      else if (w == 1) hipLaunchKernelGGL((k_enum<3, true>), g, bl, 0, c.stream, ea);
-      else hipLaunchKernelGGL((k_enum<4, true>), g, bl, 0, c.stream, ea);
-    }
-    // bucket by (frame, type)
-    const int nbk = nframes * F.ntypes;
-    int *bcount = c.b_bcount.as<int>(2 * (size_t)nbk + 1);
-    int *boffd = bcount + nbk;
-    CG_HIP(hipMemsetAsync(bcount, 0, nbk * sizeof(int), c.stream));
-    if (ncl > 0)
-      hipLaunchKernelGGL(k_bucket_rank, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.ctype, ea.cframe, ncl,
-                         F.ntypes, bcount, rank);
-    std::vector<int> hc(nbk);
-    CG_HIP(hipMemcpyAsync(hc.data(), bcount, nbk * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CG_HIP(hipStreamSynchronize(c.stream));
-    fo[w].boff.assign(nbk + 1, 0);
-    for (int i = 0; i < nbk; ++i) fo[w].boff[i + 1] = fo[w].boff[i] + hc[i];
-    upload(c, boffd, fo[w].boff.data(), nbk * sizeof(int));
-    if (ncl > 0) {
-      if (nb == 2) hipLaunchKernelGGL(k_bucket_scatter<2>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.refs, ea.ctype, ea.cframe, rank, boffd, F.ntypes, ncl, bref);
-      else if (nb == 3) hipLaunchKernelGGL(k_bucket_scatter<3>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.refs, ea.ctype, ea.cframe, rank, boffd, F.ntypes, ncl, bref);
-      else hipLaunchKernelGGL(k_bucket_scatter<4>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, ea.refs, ea.ctype, ea.cframe, rank, boffd, F.ntypes, ncl, bref);
-    }
-    fo[w].refs = bref;
+    if (F.ntypes == 0 || F.ncols == 0 || nnb == 0) continue;
+    active[w] = true;
+    EnumArgs &E = ea[w];
+    E.frames = d_frames; E.atom_frame = d_af; E.type = d_type; E.noff = d_noff; E.nlist = d_nlist;
+    E.nvec = d_nvec; E.nown = d_nown; E.nnb = nnb; E.ntypes_atom = m.ntypes; E.nkeys_type = F.ntypes;
+    E.cutoff = F.cutoff;
+    E.count = d_ccount + w * (nn1 + 1);
+    E.coff = d_coff + w * (nn1 + 1);
+    E.id_map = F.id_map; E.id_atom_perm = F.id_atom_perm; E.id_pair_map = F.id_pair_map;
+    E.smin = F.smin; E.smax = F.smax; E.int_pair_map = c.d_int_pair_map;
+    E.md = md;
+    CG_HIP(hipMemsetAsync(E.count + nnb, 0, sizeof(int), c.stream));
+    if (w == 0) hipLaunchKernelGGL((k_enum<2, false>), eg, eb, 0, c.stream, E);
+    else if (w == 1) hipLaunchKernelGGL((k_enum<3, false>), eg, eb, 0, c.stream, E);
+    else hipLaunchKernelGGL((k_enum<4, false>), eg, eb, 0, c.stream, E);
     CG_HIP(hipGetLastError());
+    exclusive_scan(c, E.count, const_cast<int *>(E.coff), nnb);
   }
+  int tot[3] = {0, 0, 0};
+  for (int w = 0; w < 3; ++w)
+    if (active[w]) CG_HIP(hipMemcpyAsync(&tot[w], ea[w].coff + nnb, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));                                    // host sync 2
+  // ---- fill, bucket by (frame, type), bucket bounds
+  std::vector<int> hbounds[3];
+  int *d_bounds[3] = {nullptr, nullptr, nullptr};
+  DBuf *fbufs[3] = {&c.b_fam[0], &c.b_fam[1], &c.b_fam[2]};
+  for (int w = 0; w < 3; ++w) {
+    const DevFamily &F = c.fam[w];
+    fo[w].ncl = 0;
+    fo[w].refs = nullptr;
+    fo[w].boff.assign((size_t)nframes * std::max(F.ntypes, 1) + 1, 0);
+    if (!active[w]) continue;
+    const int ncl = tot[w];
+    const int nb = F.nb;
+    const int nkeys = nframes * F.ntypes;
+    const bool sort = F.ntypes > 1 && ncl > 1;
+    const size_t n1 = std::max(ncl, 1);
+    // refs [n1*nb] | key [n1] | bounds [nkeys+1] | (sort) key2 [n1] | idx [n1] | idx2 [n1] | refs2 [n1*nb]
+    const size_t words = n1 * nb + n1 + (size_t)nkeys + 1 + (sort ? n1 * (3 + nb) : 0);
+    uint32_t *blk = fbufs[w]->as<uint32_t>(words);
+    if (!blk) throw Error(CG_ERR_MEMORY, "cluster list");
+    EnumArgs &E = ea[w];
+    E.refs = blk;
+    E.key = reinterpret_cast<int *>(blk + n1 * nb);
+    d_bounds[w] = E.key + n1;
+    fo[w].ncl = ncl;
+    fo[w].refs = E.refs;
+    const int *keys = E.key;
+    if (ncl > 0) {
+      if (w == 0) hipLaunchKernelGGL((k_enum<2, true>), eg, eb, 0, c.stream, E);
+      else if (w == 1) hipLaunchKernelGGL((k_enum<3, true>), eg, eb, 0, c.stream, E);
+      else hipLaunchKernelGGL((k_enum<4, true>), eg, eb, 0, c.stream, E);
+      CG_HIP(hipGetLastError());
+      if (sort) {
+        // stable radix sort by (frame, type): clusters of a bucket keep their
+        // enumeration order, so the bucketed list is deterministic
+        int *key2 = d_bounds[w] + nkeys + 1;
+        int *idx = key2 + n1, *idx2 = idx + n1;
+        uint32_t *refs2 = reinterpret_cast<uint32_t *>(idx2 + n1);
+        hipLaunchKernelGGL(k_iota, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, idx, ncl);
+        int bits = 1;
+        while ((1LL << bits) < (long long)nkeys) ++bits;
+        size_t tmp = 0;
+        CG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, E.key, key2, idx, idx2, ncl, 0, bits, c.stream));
+        void *t = c.b_scan.get(tmp);
+        if (!t) throw Error(CG_ERR_MEMORY, "sort scratch");
+        CG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tmp, E.key, key2, idx, idx2, ncl, 0, bits, c.stream));
+        if (nb == 2) hipLaunchKernelGGL(k_gather_refs<2>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, E.refs, idx2, ncl, refs2);
+        else if (nb == 3) hipLaunchKernelGGL(k_gather_refs<3>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, E.refs, idx2, ncl, refs2);
+        else hipLaunchKernelGGL(k_gather_refs<4>, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, E.refs, idx2, ncl, refs2);
+        fo[w].refs = refs2;
+        keys = key2;
+      }
+    }
+    hipLaunchKernelGGL(k_bucket_bounds, dim3((nkeys + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, keys, ncl, nkeys,
+                       d_bounds[w]);
+    CG_HIP(hipGetLastError());
+    fo[w].boff.resize((size_t)nkeys + 1);
+    CG_HIP(hipMemcpyAsync(fo[w].boff.data(), d_bounds[w], ((size_t)nkeys + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                          c.stream));
+  }
+  CG_HIP(hipStreamSynchronize(c.stream));                                    // host sync 3
   CG_HIP(hipEventRecord(c.ev[1], c.stream));
   B.natoms = natoms;
   B.rows = rows;
@@ -1059,6 +1139,7 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
   for (auto &d : hf) maxn = std::max(maxn, d.natoms);
   const size_t lds_budget = 150 * 1024;
   bool fam_ran[3] = {false, false, false};
+  std::vector<Task> atasks[3];  // host sources of the task uploads, alive until the final sync
   for (int w = 0; w < 3; ++w) c.ncl[w] = fo[w].ncl;
   for (int w = 0; w < 3; ++w) {
     const DevFamily &F = c.fam[w];
@@ -1082,18 +1163,17 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     std::stable_sort(tasks.begin(), tasks.end(), [](const Task &x, const Task &y) {
       return (long long)(x.c1 - x.c0) * x.ncols > (long long)(y.c1 - y.c0) * y.ncols;
     });
-    Task *d_tasks = c.b_tasks.as<Task>(tasks.size());
+    Task *d_tasks = c.b_tasks[w].as<Task>(tasks.size());
     if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
-    upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(Task));
+    atasks[w] = std::move(tasks);
+    upload(c, d_tasks, atasks[w].data(), atasks[w].size() * sizeof(Task));
     size_t lds = 0;
-    for (auto &tk : tasks) lds = std::max(lds, (3 * (size_t)hf[tk.frame].natoms + 56) * tk.ncols * esz);
+    for (auto &tk : atasks[w]) lds = std::max(lds, (3 * (size_t)hf[tk.frame].natoms + 56) * tk.ncols * esz);
     CG_HIP(hipEventRecord(c.fev[w][0], c.stream));
-    if (prec == CG_FP32) launch_family<float>(c, w, tasks, d_tasks, d_frames, fo[w].refs, lds, A, ld);
-    else launch_family<double>(c, w, tasks, d_tasks, d_frames, fo[w].refs, lds, A, ld);
+    if (prec == CG_FP32) launch_family<float>(c, w, atasks[w], d_tasks, d_frames, fo[w].refs, lds, A, ld);
+    else launch_family<double>(c, w, atasks[w], d_tasks, d_frames, fo[w].refs, lds, A, ld);
     CG_HIP(hipEventRecord(c.fev[w][1], c.stream));
     fam_ran[w] = true;
-    // the task buffer is reused by the next family: wait so its upload cannot race
-    CG_HIP(hipStreamSynchronize(c.stream));
   }
   if (m.n_offset_cols > 0) {
     if (prec == CG_FP32)
@@ -1126,12 +1206,13 @@ void force(Ctx &c, const double *x_host, const cg_frame *frame, double *F, doubl
   prepare(c, frame, 1, 1, B);
   c.sticky_small = saved_small;
   const int natoms = B.natoms;
-  double *d_x = c.b_tmp.as<double>(m.ncols + 3 * (size_t)natoms + 16);
+  double *d_x = c.b_force.as<double>(m.ncols + 3 * (size_t)natoms + 16);
   if (!d_x) throw Error(CG_ERR_MEMORY, "force buffers");
   double *d_F = d_x + m.ncols;
   double *d_EV = d_F + 3 * (size_t)natoms;
   CG_HIP(hipMemcpyAsync(d_x, x_host, m.ncols * sizeof(double), hipMemcpyHostToDevice, c.stream));
   CG_HIP(hipMemsetAsync(d_F, 0, (3 * (size_t)natoms + 16) * sizeof(double), c.stream));
+  std::vector<Task> ftasks[3];  // host sources of the task uploads, alive until the final sync
   for (int w = 0; w < 3; ++w) {
     const DevFamily &Fm = c.fam[w];
     if (Fm.ntypes == 0 || Fm.ncols == 0 || B.fo[w].ncl == 0) continue;
@@ -1142,9 +1223,10 @@ void force(Ctx &c, const double *x_host, const cg_frame *frame, double *F, doubl
       for (int k = c0; k < c1; k += chunk) tasks.push_back(Task{0, t, 0, 0, k, std::min(c1, k + chunk)});
     }
     if (tasks.empty()) continue;
-    Task *d_tasks = c.b_tasks.as<Task>(tasks.size());
+    Task *d_tasks = c.b_tasks[w].as<Task>(tasks.size());
     if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
-    CG_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, c.stream));
+    ftasks[w] = tasks;
+    CG_HIP(hipMemcpyAsync(d_tasks, ftasks[w].data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, c.stream));
     ForceArgs a{};
     a.tasks = d_tasks; a.frames = B.d_frames; a.wrap = c.b_wrap.as<double>(0); a.refs = B.fo[w].refs;
     a.perm = Fm.perm; a.colptr = Fm.colptr; a.col_rel = Fm.col_offset; a.col_abs = Fm.col_offset + Fm.ntypes;
@@ -1175,7 +1257,6 @@ void force(Ctx &c, const double *x_host, const cg_frame *frame, double *F, doubl
     }
 #undef CG_FLAUNCH
     CG_HIP(hipGetLastError());
-    CG_HIP(hipStreamSynchronize(c.stream));
   }
   double ev[10];
   CG_HIP(hipMemcpyAsync(F, d_F, 3 * (size_t)natoms * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1201,9 +1282,9 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 }
 
 void ctx_free(Ctx &c) {
-  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_ccount,
-                  &c.b_coff, &c.b_clist, &c.b_ctype, &c.b_crank, &c.b_bucket, &c.b_bcount, &c.b_tasks, &c.b_A,
-                  &c.b_scan, &c.b_x, &c.b_tmp, &c.b_tmp2, &c.b_tmp3})
+  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
+                  &c.b_nown, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
+                  &c.b_tasks[1], &c.b_tasks[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_tmp})
     b->release();
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,

@@ -89,8 +89,15 @@ struct Ctx {
   double penalty_scale = 1.0e4;  // PAIR CHEBYSHEV PENALTY SCALING default (chimes_md)
   double penalty_dist = 0.01;    // PAIR CHEBYSHEV PENALTY DIST default
   // batch buffers
-  DBuf b_frames, b_xyz, b_type, b_wrap, b_ncount, b_noff, b_nlist, b_ccount, b_coff, b_clist,
-      b_ctype, b_crank, b_bucket, b_bcount, b_tasks, b_A, b_scan, b_x, b_tmp, b_tmp2, b_tmp3;
+  DBuf b_frames, b_xyz, b_type, b_wrap;   // frame table, coordinates, types, wrapped coordinates
+  DBuf b_ncount, b_noff, b_nlist, b_nvec, b_nown;  // neighbour lists
+  DBuf b_ccount, b_coff;                  // per-entry cluster counts / offsets (3 families)
+  DBuf b_fam[3];                          // per-family clusters, keys and bucket bounds
+  DBuf b_tasks[3];                        // per-family assembly task lists
+  DBuf b_A, b_scan;                       // design matrix; hipcub scratch
+  DBuf b_x;                               // fit workspace
+  DBuf b_force;                           // force evaluation: x, F, energy/virial
+  DBuf b_tmp;                             // matvec output
   void *A = nullptr;          // last assembled A (device)
   int A_prec = CG_FP64;
   int64_t A_rows = 0;
