@@ -386,11 +386,53 @@ struct AsmArgs {
 
 template <class T, int N> using vecN = T __attribute__((ext_vector_type(N)));
 
+// Per-pair table of (F[i], D[i]) for powers i = 0..VW-1, packed into one
+// vector of 32-bit elements so that a wave-uniform power index is lowered to
+// GPR-index-mode moves (s_set_gpr_idx_on + one v_mov per dword) rather than
+// to a chain of v_cndmask selects, which LLVM picks for short vectors.
+template <class T, int VW, bool SPLIT = (VW > 8)> struct PairTab;
+// FP64, VW <= 8: one vector of 4*VW dwords (F lo, F hi, D lo, D hi per power)
+template <int VW> struct PairTab<double, VW, false> {
+  vecN<uint32_t, 4 * VW> v;
+  __device__ __forceinline__ void zero() { v = 0u; }
+  __device__ __forceinline__ void set(int i, double f, double d) {
+    const uint64_t fb = __builtin_bit_cast(uint64_t, f), db = __builtin_bit_cast(uint64_t, d);
+    v[4 * i] = (uint32_t)fb; v[4 * i + 1] = (uint32_t)(fb >> 32);
+    v[4 * i + 2] = (uint32_t)db; v[4 * i + 3] = (uint32_t)(db >> 32);
+  }
+  __device__ __forceinline__ void get(unsigned i, double &f, double &d) const {
+    f = __builtin_bit_cast(double, (uint64_t)v[4 * i] | ((uint64_t)v[4 * i + 1] << 32));
+    d = __builtin_bit_cast(double, (uint64_t)v[4 * i + 2] | ((uint64_t)v[4 * i + 3] << 32));
+  }
+};
+// FP64, 8 < VW <= 16: separate F and D vectors of 2*VW dwords (register
+// vectors are limited to 32 dwords; longer ones would live in scratch)
+template <int VW> struct PairTab<double, VW, true> {
+  vecN<uint32_t, 2 * VW> F, D;
+  __device__ __forceinline__ void zero() { F = 0u; D = 0u; }
+  __device__ __forceinline__ void set(int i, double f, double d) {
+    const uint64_t fb = __builtin_bit_cast(uint64_t, f), db = __builtin_bit_cast(uint64_t, d);
+    F[2 * i] = (uint32_t)fb; F[2 * i + 1] = (uint32_t)(fb >> 32);
+    D[2 * i] = (uint32_t)db; D[2 * i + 1] = (uint32_t)(db >> 32);
+  }
+  __device__ __forceinline__ void get(unsigned i, double &f, double &d) const {
+    f = __builtin_bit_cast(double, (uint64_t)F[2 * i] | ((uint64_t)F[2 * i + 1] << 32));
+    d = __builtin_bit_cast(double, (uint64_t)D[2 * i] | ((uint64_t)D[2 * i + 1] << 32));
+  }
+};
+template <int VW, bool SPLIT> struct PairTab<float, VW, SPLIT> {
+  static constexpr int N = VW < 5 ? 5 : VW;  // >= 10 elements keeps index mode
+  vecN<float, 2 * N> v;                      // F, D per power
+  __device__ __forceinline__ void zero() { v = 0.0f; }
+  __device__ __forceinline__ void set(int i, float f, float d) { v[2 * i] = f; v[2 * i + 1] = d; }
+  __device__ __forceinline__ void get(unsigned i, float &f, float &d) const { f = v[2 * i]; d = v[2 * i + 1]; }
+};
+
 // Chebyshev tables of one pair: F[i] = fc*T_{i+BASE}(x), D[i] = d/dr of that,
 // with set_polys / set_polys_out_of_range / FCUT::get_fcut semantics.
 template <class T, int VW, int BASE>
 __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double smin, double smax, double xavg,
-                                           double xdiff, double lambda, vecN<T, VW> &F, vecN<T, VW> &D) {
+                                           double xdiff, double lambda, PairTab<T, VW> &P) {
   double re = r < smin ? smin : r;
   double x, ex = 0.0, dxdr;
   if (a.transform == CG_MORSE) {
@@ -447,40 +489,59 @@ __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double sm
     if (zero_d) tdn = 0.0;
     const double tv = tn[n] + shift * tdn;
     const double tdv = tdn * dscale;
-    F[n - BASE] = (T)(fc * tv);
-    D[n - BASE] = (T)(fc * tdv + fcd * tv);
+    P.set(n - BASE, (T)(fc * tv), (T)(fc * tdv + fcd * tv));
   }
 }
 
 template <class T> __device__ __forceinline__ void lds_add(T *p, T v) { unsafeAtomicAdd(p, v); }
 
+// LDS tile geometry of k_assemble (shared with the host task builder).
+constexpr int SE_REP = 16;  // replicas of each stress / energy row
+__host__ __device__ constexpr int tile_stride(int w) { return w | 1; }
+__host__ __device__ constexpr size_t tile_bytes(int natoms, int w) {
+  return (3 * (size_t)natoms + 7 * SE_REP) * (size_t)tile_stride(w) * sizeof(double);
+}
+
 template <int NB, int VW, int BASE, class T, int BS>
 __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
+  // The tile is accumulated in FP64 for both precisions: LDS ds_add_f64 is
+  // fast on gfx950 while ds_add_f32 stalls the LDS pipeline, and the FP32
+  // path then rounds each A entry once, at the write-out.
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *acc = reinterpret_cast<T *>(smem_raw);
+  double *acc = reinterpret_cast<double *>(smem_raw);
   const Task t = a.tasks[blockIdx.x];
   const DevFrame &f = a.frames[t.frame];
   const int nf = 3 * f.natoms;
   const int W = t.ncols;
-  const int nacc = (nf + 56) * W;
-  for (int i = threadIdx.x; i < nacc; i += BS) acc[i] = (T)0;
+  // tile layout: row r (3 rows per atom, then 7 * SE_REP stress/energy
+  // replica rows) at acc[r * Wp + column]; the odd stride Wp spreads the rows
+  // of different atoms over distinct LDS banks
+  const int Wp = tile_stride(W);
+  const int nacc = (nf + 7 * SE_REP) * Wp;
+  for (int i = threadIdx.x; i < nacc; i += BS) acc[i] = 0.0;
   __syncthreads();
   const int ty = t.type;
   const int colbase = a.col_rel[ty] + t.col0;
   const T ps = (T)f.perm_scale[a.nb_scale];
   const bool full_stress = a.stress_mode == CG_STRESS_ALL;
   const bool any_stress = a.stress_mode != CG_STRESS_NONE;
-  const int rep = threadIdx.x & 7;
-  T *se = acc + (size_t)nf * W;
-  for (int base = t.c0; base < t.c1; base += BS) {
-    const int ci = base + threadIdx.x;
-    const bool valid = ci < t.c1;
-    vecN<T, VW> F[NP], D[NP];
+  const int rep = threadIdx.x & (SE_REP - 1);
+  double *se = acc + (size_t)nf * Wp;
+  // thread i takes clusters [i*R, i*R + R) of the bucket: neighbouring lanes
+  // hold clusters of different anchor atoms, so their force-row atomics do
+  // not collide on one LDS address
+  const int ncl = t.c1 - t.c0;
+  const int R = (ncl + BS - 1) / BS;
+  for (int rnd = 0; rnd < R; ++rnd) {
+    const int j = threadIdx.x * R + rnd;
+    const bool valid = j < ncl;
+    const int ci = t.c0 + j;
+    PairTab<T, VW> tab[NP];
     T rh[NP][3], rl[NP];  // unit pair vectors and lengths
     int atom[NB];
 #pragma unroll
-    for (int k = 0; k < NP; ++k) { F[k] = (T)0; D[k] = (T)0; rh[k][0] = rh[k][1] = rh[k][2] = (T)0; rl[k] = (T)1; }
+    for (int k = 0; k < NP; ++k) { tab[k].zero(); rh[k][0] = rh[k][1] = rh[k][2] = (T)0; rl[k] = (T)1; }
 #pragma unroll
     for (int s = 0; s < NB; ++s) atom[s] = 0;
     if (valid) {
@@ -500,7 +561,7 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
           rh[k][0] = (T)(v0 / r); rh[k][1] = (T)(v1 / r); rh[k][2] = (T)(v2 / r); rl[k] = (T)r;
           const int q = ty * NP + k;
-          pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], F[k], D[k]);
+          pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
           ++k;
         }
     }
@@ -516,11 +577,7 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
         if (q + 1 < pe) wnext = a.perm[q + 1];
         T fv[NP], dv[NP];
 #pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          const unsigned idx = (w >> (4 * k)) & 15u;
-          fv[k] = F[k][idx];
-          dv[k] = D[k][idx];
-        }
+        for (int k = 0; k < NP; ++k) tab[k].get((w >> (4 * k)) & 15u, fv[k], dv[k]);
         T pre[NP + 1], suf[NP + 1];
         pre[0] = (T)1;
 #pragma unroll
@@ -548,10 +605,10 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
               if (y == s) { fx0 -= g[k] * rh[k][0]; fx1 -= g[k] * rh[k][1]; fx2 -= g[k] * rh[k][2]; }
               ++k;
             }
-          T *dst = acc + (3 * atom[s]) * W + cc;
-          lds_add(dst, fx0);
-          lds_add(dst + W, fx1);
-          lds_add(dst + 2 * W, fx2);
+          double *dst = acc + (3 * atom[s]) * Wp + cc;
+          lds_add(dst, (double)fx0);
+          lds_add(dst + Wp, (double)fx1);
+          lds_add(dst + 2 * Wp, (double)fx2);
         }
         if (any_stress) {
           const int nsc = full_stress ? 6 : 3;
@@ -563,10 +620,10 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
             T sv = 0;
 #pragma unroll
             for (int k = 0; k < NP; ++k) sv -= g[k] * rl[k] * rh[k][ia] * rh[k][ib];
-            lds_add(&se[(comp * 8 + rep) * W + cc], sv);
+            lds_add(&se[(comp * SE_REP + rep) * Wp + cc], (double)sv);
           }
         }
-        lds_add(&se[(6 * 8 + rep) * W + cc], ps * e);
+        lds_add(&se[(6 * SE_REP + rep) * Wp + cc], (double)(ps * e));
       }
     }
   }
@@ -577,20 +634,20 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
   const int cabs = a.col_abs[ty] + t.col0;
   for (int i = threadIdx.x; i < nf * W; i += BS) {
     const int r = i / W, cc = i - r * W;
-    Aout[(r0 + r) * a.ld + cabs + cc] = acc[i];
+    Aout[(r0 + r) * a.ld + cabs + cc] = (T)acc[r * Wp + cc];
   }
   const int nsr = f.stress_rows, ner = f.ener_rows;
   const int sel9[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
   for (int i = threadIdx.x; i < (nsr + ner) * W; i += BS) {
     const int k = i / W, cc = i - k * W;
     int comp;
-    T scale;
-    if (k < nsr) { comp = nsr == 9 ? sel9[k] : k; scale = (T)f.inv_vol; }
-    else { comp = 6; scale = (T)1; }
-    T s = 0;
+    double scale;
+    if (k < nsr) { comp = nsr == 9 ? sel9[k] : k; scale = f.inv_vol; }
+    else { comp = 6; scale = 1.0; }
+    double s = 0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s += se[(comp * 8 + q) * W + cc];
-    Aout[(r0 + nf + k) * a.ld + cabs + cc] = s * scale;
+    for (int q = 0; q < SE_REP; ++q) s += se[(comp * SE_REP + q) * Wp + cc];
+    Aout[(r0 + nf + k) * a.ld + cabs + cc] = (T)(s * (double)scale);
   }
 }
 
@@ -633,14 +690,18 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
   double ev[10];
 #pragma unroll
   for (int q = 0; q < 10; ++q) ev[q] = 0.0;
-  for (int base = t.c0; base < t.c1; base += BS) {
-    const int ci = base + threadIdx.x;
-    const bool valid = ci < t.c1;
-    vecN<double, VW> Fv[NP], Dv[NP];
+  // strided cluster -> thread map (neighbouring lanes, different anchors)
+  const int ncl = t.c1 - t.c0;
+  const int R = (ncl + BS - 1) / BS;
+  for (int rnd = 0; rnd < R; ++rnd) {
+    const int j = threadIdx.x * R + rnd;
+    const bool valid = j < ncl;
+    const int ci = t.c0 + j;
+    PairTab<double, VW> tab[NP];
     double rh[NP][3], rl[NP];
     int atom[NB];
 #pragma unroll
-    for (int k = 0; k < NP; ++k) { Fv[k] = 0.0; Dv[k] = 0.0; rh[k][0] = rh[k][1] = rh[k][2] = 0.0; rl[k] = 1.0; }
+    for (int k = 0; k < NP; ++k) { tab[k].zero(); rh[k][0] = rh[k][1] = rh[k][2] = 0.0; rl[k] = 1.0; }
 #pragma unroll
     for (int q = 0; q < NB; ++q) atom[q] = 0;
     if (valid) {
@@ -660,7 +721,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
           rh[k][0] = v0 / r; rh[k][1] = v1 / r; rh[k][2] = v2 / r; rl[k] = r;
           const int q = ty * NP + k;
-          pair_table<double, VW, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], Fv[k], Dv[k]);
+          pair_table<double, VW, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
           ++k;
         }
     }
@@ -677,11 +738,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
         const uint32_t w = a.perm[q];
         double fv[NP], dv[NP];
 #pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          const unsigned idx = (w >> (4 * k)) & 15u;
-          fv[k] = Fv[k][idx];
-          dv[k] = Dv[k][idx];
-        }
+        for (int k = 0; k < NP; ++k) tab[k].get((w >> (4 * k)) & 15u, fv[k], dv[k]);
         double pre[NP + 1], suf[NP + 1];
         pre[0] = 1.0;
 #pragma unroll
@@ -1146,10 +1203,9 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     if (F.ntypes == 0 || F.ncols == 0 || fo[w].ncl == 0) continue;
     std::vector<Task> tasks;
     for (int fi = 0; fi < nframes; ++fi) {
-      const size_t rowsf = 3 * (size_t)hf[fi].natoms + 56;
-      int wmax = (int)(lds_budget / (rowsf * esz));
+      int wmax = 64;
+      while (wmax > 0 && tile_bytes(hf[fi].natoms, wmax) > lds_budget) --wmax;
       if (wmax < 1) throw Error(CG_ERR_UNSUPPORTED, "frame too large for the LDS tile (" + std::to_string(hf[fi].natoms) + " atoms)");
-      wmax = std::min(wmax, 64);
       for (int t = 0; t < F.ntypes; ++t) {
         int c0 = fo[w].boff[fi * F.ntypes + t], c1 = fo[w].boff[fi * F.ntypes + t + 1];
         if (c1 <= c0) continue;
@@ -1168,7 +1224,7 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     atasks[w] = std::move(tasks);
     upload(c, d_tasks, atasks[w].data(), atasks[w].size() * sizeof(Task));
     size_t lds = 0;
-    for (auto &tk : atasks[w]) lds = std::max(lds, (3 * (size_t)hf[tk.frame].natoms + 56) * tk.ncols * esz);
+    for (auto &tk : atasks[w]) lds = std::max(lds, tile_bytes(hf[tk.frame].natoms, tk.ncols));
     CG_HIP(hipEventRecord(c.fev[w][0], c.stream));
     if (prec == CG_FP32) launch_family<float>(c, w, atasks[w], d_tasks, d_frames, fo[w].refs, lds, A, ld);
     else launch_family<double>(c, w, atasks[w], d_tasks, d_frames, fo[w].refs, lds, A, ld);
