@@ -102,6 +102,7 @@ def lib(path=None):
         opt("cg_write_params", [C.c_void_p, C.c_char_p, _dp, C.c_int, C.c_int64,
                                 C.POINTER(FitResult), _dp, _dp])
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
+        opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
         _lib = L
     return _lib
 
@@ -281,3 +282,19 @@ class Device:
         f = frame.c()
         check(self.L.cg_force(self.h, dptr(x), C.byref(f), dptr(F), C.byref(E), dptr(V)), self.h)
         return F, E.value, V
+
+    def force_batch(self, x, frames):
+        """Forces (list of [natoms*3]), energies [n] and virials [n, 9] of independent configurations."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = len(frames)
+        tot = sum(f.natoms for f in frames)
+        F = np.zeros(3 * tot)
+        E = np.zeros(n)
+        V = np.zeros(9 * n)
+        cf = (Frame * n)(*[f.c() for f in frames])
+        check(self.L.cg_force_batch(self.h, dptr(x), cf, n, dptr(F), dptr(E), dptr(V)), self.h)
+        out, o = [], 0
+        for f in frames:
+            out.append(F[o:o + 3 * f.natoms])
+            o += 3 * f.natoms
+        return out, E, V.reshape(n, 9)

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "device.hpp"
@@ -129,16 +130,32 @@ __global__ void k_wrap(const DevFrame *frames, const int *atom_frame, const doub
   }
 }
 
+// Wave-wide exclusive prefix sum of one int per lane (and the wave total).
+__device__ __forceinline__ int wave_excl_scan(int v, int &total) {
+  const int lane = threadIdx.x & 63;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+
 // Neighbour images of atom i within rc: every (j, shift) with shift in
 // [-L, L]^3 (the reference's ghost layers), restricted to shifts that can lie
 // within rc along each axis.  Big cells keep j >= i (half list with the
-// reference's a1 <= PARENT[a2] rule); small cells keep every image.  Entries
-// come out sorted by parent atom j.  FILL also stores, per entry, the
-// relative vector image(j) - i with its length, and the owning atom.
+// reference's a1 <= PARENT[a2] rule); small cells keep every image.
+// One wave per atom, lanes over j; entries come out sorted by parent j, then
+// by shift (sx, sy, sz).  FILL also stores, per entry, the relative vector
+// image(j) - i with its length, and the owning atom.
 template <bool FILL>
-__global__ void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap, int natoms, double rc,
-                        int L, int *count, const int *offset, uint32_t *list, double4 *nvec, int *nown) {
-  int gi = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap,
+                                                int natoms, double rc, int L, int *count, const int *offset,
+                                                uint32_t *list, double4 *nvec, int *nown) {
+  const int gi = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
   if (gi >= natoms) return;
   const DevFrame &f = frames[atom_frame[gi]];
   const int i = gi - f.atom_off;
@@ -148,37 +165,56 @@ __global__ void k_neigh(const DevFrame *frames, const int *atom_frame, const dou
   const double rc2 = rc * rc;
   double reach[3];
   for (int r = 0; r < 3; ++r) reach[r] = rc / f.width[r] + 1e-9;
-  int n = 0;
-  int base = FILL ? offset[gi] : 0;
-  for (int j = f.small ? 0 : i; j < f.natoms; ++j) {
-    const double *wj = wrap + 3 * (f.atom_off + j);
-    double fj[3], d[3];
-    for (int r = 0; r < 3; ++r) fj[r] = f.Hi[3 * r] * wj[0] + f.Hi[3 * r + 1] * wj[1] + f.Hi[3 * r + 2] * wj[2];
-    for (int r = 0; r < 3; ++r) d[r] = fj[r] - fi[r];
-    int lo[3], hi[3];
-    for (int r = 0; r < 3; ++r) {
-      lo[r] = max(-L, (int)ceil(-d[r] - reach[r]));
-      hi[r] = min(L, (int)floor(-d[r] + reach[r]));
+  int run = FILL ? offset[gi] : 0;
+  for (int jb = f.small ? 0 : i; jb < f.natoms; jb += 64) {
+    const int j = jb + lane;
+    int n = 0;
+    int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+    const double *wj = wrap + 3 * (f.atom_off + (j < f.natoms ? j : i));
+    if (j < f.natoms) {
+      double fj[3], d[3];
+      for (int r = 0; r < 3; ++r) fj[r] = f.Hi[3 * r] * wj[0] + f.Hi[3 * r + 1] * wj[1] + f.Hi[3 * r + 2] * wj[2];
+      for (int r = 0; r < 3; ++r) d[r] = fj[r] - fi[r];
+      for (int r = 0; r < 3; ++r) {
+        lo[r] = max(-L, (int)ceil(-d[r] - reach[r]));
+        hi[r] = min(L, (int)floor(-d[r] + reach[r]));
+      }
     }
-    for (int sx = lo[0]; sx <= hi[0]; ++sx)
-      for (int sy = lo[1]; sy <= hi[1]; ++sy)
-        for (int sz = lo[2]; sz <= hi[2]; ++sz) {
-          if (j == i && sx == 0 && sy == 0 && sz == 0) continue;
-          double v[3];
-          for (int r = 0; r < 3; ++r)
-            v[r] = wj[r] - wi[r] + f.H[3 * r] * sx + f.H[3 * r + 1] * sy + f.H[3 * r + 2] * sz;
-          double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-          if (r2 < rc2) {
-            if (FILL) {
-              list[base + n] = make_ref(j, sx, sy, sz);
-              nvec[base + n] = make_double4(v[0], v[1], v[2], sqrt(r2));
-              nown[base + n] = gi;
+    // pass 1: count this lane's images; pass 2 (FILL): write them in order
+    for (int pass = 0; pass < (FILL ? 2 : 1); ++pass) {
+      int base = 0;
+      if (pass == 1) {
+        int tot;
+        base = run + wave_excl_scan(n, tot);
+        run += tot;
+      }
+      int m = 0;
+      for (int sx = lo[0]; sx <= hi[0]; ++sx)
+        for (int sy = lo[1]; sy <= hi[1]; ++sy)
+          for (int sz = lo[2]; sz <= hi[2]; ++sz) {
+            if (j == i && sx == 0 && sy == 0 && sz == 0) continue;
+            double v[3];
+            for (int r = 0; r < 3; ++r)
+              v[r] = wj[r] - wi[r] + f.H[3 * r] * sx + f.H[3 * r + 1] * sy + f.H[3 * r + 2] * sz;
+            const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (r2 < rc2) {
+              if (pass == 1) {
+                list[base + m] = make_ref(j, sx, sy, sz);
+                nvec[base + m] = make_double4(v[0], v[1], v[2], sqrt(r2));
+                nown[base + m] = gi;
+              }
+              ++m;
             }
-            ++n;
           }
-        }
+      n = m;
+    }
+    if (!FILL) {
+      int tot;
+      wave_excl_scan(n, tot);
+      run += tot;
+    }
   }
-  if (!FILL) count[gi] = n;
+  if (!FILL && lane == 0) count[gi] = run;
 }
 
 struct EnumArgs {
@@ -189,7 +225,8 @@ struct EnumArgs {
   const uint32_t *nlist;
   const double4 *nvec;  // relative vector + length per entry
   const int *nown;      // owning atom per entry
-  int nnb, ntypes_atom, nkeys_type;
+  int nnb, ntypes_atom, nkeys_type;   // nnb: capacity of the entry arrays
+  const int *ntot;      // device-side number of valid entries (<= nnb)
   double cutoff;
   int *count;           // [nnb]
   const int *coff;      // [nnb+1]
@@ -250,6 +287,10 @@ template <int NB, bool FILL>
 __global__ __launch_bounds__(256) void k_enum(EnumArgs a) {
   const int k2 = blockIdx.x * blockDim.x + threadIdx.x;
   if (k2 >= a.nnb) return;
+  if (k2 >= *a.ntot) {
+    if (!FILL) a.count[k2] = 0;
+    return;
+  }
   const double4 v2 = a.nvec[k2];
   int n = 0;
   if (v2.w < a.cutoff) {
@@ -329,6 +370,33 @@ __global__ __launch_bounds__(256) void k_enum(EnumArgs a) {
     }
   }
   if (!FILL) a.count[k2] = n;
+}
+
+// Per-family neighbour sublist: the entries of each atom's list closer than
+// the family cutoff, in list order (so still sorted by parent atom).
+__global__ void k_sub_count(const int *noff, const double4 *nvec, int natoms, double cut, int *cnt) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi > natoms) return;
+  if (gi == natoms) { cnt[gi] = 0; return; }
+  int n = 0;
+  for (int k = noff[gi]; k < noff[gi + 1]; ++k) n += nvec[k].w < cut;
+  cnt[gi] = n;
+}
+
+__global__ void k_sub_fill(const int *noff, const uint32_t *nlist, const double4 *nvec, const int *nown, int natoms,
+                           double cut, const int *soff, uint32_t *slist, double4 *svec, int *sown) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= natoms) return;
+  int o = soff[gi];
+  for (int k = noff[gi]; k < noff[gi + 1]; ++k) {
+    const double4 v = nvec[k];
+    if (v.w < cut) {
+      slist[o] = nlist[k];
+      svec[o] = v;
+      sown[o] = nown[k];
+      ++o;
+    }
+  }
 }
 
 __global__ void k_iota(int *v, int n) {
@@ -653,19 +721,25 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
 
 // Force / energy / virial of clusters with fitted coefficients
 // (Cheby::Force_all 2-body part, Force_3B, Force_4B; src/Cheby.C:1731-2675).
-// Lanes walk the clusters of one (frame, type) bucket; each lane contracts its
-// cluster's power products with the coefficients of every allowed power set
-// of the type, then scatters the pair forces to its atoms.  Energy and virial
-// are reduced over the workgroup before one atomic add per frame.
+//
+// The reference sums, for every cluster, coef[column(s)] * prod_k F_k[p_k]
+// over the allowed power sets s of the cluster type.  Here the coefficients
+// of a type are laid out as a dense tensor C[p_0, ..., p_{NP-1}] (N^NP
+// entries, zero where a power combination is not allowed), and energy and
+// pair gradients are one nested contraction of C with the per-pair tables:
+// the two innermost pairs are unrolled (table entries are plain registers,
+// C entries are wave-uniform scalar loads), the outer pairs are runtime
+// loops with one uniform table lookup per iteration.  Lanes walk clusters
+// of one (frame, type) bucket; pair forces are scattered with atomics, and
+// energy and virial are reduced over the workgroup.
 struct ForceArgs {
   const Task *tasks;
   const DevFrame *frames;
   const double *wrap;
   const uint32_t *refs;
-  const uint32_t *perm;
-  const int *colptr, *col_rel, *col_abs, *ncol_type;
   const double *smin, *smax, *xavg, *xdiff, *lambda;
-  const double *x;       // coefficients in A column order
+  const double *coef;    // [ntypes * tsize] coefficient tensors
+  int tsize;             // N^NP
   double *F;             // [batch atoms * 3]
   double *EV;            // [nframes * 10]: energy, virial xx xy xz yx yy yz zx zy zz
   int transform, fix_type, fcut_type;
@@ -674,7 +748,116 @@ struct ForceArgs {
   int nb_scale;
 };
 
-template <int NB, int VW, int BASE, int BS>
+// E = sum_p C[p] F0[p], g0 = sum_p C[p] D0[p]
+template <int N>
+__device__ __forceinline__ void contract1(const PairTab<double, N> *t, const double *__restrict__ C, double &E,
+                                          double *g) {
+  double e = 0.0, g0 = 0.0;
+#pragma unroll
+  for (int p = 0; p < N; ++p) {
+    double f, d;
+    t[0].get(p, f, d);
+    const double c = C[p];
+    e += c * f;
+    g0 += c * d;
+  }
+  E = e;
+  g[0] = g0;
+}
+
+// Inner two levels: S = sum_{a,b} C[a*N+b] Fx[a] Fy[b], Tx = sum C Dx Fy, Ty = sum C Fx Dy
+template <int N>
+__device__ __forceinline__ void contract_inner2(const PairTab<double, N> &tx, const PairTab<double, N> &ty,
+                                                const double *__restrict__ C, double &S, double &Tx, double &Ty) {
+  double s = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int pa = 0; pa < N; ++pa) {
+    double s2 = 0.0, t2 = 0.0;
+#pragma unroll
+    for (int pb = 0; pb < N; ++pb) {
+      double f, d;
+      ty.get(pb, f, d);
+      const double c = C[pa * N + pb];
+      s2 += c * f;
+      t2 += c * d;
+    }
+    double fa, da;
+    tx.get(pa, fa, da);
+    s += fa * s2;
+    sy += fa * t2;
+    sx += da * s2;
+  }
+  S = s;
+  Tx = sx;
+  Ty = sy;
+}
+
+template <int N>
+__device__ __forceinline__ void contract3(const PairTab<double, N> *t, const double *__restrict__ C, double &E,
+                                          double *g) {
+  double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  for (int p0 = 0; p0 < N; ++p0) {
+    double f0, d0;
+    t[0].get(p0, f0, d0);
+    double S, T1, T2;
+    contract_inner2<N>(t[1], t[2], C + p0 * N * N, S, T1, T2);
+    e += f0 * S;
+    g0 += d0 * S;
+    g1 += f0 * T1;
+    g2 += f0 * T2;
+  }
+  E = e;
+  g[0] = g0; g[1] = g1; g[2] = g2;
+}
+
+template <int N>
+__device__ __forceinline__ void contract6(const PairTab<double, N> *t, const double *__restrict__ C, double &E,
+                                          double *g) {
+  double e = 0.0, G[6] = {0, 0, 0, 0, 0, 0};
+  for (int p0 = 0; p0 < N; ++p0) {
+    double f0, d0;
+    t[0].get(p0, f0, d0);
+    double S1 = 0.0, T1[5] = {0, 0, 0, 0, 0};  // gradients of pairs 1..5
+    for (int p1 = 0; p1 < N; ++p1) {
+      double f1, d1;
+      t[1].get(p1, f1, d1);
+      double S2 = 0.0, T2[4] = {0, 0, 0, 0};    // pairs 2..5
+      for (int p2 = 0; p2 < N; ++p2) {
+        double f2, d2;
+        t[2].get(p2, f2, d2);
+        double S3 = 0.0, T3[3] = {0, 0, 0};     // pairs 3..5
+        const double *Cb = C + ((p0 * N + p1) * N + p2) * N * N * N;
+        for (int p3 = 0; p3 < N; ++p3) {
+          double f3, d3;
+          t[3].get(p3, f3, d3);
+          double S4, T44, T45;
+          contract_inner2<N>(t[4], t[5], Cb + p3 * N * N, S4, T44, T45);
+          S3 += f3 * S4;
+          T3[0] += d3 * S4;
+          T3[1] += f3 * T44;
+          T3[2] += f3 * T45;
+        }
+        S2 += f2 * S3;
+        T2[0] += d2 * S3;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) T2[q + 1] += f2 * T3[q];
+      }
+      S1 += f1 * S2;
+      T1[0] += d1 * S2;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) T1[q + 1] += f1 * T2[q];
+    }
+    e += f0 * S1;
+    G[0] += d0 * S1;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) G[q + 1] += f0 * T1[q];
+  }
+  E = e;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) g[q] = G[q];
+}
+
+template <int NB, int N, int BASE, int BS>
 __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
   __shared__ double red[10][BS / 64];
@@ -682,36 +865,26 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
   const DevFrame &f = a.frames[t.frame];
   const int ty = t.type;
   const double ps = f.perm_scale[a.nb_scale];
-  const int cb = a.col_rel[ty], ce = cb + a.ncol_type[ty];
-  const int cabs = a.col_abs[ty];
+  const double *C = a.coef + (size_t)ty * a.tsize;
   AsmArgs aa{};
   aa.transform = a.transform; aa.fix_type = a.fix_type; aa.fcut_type = a.fcut_type;
   aa.deriv_const = a.deriv_const; aa.smooth_dist = a.smooth_dist; aa.fcut_offset = a.fcut_offset;
   double ev[10];
 #pragma unroll
   for (int q = 0; q < 10; ++q) ev[q] = 0.0;
-  // strided cluster -> thread map (neighbouring lanes, different anchors)
-  const int ncl = t.c1 - t.c0;
-  const int R = (ncl + BS - 1) / BS;
-  for (int rnd = 0; rnd < R; ++rnd) {
-    const int j = threadIdx.x * R + rnd;
-    const bool valid = j < ncl;
-    const int ci = t.c0 + j;
-    PairTab<double, VW> tab[NP];
+  const int ci = t.c0 + threadIdx.x;
+  if (ci < t.c1) {
+    PairTab<double, N> tab[NP];
     double rh[NP][3], rl[NP];
     int atom[NB];
+    double p[NB][3];
 #pragma unroll
-    for (int k = 0; k < NP; ++k) { tab[k].zero(); rh[k][0] = rh[k][1] = rh[k][2] = 0.0; rl[k] = 1.0; }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) atom[q] = 0;
-    if (valid) {
-      double p[NB][3];
-#pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        uint32_t r = a.refs[(size_t)NB * ci + q];
-        atom[q] = ref_atom(r);
-        image_pos(f, a.wrap, r, p[q]);
-      }
+    for (int q = 0; q < NB; ++q) {
+      uint32_t r = a.refs[(size_t)NB * ci + q];
+      atom[q] = ref_atom(r);
+      image_pos(f, a.wrap, r, p[q]);
+    }
+    {
       int k = 0;
 #pragma unroll
       for (int x = 0; x < NB; ++x)
@@ -721,72 +894,45 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
           rh[k][0] = v0 / r; rh[k][1] = v1 / r; rh[k][2] = v2 / r; rl[k] = r;
           const int q = ty * NP + k;
-          pair_table<double, VW, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
+          pair_table<double, N, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
           ++k;
         }
     }
-    double e = 0.0, g[NP];
+    double e, g[NP];
+    if (NP == 1) contract1<N>(tab, C, e, g);
+    else if (NP == 3) contract3<N>(tab, C, e, g);
+    else contract6<N>(tab, C, e, g);
+    e *= ps;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) g[k] = 0.0;
-    for (int col = cb; col < ce; ++col) {
-      const double coef = a.x[cabs + (col - cb)];
-      const int pb = a.colptr[col], pe = a.colptr[col + 1];
-      double ec = 0.0, gc[NP];
-#pragma unroll
-      for (int k = 0; k < NP; ++k) gc[k] = 0.0;
-      for (int q = pb; q < pe; ++q) {
-        const uint32_t w = a.perm[q];
-        double fv[NP], dv[NP];
-#pragma unroll
-        for (int k = 0; k < NP; ++k) tab[k].get((w >> (4 * k)) & 15u, fv[k], dv[k]);
-        double pre[NP + 1], suf[NP + 1];
-        pre[0] = 1.0;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) pre[k + 1] = pre[k] * fv[k];
-        suf[NP] = 1.0;
-#pragma unroll
-        for (int k = NP - 1; k >= 0; --k) suf[k] = suf[k + 1] * fv[k];
-        ec += pre[NP];
-#pragma unroll
-        for (int k = 0; k < NP; ++k) gc[k] += dv[k] * (pre[k] * suf[k + 1]);
+    for (int k = 0; k < NP; ++k) g[k] *= ps;
+    if (NB == 2 && a.penalty_scale > 0.0) {
+      const double sm = a.smin[ty];
+      if (rl[0] - a.penalty_dist < sm) {
+        const double rp = sm + a.penalty_dist - rl[0];
+        g[0] += -3.0 * rp * rp * a.penalty_scale;
+        e += rp * rp * rp * a.penalty_scale;
       }
-      e += coef * ec;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) g[k] += coef * gc[k];
     }
-    if (valid) {
-      e *= ps;
+    int k = 0;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) g[k] *= ps;
-      if (NB == 2 && a.penalty_scale > 0.0) {
-        const double sm = a.smin[ty];
-        if (rl[0] - a.penalty_dist < sm) {
-          const double rp = sm + a.penalty_dist - rl[0];
-          g[0] += -3.0 * rp * rp * a.penalty_scale;
-          e += rp * rp * rp * a.penalty_scale;
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+      for (int y = x + 1; y < NB; ++y) {
+        double *Fa = a.F + 3 * (size_t)(f.atom_off + atom[x]);
+        double *Fb = a.F + 3 * (size_t)(f.atom_off + atom[y]);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          const double d = g[k] * rh[k][cc];
+          atomicAdd(Fa + cc, d);
+          atomicAdd(Fb + cc, -d);
         }
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+          for (int v = 0; v < 3; ++v) ev[1 + 3 * u + v] -= g[k] * rl[k] * rh[k][u] * rh[k][v];
+        ++k;
       }
-      int k = 0;
-#pragma unroll
-      for (int x = 0; x < NB; ++x)
-#pragma unroll
-        for (int y = x + 1; y < NB; ++y) {
-          double *Fa = a.F + 3 * (size_t)(f.atom_off + atom[x]);
-          double *Fb = a.F + 3 * (size_t)(f.atom_off + atom[y]);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const double d = g[k] * rh[k][c];
-            atomicAdd(Fa + c, d);
-            atomicAdd(Fb + c, -d);
-          }
-#pragma unroll
-          for (int u = 0; u < 3; ++u)
-#pragma unroll
-            for (int v = 0; v < 3; ++v) ev[1 + 3 * u + v] -= g[k] * rl[k] * rh[k][u] * rh[k][v];
-          ++k;
-        }
-      ev[0] += e;
-    }
+    ev[0] += e;
   }
   // workgroup reduction of energy + virial
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -996,7 +1142,7 @@ struct Batch {
 // + bucket the clusters of every family.  md != 0 selects the force-evaluation
 // pair set (every pair below S_MAXIM, Cheby::Force_all) instead of the
 // least-squares one (S_MINIM < r < S_MAXIM, Cheby::Deriv_2B).
-static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B) {
+static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B, bool sticky = true) {
   std::vector<DevFrame> &hf = B.hf;
   const cg_model &m = c.m;
   if (nframes <= 0) throw Error(CG_ERR_ARG, "no frames");
@@ -1021,8 +1167,15 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     d.stress_rows = (m.fit_stress != CG_STRESS_NONE && (m.nstress < 0 || fr.index < m.nstress))
                         ? (m.fit_stress == CG_STRESS_ALL ? 9 : 3) : 0;
     d.ener_rows = (m.fit_energy && (m.nener < 0 || fr.index < m.nener)) ? 3 : 0;
-    if (m.small_mode || d.min_extent <= mx) c.sticky_small = 1;
-    d.small = c.sticky_small;
+    // NEIGHBORS small-cell mode is sticky across the frames of a fit; force
+    // evaluations treat every configuration on its own
+    const int small_here = (m.small_mode || d.min_extent <= mx) ? 1 : 0;
+    if (sticky) {
+      if (small_here) c.sticky_small = 1;
+      d.small = c.sticky_small;
+    } else {
+      d.small = small_here;
+    }
     d.perm_scale[0] = d.perm_scale[1] = 1.0;
     for (int j = 2; j < 5; ++j) d.perm_scale[j] = d.small ? d.perm_scale[j - 1] / j : 1.0;
     rows += 3LL * fr.natoms + d.stress_rows + d.ener_rows;
@@ -1051,7 +1204,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   int *d_noff = c.b_noff.as<int>(natoms + 1);
   if (!d_ncount || !d_noff) throw Error(CG_ERR_MEMORY, "neighbour counts");
   CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
-  hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
+  hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
                      natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr,
                      (int *)nullptr);
   exclusive_scan(c, d_ncount, d_noff, natoms);
@@ -1061,8 +1214,35 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   double4 *d_nvec = c.b_nvec.as<double4>(nn1);
   int *d_nown = c.b_nown.as<int>(nn1);
   if (!d_nlist || !d_nvec || !d_nown) throw Error(CG_ERR_MEMORY, "neighbour list");
-  hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
+  hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
                      natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_nvec, d_nown);
+  // ---- per-family sublists (3B / 4B cutoffs below the list cutoff), sized
+  // by the full list so that no host read is needed
+  const int *d_ntot = d_noff + natoms;
+  const uint32_t *fl_list[3] = {d_nlist, d_nlist, d_nlist};
+  const double4 *fl_vec[3] = {d_nvec, d_nvec, d_nvec};
+  const int *fl_own[3] = {d_nown, d_nown, d_nown};
+  const int *fl_off[3] = {d_noff, d_noff, d_noff};
+  const int *fl_tot[3] = {d_ntot, d_ntot, d_ntot};
+  for (int w = 1; w < 3; ++w) {
+    const DevFamily &F = c.fam[w];
+    if (F.ntypes == 0 || F.ncols == 0 || nnb == 0 || !(F.cutoff < mx)) continue;
+    DBuf &sb = c.b_sub[w - 1];
+    // layout: vec [nn1] (double4) | list [nn1] | own [nn1] | cnt [natoms+1] | off [natoms+1]
+    char *base = static_cast<char *>(sb.get(nn1 * (sizeof(double4) + 2 * sizeof(int)) + 2 * ((size_t)natoms + 1) * sizeof(int) + 256));
+    if (!base) throw Error(CG_ERR_MEMORY, "neighbour sublist");
+    double4 *svec = reinterpret_cast<double4 *>(base);
+    uint32_t *slist = reinterpret_cast<uint32_t *>(svec + nn1);
+    int *sown = reinterpret_cast<int *>(slist + nn1);
+    int *scnt = sown + nn1;
+    int *soff = scnt + natoms + 1;
+    hipLaunchKernelGGL(k_sub_count, dim3((natoms + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, d_noff, d_nvec, natoms,
+                       F.cutoff, scnt);
+    exclusive_scan(c, scnt, soff, natoms);
+    hipLaunchKernelGGL(k_sub_fill, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_noff, d_nlist, d_nvec,
+                       d_nown, natoms, F.cutoff, soff, slist, svec, sown);
+    fl_list[w] = slist; fl_vec[w] = svec; fl_own[w] = sown; fl_off[w] = soff; fl_tot[w] = soff + natoms;
+  }
   // ---- per-family cluster counts (one thread per neighbour entry)
   FamOut *fo = B.fo;
   int *d_ccount = c.b_ccount.as<int>(3 * (nn1 + 1));
@@ -1076,8 +1256,9 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     if (F.ntypes == 0 || F.ncols == 0 || nnb == 0) continue;
     active[w] = true;
     EnumArgs &E = ea[w];
-    E.frames = d_frames; E.atom_frame = d_af; E.type = d_type; E.noff = d_noff; E.nlist = d_nlist;
-    E.nvec = d_nvec; E.nown = d_nown; E.nnb = nnb; E.ntypes_atom = m.ntypes; E.nkeys_type = F.ntypes;
+    E.frames = d_frames; E.atom_frame = d_af; E.type = d_type; E.noff = fl_off[w]; E.nlist = fl_list[w];
+    E.nvec = fl_vec[w]; E.nown = fl_own[w]; E.nnb = nnb; E.ntot = fl_tot[w]; E.ntypes_atom = m.ntypes;
+    E.nkeys_type = F.ntypes;
     E.cutoff = F.cutoff;
     E.count = d_ccount + w * (nn1 + 1);
     E.coff = d_coff + w * (nn1 + 1);
@@ -1254,72 +1435,114 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
 }
 
 
-void force(Ctx &c, const double *x_host, const cg_frame *frame, double *F, double *E, double *vir) {
+// Contraction order of a family: tensor dimension per pair (2-body: the
+// pair's SNUM columns; 3B/4B: largest power + 1).
+static int force_order(const Ctx &c, int w) { return w == 0 ? c.m.order2b : c.fam[w].max_pow; }
+
+// Coefficient tensors of every family for the coefficient vector x (host
+// build, uploaded when x differs from the last one seen by this context).
+static void set_coef(Ctx &c, const double *x) {
   const cg_model &m = c.m;
+  if (c.coef_valid && std::equal(x, x + m.ncols, c.h_x_last.begin())) return;
+  for (int w = 0; w < 3; ++w) {
+    const DevFamily &F = c.fam[w];
+    c.coef_size[w] = 0;
+    if (F.ntypes == 0 || F.ncols == 0) continue;
+    const int N = force_order(c, w);
+    size_t ts = 1;
+    for (int k = 0; k < F.np; ++k) ts *= (size_t)N;
+    std::vector<double> T((size_t)F.ntypes * ts, 0.0);
+    if (w == 0) {
+      for (int t = 0; t < F.ntypes; ++t)
+        for (int q = 0; q < N; ++q) T[(size_t)t * ts + q] = x[F.h_col_offset[t] + q];
+    } else {
+      const cg_cluster_set &cs = w == 1 ? m.c3 : m.c4;
+      for (int t = 0; t < cs.ntypes; ++t) {
+        if (cs.excluded[t]) continue;
+        for (int sidx = cs.perm_offset[t]; sidx < cs.perm_offset[t + 1]; ++sidx) {
+          size_t idx = 0;
+          for (int k = 0; k < cs.npairs; ++k) idx = idx * N + (size_t)cs.perm_powers[(size_t)sidx * cs.npairs + k];
+          T[(size_t)t * ts + idx] = x[cs.col_offset[t] + cs.perm_col[sidx]];
+        }
+      }
+    }
+    double *d = c.b_coef[w].as<double>(T.size());
+    if (!d) throw Error(CG_ERR_MEMORY, "coefficient tensors");
+    CG_HIP(hipMemcpy(d, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice));
+    c.coef_size[w] = (int)ts;
+  }
+  c.h_x_last.assign(x, x + m.ncols);
+  c.coef_valid = true;
+}
+
+template <int NB, int N, int BASE>
+static void launch_force_n(Ctx &c, dim3 grid, const ForceArgs &a) {
+  hipLaunchKernelGGL((k_force<NB, N, BASE, 64>), grid, dim3(64), 0, c.stream, a);
+}
+
+template <int NB, int BASE, int... Ns>
+static bool dispatch_force(Ctx &c, int N, dim3 grid, const ForceArgs &a, std::integer_sequence<int, Ns...>) {
+  bool done = false;
+  ((N == Ns + 1 ? (launch_force_n<NB, Ns + 1, BASE>(c, grid, a), done = true) : false), ...);
+  return done;
+}
+
+// Force / energy / virial of a batch of configurations, each standing alone
+// (no small-cell stickiness across them).  F: [3 * sum natoms], E: [nframes],
+// vir: [9 * nframes].
+void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
+                 double *vir) {
+  const cg_model &m = c.m;
+  set_coef(c, x_host);
   Batch B;
-  const int saved_small = c.sticky_small;
-  c.sticky_small = 0;  // each configuration stands alone
-  prepare(c, frame, 1, 1, B);
-  c.sticky_small = saved_small;
+  prepare(c, frames, nframes, 1, B, false);
   const int natoms = B.natoms;
-  double *d_x = c.b_force.as<double>(m.ncols + 3 * (size_t)natoms + 16);
-  if (!d_x) throw Error(CG_ERR_MEMORY, "force buffers");
-  double *d_F = d_x + m.ncols;
+  double *d_F = c.b_force.as<double>(3 * (size_t)natoms + 10 * (size_t)nframes + 16);
+  if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
   double *d_EV = d_F + 3 * (size_t)natoms;
-  CG_HIP(hipMemcpyAsync(d_x, x_host, m.ncols * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  CG_HIP(hipMemsetAsync(d_F, 0, (3 * (size_t)natoms + 16) * sizeof(double), c.stream));
+  CG_HIP(hipMemsetAsync(d_F, 0, (3 * (size_t)natoms + 10 * (size_t)nframes) * sizeof(double), c.stream));
   std::vector<Task> ftasks[3];  // host sources of the task uploads, alive until the final sync
   for (int w = 0; w < 3; ++w) {
     const DevFamily &Fm = c.fam[w];
     if (Fm.ntypes == 0 || Fm.ncols == 0 || B.fo[w].ncl == 0) continue;
-    std::vector<Task> tasks;
-    for (int t = 0; t < Fm.ntypes; ++t) {
-      int c0 = B.fo[w].boff[t], c1 = B.fo[w].boff[t + 1];
-      const int chunk = 4096;
-      for (int k = c0; k < c1; k += chunk) tasks.push_back(Task{0, t, 0, 0, k, std::min(c1, k + chunk)});
-    }
+    std::vector<Task> &tasks = ftasks[w];
+    const int chunk = 64;  // one wave per workgroup, one cluster per lane
+    for (int fi = 0; fi < nframes; ++fi)
+      for (int t = 0; t < Fm.ntypes; ++t) {
+        const int c0 = B.fo[w].boff[fi * Fm.ntypes + t], c1 = B.fo[w].boff[fi * Fm.ntypes + t + 1];
+        for (int k = c0; k < c1; k += chunk) tasks.push_back(Task{fi, t, 0, 0, k, std::min(c1, k + chunk)});
+      }
     if (tasks.empty()) continue;
     Task *d_tasks = c.b_tasks[w].as<Task>(tasks.size());
     if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
-    ftasks[w] = tasks;
-    CG_HIP(hipMemcpyAsync(d_tasks, ftasks[w].data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, c.stream));
+    CG_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, c.stream));
     ForceArgs a{};
     a.tasks = d_tasks; a.frames = B.d_frames; a.wrap = c.b_wrap.as<double>(0); a.refs = B.fo[w].refs;
-    a.perm = Fm.perm; a.colptr = Fm.colptr; a.col_rel = Fm.col_offset; a.col_abs = Fm.col_offset + Fm.ntypes;
-    a.ncol_type = Fm.ncol_type;
     a.smin = Fm.smin; a.smax = Fm.smax; a.xavg = Fm.xavg; a.xdiff = Fm.xdiff; a.lambda = Fm.lambda;
-    a.x = d_x; a.F = d_F; a.EV = d_EV;
+    a.coef = c.b_coef[w].as<double>(0); a.tsize = c.coef_size[w];
+    a.F = d_F; a.EV = d_EV;
     a.transform = m.transform; a.fix_type = m.fix_type; a.fcut_type = m.fcut_type;
     a.deriv_const = m.deriv_const; a.smooth_dist = m.smooth_dist; a.fcut_offset = m.fcut_offset;
     a.penalty_scale = w == 0 ? c.penalty_scale : 0.0;
     a.penalty_dist = c.penalty_dist;
     a.nb_scale = Fm.nb;
-    dim3 grid((unsigned)tasks.size());
-#define CG_FLAUNCH(NBv, VWv, BASEv, BSv) hipLaunchKernelGGL((k_force<NBv, VWv, BASEv, BSv>), grid, dim3(BSv), 0, c.stream, a)
-    if (w == 0) {
-      if (Fm.max_pow <= 16) CG_FLAUNCH(2, 16, 1, 256);
-      else throw Error(CG_ERR_UNSUPPORTED, "2-body order above 16 is not supported");
-    } else if (w == 1) {
-      if (Fm.max_pow <= 4) CG_FLAUNCH(3, 4, 0, 256);
-      else if (Fm.max_pow <= 8) CG_FLAUNCH(3, 8, 0, 256);
-      else if (Fm.max_pow <= 12) CG_FLAUNCH(3, 12, 0, 256);
-      else if (Fm.max_pow <= 16) CG_FLAUNCH(3, 16, 0, 256);
-      else throw Error(CG_ERR_UNSUPPORTED, "3-body order above 16 is not supported");
-    } else {
-      if (Fm.max_pow <= 4) CG_FLAUNCH(4, 4, 0, 256);
-      else if (Fm.max_pow <= 5) CG_FLAUNCH(4, 5, 0, 256);
-      else if (Fm.max_pow <= 8) CG_FLAUNCH(4, 8, 0, 256);
-      else throw Error(CG_ERR_UNSUPPORTED, "4-body order above 8 is not supported");
-    }
-#undef CG_FLAUNCH
+    const dim3 grid((unsigned)tasks.size());
+    const int N = force_order(c, w);
+    bool ok;
+    if (w == 0) ok = dispatch_force<2, 1>(c, N, grid, a, std::make_integer_sequence<int, 16>{});
+    else if (w == 1) ok = dispatch_force<3, 0>(c, N, grid, a, std::make_integer_sequence<int, 16>{});
+    else ok = dispatch_force<4, 0>(c, N, grid, a, std::make_integer_sequence<int, 8>{});
+    if (!ok) throw Error(CG_ERR_UNSUPPORTED, "force evaluation supports 2/3-body orders up to 16 and 4-body up to 8");
     CG_HIP(hipGetLastError());
   }
-  double ev[10];
+  std::vector<double> ev(10 * (size_t)nframes);
   CG_HIP(hipMemcpyAsync(F, d_F, 3 * (size_t)natoms * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  CG_HIP(hipMemcpyAsync(ev, d_EV, 10 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(ev.data(), d_EV, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipStreamSynchronize(c.stream));
-  if (E) *E = ev[0];
-  if (vir) for (int q = 0; q < 9; ++q) vir[q] = ev[1 + q];
+  for (int fi = 0; fi < nframes; ++fi) {
+    if (E) E[fi] = ev[10 * fi];
+    if (vir) for (int q = 0; q < 9; ++q) vir[9 * fi + q] = ev[10 * fi + 1 + q];
+  }
 }
 
 void ctx_init(Ctx &c, const cg_model *m, int device) {
@@ -1339,8 +1562,9 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 
 void ctx_free(Ctx &c) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
-                  &c.b_nown, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
-                  &c.b_tasks[1], &c.b_tasks[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_tmp})
+                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
+                  &c.b_tasks[1], &c.b_tasks[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
+                  &c.b_tmp})
     b->release();
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
@@ -1417,11 +1641,24 @@ cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host) {
   }
 }
 
+cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, int nframes, double *F, double *E,
+                         double *virial9) {
+  if (!ctx || !x || !frames || nframes <= 0 || !F) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::force_batch(ctx->c, x, frames, nframes, F, E, virial9);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
 cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f, double *F, double *E, double *virial9) {
   if (!ctx || !x || !f || !F) { cg::set_error("null argument"); return CG_ERR_ARG; }
   try {
     CG_HIP(hipSetDevice(ctx->c.device));
-    cg::force(ctx->c, x, f, F, E, virial9);
+    cg::force_batch(ctx->c, x, f, 1, F, E, virial9);
     return CG_OK;
   } catch (const cg::Error &e) {
     cg::set_error(e.what());

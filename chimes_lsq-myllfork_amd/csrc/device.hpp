@@ -91,12 +91,17 @@ struct Ctx {
   // batch buffers
   DBuf b_frames, b_xyz, b_type, b_wrap;   // frame table, coordinates, types, wrapped coordinates
   DBuf b_ncount, b_noff, b_nlist, b_nvec, b_nown;  // neighbour lists
+  DBuf b_sub[2];                          // 3B / 4B neighbour sublists
   DBuf b_ccount, b_coff;                  // per-entry cluster counts / offsets (3 families)
   DBuf b_fam[3];                          // per-family clusters, keys and bucket bounds
   DBuf b_tasks[3];                        // per-family assembly task lists
   DBuf b_A, b_scan;                       // design matrix; hipcub scratch
   DBuf b_x;                               // fit workspace
-  DBuf b_force;                           // force evaluation: x, F, energy/virial
+  DBuf b_force;                           // force evaluation: F, energy/virial
+  DBuf b_coef[3];                         // per-family coefficient tensors (force evaluation)
+  int coef_size[3] = {0, 0, 0};           // tensor entries per cluster type
+  bool coef_valid = false;
+  std::vector<double> h_x_last;           // coefficients the tensors were built from
   DBuf b_tmp;                             // matvec output
   void *A = nullptr;          // last assembled A (device)
   int A_prec = CG_FP64;

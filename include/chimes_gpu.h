@@ -238,6 +238,13 @@ cg_status cg_write_params(const cg_setup *s, /* params.txt + force.txt */ const 
 cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f,
                    double *F, double *E, double *virial9);
 
+/* The same for a batch of independent configurations in one set of launches
+ * (each configuration alone, as ZCalc does per MD step; src/functions.C:815-891).
+ * F: [3 * sum natoms] in frame order, E: [nframes], virial9: [9 * nframes].
+ * The coefficient tensors built from x stay on the device until x changes. */
+cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, int nframes,
+                         double *F, double *E, double *virial9);
+
 #ifdef __cplusplus
 }
 #endif

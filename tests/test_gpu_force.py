@@ -45,3 +45,29 @@ def test_force_penalty_below_smin(ch, synth, oracle, tmp_path):
     assert np.allclose(F, Fo, rtol=1e-10, atol=1e-10 * np.abs(Fo).max())
     assert E == pytest.approx(Eo, rel=1e-10)
     dev.close()
+
+
+def test_force_batch_matches_single_calls(ch, synth, oracle, tmp_path):
+    """A batch of configurations of different sizes (one in a small cell)
+    gives each configuration the result it gets alone, and the oracle's."""
+    text, frames, _ = synth.chno(nframes=4, nmin=40, nmax=400, seed=43)
+    s, fr = synth_setup(ch, synth, tmp_path, (text, frames, _))
+    rng = np.random.default_rng(5)
+    # a 4.9 A cubic cell (below the 5 A cutoff: small-cell lists) in the middle
+    cell = np.diag([4.9, 4.9, 4.9])
+    pos = synth.random_sequential(rng, 10, cell, 1.0)
+    small = ch.PyFrame(10, pos, rng.integers(0, 4, 10).astype(np.int32), cell)
+    fr = fr[:2] + [small] + fr[2:]
+    x = rng.normal(size=s.model.ncols)
+    dev = ch.Device(s)
+    Fb, Eb, Vb = dev.force_batch(x, fr)
+    for i, f in enumerate(fr):
+        F1, E1, V1 = dev.force(x, f)
+        Fo, Eo, Vo = oracle.force(s, f, x)
+        scale = max(np.abs(Fo).max(), 1e-300)
+        assert np.max(np.abs(Fb[i] - F1)) <= 1e-12 * scale
+        assert np.max(np.abs(Fb[i] - Fo)) <= 1e-10 * scale
+        assert Eb[i] == pytest.approx(E1, rel=1e-12, abs=1e-12 * abs(Eo))
+        assert Eb[i] == pytest.approx(Eo, rel=1e-10, abs=1e-10 * abs(Eo))
+        assert np.allclose(Vb[i], Vo, rtol=1e-10, atol=1e-10 * np.abs(Vo).max())
+    dev.close()
