@@ -103,6 +103,12 @@ def lib(path=None):
                                 C.POINTER(FitResult), _dp, _dp])
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
+        opt("cg_serve_create", [C.POINTER(Model), C.c_int, _dp, C.c_int, C.c_size_t, C.c_int,
+                                C.POINTER(C.c_void_p)])
+        opt("cg_serve_submit", [C.c_void_p, C.POINTER(Frame), C.POINTER(C.c_int64)])
+        opt("cg_serve_poll", [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp])
+        opt("cg_serve_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        opt("cg_serve_destroy", [C.c_void_p])
         _lib = L
     return _lib
 
@@ -298,3 +304,48 @@ class Device:
             out.append(F[o:o + 3 * f.natoms])
             o += 3 * f.natoms
         return out, E, V.reshape(n, 9)
+
+
+class Server:
+    """Continuous-batching force server (cg_serve_*)."""
+
+    def __init__(self, setup, x, workers=2, budget_bytes=0, max_batch_atoms=65536, device=0):
+        self.setup = setup  # keeps the model alive
+        self.L = lib()
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.h = C.c_void_p()
+        check(self.L.cg_serve_create(setup.model_ptr, device, dptr(self.x), workers, budget_bytes,
+                                     max_batch_atoms, C.byref(self.h)))
+        self.natoms = {}
+
+    def submit(self, frame):
+        t = C.c_int64()
+        f = frame.c()
+        check(self.L.cg_serve_submit(self.h, C.byref(f), C.byref(t)))
+        self.natoms[t.value] = frame.natoms
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """(F, E, virial, latency_ms) or None when not done (wait=False)."""
+        n = self.natoms[ticket]
+        F = np.zeros(3 * n)
+        E = C.c_double()
+        V = np.zeros(9)
+        lat = C.c_double()
+        done = C.c_int()
+        check(self.L.cg_serve_poll(self.h, ticket, 1 if wait else 0, C.byref(done), dptr(F), C.byref(E), dptr(V),
+                                   C.byref(lat)))
+        if not done.value:
+            return None
+        del self.natoms[ticket]
+        return F, E.value, V, lat.value
+
+    def stats(self):
+        b, r = C.c_int64(), C.c_int64()
+        check(self.L.cg_serve_stats(self.h, C.byref(b), C.byref(r)))
+        return b.value, r.value
+
+    def close(self):
+        if self.h:
+            self.L.cg_serve_destroy(self.h)
+            self.h = C.c_void_p()

@@ -149,11 +149,12 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
 // reference's a1 <= PARENT[a2] rule); small cells keep every image.
 // One wave per atom, lanes over j; entries come out sorted by parent j, then
 // by shift (sx, sy, sz).  FILL also stores, per entry, the relative vector
-// image(j) - i with its length, and the owning atom.
+// image(j) - i with its length, the owning atom and the parent's type.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap,
-                                                int natoms, double rc, int L, int *count, const int *offset,
-                                                uint32_t *list, double4 *nvec, int *nown) {
+                                                const int *type, int natoms, double rc, int L, int *count,
+                                                const int *offset, uint32_t *list, double4 *nvec, int *nown,
+                                                int *ntyp) {
   const int gi = blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (gi >= natoms) return;
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int
                 list[base + m] = make_ref(j, sx, sy, sz);
                 nvec[base + m] = make_double4(v[0], v[1], v[2], sqrt(r2));
                 nown[base + m] = gi;
+                ntyp[base + m] = type[f.atom_off + j];
               }
               ++m;
             }
@@ -225,7 +227,8 @@ struct EnumArgs {
   const uint32_t *nlist;
   const double4 *nvec;  // relative vector + length per entry
   const int *nown;      // owning atom per entry
-  int nnb, ntypes_atom, nkeys_type;   // nnb: capacity of the entry arrays
+  const int *ntyp;      // parent atom type per entry
+  int nnb, natoms, ntypes_atom, nkeys_type;   // nnb: capacity of the entry arrays
   const int *ntot;      // device-side number of valid entries (<= nnb)
   double cutoff;
   int *count;           // [nnb]
@@ -373,30 +376,200 @@ __global__ __launch_bounds__(256) void k_enum(EnumArgs a) {
 }
 
 // Per-family neighbour sublist: the entries of each atom's list closer than
-// the family cutoff, in list order (so still sorted by parent atom).
-__global__ void k_sub_count(const int *noff, const double4 *nvec, int natoms, double cut, int *cnt) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+// the family cutoff, in list order (so still sorted by parent atom).  One
+// wave per atom; COUNT writes the per-atom size, otherwise the entries are
+// compacted to soff[atom].
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_sub(const int *noff, const uint32_t *nlist, const double4 *nvec,
+                                             const int *nown, const int *ntyp, int natoms, double cut, int *cnt,
+                                             const int *soff, uint32_t *slist, double4 *svec, int *sown, int *styp) {
+  const int gi = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
   if (gi > natoms) return;
-  if (gi == natoms) { cnt[gi] = 0; return; }
-  int n = 0;
-  for (int k = noff[gi]; k < noff[gi + 1]; ++k) n += nvec[k].w < cut;
-  cnt[gi] = n;
+  if (gi == natoms) {
+    if (COUNT && lane == 0) cnt[gi] = 0;
+    return;
+  }
+  const int b = noff[gi], e = noff[gi + 1];
+  int run = COUNT ? 0 : soff[gi];
+  for (int k0 = b; k0 < e; k0 += 64) {
+    const int k = k0 + lane;
+    double4 v = make_double4(0, 0, 0, 0);
+    bool keep = false;
+    if (k < e) {
+      v = nvec[k];
+      keep = v.w < cut;
+    }
+    int tot;
+    const int pos = run + wave_excl_scan(keep ? 1 : 0, tot);
+    if (!COUNT && keep) {
+      slist[pos] = nlist[k];
+      svec[pos] = v;
+      sown[pos] = nown[k];
+      styp[pos] = ntyp[k];
+    }
+    run += tot;
+  }
+  if (COUNT && lane == 0) cnt[gi] = run;
 }
 
-__global__ void k_sub_fill(const int *noff, const uint32_t *nlist, const double4 *nvec, const int *nown, int natoms,
-                           double cut, const int *soff, uint32_t *slist, double4 *svec, int *sown) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= natoms) return;
-  int o = soff[gi];
-  for (int k = noff[gi]; k < noff[gi + 1]; ++k) {
-    const double4 v = nvec[k];
-    if (v.w < cut) {
-      slist[o] = nlist[k];
-      svec[o] = v;
-      sown[o] = nown[k];
-      ++o;
+// 3- and 4-body enumeration, one wave per anchor atom: the atom's (sub)list
+// is staged in LDS, lanes take the (k2, k3) entry pairs of the atom in order,
+// and each lane walks k4 for quadruplets.  The same rules as k_enum apply
+// (parent(a2) <= parent(a3) <= parent(a4) in big cells, all orderings in
+// small cells, every pair below the family cutoff, then the type lookup and
+// PROCEED), and the clusters of an atom are written in (k2, k3, k4) order.
+// COUNT writes the per-atom cluster count.  Lists longer than the LDS stage
+// are read from global memory instead.
+constexpr int ENUM_STAGE = 160;
+
+struct StageLds {
+  const double4 *v; const uint32_t *r; const int *t;
+  __device__ __forceinline__ double4 vec(int k) const { return v[k]; }
+  __device__ __forceinline__ uint32_t ref(int k) const { return r[k]; }
+  __device__ __forceinline__ int typ(int k) const { return t[k]; }
+};
+struct StageGlobal {
+  const double4 *v; const uint32_t *r; const int *t;
+  __device__ __forceinline__ double4 vec(int k) const { return v[k]; }
+  __device__ __forceinline__ uint32_t ref(int k) const { return r[k]; }
+  __device__ __forceinline__ int typ(int k) const { return t[k]; }
+};
+
+template <int NB, bool COUNT, class S>
+__device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, bool ordered, uint32_t r1, int t1,
+                                         int kbase, int out) {
+  const int lane = threadIdx.x & 63;
+  const double cut = a.cutoff;
+  const int npair = n * n;
+  int run = 0;
+  for (int q0 = 0; q0 < npair; q0 += 64) {
+    const int q = q0 + lane;
+    int k2 = 0, k3 = 0;
+    bool ok = q < npair;
+    double4 v2 = make_double4(0, 0, 0, 0), v3 = v2;
+    uint32_t r2 = 0, r3 = 0;
+    double r23 = 0.0;
+    if (ok) {
+      k2 = q / n;
+      k3 = q - k2 * n;
+      ok = k3 != k2;
+    }
+    if (ok) {
+      r2 = st.ref(k2);
+      r3 = st.ref(k3);
+      ok = !ordered || ref_atom(r2) <= ref_atom(r3);
+    }
+    if (ok) {
+      v2 = st.vec(k2);
+      v3 = st.vec(k3);
+      ok = v2.w < cut && v3.w < cut;
+    }
+    if (ok) {
+      r23 = dist3(v2, v3);
+      ok = r23 < cut;
+    }
+    int types[4] = {t1, 0, 0, 0};
+    if (ok) {
+      types[1] = st.typ(k2);
+      types[2] = st.typ(k3);
+    }
+    // pass 0 counts this lane's clusters, pass 1 writes them (not for COUNT)
+    int mine = 0;
+    for (int pass = 0; pass < (COUNT ? 1 : 2); ++pass) {
+      int base = 0;
+      if (pass == 1) {
+        int tot;
+        base = out + run + wave_excl_scan(mine, tot);
+        run += tot;
+      }
+      int m = 0;
+      if (ok) {
+        if (NB == 3) {
+          const double d[3] = {v2.w, v3.w, r23};
+          int id;
+          const int ct = cluster_type<3>(a, types, d, id);
+          if (ct >= 0) {
+            if (pass == 1) {
+              const uint32_t rr[3] = {r1, r2, r3};
+              const int *perm = a.id_atom_perm + id * 3;
+              for (int x = 0; x < 3; ++x) a.refs[3 * (base + m) + x] = rr[perm[x]];
+              a.key[base + m] = kbase + ct;
+            }
+            ++m;
+          }
+        } else {
+          const int p3 = ref_atom(r3);
+          for (int k4 = 0; k4 < n; ++k4) {
+            if (k4 == k2 || k4 == k3) continue;
+            const uint32_t r4 = st.ref(k4);
+            if (ordered && ref_atom(r4) < p3) continue;
+            const double4 v4 = st.vec(k4);
+            if (!(v4.w < cut)) continue;
+            const double r24 = dist3(v2, v4);
+            if (!(r24 < cut)) continue;
+            const double r34 = dist3(v3, v4);
+            if (!(r34 < cut)) continue;
+            types[3] = st.typ(k4);
+            const double d[6] = {v2.w, v3.w, v4.w, r23, r24, r34};
+            int id;
+            const int ct = cluster_type<4>(a, types, d, id);
+            if (ct < 0) continue;
+            if (pass == 1) {
+              const uint32_t rr[4] = {r1, r2, r3, r4};
+              const int *perm = a.id_atom_perm + id * 4;
+              for (int x = 0; x < 4; ++x) a.refs[4 * (base + m) + x] = rr[perm[x]];
+              a.key[base + m] = kbase + ct;
+            }
+            ++m;
+          }
+        }
+      }
+      mine = m;
+    }
+    if (COUNT) {
+      int tot;
+      wave_excl_scan(mine, tot);
+      run += tot;
     }
   }
+  return run;
+}
+
+template <int NB, bool COUNT>
+__global__ __launch_bounds__(256) void k_enum_wave(EnumArgs a) {
+  __shared__ double4 s_vec[4][ENUM_STAGE];
+  __shared__ uint32_t s_ref[4][ENUM_STAGE];
+  __shared__ int s_typ[4][ENUM_STAGE];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const int gi = blockIdx.x * 4 + wave;
+  const bool active = gi < a.natoms;
+  int b = 0, n = 0;
+  if (active) {
+    b = a.noff[gi];
+    n = a.noff[gi + 1] - b;
+  }
+  const bool staged = n <= ENUM_STAGE;
+  if (active && staged)
+    for (int k = lane; k < n; k += 64) {
+      s_vec[wave][k] = a.nvec[b + k];
+      s_ref[wave][k] = a.nlist[b + k];
+      s_typ[wave][k] = a.ntyp[b + k];
+    }
+  __syncthreads();
+  if (!active) return;
+  const int fidx = a.atom_frame[gi];
+  const DevFrame &f = a.frames[fidx];
+  const uint32_t r1 = make_ref(gi - f.atom_off, 0, 0, 0);
+  const int t1 = a.type[gi];
+  const int kbase = fidx * a.nkeys_type;
+  const int out = COUNT ? 0 : a.coff[gi];
+  int tot;
+  if (staged)
+    tot = enum_atom<NB, COUNT>(a, StageLds{s_vec[wave], s_ref[wave], s_typ[wave]}, n, !f.small, r1, t1, kbase, out);
+  else
+    tot = enum_atom<NB, COUNT>(a, StageGlobal{a.nvec + b, a.nlist + b, a.ntyp + b}, n, !f.small, r1, t1, kbase, out);
+  if (COUNT && lane == 0) a.count[gi] = tot;
 }
 
 __global__ void k_iota(int *v, int n) {
@@ -1205,51 +1378,59 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   if (!d_ncount || !d_noff) throw Error(CG_ERR_MEMORY, "neighbour counts");
   CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
   hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
-                     natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr,
-                     (int *)nullptr);
+                     (const int *)d_type, natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr,
+                     (double4 *)nullptr, (int *)nullptr, (int *)nullptr);
   exclusive_scan(c, d_ncount, d_noff, natoms);
   const int nnb = read_int(c, d_noff + natoms);                               // host sync 1
   const size_t nn1 = std::max(nnb, 1);
   uint32_t *d_nlist = c.b_nlist.as<uint32_t>(nn1);
   double4 *d_nvec = c.b_nvec.as<double4>(nn1);
-  int *d_nown = c.b_nown.as<int>(nn1);
+  int *d_nown = c.b_nown.as<int>(2 * nn1);
+  int *d_ntyp = d_nown + nn1;
   if (!d_nlist || !d_nvec || !d_nown) throw Error(CG_ERR_MEMORY, "neighbour list");
   hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
-                     natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_nvec, d_nown);
+                     (const int *)d_type, natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_nvec,
+                     d_nown, d_ntyp);
   // ---- per-family sublists (3B / 4B cutoffs below the list cutoff), sized
   // by the full list so that no host read is needed
-  const int *d_ntot = d_noff + natoms;
   const uint32_t *fl_list[3] = {d_nlist, d_nlist, d_nlist};
   const double4 *fl_vec[3] = {d_nvec, d_nvec, d_nvec};
   const int *fl_own[3] = {d_nown, d_nown, d_nown};
+  const int *fl_typ[3] = {d_ntyp, d_ntyp, d_ntyp};
   const int *fl_off[3] = {d_noff, d_noff, d_noff};
-  const int *fl_tot[3] = {d_ntot, d_ntot, d_ntot};
+  const dim3 wg((unsigned)((natoms + 1 + 3) / 4)), wb(256);  // one wave per atom (+1 for the tail slot)
   for (int w = 1; w < 3; ++w) {
     const DevFamily &F = c.fam[w];
     if (F.ntypes == 0 || F.ncols == 0 || nnb == 0 || !(F.cutoff < mx)) continue;
     DBuf &sb = c.b_sub[w - 1];
-    // layout: vec [nn1] (double4) | list [nn1] | own [nn1] | cnt [natoms+1] | off [natoms+1]
-    char *base = static_cast<char *>(sb.get(nn1 * (sizeof(double4) + 2 * sizeof(int)) + 2 * ((size_t)natoms + 1) * sizeof(int) + 256));
+    // layout: vec [nn1] (double4) | list [nn1] | own [nn1] | typ [nn1] | cnt [natoms+1] | off [natoms+1]
+    char *base = static_cast<char *>(
+        sb.get(nn1 * (sizeof(double4) + 3 * sizeof(int)) + 2 * ((size_t)natoms + 1) * sizeof(int) + 256));
     if (!base) throw Error(CG_ERR_MEMORY, "neighbour sublist");
     double4 *svec = reinterpret_cast<double4 *>(base);
     uint32_t *slist = reinterpret_cast<uint32_t *>(svec + nn1);
     int *sown = reinterpret_cast<int *>(slist + nn1);
-    int *scnt = sown + nn1;
+    int *styp = sown + nn1;
+    int *scnt = styp + nn1;
     int *soff = scnt + natoms + 1;
-    hipLaunchKernelGGL(k_sub_count, dim3((natoms + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, d_noff, d_nvec, natoms,
-                       F.cutoff, scnt);
+    hipLaunchKernelGGL(k_sub<true>, wg, wb, 0, c.stream, d_noff, d_nlist, d_nvec, d_nown, d_ntyp, natoms, F.cutoff,
+                       scnt, (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr,
+                       (int *)nullptr);
     exclusive_scan(c, scnt, soff, natoms);
-    hipLaunchKernelGGL(k_sub_fill, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_noff, d_nlist, d_nvec,
-                       d_nown, natoms, F.cutoff, soff, slist, svec, sown);
-    fl_list[w] = slist; fl_vec[w] = svec; fl_own[w] = sown; fl_off[w] = soff; fl_tot[w] = soff + natoms;
+    hipLaunchKernelGGL(k_sub<false>, wg, wb, 0, c.stream, d_noff, d_nlist, d_nvec, d_nown, d_ntyp, natoms, F.cutoff,
+                       (int *)nullptr, (const int *)soff, slist, svec, sown, styp);
+    fl_list[w] = slist; fl_vec[w] = svec; fl_own[w] = sown; fl_typ[w] = styp; fl_off[w] = soff;
   }
-  // ---- per-family cluster counts (one thread per neighbour entry)
+  // ---- per-family cluster counts: 2-body one thread per neighbour entry,
+  // 3/4-body one wave per atom over its LDS-staged list
   FamOut *fo = B.fo;
-  int *d_ccount = c.b_ccount.as<int>(3 * (nn1 + 1));
-  int *d_coff = c.b_coff.as<int>(3 * (nn1 + 1));
+  const size_t cslot = std::max(nn1, (size_t)natoms) + 1;
+  int *d_ccount = c.b_ccount.as<int>(3 * cslot);
+  int *d_coff = c.b_coff.as<int>(3 * cslot);
   if (!d_ccount || !d_coff) throw Error(CG_ERR_MEMORY, "cluster counts");
   EnumArgs ea[3] = {};
   bool active[3] = {false, false, false};
+  int nitems[3] = {0, 0, 0};  // count slots: entries (2B) or atoms (3B/4B)
   const dim3 eg((unsigned)((nnb + TB - 1) / TB)), eb(TB);
   for (int w = 0; w < 3; ++w) {
     const DevFamily &F = c.fam[w];
@@ -1257,24 +1438,26 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     active[w] = true;
     EnumArgs &E = ea[w];
     E.frames = d_frames; E.atom_frame = d_af; E.type = d_type; E.noff = fl_off[w]; E.nlist = fl_list[w];
-    E.nvec = fl_vec[w]; E.nown = fl_own[w]; E.nnb = nnb; E.ntot = fl_tot[w]; E.ntypes_atom = m.ntypes;
-    E.nkeys_type = F.ntypes;
+    E.nvec = fl_vec[w]; E.nown = fl_own[w]; E.ntyp = fl_typ[w]; E.nnb = nnb; E.ntot = d_noff + natoms;
+    E.natoms = natoms; E.ntypes_atom = m.ntypes; E.nkeys_type = F.ntypes;
     E.cutoff = F.cutoff;
-    E.count = d_ccount + w * (nn1 + 1);
-    E.coff = d_coff + w * (nn1 + 1);
+    E.count = d_ccount + w * cslot;
+    E.coff = d_coff + w * cslot;
     E.id_map = F.id_map; E.id_atom_perm = F.id_atom_perm; E.id_pair_map = F.id_pair_map;
     E.smin = F.smin; E.smax = F.smax; E.int_pair_map = c.d_int_pair_map;
     E.md = md;
-    CG_HIP(hipMemsetAsync(E.count + nnb, 0, sizeof(int), c.stream));
+    nitems[w] = w == 0 ? nnb : natoms;
+    CG_HIP(hipMemsetAsync(E.count + nitems[w], 0, sizeof(int), c.stream));
     if (w == 0) hipLaunchKernelGGL((k_enum<2, false>), eg, eb, 0, c.stream, E);
-    else if (w == 1) hipLaunchKernelGGL((k_enum<3, false>), eg, eb, 0, c.stream, E);
-    else hipLaunchKernelGGL((k_enum<4, false>), eg, eb, 0, c.stream, E);
+    else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, true>), wg, wb, 0, c.stream, E);
+    else hipLaunchKernelGGL((k_enum_wave<4, true>), wg, wb, 0, c.stream, E);
     CG_HIP(hipGetLastError());
-    exclusive_scan(c, E.count, const_cast<int *>(E.coff), nnb);
+    exclusive_scan(c, E.count, const_cast<int *>(E.coff), nitems[w]);
   }
   int tot[3] = {0, 0, 0};
   for (int w = 0; w < 3; ++w)
-    if (active[w]) CG_HIP(hipMemcpyAsync(&tot[w], ea[w].coff + nnb, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    if (active[w])
+      CG_HIP(hipMemcpyAsync(&tot[w], ea[w].coff + nitems[w], sizeof(int), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipStreamSynchronize(c.stream));                                    // host sync 2
   // ---- fill, bucket by (frame, type), bucket bounds
   std::vector<int> hbounds[3];
@@ -1304,8 +1487,8 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     const int *keys = E.key;
     if (ncl > 0) {
       if (w == 0) hipLaunchKernelGGL((k_enum<2, true>), eg, eb, 0, c.stream, E);
-      else if (w == 1) hipLaunchKernelGGL((k_enum<3, true>), eg, eb, 0, c.stream, E);
-      else hipLaunchKernelGGL((k_enum<4, true>), eg, eb, 0, c.stream, E);
+      else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, false>), wg, wb, 0, c.stream, E);
+      else hipLaunchKernelGGL((k_enum_wave<4, false>), wg, wb, 0, c.stream, E);
       CG_HIP(hipGetLastError());
       if (sort) {
         // stable radix sort by (frame, type): clusters of a bucket keep their

@@ -115,6 +115,12 @@ struct Ctx {
   long long ncl[3] = {0, 0, 0};  // clusters per family in the last batch
 };
 
+// Defined in assemble.hip.
+void ctx_init(Ctx &c, const cg_model *m, int device);
+void ctx_free(Ctx &c);
+void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
+                 double *vir);
+
 }  // namespace cg
 
 struct cg_ctx {

@@ -245,6 +245,29 @@ cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f,
 cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, int nframes,
                          double *F, double *E, double *virial9);
 
+/* ---------------------------------------------------- force-evaluation server
+ * No reference counterpart (the reference evaluates one configuration per MD
+ * step through ZCalc, src/functions.C:815-891).  Configurations are submitted
+ * from any thread; worker threads (one device context and HIP stream each)
+ * take every queued request that fits a batch (max_batch_atoms atoms, and the
+ * estimated device bytes of all batches in flight within hbm_budget_bytes;
+ * 0 = 4 GiB) and evaluate it with cg_force_batch, so batches overlap on the
+ * device and new requests are admitted as soon as a worker and budget free
+ * up.  A request larger than the budget runs alone.  x is copied. */
+typedef struct cg_server cg_server;
+cg_status cg_serve_create(const cg_model *m, int device, const double *x, int nworkers,
+                          size_t hbm_budget_bytes, int max_batch_atoms, cg_server **out);
+/* Copies the frame's positions, types and cell; returns a ticket. */
+cg_status cg_serve_submit(cg_server *s, const cg_frame *f, int64_t *ticket);
+/* done = 1 when the result is ready (wait != 0 blocks until then); the
+ * outputs are written and the ticket released on the call that sees done.
+ * latency_ms: submit to completion, measured in the library. */
+cg_status cg_serve_poll(cg_server *s, int64_t ticket, int wait, int *done, double *F, double *E,
+                        double *virial9, double *latency_ms);
+cg_status cg_serve_stats(cg_server *s, int64_t *batches, int64_t *requests);
+/* Finishes every queued request, then stops the workers. */
+void cg_serve_destroy(cg_server *s);
+
 #ifdef __cplusplus
 }
 #endif
