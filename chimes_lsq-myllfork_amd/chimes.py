@@ -32,7 +32,7 @@ class ClusterSet(C.Structure):
 
 
 class Model(C.Structure):
-    _fields_ = [("ntypes", C.c_int), ("type_names", C.c_char_p), ("npairs", C.c_int),
+    _fields_ = [("ntypes", C.c_int), ("type_names", C.c_void_p), ("npairs", C.c_int),
                 ("pair_t1", _ip), ("pair_t2", _ip), ("int_pair_map", _ip),
                 ("pair_smin", _dp), ("pair_smax", _dp), ("pair_lambda", _dp), ("pair_xavg", _dp),
                 ("pair_xdiff", _dp), ("transform", C.c_int), ("order2b", C.c_int),
@@ -71,6 +71,8 @@ def lib(path=None):
         L.cg_last_error.restype = C.c_char_p
         L.cg_last_error.argtypes = [C.c_void_p]
         L.cg_setup_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.cg_params_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.cg_params_coeffs.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.cg_setup_free.argtypes = [C.c_void_p]
         L.cg_setup_model.restype = C.POINTER(Model)
         L.cg_setup_model.argtypes = [C.c_void_p]
@@ -103,6 +105,7 @@ def lib(path=None):
                                 C.POINTER(FitResult), _dp, _dp])
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
+        opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
         opt("cg_serve_create", [C.POINTER(Model), C.c_int, _dp, C.c_int, C.c_size_t, C.c_int,
                                 C.POINTER(C.c_void_p)])
         opt("cg_serve_submit", [C.c_void_p, C.POINTER(Frame), C.POINTER(C.c_int64)])
@@ -159,14 +162,31 @@ class PyFrame:
 class Setup:
     """Parsed fm_setup.in (host C++ parser behind the C ABI)."""
 
-    def __init__(self, path):
+    def __init__(self, path, params=False):
+        """path: fm_setup.in, or a params.txt when params=True (model + coefficients)."""
         self.L = lib()
         h = C.c_void_p()
-        check(self.L.cg_setup_read(path.encode(), C.byref(h)))
+        reader = self.L.cg_params_read if params else self.L.cg_setup_read
+        check(reader(path.encode(), C.byref(h)))
         self.h = h
         self.path = path
         self.model_ptr = self.L.cg_setup_model(h)
         self.model = self.model_ptr.contents
+
+    @classmethod
+    def from_params(cls, path):
+        return cls(path, params=True)
+
+    def coeffs(self):
+        """(x in A column order, penalty_dist, penalty_scale) of a params.txt setup."""
+        x = np.zeros(self.model.ncols)
+        d, sc = C.c_double(), C.c_double()
+        check(self.L.cg_params_coeffs(self.h, dptr(x), C.byref(d), C.byref(sc)))
+        return x, d.value, sc.value
+
+    def type_names(self):
+        raw = C.string_at(self.model.type_names, 8 * self.model.ntypes)
+        return [raw[8 * i:8 * i + 8].split(b"\0")[0].decode() for i in range(self.model.ntypes)]
 
     def __del__(self):
         try:
@@ -288,6 +308,9 @@ class Device:
         f = frame.c()
         check(self.L.cg_force(self.h, dptr(x), C.byref(f), dptr(F), C.byref(E), dptr(V)), self.h)
         return F, E.value, V
+
+    def set_penalty(self, dist, scale):
+        check(self.L.cg_set_penalty(self.h, dist, scale), self.h)
 
     def force_batch(self, x, frames):
         """Forces (list of [natoms*3]), energies [n] and virials [n, 9] of independent configurations."""

@@ -1723,7 +1723,13 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
   CG_HIP(hipMemcpyAsync(ev.data(), d_EV, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipStreamSynchronize(c.stream));
   for (int fi = 0; fi < nframes; ++fi) {
-    if (E) E[fi] = ev[10 * fi];
+    double eo = 0.0;  // per-type energy offsets: atom count x offset coefficient
+    if (m.n_offset_cols > 0) {
+      std::vector<int> cnt(m.ntypes, 0);
+      for (int a = 0; a < frames[fi].natoms; ++a) ++cnt[frames[fi].type[a]];
+      for (int t = 0; t < m.n_offset_cols && t < m.ntypes; ++t) eo += cnt[t] * x_host[m.ncols_short + t];
+    }
+    if (E) E[fi] = ev[10 * fi] + eo;
     if (vir) for (int q = 0; q < 9; ++q) vir[9 * fi + q] = ev[10 * fi + 1 + q];
   }
 }
@@ -1855,6 +1861,13 @@ cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3) {
     if (ms_family3) ms_family3[w] = ctx->c.ms_fam[w];
     if (clusters3) clusters3[w] = ctx->c.ncl[w];
   }
+  return CG_OK;
+}
+
+cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale) {
+  if (!ctx || !(dist >= 0.0) || !(scale >= 0.0)) { cg::set_error("bad penalty"); return CG_ERR_ARG; }
+  ctx->c.penalty_dist = dist;
+  ctx->c.penalty_scale = scale;
   return CG_OK;
 }
 

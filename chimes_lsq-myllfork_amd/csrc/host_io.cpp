@@ -6,6 +6,7 @@
 // and CLEANUP_FILES (src/A_Matrix.C:210-760), print_param_header / print_map_file
 // (src/chimes_lsq.C:562-716), FCUT::print_header (src/Fcut.C:169-178) and
 // CLUSTER[_LIST]::print_header / print_special (src/Cluster.C:423-508,1636-1697).
+#include <algorithm>
 #include <cmath>
 #include <ctime>
 #include <cstdio>
@@ -67,6 +68,23 @@ cg_status cg_setup_read(const char *path, cg_setup **out) {
     *out = new cg_setup{read_setup(path)};
     return CG_OK;
   } catch (const Error &e) { return fail(e); } catch (const std::exception &e) { return fail(e); }
+}
+
+cg_status cg_params_read(const char *path, cg_setup **out) {
+  if (!path || !out) { g_err = "null argument"; return CG_ERR_ARG; }
+  try {
+    *out = new cg_setup{read_params(path)};
+    return CG_OK;
+  } catch (const Error &e) { return fail(e); } catch (const std::exception &e) { return fail(e); }
+}
+
+cg_status cg_params_coeffs(const cg_setup *s, double *x, double *penalty_dist, double *penalty_scale) {
+  if (!s || !x) { g_err = "null argument"; return CG_ERR_ARG; }
+  if (s->s->coeffs.empty()) { g_err = "this setup was not read from a params file"; return CG_ERR_ARG; }
+  std::copy(s->s->coeffs.begin(), s->s->coeffs.end(), x);
+  if (penalty_dist) *penalty_dist = s->s->penalty_dist;
+  if (penalty_scale) *penalty_scale = s->s->penalty_scale;
+  return CG_OK;
 }
 
 void cg_setup_free(cg_setup *s) {

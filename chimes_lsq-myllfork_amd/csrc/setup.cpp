@@ -318,6 +318,11 @@ void parse_fcut(Setup &s) {
 
 }  // namespace
 
+// Token helpers shared with the params.txt reader.
+std::vector<std::string> split_tokens(const std::string &line) { return tokenize(line); }
+int parse_int(const std::string &v) { return to_int(v); }
+double parse_double(const std::string &v) { return to_double(v); }
+
 int type_of(const Setup &s, const std::string &name) {
   for (int i = 0; i < s.ntypes; ++i)
     if (s.type_names[i] == name) return i;
@@ -327,6 +332,10 @@ int type_of(const Setup &s, const std::string &name) {
 Setup *read_setup(const std::string &path) {
   std::ifstream in(path);
   if (!in) throw Error(CG_ERR_INPUT, "ERROR: Cannot open input file: " + path);
+  return read_setup_from(in, path);
+}
+
+Setup *read_setup_from(std::istream &in, const std::string &path) {
   auto s = new Setup;
   try {
     s->path = path;

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include <fstream>
+#include <istream>
 
 #include "../../include/chimes_gpu.h"
 
@@ -96,6 +97,11 @@ struct Setup {
     std::vector<double> smin, smax, xavg, xdiff;
   } f3, f4;
 
+  // coefficients read from a params.txt (A column order) and the chimes_md
+  // 2-body penalty settings found there (defaults of src/chimes_md.C:1747-1748)
+  std::vector<double> coeffs;
+  double penalty_dist = 0.01, penalty_scale = 1.0e4;
+
   // text output state
   std::ofstream fA, fb, fbl, fna;
   int64_t data_count = 0;
@@ -104,6 +110,12 @@ struct Setup {
 
 void set_error(const std::string &m);  // thread-local message for cg_last_error
 Setup *read_setup(const std::string &path);
+Setup *read_setup_from(std::istream &in, const std::string &path);
+Setup *read_params(const std::string &path);  // params.txt -> model + coefficients
+// fm_setup.in token rules ('!' and '##' comments stripped, whitespace split)
+std::vector<std::string> split_tokens(const std::string &line);
+int parse_int(const std::string &v);
+double parse_double(const std::string &v);
 void flatten(Setup &s);
 int type_of(const Setup &s, const std::string &name); // -1 if unknown
 void write_header(const Setup &s, const std::string &dir);

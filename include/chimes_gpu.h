@@ -140,6 +140,15 @@ typedef struct cg_frame {
 typedef struct cg_setup cg_setup;
 cg_status cg_setup_read(const char *fm_setup_path, cg_setup **out);
 void cg_setup_free(cg_setup *s);
+/* Model + coefficients from a params.txt (the file chimes_md reads,
+ * src/chimes_md.C:1474-2179): the model is rebuilt as chimes_lsq would for the
+ * same species, pair table, orders, cutoffs and exclusions, and every
+ * coefficient block is checked against it.  Coulomb, over-coordination and
+ * 1B/2B exclusion terms are refused (CG_ERR_UNSUPPORTED). */
+cg_status cg_params_read(const char *path, cg_setup **out);
+/* Coefficients in A column order (x[ncols]; energy offsets last) and the
+ * file's 2-body penalty settings. */
+cg_status cg_params_coeffs(const cg_setup *s, double *x, double *penalty_dist, double *penalty_scale);
 const cg_model *cg_setup_model(const cg_setup *s);
 int cg_setup_nframes(const cg_setup *s);
 /* rows of A produced by a frame (3N + stress rows + energy rows) */
@@ -244,6 +253,11 @@ cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f,
  * The coefficient tensors built from x stay on the device until x changes. */
 cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, int nframes,
                          double *F, double *E, double *virial9);
+
+/* 2-body penalty of the force evaluation, A_p (s_min + d_p - r)^3 for
+ * r < s_min + d_p (chimes_md defaults d_p = 0.01, A_p = 1e4;
+ * src/chimes_md.C:1747-1748, overridden by PAIR CHEBYSHEV PENALTY lines). */
+cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale);
 
 /* ---------------------------------------------------- force-evaluation server
  * No reference counterpart (the reference evaluates one configuration per MD

@@ -641,6 +641,11 @@ int orc_force(const cg_model *m, const cg_frame *f, const double *x, double pena
         sv[3] -= coeff * rab[0] * rab[1] / r; sv[4] -= coeff * rab[0] * rab[2] / r; sv[5] -= coeff * rab[1] * rab[2] / r;
       }
     }
+  /* per-atom energy offsets (ZCalc with INCLUDE_ATOM_OFFSETS, its default:
+   * src/functions.C:843-852, src/input.C:2097); they are the energy-offset
+   * columns of x, after the short-range ones */
+  if (m->n_offset_cols > 0)
+    for (int a = 0; a < f->natoms; ++a) e += x[m->ncols_short + f->type[a]];
   *E = e;
   /* xx xy xz / yx yy yz / zx zy zz */
   vir9[0] = sv[0]; vir9[1] = sv[3]; vir9[2] = sv[4];

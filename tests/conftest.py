@@ -111,3 +111,30 @@ def synth_setup(ch, synth, tmp_path, gen):
     fr = [ch.PyFrame(f["natoms"], f["xyz"], f["types"], f["cell"], f["forces"], f["stress"], f["energy"], f["index"])
           for f in frames]
     return s, fr
+
+
+def md_dir(name):
+    return os.path.join(GOLDEN, "lsq", "..", "md", name)
+
+
+MD_CASES = ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b", "stress-and-ener-4b2"]
+
+
+def md_case(ch, name, tmp_path):
+    """(Setup from params.txt, PyFrame of the first coordinate frame, golden forces)."""
+    import gzip
+    d = md_dir(name)
+    p = os.path.join(str(tmp_path), "params.txt")
+    with gzip.open(os.path.join(d, "params.txt.gz"), "rb") as g:
+        open(p, "wb").write(g.read())
+    s = ch.Setup.from_params(p)
+    names = s.type_names()
+    lines = open(os.path.join(d, "frame.xyzf")).read().split("\n")
+    n = int(lines[0].split()[0])
+    box = [float(v) for v in lines[1].split()[:3]]
+    xyz = np.array([[float(v) for v in lines[2 + i].split()[1:4]] for i in range(n)])
+    typ = np.array([names.index(lines[2 + i].split()[0]) for i in range(n)], dtype=np.int32)
+    f = ch.PyFrame(n, xyz, typ, np.diag(box))
+    ref = np.array([float(v) for v in open(os.path.join(d, "forceout.txt")).read().split()])
+    assert ref.size == 3 * n
+    return s, f, ref

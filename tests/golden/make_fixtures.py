@@ -14,7 +14,12 @@ expected outputs), trimmed to the frames the tests need:
                            12 golden A rows (12x1308).
   lsq/fitstrs/             last frame (96 atoms, diagonal stress rows) and the
                            176 golden A rows that survive in A.txt.ab.
+  md/<test>/               force-evaluation cases of test_suite-md without
+                           Coulomb terms: params.txt (gzip), the first frame
+                           of the run's coordinate file and the reference's
+                           forceout.txt (forces, kcal/mol/A, 14 digits).
 """
+import gzip
 import hashlib
 import json
 import os
@@ -26,6 +31,9 @@ import numpy as np
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 LSQ = os.path.join(REF, "test_suite-lsq")
+
+MD_TESTS = ["verify-lsq-forces-special3b", "verify-lsq-forces-special4b", "verify-lsq-forces-stress-and-ener-2b1",
+            "verify-lsq-forces-stress-and-ener-4b", "verify-lsq-forces-stress-and-ener-4b2"]
 
 HEADER_TESTS = ["nonorth2", "test_4atoms", "test_4atoms.2", "fitstrs", "fitstrs3b", "h2o-invr", "special3b",
                 "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2", "stress-and-ener-2b3",
@@ -114,6 +122,38 @@ def main():
         open(os.path.join(out, "fm_setup.in"), "w").write(text)
         rows = tail_rows(os.path.join(src, "correct_output", apiece))
         np.savez_compressed(os.path.join(out, "golden.npz"), A=rows, row0=frame_rows - rows.shape[0])
+    # force-evaluation cases: params.txt + first coordinate frame + forceout.txt
+    md = os.path.join(REF, "test_suite-md")
+    for t in MD_TESTS:
+        src = os.path.join(md, t)
+        out = os.path.join(HERE, "md", t.replace("verify-lsq-forces-", ""))
+        os.makedirs(out, exist_ok=True)
+        run = open(os.path.join(src, "run_md.in")).read()
+        crd = re.search(r"# CRDFILE #[^\n]*\n\s*(\S+)", run).group(1)
+        open(os.path.join(out, "frame.xyzf"), "w").write(frames_of(os.path.join(src, crd))[0])
+        with open(os.path.join(out, "params.txt.gz"), "wb") as raw:
+            with gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0, filename="") as g:
+                g.write(open(os.path.join(src, "correct_output", "params.txt"), "rb").read())
+        open(os.path.join(out, "forceout.txt"), "w").write(
+            open(os.path.join(src, "correct_output", "forceout.txt")).read())
+        # |E - E_QM| / N printed by the reference run (6 decimals), when it fits energies
+        m = re.search(r"Absolute energy error = (\S+) kcal/mol/atom",
+                      open(os.path.join(src, "correct_output", "run_md.out")).read())
+        meta = {"abs_energy_error_per_atom": float(m.group(1)) if m else None}
+        if m:
+            # the run compares with the energy row of its CMPRFRC file (force.txt, the
+            # fit's A x at 7 digits): frame 0's first value repeated on three rows
+            # after the 3N force rows and the stress rows
+            n = int(open(os.path.join(out, "frame.xyzf")).readline().split()[0])
+            ft = os.path.join(src, "force.txt")
+            if not os.path.exists(ft):
+                ft = os.path.join(src, "correct_output", "force.txt")
+            y = [float(v) for v in open(ft).read().split()[:3 * n + 12]]
+            for k in (3 * n, 3 * n + 3, 3 * n + 9):
+                if y[k] == y[k + 1] == y[k + 2]:
+                    meta["forcetxt_energy"] = y[k]
+                    break
+        json.dump(meta, open(os.path.join(out, "expected.json"), "w"), indent=1, sort_keys=True)
     print("fixtures written to", HERE)
 
 

@@ -71,3 +71,26 @@ def test_force_batch_matches_single_calls(ch, synth, oracle, tmp_path):
         assert Eb[i] == pytest.approx(Eo, rel=1e-10, abs=1e-10 * abs(Eo))
         assert np.allclose(Vb[i], Vo, rtol=1e-10, atol=1e-10 * np.abs(Vo).max())
     dev.close()
+
+
+@pytest.mark.parametrize("case", ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b",
+                                  "stress-and-ener-4b2"])
+def test_force_matches_reference_forceout(ch, tmp_path, case):
+    """GPU forces with a reference params.txt reproduce chimes_md's forceout.txt."""
+    from conftest import md_case
+    s, f, ref = md_case(ch, case, tmp_path)
+    x, pd, ps = s.coeffs()
+    dev = ch.Device(s)
+    dev.set_penalty(pd, ps)
+    F, E, V = dev.force(x, f)
+    dev.close()
+    assert np.max(np.abs(F - ref)) <= 1e-10 * np.abs(ref).max()
+    # energy incl. per-atom offsets, as the reference run checked it: |E - E_cmp| / N
+    # to 6 decimals, E_cmp = energy row of its force.txt
+    import json
+    import os
+    from conftest import md_dir
+    meta = json.load(open(os.path.join(md_dir(case), "expected.json")))
+    if meta["abs_energy_error_per_atom"] is not None:
+        assert round(abs(E - meta["forcetxt_energy"]) / f.natoms, 6) == pytest.approx(
+            meta["abs_energy_error_per_atom"], abs=1e-12)

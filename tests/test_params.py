@@ -1,0 +1,53 @@
+"""params.txt reader (the file chimes_md reads): model rebuilt from its header,
+coefficients placed in A column order, checked against the reference's own
+fitted values and, through the oracle's force evaluation, its forceout.txt."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import MD_CASES, golden_dir, md_case
+
+
+def test_nonorth2_coefficients_match_test_suite_params(ch):
+    d = golden_dir("nonorth2")
+    s = ch.Setup.from_params(os.path.join(d, "params.txt"))
+    x, pd, ps = s.coeffs()
+    xr = np.array([float(l.split()[1]) for l in open(os.path.join(d, "test_suite_params.txt")) if l.strip()])
+    assert x.shape == xr.shape
+    assert np.array_equal(x, xr)
+    assert (pd, ps) == (0.01, 1.0e4)
+    # the rebuilt model is the one the fm_setup.in of the fit describes
+    sf = ch.Setup(os.path.join(d, "fm_setup.in"))
+    assert sf.model.ncols == s.model.ncols and sf.model.order2b == s.model.order2b
+
+
+@pytest.mark.parametrize("case", MD_CASES)
+def test_params_reader_oracle_forces(ch, oracle, tmp_path, case):
+    """Oracle forces with the coefficients read from params.txt reproduce the
+    reference chimes_md forceout.txt (14 digits)."""
+    s, f, ref = md_case(ch, case, tmp_path)
+    x, pd, ps = s.coeffs()
+    F, E, V = oracle.force(s, f, x, penalty_scale=ps, penalty_dist=pd)
+    assert np.max(np.abs(F - ref)) <= 1e-10 * np.abs(ref).max()
+    # energy (with the per-atom offsets): the reference run printed
+    # |E - E_cmp| / N to 6 decimals, E_cmp being the energy row of its force.txt
+    import json
+    from conftest import md_dir
+    meta = json.load(open(os.path.join(md_dir(case), "expected.json")))
+    if meta["abs_energy_error_per_atom"] is not None:
+        err = abs(E - meta["forcetxt_energy"]) / f.natoms
+        assert round(err, 6) == pytest.approx(meta["abs_energy_error_per_atom"], abs=1e-12)
+
+
+def test_params_reader_refuses_coulomb(ch, tmp_path):
+    import gzip
+    from conftest import md_dir
+    text = gzip.open(os.path.join(md_dir("special4b"), "params.txt.gz"), "rt").read()
+    p = tmp_path / "params.txt"
+    p.write_text(text.replace("USECOUL: false", "USECOUL: true"))
+    with pytest.raises(RuntimeError, match="UNSUPPORTED"):
+        ch.Setup.from_params(str(p))
+    p.write_text(text.replace("   PAIRS: OO OO OO UNIQUE: 7 TOTAL: 20", "   PAIRS: OO OO OO UNIQUE: 8 TOTAL: 20"))
+    with pytest.raises(RuntimeError, match="UNIQUE/TOTAL"):
+        ch.Setup.from_params(str(p))
