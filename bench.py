@@ -7,13 +7,17 @@ seeded inputs.  `value` is A rows per second for that whole batch, timed around
 `cg_assemble` (frames handed over, neighbour build, cluster enumeration,
 2/3/4-body assembly, A resident in HBM afterwards).
 
-Also reported: the FP32 assembly rate, the fit time of the FP64 path and of the
-FP32 + split-BF16x3 Gram + FP64 refinement path with their force-RMS agreement,
-force-evaluation latency p50/p99 of 1k-atom C/H/N/O configurations (orders
-12/6/3), the 4-body kernel's roofline position, and the CPU oracle timed on
-the host cores on a bounded sample.
+Also reported: the FP32 assembly rate; the fit time of the FP64 path, of the
+FP32-A path (FP64-product Gram + FP64 refinement) and of the split-BF16x3 Gram
+variant, with their force-RMS agreement; force-evaluation latency p50/p99 of
+1k-atom C/H/N/O configurations (orders 12/6/3), one call at a time and as a
+Poisson stream of 64-4096-atom requests through the batching server; the
+4-body kernel's roofline position (HIP events on the context stream; HBM
+traffic from the committed counter summary); and the CPU oracle timed on the
+host cores on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames F] [--no-cpu]
+    python bench.py --counters     (one assembly + calibration copy, for rocprofv3 --pmc)
 """
 import argparse
 import ctypes
@@ -30,6 +34,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "chimes_lsq-myllfork_amd")
 METRIC = "A-matrix rows/s + fit time at FP64-oracle tol vs plain FP64; force-eval p50/p99"
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+COPY_PROBE_BYTES = 1 << 30  # calibration copy of the counter passes
+COUNTERS = os.path.join(ROOT, "profiles", "counters_k_assemble_4b_carbon_100x512.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the 4-body kernel from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the calibration copy), or None."""
+    try:
+        for line in open(COUNTERS):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def _load(name, path):
@@ -138,6 +156,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="one FP64 assembly of the workload plus a 1 GiB device copy, for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the design-matrix path of this build is single-device "
@@ -151,6 +171,12 @@ def main():
     fr = make_frames(ch, frames)
     rows = sum(setup.frame_rows(f.natoms, f.index) for f in fr)
     dev = ch.Device(setup)
+    if args.counters:
+        dev.assemble(fr, prec=ch.CG_FP64, copy=False)
+        dev.copy_probe(COPY_PROBE_BYTES)
+        dev.close()
+        log("counter pass done: assembly + %d-byte copy" % COPY_PROBE_BYTES)
+        return
 
     def step(prec):
         dev.assemble(fr, prec=prec, copy=False)
@@ -202,15 +228,23 @@ def main():
         x32, r32 = dev.fit(b, eps=1e-5, refine=4)
         t_fit32 = time.perf_counter() - t0
         t_asm32 = rows / value32
+        t0 = time.perf_counter()
+        xb, rb = dev.fit(b, eps=1e-5, refine=4, gram=ch.CG_GRAM_BF16X3)
+        t_fitb = time.perf_counter() - t0
         extra["fit"] = {
             "fp64": {"fit_s": t_fit64, "gram_ms": r64.ms_gram, "solve_ms": r64.ms_solve, "refine_ms": r64.ms_refine,
                      "rms": r64.rms, "nvars": r64.nvars, "assembly_plus_fit_rows_per_s": rows / (t_asm64 + t_fit64)},
-            "fp32_bf16x3_refined": {"fit_s": t_fit32, "gram_ms": r32.ms_gram, "solve_ms": r32.ms_solve,
-                                    "refine_ms": r32.ms_refine, "rms": r32.rms, "nvars": r32.nvars,
-                                    "rms_rel_err_vs_fp64": abs(r32.rms - r64.rms) / r64.rms,
-                                    "assembly_plus_fit_rows_per_s": rows / (t_asm32 + t_fit32)},
+            "fp32_A_fp64_gram_refined": {
+                "fit_s": t_fit32, "gram_ms": r32.ms_gram, "solve_ms": r32.ms_solve, "refine_ms": r32.ms_refine,
+                "rms": r32.rms, "nvars": r32.nvars, "rms_rel_err_vs_fp64": abs(r32.rms - r64.rms) / r64.rms,
+                "coef_rel_err_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
+                "assembly_plus_fit_rows_per_s": rows / (t_asm32 + t_fit32)},
+            "fp32_A_bf16x3_gram_refined": {
+                "fit_s": t_fitb, "gram_ms": rb.ms_gram, "rms": rb.rms, "nvars": rb.nvars, "iters": rb.iters,
+                "rms_rel_err_vs_fp64": abs(rb.rms - r64.rms) / r64.rms},
         }
-        log("fit FP64 %.3f s rms %.6e | FP32+refine %.3f s rms %.6e" % (t_fit64, r64.rms, t_fit32, r32.rms))
+        log("fit FP64 %.3f s rms %.6e | FP32 A + FP64 Gram + refine %.3f s rms %.6e | split-BF16x3 Gram rms %.6e" % (
+            t_fit64, r64.rms, t_fit32, r32.rms, rb.rms))
     dev.close()
     if not args.no_serve:
         one = serving_latency(ch, synth)
@@ -234,7 +268,8 @@ def main():
                                    args.frames, rows, setup.model.ncols),
                    "frames": args.frames, "natoms": 512, "rows": rows, "cols": int(setup.model.ncols)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
+                     "pipe": "FP64 VALU (same 78.6 TFLOP/s dense peak as FP64 MFMA)",
                      "kernel": "k_assemble<4,5,0,double> (4-body rows)", "kernel_ms": ms4,
                      "algorithmic_flops": flops4},
         "cpu_baseline": cpu,

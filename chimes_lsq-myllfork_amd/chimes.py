@@ -106,6 +106,7 @@ def lib(path=None):
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
         opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
+        opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
         opt("cg_serve_create", [C.POINTER(Model), C.c_int, _dp, C.c_int, C.c_size_t, C.c_int,
                                 C.POINTER(C.c_void_p)])
         opt("cg_serve_submit", [C.c_void_p, C.POINTER(Frame), C.POINTER(C.c_int64)])
@@ -308,6 +309,9 @@ class Device:
         f = frame.c()
         check(self.L.cg_force(self.h, dptr(x), C.byref(f), dptr(F), C.byref(E), dptr(V)), self.h)
         return F, E.value, V
+
+    def copy_probe(self, nbytes):
+        check(self.L.cg_copy_probe(self.h, nbytes), self.h)
 
     def set_penalty(self, dist, scale):
         check(self.L.cg_set_penalty(self.h, dist, scale), self.h)

@@ -1864,6 +1864,24 @@ cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3) {
   return CG_OK;
 }
 
+cg_status cg_copy_probe(cg_ctx *ctx, size_t bytes) {
+  if (!ctx || bytes == 0) { cg::set_error("bad argument"); return CG_ERR_ARG; }
+  try {
+    cg::Ctx &c = ctx->c;
+    CG_HIP(hipSetDevice(c.device));
+    char *buf = static_cast<char *>(c.b_tmp.get(2 * bytes));
+    if (!buf) throw cg::Error(CG_ERR_MEMORY, "copy probe buffers");
+    CG_HIP(hipMemsetAsync(buf, 1, bytes, c.stream));
+    CG_HIP(hipStreamSynchronize(c.stream));
+    CG_HIP(hipMemcpyAsync(buf + bytes, buf, bytes, hipMemcpyDeviceToDevice, c.stream));
+    CG_HIP(hipStreamSynchronize(c.stream));
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
 cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale) {
   if (!ctx || !(dist >= 0.0) || !(scale >= 0.0)) { cg::set_error("bad penalty"); return CG_ERR_ARG; }
   ctx->c.penalty_dist = dist;

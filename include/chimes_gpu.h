@@ -259,6 +259,11 @@ cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, i
  * src/chimes_md.C:1747-1748, overridden by PAIR CHEBYSHEV PENALTY lines). */
 cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale);
 
+/* Device-to-device copy of `bytes` bytes on the context stream (one kernel
+ * that reads and writes exactly `bytes`): the reference of known traffic
+ * used to calibrate HBM byte counters.  No reference counterpart. */
+cg_status cg_copy_probe(cg_ctx *ctx, size_t bytes);
+
 /* ---------------------------------------------------- force-evaluation server
  * No reference counterpart (the reference evaluates one configuration per MD
  * step through ZCalc, src/functions.C:815-891).  Configurations are submitted
