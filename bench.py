@@ -147,6 +147,63 @@ def serving_stream(ch, synth, n_req=300, qps=400.0, seed=106):
             "natoms": "64-4096 log-uniform", "orders": "12/6/3", "workers": 2}
 
 
+def assemble_rate(ch, dev, fr, rows, prec, steps=2):
+    dev.assemble(fr, prec=prec, copy=False)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.assemble(fr, prec=prec, copy=False)
+    dev.synchronize()
+    return rows / ((time.perf_counter() - t0) / steps)
+
+
+def config_c3(ch, synth, nframes=600):
+    """Precision form of the carbon workload at 600 frames: FP32 A + FP64-product
+    Gram + FP64 refinement against the plain FP64 path (assembly and fit)."""
+    text, frames, _ = synth.carbon(nframes=nframes, natoms=512, seed=103)
+    s = setup_from(ch, text)
+    fr = make_frames(ch, frames)
+    rows = sum(s.frame_rows(f.natoms, f.index) for f in fr)
+    dev = ch.Device(s)
+    out = {"workload": "carbon %d frames x 512 atoms, orders 16/9/5" % nframes, "rows": rows,
+           "cols": int(s.model.ncols)}
+    for prec, tag in [(ch.CG_FP64, "fp64"), (ch.CG_FP32, "fp32")]:
+        rate = assemble_rate(ch, dev, fr, rows, prec, steps=1)
+        _, b = dev.assemble(fr, prec=prec, copy=False)
+        dev.fit(b, eps=1e-5, refine=0)
+        t0 = time.perf_counter()
+        x, r = dev.fit(b, eps=1e-5, refine=4)
+        tf = time.perf_counter() - t0
+        out[tag] = {"assembly_rows_per_s": rate, "fit_s": tf, "rms": r.rms, "nvars": r.nvars,
+                    "assembly_plus_fit_rows_per_s": rows / (rows / rate + tf), "x": x}
+    x64, x32 = out["fp64"].pop("x"), out["fp32"].pop("x")
+    out["fp32"]["rms_rel_err_vs_fp64"] = abs(out["fp32"]["rms"] - out["fp64"]["rms"]) / out["fp64"]["rms"]
+    out["fp32"]["coef_rel_err_vs_fp64"] = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+    out["fp32_over_fp64_rows_per_s"] = (out["fp32"]["assembly_plus_fit_rows_per_s"] /
+                                         out["fp64"]["assembly_plus_fit_rows_per_s"])
+    dev.close()
+    return out
+
+
+def config_c4(ch, synth, nframes=256):
+    """Batched assembly of frames of uneven size and species in one call: C/H/N/O,
+    32-2048 atoms (log-uniform), triclinic cells, orders 12/6/3."""
+    text, frames, _ = synth.chno(nframes=nframes, nmin=32, nmax=2048, seed=104)
+    s = setup_from(ch, text)
+    fr = make_frames(ch, frames)
+    rows = sum(s.frame_rows(f.natoms, f.index) for f in fr)
+    dev = ch.Device(s)
+    out = {"workload": "C/H/N/O %d frames, 32-2048 atoms, orders 12/6/3" % nframes, "rows": rows,
+           "cols": int(s.model.ncols), "atoms": sum(f.natoms for f in fr)}
+    for prec, tag in [(ch.CG_FP64, "fp64_rows_per_s"), (ch.CG_FP32, "fp32_rows_per_s")]:
+        out[tag] = assemble_rate(ch, dev, fr, rows, prec, steps=1)
+        ms, ncl = dev.detail()
+        out[tag.replace("rows_per_s", "family_ms")] = dict(zip(["2B", "3B", "4B"], ms))
+    out["clusters"] = dict(zip(["pairs", "triplets", "quadruplets"], dev.detail()[1]))
+    dev.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 600-frame and mixed-batch legs")
     ap.add_argument("--counters", action="store_true",
                     help="one FP64 assembly of the workload plus a 1 GiB device copy, for rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -253,6 +311,16 @@ def main():
         log("serving Poisson stream %.0f req/s: p50 %.3f ms p99 %.3f ms, %.3g atoms/s, mean batch %.2f" % (
             stream["qps_offered"], stream["p50_ms"], stream["p99_ms"], stream["atoms_per_s"], stream["mean_batch"]))
         extra["serving"] = {"single_1k_atoms": one, "poisson_stream": stream}
+    if not args.no_configs:
+        c3 = config_c3(ch, synth)
+        log("600-frame carbon: FP64 %.3g rows/s + fit %.3f s | FP32 %.3g rows/s + fit %.3f s, RMS rel err %.2e" % (
+            c3["fp64"]["assembly_rows_per_s"], c3["fp64"]["fit_s"], c3["fp32"]["assembly_rows_per_s"],
+            c3["fp32"]["fit_s"], c3["fp32"]["rms_rel_err_vs_fp64"]))
+        c4 = config_c4(ch, synth)
+        log("mixed C/H/N/O batch (%d atoms, %d rows x %d cols): FP64 %.3g rows/s, FP32 %.3g rows/s" % (
+            c4["atoms"], c4["rows"], c4["cols"], c4["fp64_rows_per_s"], c4["fp32_rows_per_s"]))
+        extra["carbon_600_frames_precision"] = c3
+        extra["chno_mixed_batch"] = c4
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle on the host")

@@ -1,0 +1,52 @@
+"""The drop-in executable: `chimes_lsq_gpu fm_setup.in --fit` in a copy of the
+reference's nonorth2 case writes the files chimes_lsq + chimes_lsq.py write,
+with the reference's contents."""
+import hashlib
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG, colscaled_error, golden_dir, headers
+
+pytestmark = pytest.mark.gpu
+
+
+def _numbers(line):
+    return [float(t) for t in re.findall(r"[-+]?\d+\.\d+e[-+]\d+", line)]
+
+
+def test_cli_nonorth2_end_to_end(tmp_path):
+    d = golden_dir("nonorth2")
+    for fn in ["fm_setup.in", "input.xyzf"]:
+        shutil.copy(os.path.join(d, fn), tmp_path / fn)
+    exe = os.path.join(PKG, "chimes_lsq_gpu")
+    r = subprocess.run([exe, "fm_setup.in", "--fit"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = np.load(os.path.join(d, "golden.npz"))
+    A = np.loadtxt(tmp_path / "A.txt")
+    assert A.shape == g["A"].shape
+    assert colscaled_error(A, g["A"]) < 1e-10
+    # A.txt text layout: 16-digit scientific, three-space separators on force rows
+    first = open(tmp_path / "A.txt").readline()
+    assert re.match(r"^ ?-?\d\.\d{16}e[-+]\d\d   ", first)
+    assert np.array_equal(np.loadtxt(tmp_path / "b.txt"), g["b"])
+    assert open(tmp_path / "dim.txt").read().split() == open(os.path.join(d, "dim.txt")).read().split()
+    h = headers()["nonorth2"]
+    for fn in ["params.header", "ff_groups.map"]:
+        data = open(tmp_path / fn, "rb").read()
+        assert hashlib.sha256(data).hexdigest() == h[fn]["sha256"], fn
+    # params.txt: same lines as the reference's fit; numbers within the fit's tolerance
+    ours = open(tmp_path / "params.txt").read().split("\n")[1:]
+    ref = open(os.path.join(d, "params.txt")).read().split("\n")[1:]
+    assert len(ours) == len(ref)
+    for a, b in zip(ours, ref):
+        na, nb = _numbers(a), _numbers(b)
+        assert re.sub(r"[-+]?\d+\.\d+e[-+]\d+", "#", a) == re.sub(r"[-+]?\d+\.\d+e[-+]\d+", "#", b), (a, b)
+        for u, v in zip(na, nb):
+            assert u == pytest.approx(v, rel=1e-4, abs=1e-9 * 358.7), (a, b)
+    y = np.loadtxt(tmp_path / "force.txt")
+    assert np.max(np.abs(y - g["force"]) / np.maximum(np.abs(g["force"]), 1e-6)) < 1e-5
