@@ -130,6 +130,24 @@ __global__ void k_wrap(const DevFrame *frames, const int *atom_frame, const doub
   }
 }
 
+// Cell list: bin of each atom in fractional coordinates (frames that use the
+// cell search), as a sort key; other frames sort after every bin.
+__global__ void k_bin_keys(const DevFrame *frames, const int *atom_frame, const double *wrap, int natoms,
+                           int nbins_total, int *key, int *val) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= natoms) return;
+  const DevFrame &f = frames[atom_frame[gi]];
+  val[gi] = gi;
+  if (!f.cell) { key[gi] = nbins_total; return; }
+  const double *w = wrap + 3 * gi;
+  int b[3];
+  for (int r = 0; r < 3; ++r) {
+    const double s = f.Hi[3 * r] * w[0] + f.Hi[3 * r + 1] * w[1] + f.Hi[3 * r + 2] * w[2];
+    b[r] = min(f.nbin[r] - 1, max(0, (int)floor(s * f.nbin[r])));
+  }
+  key[gi] = f.bin_off + (b[2] * f.nbin[1] + b[1]) * f.nbin[0] + b[0];
+}
+
 // Wave-wide exclusive prefix sum of one int per lane (and the wave total).
 __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
   const int lane = threadIdx.x & 63;
@@ -147,14 +165,17 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
 // [-L, L]^3 (the reference's ghost layers), restricted to shifts that can lie
 // within rc along each axis.  Big cells keep j >= i (half list with the
 // reference's a1 <= PARENT[a2] rule); small cells keep every image.
-// One wave per atom, lanes over j; entries come out sorted by parent j, then
-// by shift (sx, sy, sz).  FILL also stores, per entry, the relative vector
+// Frames at least three list cutoffs thick along every axis search a cell
+// list (27 fractional-coordinate bins around the atom, the entries in bin
+// order); the others scan every atom, one wave per atom with lanes over j,
+// entries sorted by parent j then shift (sx, sy, sz).  Both give the same set
+// of (j, shift) entries.  FILL also stores, per entry, the relative vector
 // image(j) - i with its length, the owning atom and the parent's type.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap,
-                                                const int *type, int natoms, double rc, int L, int *count,
-                                                const int *offset, uint32_t *list, double4 *nvec, int *nown,
-                                                int *ntyp) {
+                                                const int *type, const int *bin_start, const int *bin_atom,
+                                                int natoms, double rc, int L, int *count, const int *offset,
+                                                uint32_t *list, double4 *nvec, int *nown, int *ntyp) {
   const int gi = blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (gi >= natoms) return;
@@ -167,6 +188,52 @@ __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int
   double reach[3];
   for (int r = 0; r < 3; ++r) reach[r] = rc / f.width[r] + 1e-9;
   int run = FILL ? offset[gi] : 0;
+  if (f.cell) {
+    // 27 neighbouring bins (each >= rc thick along its axis, at least 3 per
+    // axis): every image within rc lies there, with shift -1/0/+1 per axis
+    // where the stencil crosses the cell face
+    int bi[3];
+    for (int r = 0; r < 3; ++r) bi[r] = min(f.nbin[r] - 1, max(0, (int)floor(fi[r] * f.nbin[r])));
+    for (int st = 0; st < 27; ++st) {
+      int bn[3], sh[3];
+      const int d[3] = {st % 3 - 1, (st / 3) % 3 - 1, st / 9 - 1};
+      for (int r = 0; r < 3; ++r) {
+        bn[r] = bi[r] + d[r];
+        sh[r] = 0;
+        if (bn[r] < 0) { bn[r] += f.nbin[r]; sh[r] = -1; }
+        else if (bn[r] >= f.nbin[r]) { bn[r] -= f.nbin[r]; sh[r] = 1; }
+      }
+      const int bin = f.bin_off + (bn[2] * f.nbin[1] + bn[1]) * f.nbin[0] + bn[0];
+      const int c0 = bin_start[bin], c1 = bin_start[bin + 1];
+      for (int cb = c0; cb < c1; cb += 64) {
+        const int c = cb + lane;
+        int n = 0, j = -1;
+        double v[3] = {0, 0, 0}, r2 = 0.0;
+        if (c < c1) {
+          const int gj = bin_atom[c];
+          j = gj - f.atom_off;
+          if (j >= i && !(j == i && sh[0] == 0 && sh[1] == 0 && sh[2] == 0)) {
+            const double *wj = wrap + 3 * gj;
+            for (int r = 0; r < 3; ++r)
+              v[r] = wj[r] - wi[r] + f.H[3 * r] * sh[0] + f.H[3 * r + 1] * sh[1] + f.H[3 * r + 2] * sh[2];
+            r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            n = r2 < rc2 ? 1 : 0;
+          }
+        }
+        int tot;
+        const int pos = run + wave_excl_scan(n, tot);
+        if (FILL && n) {
+          list[pos] = make_ref(j, sh[0], sh[1], sh[2]);
+          nvec[pos] = make_double4(v[0], v[1], v[2], sqrt(r2));
+          nown[pos] = gi;
+          ntyp[pos] = type[f.atom_off + j];
+        }
+        run += tot;
+      }
+    }
+    if (!FILL && lane == 0) count[gi] = run;
+    return;
+  }
   for (int jb = f.small ? 0 : i; jb < f.natoms; jb += 64) {
     const int j = jb + lane;
     int n = 0;
@@ -1326,6 +1393,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   std::vector<int> type;
   long long rows = 0;
   int natoms = 0;
+  int nbins_total = 0;
   double mx = m.max_cutoff_2b;
   if (m.c3.nbody) mx = std::max(mx, m.c3.max_cutoff);
   if (m.c4.nbody) mx = std::max(mx, m.c4.max_cutoff);
@@ -1348,6 +1416,33 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
       d.small = c.sticky_small;
     } else {
       d.small = small_here;
+    }
+    // cell list when every axis holds at least three bins of thickness >= mx
+    // and there are at least 125 bins (the 27-bin stencil then visits under a
+    // quarter of the frame; below that the direct scan is as cheap), with the
+    // bins per frame kept to O(atoms) for sparse cells
+    d.cell = 0;
+    d.nbin[0] = d.nbin[1] = d.nbin[2] = 1;
+    d.bin_off = 0;
+    if (!d.small && mx > 0.0) {
+      int nb[3];
+      bool ok = true;
+      for (int r = 0; r < 3; ++r) {
+        nb[r] = (int)std::min(256.0, std::floor(d.width[r] / mx));
+        ok = ok && nb[r] >= 3;
+      }
+      if (ok && (long long)nb[0] * nb[1] * nb[2] < 125) ok = false;
+      if (ok) {
+        while ((long long)nb[0] * nb[1] * nb[2] > 4LL * fr.natoms + 125) {
+          int r = nb[0] >= nb[1] && nb[0] >= nb[2] ? 0 : (nb[1] >= nb[2] ? 1 : 2);
+          if (nb[r] <= 5) break;
+          --nb[r];
+        }
+        d.cell = 1;
+        for (int r = 0; r < 3; ++r) d.nbin[r] = nb[r];
+        d.bin_off = nbins_total;
+        nbins_total += nb[0] * nb[1] * nb[2];
+      }
     }
     d.perm_scale[0] = d.perm_scale[1] = 1.0;
     for (int j = 2; j < 5; ++j) d.perm_scale[j] = d.small ? d.perm_scale[j - 1] / j : 1.0;
@@ -1372,14 +1467,36 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   upload(c, d_xyz, xyz.data(), xyz.size() * sizeof(double));
   const int TB = 256;
   hipLaunchKernelGGL(k_wrap, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_xyz, d_wrap, natoms);
+  // ---- cell lists: atoms sorted (stably) by bin, bin starts
+  const int *d_bin_start = nullptr, *d_bin_atom = nullptr;
+  if (nbins_total > 0) {
+    char *bb = static_cast<char *>(c.b_bins.get((4 * (size_t)natoms + nbins_total + 1) * sizeof(int) + 256));
+    if (!bb) throw Error(CG_ERR_MEMORY, "cell lists");
+    int *key = reinterpret_cast<int *>(bb), *val = key + natoms, *key2 = val + natoms, *val2 = key2 + natoms;
+    int *bstart = val2 + natoms;
+    hipLaunchKernelGGL(k_bin_keys, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_wrap,
+                       natoms, nbins_total, key, val);
+    int bits = 1;
+    while ((1LL << bits) <= (long long)nbins_total) ++bits;
+    size_t tmp = 0;
+    CG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, val, val2, natoms, 0, bits, c.stream));
+    void *t = c.b_scan.get(tmp);
+    if (!t) throw Error(CG_ERR_MEMORY, "sort scratch");
+    CG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tmp, key, key2, val, val2, natoms, 0, bits, c.stream));
+    hipLaunchKernelGGL(k_bucket_bounds, dim3((nbins_total + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, key2, natoms,
+                       nbins_total, bstart);
+    CG_HIP(hipGetLastError());
+    d_bin_start = bstart;
+    d_bin_atom = val2;
+  }
   // ---- neighbour lists (one list at the largest family cutoff)
   int *d_ncount = c.b_ncount.as<int>(natoms + 1);
   int *d_noff = c.b_noff.as<int>(natoms + 1);
   if (!d_ncount || !d_noff) throw Error(CG_ERR_MEMORY, "neighbour counts");
   CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
   hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
-                     (const int *)d_type, natoms, mx, m.nlayers, d_ncount, (const int *)nullptr, (uint32_t *)nullptr,
-                     (double4 *)nullptr, (int *)nullptr, (int *)nullptr);
+                     (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, d_ncount,
+                     (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr, (int *)nullptr);
   exclusive_scan(c, d_ncount, d_noff, natoms);
   const int nnb = read_int(c, d_noff + natoms);                               // host sync 1
   const size_t nn1 = std::max(nnb, 1);
@@ -1389,8 +1506,8 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   int *d_ntyp = d_nown + nn1;
   if (!d_nlist || !d_nvec || !d_nown) throw Error(CG_ERR_MEMORY, "neighbour list");
   hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
-                     (const int *)d_type, natoms, mx, m.nlayers, (int *)nullptr, (const int *)d_noff, d_nlist, d_nvec,
-                     d_nown, d_ntyp);
+                     (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, (int *)nullptr,
+                     (const int *)d_noff, d_nlist, d_nvec, d_nown, d_ntyp);
   // ---- per-family sublists (3B / 4B cutoffs below the list cutoff), sized
   // by the full list so that no host read is needed
   const uint32_t *fl_list[3] = {d_nlist, d_nlist, d_nlist};
@@ -1751,7 +1868,7 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 
 void ctx_free(Ctx &c) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
-                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
+                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
                   &c.b_tasks[1], &c.b_tasks[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
                   &c.b_tmp})
     b->release();

@@ -54,6 +54,9 @@ struct DevFrame {
   double min_extent;   // smallest LAMMPS extent (small-cell switch)
   double inv_vol;
   double perm_scale[5];
+  int cell;            // neighbour search through fractional-coordinate bins
+  int nbin[3];         // bins per cell axis (each at least the list cutoff thick)
+  int bin_off;         // first bin of the frame in the batch bin arrays
 };
 
 // Device copy of one cluster family (3B or 4B) or the 2B pair family.
@@ -92,6 +95,7 @@ struct Ctx {
   DBuf b_frames, b_xyz, b_type, b_wrap;   // frame table, coordinates, types, wrapped coordinates
   DBuf b_ncount, b_noff, b_nlist, b_nvec, b_nown;  // neighbour lists
   DBuf b_sub[2];                          // 3B / 4B neighbour sublists
+  DBuf b_bins;                            // cell-list keys, atoms and bin starts
   DBuf b_ccount, b_coff;                  // per-entry cluster counts / offsets (3 families)
   DBuf b_fam[3];                          // per-family clusters, keys and bucket bounds
   DBuf b_tasks[3];                        // per-family assembly task lists

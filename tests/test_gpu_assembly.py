@@ -77,6 +77,27 @@ def test_synthetic_cases(ch, synth, oracle, tmp_path, case):
     dev.close()
 
 
+def test_cell_list_frame_matches_oracle(ch, synth, oracle, tmp_path):
+    """A frame large enough for the cell-list neighbour search (1400 carbon
+    atoms, 23.4 A cube: 5 bins of >= 4.5 A per axis) against the oracle's
+    all-pairs search; a 2+3-body model keeps the oracle fast."""
+    text, frames, names = synth.carbon(nframes=1, natoms=1400, seed=26, o2=8, o3=4, o4=0)
+    s, fr = synth_setup(ch, synth, tmp_path, (text, frames, names))
+    assert np.floor(np.asarray(fr[0].cell)[0] / 4.5) >= 5
+    dev = _dev(ch, s)
+    A, b = dev.assemble(fr)
+    Ao = oracle.assemble(s, fr)
+    assert colscaled_error(A, Ao) < TOL64
+    ms, ncl = dev.detail()
+    import ctypes
+    cnt = (ctypes.c_longlong * 3)()
+    cf = fr[0].c()
+    small = ctypes.c_int(0)
+    assert oracle.L.orc_count_clusters(s.model_ptr, ctypes.byref(cf), ctypes.byref(small), cnt) == 0
+    assert list(ncl[:2]) == list(cnt[:2])
+    dev.close()
+
+
 def test_newton_third_law_full_size(ch, synth, tmp_path):
     """At the benchmark size (512-atom carbon frames, orders 16/9/5) the force
     rows of every column sum to zero over the atoms of a frame."""
