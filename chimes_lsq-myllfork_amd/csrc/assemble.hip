@@ -303,8 +303,13 @@ struct EnumArgs {
   uint32_t *refs;       // [ncl * nb]
   int *key;             // [ncl] frame * types + type
   // family tables
-  const int *id_map, *id_atom_perm, *id_pair_map;
   const double *smin, *smax;
+  // per type tuple (index sum_q t_q * ntypes_atom^q of the tuple in
+  // enumeration order): cluster type (-1: none), S_MAXIM of each enumerated
+  // pair slot, canonical slot permutation
+  const int *cct;
+  const double *csmax;
+  const int *cperm;
   const int *int_pair_map;
   int md;               // force-evaluation pair set
 };
@@ -315,46 +320,32 @@ __device__ __forceinline__ double dist3(const double4 &u, const double4 &v) {
 }
 
 // Cluster-type lookup (make_id_int + INT_MAP) and PROCEED on every pair with
-// its canonical pair's S_MAXIM (src/Cheby.C:1015-1054, 1387-1500).  d[] holds
-// the pair distances in enumeration order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
+// its canonical pair's S_MAXIM (src/Cheby.C:1015-1054, 1387-1500), through
+// tables built on the host per type tuple (index sum_q t_q * ntypes^q, types
+// in enumeration order): one small cached row per candidate.  d[] holds the
+// pair distances in enumeration order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
 template <int NB>
-__device__ __forceinline__ int cluster_type(const EnumArgs &a, const int *types, const double *d, int &id) {
+__device__ __forceinline__ int cluster_type_c(const EnumArgs &a, const int *types, const double *d,
+                                              const int *&perm) {
   constexpr int NP = NB * (NB - 1) / 2;
-  id = 0;
-  int w = 1;
+  int cid = 0;
 #pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    id += w * (types[q] + 1);
-    w *= CG_MAX_ATOM_TYPES;
-  }
-  const int ct = a.id_map[id];
+  for (int q = NB - 1; q >= 0; --q) cid = cid * a.ntypes_atom + types[q];
+  const int ct = a.cct[cid];
   if (ct < 0) return -1;
+  const double *row = a.csmax + (size_t)cid * NP;
 #pragma unroll
-  for (int pf = 0; pf < NP; ++pf) {
-    const int cp = a.id_pair_map[id * NP + pf];
-    if (!(d[pf] < a.smax[ct * NP + cp])) return -1;
-  }
+  for (int pf = 0; pf < NP; ++pf)
+    if (!(d[pf] < row[pf])) return -1;
+  perm = a.cperm + cid * NB;
   return ct;
 }
 
-// First entry of atom gi's list in [b, k] that has the same parent as entry k
-// (lists are sorted by parent), so [start, e) holds every entry whose parent
-// is not below parent(k).
-__device__ __forceinline__ int group_start(const uint32_t *nlist, int b, int k) {
-  const int p = ref_atom(nlist[k]);
-  while (k > b && ref_atom(nlist[k - 1]) == p) --k;
-  return k;
-}
-
-// Enumerate the clusters anchored at each real atom (the reference's
-// Deriv_2B/3B/4B loop structure: a1 real, a2 <= a3 <= a4 by parent in big
-// cells, all orderings in small cells) and keep those whose pairs all pass
-// PROCEED.  One thread per neighbour entry (the atom's second member); the
-// inner members are pruned by the family cutoff before the type lookup.
-// FILL stores each cluster in the canonical atom order of its type, with its
-// (frame, type) key; clusters come out ordered by (atom, entry).
-template <int NB, bool FILL>
-__global__ __launch_bounds__(256) void k_enum(EnumArgs a) {
+// 2-body enumeration (the pair loop of Deriv_2B / Force_all): one thread per
+// neighbour entry; LSQ keeps S_MINIM < r < S_MAXIM, force evaluation every
+// r < S_MAXIM.  FILL stores the pair with its (frame, pair type) key.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_enum2(EnumArgs a) {
   const int k2 = blockIdx.x * blockDim.x + threadIdx.x;
   if (k2 >= a.nnb) return;
   if (k2 >= *a.ntot) {
@@ -367,76 +358,17 @@ __global__ __launch_bounds__(256) void k_enum(EnumArgs a) {
     const int gi = a.nown[k2];
     const int fidx = a.atom_frame[gi];
     const DevFrame &f = a.frames[fidx];
-    const int i = gi - f.atom_off;
-    const uint32_t r1 = make_ref(i, 0, 0, 0);
     const uint32_t r2 = a.nlist[k2];
-    int types[4];
-    types[0] = a.type[gi];
-    types[1] = a.type[f.atom_off + ref_atom(r2)];
-    const int out = FILL ? a.coff[k2] : 0;
-    const int kbase = fidx * a.nkeys_type;
-    if (NB == 2) {
-      const int p = a.int_pair_map[types[0] * a.ntypes_atom + types[1]];
-      const double r = v2.w;
-      if (r < a.smax[p] && (a.md || r > a.smin[p])) {
-        if (FILL) {
-          a.refs[2 * out] = r1;
-          a.refs[2 * out + 1] = r2;
-          a.key[out] = kbase + p;
-        }
-        n = 1;
+    const int p = a.int_pair_map[a.type[gi] * a.ntypes_atom + a.ntyp[k2]];
+    const double r = v2.w;
+    if (r < a.smax[p] && (a.md || r > a.smin[p])) {
+      if (FILL) {
+        const int out = a.coff[k2];
+        a.refs[2 * out] = make_ref(gi - f.atom_off, 0, 0, 0);
+        a.refs[2 * out + 1] = r2;
+        a.key[out] = fidx * a.nkeys_type + p;
       }
-    } else {
-      const int b = a.noff[gi], e = a.noff[gi + 1];
-      const bool ordered = !f.small;
-      const double cut = a.cutoff;
-      const int s3 = ordered ? group_start(a.nlist, b, k2) : b;
-      for (int k3 = s3; k3 < e; ++k3) {
-        if (k3 == k2) continue;
-        const double4 v3 = a.nvec[k3];
-        if (v3.w >= cut) continue;
-        const double r23 = dist3(v2, v3);
-        if (r23 >= cut) continue;
-        const uint32_t r3 = a.nlist[k3];
-        types[2] = a.type[f.atom_off + ref_atom(r3)];
-        if (NB == 3) {
-          const double d[3] = {v2.w, v3.w, r23};
-          int id;
-          const int ct = cluster_type<3>(a, types, d, id);
-          if (ct < 0) continue;
-          if (FILL) {
-            const uint32_t rr[3] = {r1, r2, r3};
-            const int *perm = a.id_atom_perm + id * 3;
-            for (int q = 0; q < 3; ++q) a.refs[3 * (out + n) + q] = rr[perm[q]];
-            a.key[out + n] = kbase + ct;
-          }
-          ++n;
-        } else {
-          const int s4 = ordered ? group_start(a.nlist, b, k3) : b;
-          for (int k4 = s4; k4 < e; ++k4) {
-            if (k4 == k2 || k4 == k3) continue;
-            const double4 v4 = a.nvec[k4];
-            if (v4.w >= cut) continue;
-            const double r24 = dist3(v2, v4);
-            if (r24 >= cut) continue;
-            const double r34 = dist3(v3, v4);
-            if (r34 >= cut) continue;
-            const uint32_t r4 = a.nlist[k4];
-            types[3] = a.type[f.atom_off + ref_atom(r4)];
-            const double d[6] = {v2.w, v3.w, v4.w, r23, r24, r34};
-            int id;
-            const int ct = cluster_type<4>(a, types, d, id);
-            if (ct < 0) continue;
-            if (FILL) {
-              const uint32_t rr[4] = {r1, r2, r3, r4};
-              const int *perm = a.id_atom_perm + id * 4;
-              for (int q = 0; q < 4; ++q) a.refs[4 * (out + n) + q] = rr[perm[q]];
-              a.key[out + n] = kbase + ct;
-            }
-            ++n;
-          }
-        }
-      }
+      n = 1;
     }
   }
   if (!FILL) a.count[k2] = n;
@@ -503,9 +435,11 @@ struct StageGlobal {
   __device__ __forceinline__ int typ(int k) const { return t[k]; }
 };
 
+constexpr int ENUM_ADJ_WORDS = (ENUM_STAGE + 63) / 64;
+
 template <int NB, bool COUNT, class S>
 __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, bool ordered, uint32_t r1, int t1,
-                                         int kbase, int out) {
+                                         int kbase, int out, const unsigned long long *adj) {
   const int lane = threadIdx.x & 63;
   const double cut = a.cutoff;
   const int npair = n * n;
@@ -541,8 +475,13 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
       types[1] = st.typ(k2);
       types[2] = st.typ(k3);
     }
-    // pass 0 counts this lane's clusters, pass 1 writes them (not for COUNT)
+    // pass 0 counts this lane's clusters, pass 1 writes them (not for COUNT);
+    // with an adjacency table pass 0 also records the accepted k4 so that
+    // pass 1 revisits only those
     int mine = 0;
+    unsigned long long kept[ENUM_ADJ_WORDS];
+#pragma unroll
+    for (int w = 0; w < ENUM_ADJ_WORDS; ++w) kept[w] = 0ull;
     for (int pass = 0; pass < (COUNT ? 1 : 2); ++pass) {
       int base = 0;
       if (pass == 1) {
@@ -554,12 +493,11 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
       if (ok) {
         if (NB == 3) {
           const double d[3] = {v2.w, v3.w, r23};
-          int id;
-          const int ct = cluster_type<3>(a, types, d, id);
+          const int *perm;
+          const int ct = cluster_type_c<3>(a, types, d, perm);
           if (ct >= 0) {
             if (pass == 1) {
               const uint32_t rr[3] = {r1, r2, r3};
-              const int *perm = a.id_atom_perm + id * 3;
               for (int x = 0; x < 3; ++x) a.refs[3 * (base + m) + x] = rr[perm[x]];
               a.key[base + m] = kbase + ct;
             }
@@ -567,7 +505,24 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
           }
         } else {
           const int p3 = ref_atom(r3);
-          for (int k4 = 0; k4 < n; ++k4) {
+          // candidates k4: with an adjacency table, the entries within the
+          // cutoff of both k2 and k3 (set bits, in index order); else all
+          const bool replay = adj && pass == 1;
+          int word = 0;
+          unsigned long long bits = 0;
+          if (adj) bits = replay ? kept[0] : (adj[k2 * ENUM_ADJ_WORDS] & adj[k3 * ENUM_ADJ_WORDS]);
+          for (int kk = 0;; ++kk) {
+            int k4;
+            if (adj) {
+              while (!bits && ++word < ENUM_ADJ_WORDS)
+                bits = replay ? kept[word] : (adj[k2 * ENUM_ADJ_WORDS + word] & adj[k3 * ENUM_ADJ_WORDS + word]);
+              if (!bits) break;
+              k4 = word * 64 + __builtin_ctzll(bits);
+              bits &= bits - 1;
+            } else {
+              if (kk >= n) break;
+              k4 = kk;
+            }
             if (k4 == k2 || k4 == k3) continue;
             const uint32_t r4 = st.ref(k4);
             if (ordered && ref_atom(r4) < p3) continue;
@@ -579,12 +534,16 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
             if (!(r34 < cut)) continue;
             types[3] = st.typ(k4);
             const double d[6] = {v2.w, v3.w, v4.w, r23, r24, r34};
-            int id;
-            const int ct = cluster_type<4>(a, types, d, id);
+            const int *perm;
+            const int ct = cluster_type_c<4>(a, types, d, perm);
             if (ct < 0) continue;
+            if (pass == 0 && adj) {
+#pragma unroll
+              for (int w = 0; w < ENUM_ADJ_WORDS; ++w)
+                if (k4 / 64 == w) kept[w] |= 1ull << (k4 % 64);
+            }
             if (pass == 1) {
               const uint32_t rr[4] = {r1, r2, r3, r4};
-              const int *perm = a.id_atom_perm + id * 4;
               for (int x = 0; x < 4; ++x) a.refs[4 * (base + m) + x] = rr[perm[x]];
               a.key[base + m] = kbase + ct;
             }
@@ -608,6 +567,8 @@ __global__ __launch_bounds__(256) void k_enum_wave(EnumArgs a) {
   __shared__ double4 s_vec[4][ENUM_STAGE];
   __shared__ uint32_t s_ref[4][ENUM_STAGE];
   __shared__ int s_typ[4][ENUM_STAGE];
+  // 4-body: bit l of row k set when entries k and l are closer than the cutoff
+  __shared__ unsigned long long s_adj[NB == 4 ? 4 : 1][NB == 4 ? ENUM_STAGE * ENUM_ADJ_WORDS : 1];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const int gi = blockIdx.x * 4 + wave;
   const bool active = gi < a.natoms;
@@ -624,6 +585,19 @@ __global__ __launch_bounds__(256) void k_enum_wave(EnumArgs a) {
       s_typ[wave][k] = a.ntyp[b + k];
     }
   __syncthreads();
+  if (NB == 4 && active && staged) {
+    // one row per lane (rows k, k+64, ...), written by its owner lane only
+    for (int k = lane; k < n; k += 64) {
+      const double4 vk = s_vec[wave][k];
+      for (int w = 0; w < ENUM_ADJ_WORDS; ++w) {
+        unsigned long long m = 0;
+        for (int l = w * 64; l < min(n, w * 64 + 64); ++l)
+          if (l != k && dist3(vk, s_vec[wave][l]) < a.cutoff) m |= 1ull << (l - w * 64);
+        s_adj[NB == 4 ? wave : 0][k * ENUM_ADJ_WORDS + w] = m;
+      }
+    }
+  }
+  __syncthreads();
   if (!active) return;
   const int fidx = a.atom_frame[gi];
   const DevFrame &f = a.frames[fidx];
@@ -633,9 +607,11 @@ __global__ __launch_bounds__(256) void k_enum_wave(EnumArgs a) {
   const int out = COUNT ? 0 : a.coff[gi];
   int tot;
   if (staged)
-    tot = enum_atom<NB, COUNT>(a, StageLds{s_vec[wave], s_ref[wave], s_typ[wave]}, n, !f.small, r1, t1, kbase, out);
+    tot = enum_atom<NB, COUNT>(a, StageLds{s_vec[wave], s_ref[wave], s_typ[wave]}, n, !f.small, r1, t1, kbase, out,
+                               NB == 4 ? s_adj[NB == 4 ? wave : 0] : nullptr);
   else
-    tot = enum_atom<NB, COUNT>(a, StageGlobal{a.nvec + b, a.nlist + b, a.ntyp + b}, n, !f.small, r1, t1, kbase, out);
+    tot = enum_atom<NB, COUNT>(a, StageGlobal{a.nvec + b, a.nlist + b, a.ntyp + b}, n, !f.small, r1, t1, kbase, out,
+                               nullptr);
   if (COUNT && lane == 0) a.count[gi] = tot;
 }
 
@@ -1032,80 +1008,162 @@ __device__ __forceinline__ void contract_inner2(const PairTab<double, N> &tx, co
   Ty = sy;
 }
 
+// Inner three levels fully unrolled (small orders): S = sum C Fx Fy Fz over
+// N^3 entries plus the three partial gradients.
+template <int N>
+__device__ __forceinline__ void contract_inner3(const PairTab<double, N> &tx, const PairTab<double, N> &ty,
+                                                const PairTab<double, N> &tz, const double *__restrict__ C,
+                                                double &S, double &Tx, double &Ty, double &Tz) {
+  double s = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+  for (int pa = 0; pa < N; ++pa) {
+    double S2, T2y, T2z;
+    contract_inner2<N>(ty, tz, C + pa * N * N, S2, T2y, T2z);
+    double fa, da;
+    tx.get(pa, fa, da);
+    s += fa * S2;
+    sx += da * S2;
+    sy += fa * T2y;
+    sz += fa * T2z;
+  }
+  S = s;
+  Tx = sx;
+  Ty = sy;
+  Tz = sz;
+}
+
 template <int N>
 __device__ __forceinline__ void contract3(const PairTab<double, N> *t, const double *__restrict__ C, double &E,
                                           double *g) {
-  double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-  for (int p0 = 0; p0 < N; ++p0) {
-    double f0, d0;
-    t[0].get(p0, f0, d0);
-    double S, T1, T2;
-    contract_inner2<N>(t[1], t[2], C + p0 * N * N, S, T1, T2);
-    e += f0 * S;
-    g0 += d0 * S;
-    g1 += f0 * T1;
-    g2 += f0 * T2;
+  if constexpr (N <= 8) {  // whole tensor unrolled: one batch of scalar loads
+    double S, T0, T1, T2;
+    contract_inner3<N>(t[0], t[1], t[2], C, S, T0, T1, T2);
+    E = S;
+    g[0] = T0; g[1] = T1; g[2] = T2;
+    return;
+  } else {
+    double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int p0 = 0; p0 < N; ++p0) {
+      double f0, d0;
+      t[0].get(p0, f0, d0);
+      double S, T1, T2;
+      contract_inner2<N>(t[1], t[2], C + p0 * N * N, S, T1, T2);
+      e += f0 * S;
+      g0 += d0 * S;
+      g1 += f0 * T1;
+      g2 += f0 * T2;
+    }
+    E = e;
+    g[0] = g0; g[1] = g1; g[2] = g2;
   }
-  E = e;
-  g[0] = g0; g[1] = g1; g[2] = g2;
 }
 
 template <int N>
 __device__ __forceinline__ void contract6(const PairTab<double, N> *t, const double *__restrict__ C, double &E,
                                           double *g) {
   double e = 0.0, G[6] = {0, 0, 0, 0, 0, 0};
-  for (int p0 = 0; p0 < N; ++p0) {
-    double f0, d0;
-    t[0].get(p0, f0, d0);
-    double S1 = 0.0, T1[5] = {0, 0, 0, 0, 0};  // gradients of pairs 1..5
-    for (int p1 = 0; p1 < N; ++p1) {
-      double f1, d1;
-      t[1].get(p1, f1, d1);
-      double S2 = 0.0, T2[4] = {0, 0, 0, 0};    // pairs 2..5
-      for (int p2 = 0; p2 < N; ++p2) {
-        double f2, d2;
-        t[2].get(p2, f2, d2);
-        double S3 = 0.0, T3[3] = {0, 0, 0};     // pairs 3..5
-        const double *Cb = C + ((p0 * N + p1) * N + p2) * N * N * N;
-        for (int p3 = 0; p3 < N; ++p3) {
-          double f3, d3;
-          t[3].get(p3, f3, d3);
-          double S4, T44, T45;
-          contract_inner2<N>(t[4], t[5], Cb + p3 * N * N, S4, T44, T45);
-          S3 += f3 * S4;
-          T3[0] += d3 * S4;
-          T3[1] += f3 * T44;
-          T3[2] += f3 * T45;
+  if constexpr (N <= 5) {  // pairs 3..5 unrolled (N^3 coefficients per outer iteration)
+    for (int p0 = 0; p0 < N; ++p0) {
+      double f0, d0;
+      t[0].get(p0, f0, d0);
+      double S1 = 0.0, T1[5] = {0, 0, 0, 0, 0};
+      for (int p1 = 0; p1 < N; ++p1) {
+        double f1, d1;
+        t[1].get(p1, f1, d1);
+        double S2 = 0.0, T2[4] = {0, 0, 0, 0};
+        for (int p2 = 0; p2 < N; ++p2) {
+          double f2, d2;
+          t[2].get(p2, f2, d2);
+          double S3, T33, T34, T35;
+          contract_inner3<N>(t[3], t[4], t[5], C + ((p0 * N + p1) * N + p2) * N * N * N, S3, T33, T34, T35);
+          S2 += f2 * S3;
+          T2[0] += d2 * S3;
+          T2[1] += f2 * T33;
+          T2[2] += f2 * T34;
+          T2[3] += f2 * T35;
         }
-        S2 += f2 * S3;
-        T2[0] += d2 * S3;
+        S1 += f1 * S2;
+        T1[0] += d1 * S2;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) T2[q + 1] += f2 * T3[q];
+        for (int q = 0; q < 4; ++q) T1[q + 1] += f1 * T2[q];
       }
-      S1 += f1 * S2;
-      T1[0] += d1 * S2;
+      e += f0 * S1;
+      G[0] += d0 * S1;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) T1[q + 1] += f1 * T2[q];
+      for (int q = 0; q < 5; ++q) G[q + 1] += f0 * T1[q];
     }
-    e += f0 * S1;
-    G[0] += d0 * S1;
+    E = e;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) G[q + 1] += f0 * T1[q];
+    for (int q = 0; q < 6; ++q) g[q] = G[q];
+    return;
+  } else {
+    for (int p0 = 0; p0 < N; ++p0) {
+      double f0, d0;
+      t[0].get(p0, f0, d0);
+      double S1 = 0.0, T1[5] = {0, 0, 0, 0, 0};  // gradients of pairs 1..5
+      for (int p1 = 0; p1 < N; ++p1) {
+        double f1, d1;
+        t[1].get(p1, f1, d1);
+        double S2 = 0.0, T2[4] = {0, 0, 0, 0};    // pairs 2..5
+        for (int p2 = 0; p2 < N; ++p2) {
+          double f2, d2;
+          t[2].get(p2, f2, d2);
+          double S3 = 0.0, T3[3] = {0, 0, 0};     // pairs 3..5
+          const double *Cb = C + ((p0 * N + p1) * N + p2) * N * N * N;
+          for (int p3 = 0; p3 < N; ++p3) {
+            double f3, d3;
+            t[3].get(p3, f3, d3);
+            double S4, T44, T45;
+            contract_inner2<N>(t[4], t[5], Cb + p3 * N * N, S4, T44, T45);
+            S3 += f3 * S4;
+            T3[0] += d3 * S4;
+            T3[1] += f3 * T44;
+            T3[2] += f3 * T45;
+          }
+          S2 += f2 * S3;
+          T2[0] += d2 * S3;
+  #pragma unroll
+          for (int q = 0; q < 3; ++q) T2[q + 1] += f2 * T3[q];
+        }
+        S1 += f1 * S2;
+        T1[0] += d1 * S2;
+  #pragma unroll
+        for (int q = 0; q < 4; ++q) T1[q + 1] += f1 * T2[q];
+      }
+      e += f0 * S1;
+      G[0] += d0 * S1;
+  #pragma unroll
+      for (int q = 0; q < 5; ++q) G[q + 1] += f0 * T1[q];
+    }
+    E = e;
+  #pragma unroll
+    for (int q = 0; q < 6; ++q) g[q] = G[q];
   }
-  E = e;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) g[q] = G[q];
 }
+
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+constexpr int FORCE_LDS_COEF = 4096;  // coefficient tensors up to 32 KB are staged in LDS
 
 template <int NB, int N, int BASE, int BS>
 __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
+  constexpr int TS = ipow(N, NP);
+  constexpr bool STAGE = TS <= FORCE_LDS_COEF;
   __shared__ double red[10][BS / 64];
+  // The workgroup's clusters share one type: its coefficient tensor is read
+  // once into LDS (uniform-address reads are broadcasts) instead of through
+  // the scalar cache, which thrashes between the tensors of different types.
+  __shared__ double sC[STAGE ? TS : 1];
   const Task t = a.tasks[blockIdx.x];
   const DevFrame &f = a.frames[t.frame];
   const int ty = t.type;
   const double ps = f.perm_scale[a.nb_scale];
   const double *C = a.coef + (size_t)ty * a.tsize;
+  if constexpr (STAGE) {
+    for (int i = threadIdx.x; i < TS; i += BS) sC[i] = C[i];
+    __syncthreads();
+    C = sC;
+  }
   AsmArgs aa{};
   aa.transform = a.transform; aa.fix_type = a.fix_type; aa.fcut_type = a.fcut_type;
   aa.deriv_const = a.deriv_const; aa.smooth_dist = a.smooth_dist; aa.fcut_offset = a.fcut_offset;
@@ -1153,18 +1211,20 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
         e += rp * rp * rp * a.penalty_scale;
       }
     }
+    // per-atom sums of the pair forces first: 3 atomics per atom of the cluster
+    double fa[NB][3];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) fa[q][0] = fa[q][1] = fa[q][2] = 0.0;
     int k = 0;
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll
       for (int y = x + 1; y < NB; ++y) {
-        double *Fa = a.F + 3 * (size_t)(f.atom_off + atom[x]);
-        double *Fb = a.F + 3 * (size_t)(f.atom_off + atom[y]);
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
           const double d = g[k] * rh[k][cc];
-          atomicAdd(Fa + cc, d);
-          atomicAdd(Fb + cc, -d);
+          fa[x][cc] += d;
+          fa[y][cc] -= d;
         }
 #pragma unroll
         for (int u = 0; u < 3; ++u)
@@ -1172,6 +1232,12 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
           for (int v = 0; v < 3; ++v) ev[1 + 3 * u + v] -= g[k] * rl[k] * rh[k][u] * rh[k][v];
         ++k;
       }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      double *Fq = a.F + 3 * (size_t)(f.atom_off + atom[q]);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) atomicAdd(Fq + cc, fa[q][cc]);
+    }
     ev[0] += e;
   }
   // workgroup reduction of energy + virial
@@ -1229,7 +1295,6 @@ static void build_family(Ctx &c, int which) {
   std::vector<int> colptr, col_rel, col_abs;
   std::vector<uint32_t> perm;
   std::vector<double> smin, smax, xavg, xdiff, lam;
-  std::vector<int> idm, idap, idpm;
   if (which == 0) {
     F.np = 1; F.nb = 2; F.ntypes = m.npairs; F.ncols = m.n2b_cols; F.max_pow = m.order2b;
     F.cutoff = m.max_cutoff_2b;
@@ -1279,12 +1344,36 @@ static void build_family(Ctx &c, int which) {
       }
     }
     colptr.push_back((int)perm.size());
-    idm.assign(cs.id_map, cs.id_map + cs.id_size);
-    idap.assign(cs.id_atom_perm, cs.id_atom_perm + (size_t)cs.id_size * cs.nbody);
-    idpm.assign(cs.id_pair_map, cs.id_pair_map + (size_t)cs.id_size * cs.npairs);
-    F.id_map = dalloc_copy(idm);
-    F.id_atom_perm = dalloc_copy(idap);
-    F.id_pair_map = dalloc_copy(idpm);
+    // Type-tuple tables for the enumeration: the reference's INT_MAP,
+    // INT_MAP atom and pair maps are indexed by sum_q (t_q+1) 10^q; here the
+    // tuple index is dense, sum_q t_q ntypes^q, and the pair map is resolved
+    // to the canonical pair's S_MAXIM.
+    const int nt = m.ntypes, nb = cs.nbody, np = cs.npairs;
+    int ntup = 1;
+    for (int q = 0; q < nb; ++q) ntup *= nt;
+    std::vector<int> cct(ntup, -1), cperm((size_t)ntup * nb, 0);
+    std::vector<double> csmax((size_t)ntup * np, -1.0);
+    for (int cid = 0; cid < ntup; ++cid) {
+      int id = 0, w = 1, r = cid;
+      for (int q = 0; q < nb; ++q) {
+        id += w * (r % nt + 1);
+        r /= nt;
+        w *= CG_MAX_ATOM_TYPES;
+      }
+      if (id >= cs.id_size) throw Error(CG_ERR_INPUT, "cluster id table too small");
+      const int ctp = cs.id_map[id];
+      cct[cid] = ctp;
+      if (ctp < 0) continue;
+      for (int q = 0; q < nb; ++q) cperm[(size_t)cid * nb + q] = cs.id_atom_perm[(size_t)id * nb + q];
+      for (int pf = 0; pf < np; ++pf) {
+        const int cp = cs.id_pair_map[(size_t)id * np + pf];
+        if (cp < 0 || cp >= np) throw Error(CG_ERR_INPUT, "bad cluster pair map");
+        csmax[(size_t)cid * np + pf] = cs.s_max[ctp * np + cp];
+      }
+    }
+    F.cct = dalloc_copy(cct);
+    F.cperm = dalloc_copy(cperm);
+    F.csmax = dalloc_copy(csmax);
   }
   F.colptr = dalloc_copy(colptr);
   F.perm = dalloc_copy(perm);
@@ -1560,12 +1649,12 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     E.cutoff = F.cutoff;
     E.count = d_ccount + w * cslot;
     E.coff = d_coff + w * cslot;
-    E.id_map = F.id_map; E.id_atom_perm = F.id_atom_perm; E.id_pair_map = F.id_pair_map;
+    E.cct = F.cct; E.csmax = F.csmax; E.cperm = F.cperm;
     E.smin = F.smin; E.smax = F.smax; E.int_pair_map = c.d_int_pair_map;
     E.md = md;
     nitems[w] = w == 0 ? nnb : natoms;
     CG_HIP(hipMemsetAsync(E.count + nitems[w], 0, sizeof(int), c.stream));
-    if (w == 0) hipLaunchKernelGGL((k_enum<2, false>), eg, eb, 0, c.stream, E);
+    if (w == 0) hipLaunchKernelGGL((k_enum2<false>), eg, eb, 0, c.stream, E);
     else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, true>), wg, wb, 0, c.stream, E);
     else hipLaunchKernelGGL((k_enum_wave<4, true>), wg, wb, 0, c.stream, E);
     CG_HIP(hipGetLastError());
@@ -1603,7 +1692,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     fo[w].refs = E.refs;
     const int *keys = E.key;
     if (ncl > 0) {
-      if (w == 0) hipLaunchKernelGGL((k_enum<2, true>), eg, eb, 0, c.stream, E);
+      if (w == 0) hipLaunchKernelGGL((k_enum2<true>), eg, eb, 0, c.stream, E);
       else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, false>), wg, wb, 0, c.stream, E);
       else hipLaunchKernelGGL((k_enum_wave<4, false>), wg, wb, 0, c.stream, E);
       CG_HIP(hipGetLastError());
@@ -1775,9 +1864,11 @@ static void set_coef(Ctx &c, const double *x) {
   c.coef_valid = true;
 }
 
+constexpr int FORCE_BS = 256;  // threads (= clusters) per force workgroup
+
 template <int NB, int N, int BASE>
 static void launch_force_n(Ctx &c, dim3 grid, const ForceArgs &a) {
-  hipLaunchKernelGGL((k_force<NB, N, BASE, 64>), grid, dim3(64), 0, c.stream, a);
+  hipLaunchKernelGGL((k_force<NB, N, BASE, FORCE_BS>), grid, dim3(FORCE_BS), 0, c.stream, a);
 }
 
 template <int NB, int BASE, int... Ns>
@@ -1806,7 +1897,7 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
     const DevFamily &Fm = c.fam[w];
     if (Fm.ntypes == 0 || Fm.ncols == 0 || B.fo[w].ncl == 0) continue;
     std::vector<Task> &tasks = ftasks[w];
-    const int chunk = 64;  // one wave per workgroup, one cluster per lane
+    const int chunk = FORCE_BS;  // one cluster per thread, one type per workgroup
     for (int fi = 0; fi < nframes; ++fi)
       for (int t = 0; t < Fm.ntypes; ++t) {
         const int c0 = B.fo[w].boff[fi * Fm.ntypes + t], c1 = B.fo[w].boff[fi * Fm.ntypes + t + 1];
@@ -1874,8 +1965,8 @@ void ctx_free(Ctx &c) {
     b->release();
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
-                    (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.id_map, (void *)F.id_atom_perm,
-                    (void *)F.id_pair_map})
+                    (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.cct, (void *)F.csmax,
+                    (void *)F.cperm})
       if (p) (void)hipFree(p);
   }
   if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);

@@ -76,9 +76,9 @@ struct DevFamily {
   int *colptr = nullptr;      // [ncols+1] first perm of each family column (family-relative)
   uint32_t *perm = nullptr;   // [nperm] packed powers, 4 bits per canonical pair
   double *smin = nullptr, *smax = nullptr, *xavg = nullptr, *xdiff = nullptr, *lambda = nullptr; // [ntypes*np]
-  int *id_map = nullptr;      // [10^nb]
-  int *id_atom_perm = nullptr;// [10^nb * nb]
-  int *id_pair_map = nullptr; // [10^nb * np]
+  int *cct = nullptr;         // [ntypes_atom^nb] cluster type of a type tuple, -1 if none
+  double *csmax = nullptr;    // [ntypes_atom^nb * np] S_MAXIM of each enumerated pair slot
+  int *cperm = nullptr;       // [ntypes_atom^nb * nb] canonical slot of each enumerated atom
 };
 
 struct Ctx {
