@@ -17,6 +17,7 @@
 // force / stress / energy rows of the tile in LDS.  The tile is written to A
 // once, coalesced along its columns.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -657,6 +658,7 @@ struct AsmArgs {
   const double *wrap;
   const uint32_t *refs;     // bucketed cluster refs [ncl*NB]
   const uint32_t *perm;     // packed powers
+  const int *sets8;         // 4-body: per allowed set 8 ints, 2 * power of pairs 0..5, then 0, 0
   const int *colptr;        // [family ncols + 1]
   const int *col_rel;       // [ntypes] family-relative first column of a type
   const int *col_abs;       // [ntypes] absolute first column of a type
@@ -714,9 +716,9 @@ template <int VW, bool SPLIT> struct PairTab<float, VW, SPLIT> {
 
 // Chebyshev tables of one pair: F[i] = fc*T_{i+BASE}(x), D[i] = d/dr of that,
 // with set_polys / set_polys_out_of_range / FCUT::get_fcut semantics.
-template <class T, int VW, int BASE>
+template <class T, int VW, int BASE, class Tab>
 __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double smin, double smax, double xavg,
-                                           double xdiff, double lambda, PairTab<T, VW> &P) {
+                                           double xdiff, double lambda, Tab &P) {
   double re = r < smin ? smin : r;
   double x, ex = 0.0, dxdr;
   if (a.transform == CG_MORSE) {
@@ -777,6 +779,122 @@ __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double sm
   }
 }
 
+// 4-body FP64 tables for the hand-scheduled set loop: F and D of the six
+// pairs in four 32-dword register blocks held in fixed registers
+// v[128:255] (F of pairs 0-2, F of pairs 3-5, D of pairs 0-2, D of pairs
+// 3-5; pair slot of 10 dwords, entry of power p at 2p).  A wave-uniform
+// power selects an entry through GPR-index mode applied to src0 of the
+// arithmetic itself, so a table read costs no instruction of its own.
+typedef vecN<uint32_t, 32> v32u;
+struct Tab4 {
+  v32u FA, FB, DA, DB;
+  __device__ __forceinline__ void zero() { FA = 0u; FB = 0u; DA = 0u; DB = 0u; }
+};
+struct Tab4Pair {
+  Tab4 &t;
+  int k;
+  __device__ __forceinline__ void set(int i, double f, double d) {
+    const uint64_t fb = __builtin_bit_cast(uint64_t, f), db = __builtin_bit_cast(uint64_t, d);
+    const int o = 10 * (k % 3) + 2 * i;
+    if (k < 3) {
+      t.FA[o] = (uint32_t)fb; t.FA[o + 1] = (uint32_t)(fb >> 32);
+      t.DA[o] = (uint32_t)db; t.DA[o + 1] = (uint32_t)(db >> 32);
+    } else {
+      t.FB[o] = (uint32_t)fb; t.FB[o + 1] = (uint32_t)(fb >> 32);
+      t.DB[o] = (uint32_t)db; t.DB[o + 1] = (uint32_t)(db >> 32);
+    }
+  }
+};
+
+// One allowed set (indices i0..i5 = 2 * power of each pair in SGPRs):
+// E += prod_k F_k, G_k += D_k prod_{j != k} F_j, as L = F0 F1 F2 and
+// R = F3 F4 F5 with their partial derivatives, each pair's index region
+// entered once.  F_k sits at v[128 + 10k] (k < 3) / v[160 + 10(k-3)], D_k
+// 64 registers higher.
+#define CG_SET4(i0, i1, i2, i3, i4, i5)                      \
+  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
+  "v_mov_b64 %[X0], v[128:129]\n\t"                          \
+  "v_mov_b64 %[Y0], v[192:193]\n\t"                          \
+  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
+  "v_mov_b64 %[X5], v[180:181]\n\t"                          \
+  "v_mov_b64 %[Y5], v[244:245]\n\t"                          \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_mul_f64 %[M], v[202:203], %[X0]\n\t"                    \
+  "v_mul_f64 %[N], v[138:139], %[Y0]\n\t"                    \
+  "v_mul_f64 %[L], v[138:139], %[X0]\n\t"                    \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_mul_f64 %[M4], v[234:235], %[X5]\n\t"                   \
+  "v_mul_f64 %[N4], v[170:171], %[Y5]\n\t"                   \
+  "v_mul_f64 %[R], v[170:171], %[X5]\n\t"                    \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_mul_f64 %[DL2], v[212:213], %[L]\n\t"                   \
+  "v_mul_f64 %[L], v[148:149], %[L]\n\t"                     \
+  "v_mul_f64 %[M], v[148:149], %[M]\n\t"                     \
+  "v_mul_f64 %[N], v[148:149], %[N]\n\t"                     \
+  "s_set_gpr_idx_idx " #i3 "\n\t"                            \
+  "v_mul_f64 %[DR3], v[224:225], %[R]\n\t"                   \
+  "v_mul_f64 %[R], v[160:161], %[R]\n\t"                     \
+  "v_mul_f64 %[M4], v[160:161], %[M4]\n\t"                   \
+  "v_mul_f64 %[N4], v[160:161], %[N4]\n\t"                   \
+  "s_set_gpr_idx_off\n\t"                                    \
+  "v_fmac_f64 %[E], %[L], %[R]\n\t"                          \
+  "v_fmac_f64 %[G0], %[N], %[R]\n\t"                         \
+  "v_fmac_f64 %[G1], %[M], %[R]\n\t"                         \
+  "v_fmac_f64 %[G2], %[DL2], %[R]\n\t"                       \
+  "v_fmac_f64 %[G3], %[DR3], %[L]\n\t"                       \
+  "v_fmac_f64 %[G4], %[M4], %[L]\n\t"                        \
+  "v_fmac_f64 %[G5], %[N4], %[L]\n\t"
+
+// Sum over the allowed sets of one column: cp points at the column's entry
+// of colptr (first and end set), records are 32 bytes at sets8 = (lo, hi);
+// the two records after a column's last are read ahead and must exist.  Set
+// records are loaded one ahead into alternating SGPR blocks.
+__device__ __forceinline__ void set_loop4(const Tab4 &t, const int *cp, uint32_t lo, uint32_t hi, double &E,
+                                          double *G) {
+  double X0, Y0, X5, Y5, L, M, N, DL2, R, M4, N4, DR3;
+  double G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
+  asm volatile(
+      "s_mov_b32 s59, m0\n\t"
+      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_sub_i32 s58, s41, s40\n\t"
+      "s_cmp_lt_i32 s58, 1\n\t"
+      "s_cbranch_scc1 L_set4_done_%=\n\t"
+      "s_mov_b32 s41, 0\n\t"
+      "s_lshl_b64 s[40:41], s[40:41], 5\n\t"
+      "s_add_u32 s56, s40, %[lo]\n\t"
+      "s_addc_u32 s57, s41, %[hi]\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_set4_%=:\n\t"
+      "s_load_dwordx8 s[48:55], s[56:57], 0x20\n\t"
+      CG_SET4(s40, s41, s42, s43, s44, s45)
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_cbranch_scc1 L_set4_done_%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x40\n\t"
+      CG_SET4(s48, s49, s50, s51, s52, s53)
+      "s_add_u32 s56, s56, 0x40\n\t"
+      "s_addc_u32 s57, s57, 0\n\t"
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cbranch_scc0 L_set4_%=\n"
+      "L_set4_done_%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, s59\n\t"
+      : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
+        [X0] "=&v"(X0), [Y0] "=&v"(Y0), [X5] "=&v"(X5), [Y5] "=&v"(Y5), [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N),
+        [DL2] "=&v"(DL2), [R] "=&v"(R), [M4] "=&v"(M4), [N4] "=&v"(N4), [DR3] "=&v"(DR3)
+      : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[128:159]}"(t.FA), "{v[160:191]}"(t.FB), "{v[192:223]}"(t.DA),
+        "{v[224:255]}"(t.DB)
+      : "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s55",
+        "s56", "s57", "s58", "s59", "scc");
+  G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
+}
+#undef CG_SET4
+
 template <class T> __device__ __forceinline__ void lds_add(T *p, T v) { unsafeAtomicAdd(p, v); }
 
 // LDS tile geometry of k_assemble (shared with the host task builder).
@@ -821,11 +939,19 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
     const int j = threadIdx.x * R + rnd;
     const bool valid = j < ncl;
     const int ci = t.c0 + j;
-    PairTab<T, VW> tab[NP];
+    // 4-body FP64 with powers below 5 takes the hand-scheduled set loop
+    constexpr bool ASM4 = NB == 4 && VW <= 5 && std::is_same<T, double>::value;
+    PairTab<T, VW> tab[ASM4 ? 1 : NP];
+    Tab4 tab4;
     T rh[NP][3], rl[NP];  // unit pair vectors and lengths
     int atom[NB];
+    if constexpr (ASM4) tab4.zero();
+    else {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) { tab[k].zero(); rh[k][0] = rh[k][1] = rh[k][2] = (T)0; rl[k] = (T)1; }
+      for (int k = 0; k < NP; ++k) tab[k].zero();
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) { rh[k][0] = rh[k][1] = rh[k][2] = (T)0; rl[k] = (T)1; }
 #pragma unroll
     for (int s = 0; s < NB; ++s) atom[s] = 0;
     if (valid) {
@@ -845,16 +971,24 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
           rh[k][0] = (T)(v0 / r); rh[k][1] = (T)(v1 / r); rh[k][2] = (T)(v2 / r); rl[k] = (T)r;
           const int q = ty * NP + k;
-          pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
+          if constexpr (ASM4) {
+            Tab4Pair tp{tab4, k};
+            pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
+          } else {
+            pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
+          }
           ++k;
         }
     }
     for (int cc = 0; cc < W; ++cc) {
       const int col = colbase + cc;
-      const int pb = a.colptr[col], pe = a.colptr[col + 1];
       T e = (T)0, g[NP];
 #pragma unroll
       for (int k = 0; k < NP; ++k) g[k] = (T)0;
+      if constexpr (ASM4) {
+        set_loop4(tab4, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
+      } else {
+      const int pb = a.colptr[col], pe = a.colptr[col + 1];
       uint32_t wnext = pb < pe ? a.perm[pb] : 0u;
       for (int q = pb; q < pe; ++q) {
         const uint32_t w = wnext;
@@ -872,6 +1006,7 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
         e += pre[NP];
 #pragma unroll
         for (int k = 0; k < NP; ++k) g[k] += dv[k] * (pre[k] * suf[k + 1]);
+      }
       }
       if (valid) {
 #pragma unroll
@@ -1377,6 +1512,17 @@ static void build_family(Ctx &c, int which) {
   }
   F.colptr = dalloc_copy(colptr);
   F.perm = dalloc_copy(perm);
+  if (F.nb == 4 && F.max_pow <= 5) {
+    // records of the hand-scheduled 4-body set loop, plus the two it reads ahead
+    std::vector<int> s8((perm.size() + 2) * 8, 0);
+    for (size_t i = 0; i < perm.size(); ++i)
+      for (int k = 0; k < 6; ++k) {
+        const int pw = (perm[i] >> (4 * k)) & 15u;
+        if (pw >= F.max_pow) throw Error(CG_ERR_INPUT, "4-body power out of range");
+        s8[8 * i + k] = 2 * pw;
+      }
+    F.sets8 = dalloc_copy(s8);
+  }
   F.smin = dalloc_copy(smin); F.smax = dalloc_copy(smax);
   F.xavg = dalloc_copy(xavg); F.xdiff = dalloc_copy(xdiff); F.lambda = dalloc_copy(lam);
   std::vector<int> both(col_rel);
@@ -1396,6 +1542,7 @@ static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, con
   a.wrap = c.b_wrap.as<double>(0);
   a.refs = refs;
   a.perm = F.perm;
+  a.sets8 = F.sets8;
   a.colptr = F.colptr;
   a.col_rel = F.col_offset;
   a.col_abs = F.col_offset + F.ntypes;
@@ -1966,7 +2113,7 @@ void ctx_free(Ctx &c) {
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
                     (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.cct, (void *)F.csmax,
-                    (void *)F.cperm})
+                    (void *)F.cperm, (void *)F.sets8})
       if (p) (void)hipFree(p);
   }
   if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);
