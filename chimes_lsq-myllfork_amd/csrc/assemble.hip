@@ -907,6 +907,11 @@ __host__ __device__ constexpr size_t tile_bytes(int natoms, int w) {
 template <int NB, int VW, int BASE, class T, int BS>
 __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
+  // 4-body with powers below 5 takes the hand-scheduled FP64 set loop for
+  // both precisions (FP32 A is then rounded once, at the write-out); other
+  // families compute tables, products and projections in T
+  constexpr bool ASM4 = NB == 4 && VW <= 5;
+  using TP = typename std::conditional<ASM4, double, T>::type;
   // The tile is accumulated in FP64 for both precisions: LDS ds_add_f64 is
   // fast on gfx950 while ds_add_f32 stalls the LDS pipeline, and the FP32
   // path then rounds each A entry once, at the write-out.
@@ -925,7 +930,7 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
   __syncthreads();
   const int ty = t.type;
   const int colbase = a.col_rel[ty] + t.col0;
-  const T ps = (T)f.perm_scale[a.nb_scale];
+  const TP ps = (TP)f.perm_scale[a.nb_scale];
   const bool full_stress = a.stress_mode == CG_STRESS_ALL;
   const bool any_stress = a.stress_mode != CG_STRESS_NONE;
   const int rep = threadIdx.x & (SE_REP - 1);
@@ -940,10 +945,12 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
     const bool valid = j < ncl;
     const int ci = t.c0 + j;
     // 4-body FP64 with powers below 5 takes the hand-scheduled set loop
-    constexpr bool ASM4 = NB == 4 && VW <= 5 && std::is_same<T, double>::value;
     PairTab<T, VW> tab[ASM4 ? 1 : NP];
     Tab4 tab4;
-    T rh[NP][3], rl[NP];  // unit pair vectors and lengths
+    // positions relative to the cluster's first atom and, per pair,
+    // PERM_SCALE / r: pair vector v = d[y] - d[x] (d[0] = 0), force on x is
+    // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
+    TP dpos[NB][3], sc[NP];
     int atom[NB];
     if constexpr (ASM4) tab4.zero();
     else {
@@ -951,9 +958,9 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
       for (int k = 0; k < NP; ++k) tab[k].zero();
     }
 #pragma unroll
-    for (int k = 0; k < NP; ++k) { rh[k][0] = rh[k][1] = rh[k][2] = (T)0; rl[k] = (T)1; }
+    for (int k = 0; k < NP; ++k) sc[k] = (TP)0;
 #pragma unroll
-    for (int s = 0; s < NB; ++s) atom[s] = 0;
+    for (int s = 0; s < NB; ++s) { atom[s] = 0; dpos[s][0] = dpos[s][1] = dpos[s][2] = (TP)0; }
     if (valid) {
       double p[NB][3];
 #pragma unroll
@@ -962,6 +969,10 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
         atom[s] = ref_atom(r);
         image_pos(f, a.wrap, r, p[s]);
       }
+#pragma unroll
+      for (int s = 1; s < NB; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dpos[s][c] = (TP)(p[s][c] - p[0][c]);
       int k = 0;
 #pragma unroll
       for (int x = 0; x < NB; ++x)
@@ -969,11 +980,11 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
         for (int y = x + 1; y < NB; ++y) {
           double v0 = p[y][0] - p[x][0], v1 = p[y][1] - p[x][1], v2 = p[y][2] - p[x][2];
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-          rh[k][0] = (T)(v0 / r); rh[k][1] = (T)(v1 / r); rh[k][2] = (T)(v2 / r); rl[k] = (T)r;
+          sc[k] = (TP)((double)ps / r);
           const int q = ty * NP + k;
           if constexpr (ASM4) {
             Tab4Pair tp{tab4, k};
-            pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
+            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
           } else {
             pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
           }
@@ -982,7 +993,7 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
     }
     for (int cc = 0; cc < W; ++cc) {
       const int col = colbase + cc;
-      T e = (T)0, g[NP];
+      TP e = (TP)0, g[NP];
 #pragma unroll
       for (int k = 0; k < NP; ++k) g[k] = (T)0;
       if constexpr (ASM4) {
@@ -1009,25 +1020,32 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
       }
       }
       if (valid) {
+        // force on each canonical atom: sum over its pairs of +-g v / r
+        TP F[NB][3];
 #pragma unroll
-        for (int k = 0; k < NP; ++k) g[k] *= ps;
-        // force on each canonical atom: sum over its pairs of +-g * rhat
-#pragma unroll
-        for (int s = 0; s < NB; ++s) {
-          T fx0 = 0, fx1 = 0, fx2 = 0;
+        for (int s = 0; s < NB; ++s) F[s][0] = F[s][1] = F[s][2] = (TP)0;
+        {
           int k = 0;
 #pragma unroll
           for (int x = 0; x < NB; ++x)
 #pragma unroll
             for (int y = x + 1; y < NB; ++y) {
-              if (x == s) { fx0 += g[k] * rh[k][0]; fx1 += g[k] * rh[k][1]; fx2 += g[k] * rh[k][2]; }
-              if (y == s) { fx0 -= g[k] * rh[k][0]; fx1 -= g[k] * rh[k][1]; fx2 -= g[k] * rh[k][2]; }
+              const TP gk = g[k] * sc[k];
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const TP v = x == 0 ? dpos[y][c] : dpos[y][c] - dpos[x][c];
+                F[x][c] += gk * v;
+                F[y][c] -= gk * v;
+              }
               ++k;
             }
+        }
+#pragma unroll
+        for (int s = 0; s < NB; ++s) {
           double *dst = acc + (3 * atom[s]) * Wp + cc;
-          lds_add(dst, (double)fx0);
-          lds_add(dst + Wp, (double)fx1);
-          lds_add(dst + 2 * Wp, (double)fx2);
+          lds_add(dst, (double)F[s][0]);
+          lds_add(dst + Wp, (double)F[s][1]);
+          lds_add(dst + 2 * Wp, (double)F[s][2]);
         }
         if (any_stress) {
           const int nsc = full_stress ? 6 : 3;
@@ -1036,9 +1054,10 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
             if (comp >= nsc) break;
             const int ia = comp < 3 ? comp : (comp == 5 ? 1 : 0);
             const int ib = comp < 3 ? comp : (comp == 3 ? 1 : 2);
-            T sv = 0;
+            // -sum_pairs g r_a r_b / r = sum_atoms F_a (p - p_0)_b
+            TP sv = 0;
 #pragma unroll
-            for (int k = 0; k < NP; ++k) sv -= g[k] * rl[k] * rh[k][ia] * rh[k][ib];
+            for (int s = 1; s < NB; ++s) sv += F[s][ia] * dpos[s][ib];
             lds_add(&se[(comp * SE_REP + rep) * Wp + cc], (double)sv);
           }
         }
