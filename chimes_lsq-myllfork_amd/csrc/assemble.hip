@@ -658,7 +658,7 @@ struct AsmArgs {
   const double *wrap;
   const uint32_t *refs;     // bucketed cluster refs [ncl*NB]
   const uint32_t *perm;     // packed powers
-  const int *sets8;         // 4-body: per allowed set 8 ints, 2 * power of pairs 0..5, then 0, 0
+  const int *sets8;         // set records: 4-body 8 ints (2 * power of pairs 0..5, 0, 0), 3-body 4 ints
   const int *colptr;        // [family ncols + 1]
   const int *col_rel;       // [ntypes] family-relative first column of a type
   const int *col_abs;       // [ntypes] absolute first column of a type
@@ -895,6 +895,94 @@ __device__ __forceinline__ void set_loop4(const Tab4 &t, const int *cp, uint32_t
 }
 #undef CG_SET4
 
+// 3-body FP64 tables for the hand-scheduled set loop (powers below 9): F
+// of pair k at v[128 + 18k], D at v[182 + 18k], entry of power p at 2p, in
+// three 32-dword blocks and one 12-dword block held in v[128:235].
+struct Tab3 {
+  v32u B[3];
+  vecN<uint32_t, 12> B3;
+  __device__ __forceinline__ void zero() { B[0] = 0u; B[1] = 0u; B[2] = 0u; B3 = 0u; }
+};
+struct Tab3Pair {
+  Tab3 &t;
+  int k;
+  __device__ __forceinline__ void put(int o, double x) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    if (o >= 96) {
+      t.B3[o - 96] = (uint32_t)b;
+      t.B3[o - 95] = (uint32_t)(b >> 32);
+    } else {
+      t.B[o >> 5][o & 31] = (uint32_t)b;
+      t.B[o >> 5][(o & 31) + 1] = (uint32_t)(b >> 32);
+    }
+  }
+  __device__ __forceinline__ void set(int i, double f, double d) {
+    put(18 * k + 2 * i, f);
+    put(54 + 18 * k + 2 * i, d);
+  }
+};
+
+// One allowed 3-body set: E += F0 F1 F2, G_k += D_k prod_{j != k} F_j.
+#define CG_SET3(i0, i1, i2)                                  \
+  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
+  "v_mov_b64 %[X0], v[128:129]\n\t"                          \
+  "v_mov_b64 %[Y0], v[182:183]\n\t"                          \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_mul_f64 %[L], v[146:147], %[X0]\n\t"                    \
+  "v_mul_f64 %[M], v[200:201], %[X0]\n\t"                    \
+  "v_mul_f64 %[N], v[146:147], %[Y0]\n\t"                    \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_fmac_f64 %[E], v[164:165], %[L]\n\t"                    \
+  "v_fmac_f64 %[G2], v[218:219], %[L]\n\t"                   \
+  "v_fmac_f64 %[G1], v[164:165], %[M]\n\t"                   \
+  "v_fmac_f64 %[G0], v[164:165], %[N]\n\t"                   \
+  "s_set_gpr_idx_off\n\t"
+
+// As set_loop4 for 3-body records of 16 bytes (2 * power of pairs 0..2, 0).
+__device__ __forceinline__ void set_loop3(const Tab3 &t, const int *cp, uint32_t lo, uint32_t hi, double &E,
+                                          double *G) {
+  double X0, Y0, L, M, N;
+  double G0 = G[0], G1 = G[1], G2 = G[2];
+  asm volatile(
+      "s_mov_b32 s59, m0\n\t"
+      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_sub_i32 s58, s41, s40\n\t"
+      "s_cmp_lt_i32 s58, 1\n\t"
+      "s_cbranch_scc1 L_set3_done_%=\n\t"
+      "s_mov_b32 s41, 0\n\t"
+      "s_lshl_b64 s[40:41], s[40:41], 4\n\t"
+      "s_add_u32 s56, s40, %[lo]\n\t"
+      "s_addc_u32 s57, s41, %[hi]\n\t"
+      "s_load_dwordx4 s[40:43], s[56:57], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_set3_%=:\n\t"
+      "s_load_dwordx4 s[44:47], s[56:57], 0x10\n\t"
+      CG_SET3(s40, s41, s42)
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_cbranch_scc1 L_set3_done_%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_load_dwordx4 s[40:43], s[56:57], 0x20\n\t"
+      CG_SET3(s44, s45, s46)
+      "s_add_u32 s56, s56, 0x20\n\t"
+      "s_addc_u32 s57, s57, 0\n\t"
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cbranch_scc0 L_set3_%=\n"
+      "L_set3_done_%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, s59\n\t"
+      : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [X0] "=&v"(X0), [Y0] "=&v"(Y0), [L] "=&v"(L),
+        [M] "=&v"(M), [N] "=&v"(N)
+      : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[128:159]}"(t.B[0]), "{v[160:191]}"(t.B[1]),
+        "{v[192:223]}"(t.B[2]), "{v[224:235]}"(t.B3)
+      : "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s56", "s57", "s58", "s59", "scc");
+  G[0] = G0; G[1] = G1; G[2] = G2;
+}
+#undef CG_SET3
+
 template <class T> __device__ __forceinline__ void lds_add(T *p, T v) { unsafeAtomicAdd(p, v); }
 
 // LDS tile geometry of k_assemble (shared with the host task builder).
@@ -907,11 +995,13 @@ __host__ __device__ constexpr size_t tile_bytes(int natoms, int w) {
 template <int NB, int VW, int BASE, class T, int BS>
 __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
-  // 4-body with powers below 5 takes the hand-scheduled FP64 set loop for
-  // both precisions (FP32 A is then rounded once, at the write-out); other
-  // families compute tables, products and projections in T
+  // 4-body with powers below 5 and 3-body with powers below 9 take the
+  // hand-scheduled FP64 set loops for both precisions (FP32 A is then
+  // rounded once, at the write-out); other kernels compute tables, products
+  // and projections in T
   constexpr bool ASM4 = NB == 4 && VW <= 5;
-  using TP = typename std::conditional<ASM4, double, T>::type;
+  constexpr bool ASM3 = NB == 3 && VW <= 9;
+  using TP = typename std::conditional<ASM4 || ASM3, double, T>::type;
   // The tile is accumulated in FP64 for both precisions: LDS ds_add_f64 is
   // fast on gfx950 while ds_add_f32 stalls the LDS pipeline, and the FP32
   // path then rounds each A entry once, at the write-out.
@@ -945,14 +1035,16 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
     const bool valid = j < ncl;
     const int ci = t.c0 + j;
     // 4-body FP64 with powers below 5 takes the hand-scheduled set loop
-    PairTab<T, VW> tab[ASM4 ? 1 : NP];
+    PairTab<T, VW> tab[ASM4 || ASM3 ? 1 : NP];
     Tab4 tab4;
+    Tab3 tab3;
     // positions relative to the cluster's first atom and, per pair,
     // PERM_SCALE / r: pair vector v = d[y] - d[x] (d[0] = 0), force on x is
     // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
     TP dpos[NB][3], sc[NP];
     int atom[NB];
     if constexpr (ASM4) tab4.zero();
+    else if constexpr (ASM3) tab3.zero();
     else {
 #pragma unroll
       for (int k = 0; k < NP; ++k) tab[k].zero();
@@ -985,6 +1077,9 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
           if constexpr (ASM4) {
             Tab4Pair tp{tab4, k};
             pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
+          } else if constexpr (ASM3) {
+            Tab3Pair tp{tab3, k};
+            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
           } else {
             pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
           }
@@ -998,6 +1093,8 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
       for (int k = 0; k < NP; ++k) g[k] = (T)0;
       if constexpr (ASM4) {
         set_loop4(tab4, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
+      } else if constexpr (ASM3) {
+        set_loop3(tab3, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
       } else {
       const int pb = a.colptr[col], pe = a.colptr[col + 1];
       uint32_t wnext = pb < pe ? a.perm[pb] : 0u;
@@ -1531,6 +1628,17 @@ static void build_family(Ctx &c, int which) {
   }
   F.colptr = dalloc_copy(colptr);
   F.perm = dalloc_copy(perm);
+  if (F.nb == 3 && F.max_pow <= 9) {
+    // records of the hand-scheduled 3-body set loop, plus the two it reads ahead
+    std::vector<int> s4((perm.size() + 2) * 4, 0);
+    for (size_t i = 0; i < perm.size(); ++i)
+      for (int k = 0; k < 3; ++k) {
+        const int pw = (perm[i] >> (4 * k)) & 15u;
+        if (pw >= F.max_pow) throw Error(CG_ERR_INPUT, "3-body power out of range");
+        s4[4 * i + k] = 2 * pw;
+      }
+    F.sets8 = dalloc_copy(s4);
+  }
   if (F.nb == 4 && F.max_pow <= 5) {
     // records of the hand-scheduled 4-body set loop, plus the two it reads ahead
     std::vector<int> s8((perm.size() + 2) * 8, 0);
@@ -1590,6 +1698,7 @@ static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, con
   } else if (which == 1) {
     if (F.max_pow <= 4) CG_LAUNCH(3, 4, 0);
     else if (F.max_pow <= 8) CG_LAUNCH(3, 8, 0);
+    else if (F.max_pow <= 9) CG_LAUNCH(3, 9, 0);
     else if (F.max_pow <= 12) CG_LAUNCH(3, 12, 0);
     else if (F.max_pow <= 16) CG_LAUNCH(3, 16, 0);
     else throw Error(CG_ERR_UNSUPPORTED, "3-body order above 16 is not supported");

@@ -75,7 +75,7 @@ struct DevFamily {
   int *ncol_type = nullptr;   // [ntypes] columns per type
   int *colptr = nullptr;      // [ncols+1] first perm of each family column (family-relative)
   uint32_t *perm = nullptr;   // [nperm] packed powers, 4 bits per canonical pair
-  int *sets8 = nullptr;       // 4-body, powers < 5: [nperm + 2][8] set records (2 * power per pair)
+  int *sets8 = nullptr;       // set records (2 * power per pair): 4-body powers < 5 [nperm + 2][8], 3-body powers < 9 [nperm + 2][4]
   double *smin = nullptr, *smax = nullptr, *xavg = nullptr, *xdiff = nullptr, *lambda = nullptr; // [ntypes*np]
   int *cct = nullptr;         // [ntypes_atom^nb] cluster type of a type tuple, -1 if none
   double *csmax = nullptr;    // [ntypes_atom^nb * np] S_MAXIM of each enumerated pair slot
