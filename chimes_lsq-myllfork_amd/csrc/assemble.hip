@@ -176,7 +176,7 @@ template <bool FILL>
 __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap,
                                                 const int *type, const int *bin_start, const int *bin_atom,
                                                 int natoms, double rc, int L, int *count, const int *offset,
-                                                uint32_t *list, double4 *nvec, int *nown, int *ntyp) {
+                                                uint32_t *list, double4 *nvec, int *nown, int *ntyp, int cap) {
   const int gi = blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (gi >= natoms) return;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int
         }
         int tot;
         const int pos = run + wave_excl_scan(n, tot);
-        if (FILL && n) {
+        if (FILL && n && pos < cap) {
           list[pos] = make_ref(j, sh[0], sh[1], sh[2]);
           nvec[pos] = make_double4(v[0], v[1], v[2], sqrt(r2));
           nown[pos] = gi;
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int
               v[r] = wj[r] - wi[r] + f.H[3 * r] * sx + f.H[3 * r + 1] * sy + f.H[3 * r + 2] * sz;
             const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
             if (r2 < rc2) {
-              if (pass == 1) {
+              if (pass == 1 && base + m < cap) {
                 list[base + m] = make_ref(j, sx, sy, sz);
                 nvec[base + m] = make_double4(v[0], v[1], v[2], sqrt(r2));
                 nown[base + m] = gi;
@@ -297,6 +297,7 @@ struct EnumArgs {
   const int *nown;      // owning atom per entry
   const int *ntyp;      // parent atom type per entry
   int nnb, natoms, ntypes_atom, nkeys_type;   // nnb: capacity of the entry arrays
+  int ccap;             // capacity of the cluster arrays (fill passes write below it)
   const int *ntot;      // device-side number of valid entries (<= nnb)
   double cutoff;
   int *count;           // [nnb]
@@ -363,8 +364,8 @@ __global__ __launch_bounds__(256) void k_enum2(EnumArgs a) {
     const int p = a.int_pair_map[a.type[gi] * a.ntypes_atom + a.ntyp[k2]];
     const double r = v2.w;
     if (r < a.smax[p] && (a.md || r > a.smin[p])) {
-      if (FILL) {
-        const int out = a.coff[k2];
+      const int out = FILL ? a.coff[k2] : 0;
+      if (FILL && out < a.ccap) {
         a.refs[2 * out] = make_ref(gi - f.atom_off, 0, 0, 0);
         a.refs[2 * out + 1] = r2;
         a.key[out] = fidx * a.nkeys_type + p;
@@ -382,7 +383,8 @@ __global__ __launch_bounds__(256) void k_enum2(EnumArgs a) {
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_sub(const int *noff, const uint32_t *nlist, const double4 *nvec,
                                              const int *nown, const int *ntyp, int natoms, double cut, int *cnt,
-                                             const int *soff, uint32_t *slist, double4 *svec, int *sown, int *styp) {
+                                             const int *soff, uint32_t *slist, double4 *svec, int *sown, int *styp,
+                                             int cap) {
   const int gi = blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (gi > natoms) return;
@@ -390,7 +392,9 @@ __global__ __launch_bounds__(256) void k_sub(const int *noff, const uint32_t *nl
     if (COUNT && lane == 0) cnt[gi] = 0;
     return;
   }
-  const int b = noff[gi], e = noff[gi + 1];
+  // entries past the list capacity were not stored (the caller detects the
+  // overflow from the device totals and repeats the call with exact sizes)
+  const int b = min(noff[gi], cap), e = min(noff[gi + 1], cap);
   int run = COUNT ? 0 : soff[gi];
   for (int k0 = b; k0 < e; k0 += 64) {
     const int k = k0 + lane;
@@ -439,16 +443,15 @@ struct StageGlobal {
 constexpr int ENUM_ADJ_WORDS = (ENUM_STAGE + 63) / 64;
 
 template <int NB, bool COUNT, class S>
-__device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, bool ordered, uint32_t r1, int t1,
-                                         int kbase, int out, const unsigned long long *adj) {
+__device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, int qa, int qb, bool ordered,
+                                         uint32_t r1, int t1, int kbase, int out, const unsigned long long *adj) {
   const int lane = threadIdx.x & 63;
   const double cut = a.cutoff;
-  const int npair = n * n;
   int run = 0;
-  for (int q0 = 0; q0 < npair; q0 += 64) {
+  for (int q0 = qa; q0 < qb; q0 += 64) {
     const int q = q0 + lane;
     int k2 = 0, k3 = 0;
-    bool ok = q < npair;
+    bool ok = q < qb;
     double4 v2 = make_double4(0, 0, 0, 0), v3 = v2;
     uint32_t r2 = 0, r3 = 0;
     double r23 = 0.0;
@@ -497,7 +500,7 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
           const int *perm;
           const int ct = cluster_type_c<3>(a, types, d, perm);
           if (ct >= 0) {
-            if (pass == 1) {
+            if (pass == 1 && base + m < a.ccap) {
               const uint32_t rr[3] = {r1, r2, r3};
               for (int x = 0; x < 3; ++x) a.refs[3 * (base + m) + x] = rr[perm[x]];
               a.key[base + m] = kbase + ct;
@@ -543,7 +546,7 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
               for (int w = 0; w < ENUM_ADJ_WORDS; ++w)
                 if (k4 / 64 == w) kept[w] |= 1ull << (k4 % 64);
             }
-            if (pass == 1) {
+            if (pass == 1 && base + m < a.ccap) {
               const uint32_t rr[4] = {r1, r2, r3, r4};
               for (int x = 0; x < 4; ++x) a.refs[4 * (base + m) + x] = rr[perm[x]];
               a.key[base + m] = kbase + ct;
@@ -563,57 +566,60 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
   return run;
 }
 
+// One workgroup per anchor atom: the atom's list is staged in LDS (and, for
+// 4-body, its adjacency bitmasks built) by the whole workgroup, then wave w
+// takes the w-th quarter of the (k2, k3) entry pairs.  Counts and outputs are
+// per (atom, wave), so the clusters of an atom keep their (k2, k3, k4) order.
 template <int NB, bool COUNT>
 __global__ __launch_bounds__(256) void k_enum_wave(EnumArgs a) {
-  __shared__ double4 s_vec[4][ENUM_STAGE];
-  __shared__ uint32_t s_ref[4][ENUM_STAGE];
-  __shared__ int s_typ[4][ENUM_STAGE];
+  __shared__ double4 s_vec[ENUM_STAGE];
+  __shared__ uint32_t s_ref[ENUM_STAGE];
+  __shared__ int s_typ[ENUM_STAGE];
   // 4-body: bit l of row k set when entries k and l are closer than the cutoff
-  __shared__ unsigned long long s_adj[NB == 4 ? 4 : 1][NB == 4 ? ENUM_STAGE * ENUM_ADJ_WORDS : 1];
+  __shared__ unsigned long long s_adj[NB == 4 ? ENUM_STAGE * ENUM_ADJ_WORDS : 1];
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  const int gi = blockIdx.x * 4 + wave;
-  const bool active = gi < a.natoms;
-  int b = 0, n = 0;
-  if (active) {
-    b = a.noff[gi];
-    n = a.noff[gi + 1] - b;
-  }
+  const int gi = blockIdx.x;
+  if (gi >= a.natoms) return;  // whole workgroup
+  const int b = min(a.noff[gi], a.nnb);
+  const int n = min(a.noff[gi + 1], a.nnb) - b;
   const bool staged = n <= ENUM_STAGE;
-  if (active && staged)
-    for (int k = lane; k < n; k += 64) {
-      s_vec[wave][k] = a.nvec[b + k];
-      s_ref[wave][k] = a.nlist[b + k];
-      s_typ[wave][k] = a.ntyp[b + k];
+  if (staged)
+    for (int k = threadIdx.x; k < n; k += 256) {
+      s_vec[k] = a.nvec[b + k];
+      s_ref[k] = a.nlist[b + k];
+      s_typ[k] = a.ntyp[b + k];
     }
   __syncthreads();
-  if (NB == 4 && active && staged) {
-    // one row per lane (rows k, k+64, ...), written by its owner lane only
-    for (int k = lane; k < n; k += 64) {
-      const double4 vk = s_vec[wave][k];
+  if (NB == 4 && staged) {
+    // one row per thread (rows k, k+256, ...), written by its owner only
+    for (int k = threadIdx.x; k < n; k += 256) {
+      const double4 vk = s_vec[k];
       for (int w = 0; w < ENUM_ADJ_WORDS; ++w) {
         unsigned long long m = 0;
         for (int l = w * 64; l < min(n, w * 64 + 64); ++l)
-          if (l != k && dist3(vk, s_vec[wave][l]) < a.cutoff) m |= 1ull << (l - w * 64);
-        s_adj[NB == 4 ? wave : 0][k * ENUM_ADJ_WORDS + w] = m;
+          if (l != k && dist3(vk, s_vec[l]) < a.cutoff) m |= 1ull << (l - w * 64);
+        s_adj[k * ENUM_ADJ_WORDS + w] = m;
       }
     }
   }
   __syncthreads();
-  if (!active) return;
   const int fidx = a.atom_frame[gi];
   const DevFrame &f = a.frames[fidx];
   const uint32_t r1 = make_ref(gi - f.atom_off, 0, 0, 0);
   const int t1 = a.type[gi];
   const int kbase = fidx * a.nkeys_type;
-  const int out = COUNT ? 0 : a.coff[gi];
+  const int slot = 4 * gi + wave;
+  const int out = COUNT ? 0 : a.coff[slot];
+  const int npair = n * n, per = (npair + 3) / 4;
+  const int qa = min(npair, wave * per), qb = min(npair, qa + per);
   int tot;
   if (staged)
-    tot = enum_atom<NB, COUNT>(a, StageLds{s_vec[wave], s_ref[wave], s_typ[wave]}, n, !f.small, r1, t1, kbase, out,
-                               NB == 4 ? s_adj[NB == 4 ? wave : 0] : nullptr);
+    tot = enum_atom<NB, COUNT>(a, StageLds{s_vec, s_ref, s_typ}, n, qa, qb, !f.small, r1, t1, kbase, out,
+                               NB == 4 ? s_adj : nullptr);
   else
-    tot = enum_atom<NB, COUNT>(a, StageGlobal{a.nvec + b, a.nlist + b, a.ntyp + b}, n, !f.small, r1, t1, kbase, out,
-                               nullptr);
-  if (COUNT && lane == 0) a.count[gi] = tot;
+    tot = enum_atom<NB, COUNT>(a, StageGlobal{a.nvec + b, a.nlist + b, a.ntyp + b}, n, qa, qb, !f.small, r1, t1,
+                               kbase, out, nullptr);
+  if (COUNT && lane == 0) a.count[slot] = tot;
 }
 
 __global__ void k_iota(int *v, int n) {
@@ -643,6 +649,24 @@ __global__ void k_bucket_bounds(const int *key, int ncl, int nkeys, int *boff) {
   boff[q] = lo;
 }
 
+// Keys of the slots in [*tot, cap) of a capacity-sized cluster list get the
+// sentinel key (one past the last bucket), so that sorting and bucket bounds
+// over the whole capacity see only the valid clusters in the buckets.
+__global__ void k_pad_keys(int *key, const int *tot, int cap, int sentinel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap && i >= *tot) key[i] = sentinel;
+}
+
+// Device-side task list of the force kernel: chunks of at most `chunk`
+// clusters of one (frame, type) bucket.  k_task_counts gives the chunks per
+// bucket (scanned into toff, toff[nkeys] = number of tasks), k_task_fill one
+// task per thread.
+__global__ void k_task_counts(const int *bounds, int nkeys, int chunk, int *cnt) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nkeys) cnt[q] = (bounds[q + 1] - bounds[q] + chunk - 1) / chunk;
+  else if (q == nkeys) cnt[q] = 0;
+}
+
 // ------------------------------------------------------------ assembly kernel
 struct Task {
   int frame;
@@ -651,6 +675,29 @@ struct Task {
   int ncols;   // tile width
   int c0, c1;  // cluster range (bucketed arrays)
 };
+
+__global__ void k_task_fill(const int *bounds, const int *toff, int nkeys, int ntypes, int chunk, int max_tasks,
+                            Task *tasks) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= max_tasks || t >= toff[nkeys]) return;
+  int lo = 0, hi = nkeys - 1;  // last bucket with toff[q] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (toff[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const int j = t - toff[lo];
+  const int c0 = bounds[lo] + j * chunk;
+  tasks[t] = Task{lo / ntypes, lo % ntypes, 0, 0, c0, min(bounds[lo + 1], c0 + chunk)};
+}
+
+// Totals of a call gathered for one copy back: entries of the neighbour list
+// and clusters of each family (-1 where a pointer is null).
+struct TotalPtrs { const int *p[4]; };
+__global__ void k_totals(TotalPtrs t, double *out) {
+  const int q = threadIdx.x;
+  if (q < 4) out[q] = t.p[q] ? (double)*t.p[q] : -1.0;
+}
 
 struct AsmArgs {
   const Task *tasks;
@@ -1201,6 +1248,7 @@ __global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
 // energy and virial are reduced over the workgroup.
 struct ForceArgs {
   const Task *tasks;
+  const int *ntasks;     // device-side task count (the grid may be larger)
   const DevFrame *frames;
   const double *wrap;
   const uint32_t *refs;
@@ -1405,6 +1453,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
   // once into LDS (uniform-address reads are broadcasts) instead of through
   // the scalar cache, which thrashes between the tensors of different types.
   __shared__ double sC[STAGE ? TS : 1];
+  if ((int)blockIdx.x >= *a.ntasks) return;  // whole workgroup, before any barrier
   const Task t = a.tasks[blockIdx.x];
   const DevFrame &f = a.frames[t.frame];
   const int ty = t.type;
@@ -1728,9 +1777,19 @@ static int read_int(Ctx &c, const int *d) {
 }
 
 struct FamOut {
-  int ncl = 0;
+  int ncl = 0;                 // clusters (exact mode) or capacity of the list (capacity mode)
   uint32_t *refs = nullptr;
-  std::vector<int> boff;
+  std::vector<int> boff;       // host bucket bounds (exact mode only)
+  int *d_bounds = nullptr;     // device bucket bounds [nkeys + 1]
+  int nkeys = 0;
+};
+
+// Sizes of a call without host reads: capacity of the neighbour list and of
+// each family's cluster list.  The kernels write nothing past them; the
+// device totals tell afterwards whether the call fitted.
+struct Caps {
+  int nnb = 0;
+  int ncl[3] = {0, 0, 0};
 };
 
 struct Batch {
@@ -1740,13 +1799,15 @@ struct Batch {
   DevFrame *d_frames = nullptr;
   int *d_type = nullptr;
   FamOut fo[3];
+  TotalPtrs tot{};             // device totals: neighbour entries, clusters per family
 };
 
 // Upload a batch of frames, wrap atoms, build neighbour lists and enumerate
 // + bucket the clusters of every family.  md != 0 selects the force-evaluation
 // pair set (every pair below S_MAXIM, Cheby::Force_all) instead of the
 // least-squares one (S_MINIM < r < S_MAXIM, Cheby::Deriv_2B).
-static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B, bool sticky = true) {
+static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B, bool sticky = true,
+                    const Caps *caps = nullptr) {
   std::vector<DevFrame> &hf = B.hf;
   const cg_model &m = c.m;
   if (nframes <= 0) throw Error(CG_ERR_ARG, "no frames");
@@ -1860,10 +1921,13 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
   hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
                      (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, d_ncount,
-                     (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr, (int *)nullptr);
+                     (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr, (int *)nullptr, 0);
   exclusive_scan(c, d_ncount, d_noff, natoms);
-  const int nnb = read_int(c, d_noff + natoms);                               // host sync 1
+  // exact mode reads the list size back (host sync); capacity mode sizes the
+  // list from the caller's capacity
+  const int nnb = caps ? caps->nnb : read_int(c, d_noff + natoms);
   const size_t nn1 = std::max(nnb, 1);
+  B.tot = TotalPtrs{{d_noff + natoms, nullptr, nullptr, nullptr}};
   uint32_t *d_nlist = c.b_nlist.as<uint32_t>(nn1);
   double4 *d_nvec = c.b_nvec.as<double4>(nn1);
   int *d_nown = c.b_nown.as<int>(2 * nn1);
@@ -1871,7 +1935,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   if (!d_nlist || !d_nvec || !d_nown) throw Error(CG_ERR_MEMORY, "neighbour list");
   hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
                      (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, (int *)nullptr,
-                     (const int *)d_noff, d_nlist, d_nvec, d_nown, d_ntyp);
+                     (const int *)d_noff, d_nlist, d_nvec, d_nown, d_ntyp, (int)nn1);
   // ---- per-family sublists (3B / 4B cutoffs below the list cutoff), sized
   // by the full list so that no host read is needed
   const uint32_t *fl_list[3] = {d_nlist, d_nlist, d_nlist};
@@ -1896,23 +1960,24 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     int *soff = scnt + natoms + 1;
     hipLaunchKernelGGL(k_sub<true>, wg, wb, 0, c.stream, d_noff, d_nlist, d_nvec, d_nown, d_ntyp, natoms, F.cutoff,
                        scnt, (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr,
-                       (int *)nullptr);
+                       (int *)nullptr, (int)nn1);
     exclusive_scan(c, scnt, soff, natoms);
     hipLaunchKernelGGL(k_sub<false>, wg, wb, 0, c.stream, d_noff, d_nlist, d_nvec, d_nown, d_ntyp, natoms, F.cutoff,
-                       (int *)nullptr, (const int *)soff, slist, svec, sown, styp);
+                       (int *)nullptr, (const int *)soff, slist, svec, sown, styp, (int)nn1);
     fl_list[w] = slist; fl_vec[w] = svec; fl_own[w] = sown; fl_typ[w] = styp; fl_off[w] = soff;
   }
   // ---- per-family cluster counts: 2-body one thread per neighbour entry,
   // 3/4-body one wave per atom over its LDS-staged list
   FamOut *fo = B.fo;
-  const size_t cslot = std::max(nn1, (size_t)natoms) + 1;
+  const size_t cslot = std::max(nn1, 4 * (size_t)natoms) + 1;
   int *d_ccount = c.b_ccount.as<int>(3 * cslot);
   int *d_coff = c.b_coff.as<int>(3 * cslot);
   if (!d_ccount || !d_coff) throw Error(CG_ERR_MEMORY, "cluster counts");
   EnumArgs ea[3] = {};
   bool active[3] = {false, false, false};
-  int nitems[3] = {0, 0, 0};  // count slots: entries (2B) or atoms (3B/4B)
+  int nitems[3] = {0, 0, 0};  // count slots: entries (2B) or (atom, wave) pairs (3B/4B)
   const dim3 eg((unsigned)((nnb + TB - 1) / TB)), eb(TB);
+  const dim3 ag((unsigned)natoms), ab(256);  // 3/4-body enumeration: one workgroup per atom
   for (int w = 0; w < 3; ++w) {
     const DevFamily &F = c.fam[w];
     if (F.ntypes == 0 || F.ncols == 0 || nnb == 0) continue;
@@ -1927,19 +1992,25 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     E.cct = F.cct; E.csmax = F.csmax; E.cperm = F.cperm;
     E.smin = F.smin; E.smax = F.smax; E.int_pair_map = c.d_int_pair_map;
     E.md = md;
-    nitems[w] = w == 0 ? nnb : natoms;
+    nitems[w] = w == 0 ? nnb : 4 * natoms;
     CG_HIP(hipMemsetAsync(E.count + nitems[w], 0, sizeof(int), c.stream));
     if (w == 0) hipLaunchKernelGGL((k_enum2<false>), eg, eb, 0, c.stream, E);
-    else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, true>), wg, wb, 0, c.stream, E);
-    else hipLaunchKernelGGL((k_enum_wave<4, true>), wg, wb, 0, c.stream, E);
+    else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, true>), ag, ab, 0, c.stream, E);
+    else hipLaunchKernelGGL((k_enum_wave<4, true>), ag, ab, 0, c.stream, E);
     CG_HIP(hipGetLastError());
     exclusive_scan(c, E.count, const_cast<int *>(E.coff), nitems[w]);
   }
   int tot[3] = {0, 0, 0};
   for (int w = 0; w < 3; ++w)
-    if (active[w])
-      CG_HIP(hipMemcpyAsync(&tot[w], ea[w].coff + nitems[w], sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  CG_HIP(hipStreamSynchronize(c.stream));                                    // host sync 2
+    if (active[w]) B.tot.p[1 + w] = ea[w].coff + nitems[w];
+  if (caps) {
+    for (int w = 0; w < 3; ++w) tot[w] = caps->ncl[w];
+  } else {
+    for (int w = 0; w < 3; ++w)
+      if (active[w])
+        CG_HIP(hipMemcpyAsync(&tot[w], ea[w].coff + nitems[w], sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CG_HIP(hipStreamSynchronize(c.stream));                                  // host sync 2 (exact mode)
+  }
   // ---- fill, bucket by (frame, type), bucket bounds
   std::vector<int> hbounds[3];
   int *d_bounds[3] = {nullptr, nullptr, nullptr};
@@ -1954,6 +2025,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     const int nb = F.nb;
     const int nkeys = nframes * F.ntypes;
     const bool sort = F.ntypes > 1 && ncl > 1;
+    // capacity mode: slots past the device total carry the sentinel key
     const size_t n1 = std::max(ncl, 1);
     // refs [n1*nb] | key [n1] | bounds [nkeys+1] | (sort) key2 [n1] | idx [n1] | idx2 [n1] | refs2 [n1*nb]
     const size_t words = n1 * nb + n1 + (size_t)nkeys + 1 + (sort ? n1 * (3 + nb) : 0);
@@ -1965,12 +2037,18 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     d_bounds[w] = E.key + n1;
     fo[w].ncl = ncl;
     fo[w].refs = E.refs;
+    fo[w].nkeys = nkeys;
+    fo[w].d_bounds = d_bounds[w];
+    E.ccap = ncl;
     const int *keys = E.key;
     if (ncl > 0) {
       if (w == 0) hipLaunchKernelGGL((k_enum2<true>), eg, eb, 0, c.stream, E);
-      else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, false>), wg, wb, 0, c.stream, E);
-      else hipLaunchKernelGGL((k_enum_wave<4, false>), wg, wb, 0, c.stream, E);
+      else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, false>), ag, ab, 0, c.stream, E);
+      else hipLaunchKernelGGL((k_enum_wave<4, false>), ag, ab, 0, c.stream, E);
       CG_HIP(hipGetLastError());
+      if (caps)
+        hipLaunchKernelGGL(k_pad_keys, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, E.key,
+                           (const int *)(ea[w].coff + nitems[w]), ncl, nkeys);
       if (sort) {
         // stable radix sort by (frame, type): clusters of a bucket keep their
         // enumeration order, so the bucketed list is deterministic
@@ -1979,7 +2057,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
         uint32_t *refs2 = reinterpret_cast<uint32_t *>(idx2 + n1);
         hipLaunchKernelGGL(k_iota, dim3((ncl + TB - 1) / TB), dim3(TB), 0, c.stream, idx, ncl);
         int bits = 1;
-        while ((1LL << bits) < (long long)nkeys) ++bits;
+        while ((1LL << bits) <= (long long)nkeys) ++bits;  // keys 0..nkeys (nkeys: sentinel)
         size_t tmp = 0;
         CG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, E.key, key2, idx, idx2, ncl, 0, bits, c.stream));
         void *t = c.b_scan.get(tmp);
@@ -1995,11 +2073,13 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     hipLaunchKernelGGL(k_bucket_bounds, dim3((nkeys + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, keys, ncl, nkeys,
                        d_bounds[w]);
     CG_HIP(hipGetLastError());
-    fo[w].boff.resize((size_t)nkeys + 1);
-    CG_HIP(hipMemcpyAsync(fo[w].boff.data(), d_bounds[w], ((size_t)nkeys + 1) * sizeof(int), hipMemcpyDeviceToHost,
-                          c.stream));
+    if (!caps) {
+      fo[w].boff.resize((size_t)nkeys + 1);
+      CG_HIP(hipMemcpyAsync(fo[w].boff.data(), d_bounds[w], ((size_t)nkeys + 1) * sizeof(int),
+                            hipMemcpyDeviceToHost, c.stream));
+    }
   }
-  CG_HIP(hipStreamSynchronize(c.stream));                                    // host sync 3
+  if (!caps) CG_HIP(hipStreamSynchronize(c.stream));                         // host sync 3 (exact mode)
   CG_HIP(hipEventRecord(c.ev[1], c.stream));
   B.natoms = natoms;
   B.rows = rows;
@@ -2156,34 +2236,51 @@ static bool dispatch_force(Ctx &c, int N, dim3 grid, const ForceArgs &a, std::in
 // Force / energy / virial of a batch of configurations, each standing alone
 // (no small-cell stickiness across them).  F: [3 * sum natoms], E: [nframes],
 // vir: [9 * nframes].
-void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
-                 double *vir) {
+//
+// Once the context has seen a call, the neighbour and cluster lists are
+// sized from per-atom rates (the largest seen, plus a margin) instead of
+// being read back, the force tasks are built on the device, and the only
+// host synchronisation is the copy of the results.  The device totals come
+// back with them; a call that did not fit is repeated with exact sizes.
+static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F,
+                             double *E, double *vir, bool exact) {
   const cg_model &m = c.m;
   set_coef(c, x_host);
+  int natoms = 0;
+  for (int i = 0; i < nframes; ++i) natoms += frames[i].natoms;
+  Caps caps;
+  const bool capmode = !exact && c.rate_nb > 0.0;
+  if (capmode) {
+    caps.nnb = (int)std::min(2.0e9, natoms * c.rate_nb * 1.25 + 1024.0);
+    for (int w = 0; w < 3; ++w) caps.ncl[w] = (int)std::min(2.0e9, natoms * c.rate_cl[w] * 1.25 + 1024.0);
+  }
   Batch B;
-  prepare(c, frames, nframes, 1, B, false);
-  const int natoms = B.natoms;
-  double *d_F = c.b_force.as<double>(3 * (size_t)natoms + 10 * (size_t)nframes + 16);
+  prepare(c, frames, nframes, 1, B, false, capmode ? &caps : nullptr);
+  const size_t nout = 3 * (size_t)natoms + 10 * (size_t)nframes;
+  double *d_F = c.b_force.as<double>(nout + 16);
   if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
   double *d_EV = d_F + 3 * (size_t)natoms;
-  CG_HIP(hipMemsetAsync(d_F, 0, (3 * (size_t)natoms + 10 * (size_t)nframes) * sizeof(double), c.stream));
-  std::vector<Task> ftasks[3];  // host sources of the task uploads, alive until the final sync
+  CG_HIP(hipMemsetAsync(d_F, 0, nout * sizeof(double), c.stream));
+  const int TB = 256;
   for (int w = 0; w < 3; ++w) {
     const DevFamily &Fm = c.fam[w];
-    if (Fm.ntypes == 0 || Fm.ncols == 0 || B.fo[w].ncl == 0) continue;
-    std::vector<Task> &tasks = ftasks[w];
-    const int chunk = FORCE_BS;  // one cluster per thread, one type per workgroup
-    for (int fi = 0; fi < nframes; ++fi)
-      for (int t = 0; t < Fm.ntypes; ++t) {
-        const int c0 = B.fo[w].boff[fi * Fm.ntypes + t], c1 = B.fo[w].boff[fi * Fm.ntypes + t + 1];
-        for (int k = c0; k < c1; k += chunk) tasks.push_back(Task{fi, t, 0, 0, k, std::min(c1, k + chunk)});
-      }
-    if (tasks.empty()) continue;
-    Task *d_tasks = c.b_tasks[w].as<Task>(tasks.size());
-    if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
-    CG_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, c.stream));
+    const FamOut &fo = B.fo[w];
+    if (Fm.ntypes == 0 || Fm.ncols == 0 || fo.ncl == 0 || !fo.d_bounds) continue;
+    // tasks: chunks of at most FORCE_BS clusters of one (frame, type) bucket
+    const int nkeys = fo.nkeys;
+    const int max_tasks = nkeys + (fo.ncl + FORCE_BS - 1) / FORCE_BS;
+    int *d_cnt = c.b_tcount[w].as<int>(2 * ((size_t)nkeys + 1));
+    Task *d_tasks = c.b_tasks[w].as<Task>(max_tasks);
+    if (!d_cnt || !d_tasks) throw Error(CG_ERR_MEMORY, "task list");
+    int *d_toff = d_cnt + nkeys + 1;
+    hipLaunchKernelGGL(k_task_counts, dim3((nkeys + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, fo.d_bounds, nkeys,
+                       FORCE_BS, d_cnt);
+    exclusive_scan(c, d_cnt, d_toff, nkeys);
+    hipLaunchKernelGGL(k_task_fill, dim3((max_tasks + TB - 1) / TB), dim3(TB), 0, c.stream, fo.d_bounds,
+                       (const int *)d_toff, nkeys, Fm.ntypes, FORCE_BS, max_tasks, d_tasks);
     ForceArgs a{};
-    a.tasks = d_tasks; a.frames = B.d_frames; a.wrap = c.b_wrap.as<double>(0); a.refs = B.fo[w].refs;
+    a.tasks = d_tasks; a.ntasks = d_toff + nkeys;
+    a.frames = B.d_frames; a.wrap = c.b_wrap.as<double>(0); a.refs = fo.refs;
     a.smin = Fm.smin; a.smax = Fm.smax; a.xavg = Fm.xavg; a.xdiff = Fm.xdiff; a.lambda = Fm.lambda;
     a.coef = c.b_coef[w].as<double>(0); a.tsize = c.coef_size[w];
     a.F = d_F; a.EV = d_EV;
@@ -2192,7 +2289,7 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
     a.penalty_scale = w == 0 ? c.penalty_scale : 0.0;
     a.penalty_dist = c.penalty_dist;
     a.nb_scale = Fm.nb;
-    const dim3 grid((unsigned)tasks.size());
+    const dim3 grid((unsigned)max_tasks);
     const int N = force_order(c, w);
     bool ok;
     if (w == 0) ok = dispatch_force<2, 1>(c, N, grid, a, std::make_integer_sequence<int, 16>{});
@@ -2201,10 +2298,28 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
     if (!ok) throw Error(CG_ERR_UNSUPPORTED, "force evaluation supports 2/3-body orders up to 16 and 4-body up to 8");
     CG_HIP(hipGetLastError());
   }
-  std::vector<double> ev(10 * (size_t)nframes);
-  CG_HIP(hipMemcpyAsync(F, d_F, 3 * (size_t)natoms * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  CG_HIP(hipMemcpyAsync(ev.data(), d_EV, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  // results and the call's totals in one copy
+  double *d_tot = d_F + nout;
+  hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, c.stream, B.tot, d_tot);
+  c.h_out.resize(nout + 4);
+  CG_HIP(hipMemcpyAsync(c.h_out.data(), d_F, (nout + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipStreamSynchronize(c.stream));
+  const double *tot = c.h_out.data() + nout;
+  const double na = std::max(natoms, 1);
+  bool fits = true;
+  if (capmode) {
+    fits = tot[0] <= caps.nnb;
+    for (int w = 0; w < 3; ++w) fits = fits && tot[1 + w] <= caps.ncl[w];
+  }
+  c.rate_nb = std::max(c.rate_nb, tot[0] / na);
+  for (int w = 0; w < 3; ++w) c.rate_cl[w] = std::max(c.rate_cl[w], tot[1 + w] / na);
+  if (!fits) {
+    ++c.n_repeat;
+    force_batch_impl(c, x_host, frames, nframes, F, E, vir, true);
+    return;
+  }
+  if (F) std::memcpy(F, c.h_out.data(), 3 * (size_t)natoms * sizeof(double));
+  const double *ev = c.h_out.data() + 3 * (size_t)natoms;
   for (int fi = 0; fi < nframes; ++fi) {
     double eo = 0.0;  // per-type energy offsets: atom count x offset coefficient
     if (m.n_offset_cols > 0) {
@@ -2215,6 +2330,12 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
     if (E) E[fi] = ev[10 * fi] + eo;
     if (vir) for (int q = 0; q < 9; ++q) vir[9 * fi + q] = ev[10 * fi + 1 + q];
   }
+}
+
+void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
+                 double *vir) {
+  ++c.n_force;
+  force_batch_impl(c, x_host, frames, nframes, F, E, vir, false);
 }
 
 void ctx_init(Ctx &c, const cg_model *m, int device) {
@@ -2235,7 +2356,7 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 void ctx_free(Ctx &c) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
-                  &c.b_tasks[1], &c.b_tasks[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
+                  &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1], &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
                   &c.b_tmp})
     b->release();
   for (auto &F : c.fam) {
@@ -2363,6 +2484,13 @@ cg_status cg_copy_probe(cg_ctx *ctx, size_t bytes) {
     cg::set_error(e.what());
     return e.code;
   }
+}
+
+cg_status cg_force_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats) {
+  if (!ctx) return CG_ERR_ARG;
+  if (calls) *calls = ctx->c.n_force;
+  if (repeats) *repeats = ctx->c.n_repeat;
+  return CG_OK;
 }
 
 cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale) {

@@ -99,10 +99,15 @@ struct Ctx {
   DBuf b_bins;                            // cell-list keys, atoms and bin starts
   DBuf b_ccount, b_coff;                  // per-entry cluster counts / offsets (3 families)
   DBuf b_fam[3];                          // per-family clusters, keys and bucket bounds
-  DBuf b_tasks[3];                        // per-family assembly task lists
+  DBuf b_tasks[3];                        // per-family assembly / force task lists
+  DBuf b_tcount[3];                       // force tasks: chunks per bucket and their offsets
   DBuf b_A, b_scan;                       // design matrix; hipcub scratch
   DBuf b_x;                               // fit workspace
-  DBuf b_force;                           // force evaluation: F, energy/virial
+  DBuf b_force;                           // force evaluation: F, energy/virial, call totals
+  std::vector<double> h_out;              // host copy of the force results and totals
+  double rate_nb = 0.0;                   // largest neighbour entries per atom seen (0: none yet)
+  double rate_cl[3] = {0.0, 0.0, 0.0};    // largest clusters per atom seen, per family
+  int64_t n_force = 0, n_repeat = 0;      // force calls, and those repeated with exact sizes
   DBuf b_coef[3];                         // per-family coefficient tensors (force evaluation)
   int coef_size[3] = {0, 0, 0};           // tensor entries per cluster type
   bool coef_valid = false;

@@ -94,3 +94,39 @@ def test_force_matches_reference_forceout(ch, tmp_path, case):
     if meta["abs_energy_error_per_atom"] is not None:
         assert round(abs(E - meta["forcetxt_energy"]) / f.natoms, 6) == pytest.approx(
             meta["abs_energy_error_per_atom"], abs=1e-12)
+
+
+def test_force_list_capacity_overflow_repeats_exactly(ch, synth, oracle, tmp_path):
+    """After a dilute configuration the context sizes the lists of the next
+    call from its per-atom counts; a dense configuration overflows them, is
+    repeated with exact sizes and gives the result of a fresh context (and
+    the oracle's); the next call of the same size then fits."""
+    text = synth.fm_setup(["C", "H", "N", "O"], [12.011, 1.008, 14.007, 15.999], 12, 6, 3, 0.8, 5.0, 1.4,
+                          smax3=4.0, smax4=3.5)
+    p = tmp_path / "fm_setup.in"
+    open(p, "w").write(text)
+    s = ch.Setup(str(p))
+    rng = np.random.default_rng(6)
+    cell = np.diag([20.0, 20.0, 20.0])
+    dil = ch.PyFrame(60, synth.random_sequential(rng, 60, cell, 0.9), rng.integers(0, 4, 60).astype(np.int32), cell)
+    den = ch.PyFrame(900, synth.random_sequential(rng, 900, cell, 0.9), rng.integers(0, 4, 900).astype(np.int32),
+                     cell)
+    x = rng.normal(size=s.model.ncols)
+    dev = ch.Device(s)
+    dev.force(x, dil)
+    F1, E1, V1 = dev.force(x, den)
+    assert dev.force_stats() == (2, 1)
+    F2, E2, V2 = dev.force(x, den)
+    assert dev.force_stats() == (3, 1)
+    fresh = ch.Device(s)
+    F0, E0, V0 = fresh.force(x, den)
+    fresh.close()
+    scale = np.abs(F0).max()
+    for F, E, V in ((F1, E1, V1), (F2, E2, V2)):
+        assert np.max(np.abs(F - F0)) <= 1e-12 * scale
+        assert E == pytest.approx(E0, rel=1e-12, abs=1e-12 * abs(E0))
+        assert np.allclose(V, V0, rtol=1e-12, atol=1e-12 * np.abs(V0).max())
+    Fo, Eo, Vo = oracle.force(s, den, x)
+    assert np.max(np.abs(F0 - Fo)) <= 1e-10 * scale
+    assert E0 == pytest.approx(Eo, rel=1e-10, abs=1e-10 * abs(Eo))
+    dev.close()
