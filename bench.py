@@ -114,6 +114,38 @@ def serving_latency(ch, synth, n_req=40, natoms=1000, seed=105):
             "orders": "12/6/3", "ncols": int(s.model.ncols)}
 
 
+def md_steps(ch, synth, n_steps=60, natoms=1000, seed=107):
+    """MD-style force calls: one 1k-atom C/H/N/O configuration (fixed cell and atom
+    count) with atoms moved between calls.  The context captures the call into a HIP
+    graph and replays it; the same steps on a second context that alternates with a
+    different shape stay eager, for comparison."""
+    text, frames, _ = synth.chno(nframes=1, nmin=natoms, nmax=natoms, seed=seed, target_density=0.1)
+    s = setup_from(ch, text)
+    f0 = make_frames(ch, frames)[0]
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1e-3, s.model.ncols)
+    other = ch.PyFrame(64, synth.random_sequential(rng, 64, np.diag([12.0] * 3), 0.9),
+                       rng.integers(0, 4, 64).astype(np.int32), np.diag([12.0] * 3))
+    xyz = f0.xyz.reshape(-1, 3)
+    steps = [ch.PyFrame(natoms, xyz + rng.normal(0, 0.01, xyz.shape), f0.types, f0.cell) for _ in range(n_steps)]
+    out = {"natoms": natoms, "steps": n_steps, "orders": "12/6/3"}
+    for tag, alternate in (("graph", False), ("eager", True)):
+        dev = ch.Device(s)
+        lat = []
+        for i, f in enumerate(steps):
+            t1 = time.perf_counter()
+            dev.force(x, f)
+            if i >= 5:
+                lat.append((time.perf_counter() - t1) * 1e3)
+            if alternate:
+                dev.force(x, other)
+        out[tag + "_p50_ms"] = float(np.percentile(lat, 50))
+        out[tag + "_p99_ms"] = float(np.percentile(lat, 99))
+        out[tag + "_replays"] = dev.force_stats()[2]
+        dev.close()
+    return out
+
+
 def serving_stream(ch, synth, n_req=300, qps=400.0, seed=106):
     """Poisson stream of C/H/N/O configurations (64-4096 atoms, log-uniform)
     through the continuous-batching server (2 workers); latency is measured in
@@ -310,7 +342,11 @@ def main():
         stream = serving_stream(ch, synth)
         log("serving Poisson stream %.0f req/s: p50 %.3f ms p99 %.3f ms, %.3g atoms/s, mean batch %.2f" % (
             stream["qps_offered"], stream["p50_ms"], stream["p99_ms"], stream["atoms_per_s"], stream["mean_batch"]))
-        extra["serving"] = {"single_1k_atoms": one, "poisson_stream": stream}
+        md = md_steps(ch, synth)
+        log("MD-style steps, 1k atoms: graph replay p50 %.3f ms p99 %.3f ms (%d replays) | eager p50 %.3f ms "
+            "p99 %.3f ms" % (md["graph_p50_ms"], md["graph_p99_ms"], md["graph_replays"], md["eager_p50_ms"],
+                             md["eager_p99_ms"]))
+        extra["serving"] = {"single_1k_atoms": one, "poisson_stream": stream, "md_steps_1k_atoms": md}
     if not args.no_configs:
         c3 = config_c3(ch, synth)
         log("600-frame carbon: FP64 %.3g rows/s + fit %.3f s | FP32 %.3g rows/s + fit %.3f s, RMS rel err %.2e" % (

@@ -106,7 +106,7 @@ def lib(path=None):
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
         opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
-        opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
         opt("cg_serve_create", [C.POINTER(Model), C.c_int, _dp, C.c_int, C.c_size_t, C.c_int,
                                 C.POINTER(C.c_void_p)])
@@ -318,10 +318,10 @@ class Device:
         check(self.L.cg_set_penalty(self.h, dist, scale), self.h)
 
     def force_stats(self):
-        """(force calls, calls repeated with exact list sizes) of this context."""
-        calls, rep = C.c_int64(), C.c_int64()
-        check(self.L.cg_force_stats(self.h, C.byref(calls), C.byref(rep)), self.h)
-        return calls.value, rep.value
+        """(force calls, calls repeated with exact list sizes, calls served by a graph replay)."""
+        calls, rep, gr = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self.L.cg_force_stats(self.h, C.byref(calls), C.byref(rep), C.byref(gr)), self.h)
+        return calls.value, rep.value, gr.value
 
     def force_batch(self, x, frames):
         """Forces (list of [natoms*3]), energies [n] and virials [n, 9] of independent configurations."""

@@ -33,6 +33,9 @@
 
 namespace cg {
 
+std::atomic<uint64_t> g_buffer_generation{0};
+thread_local bool t_no_grow = false;
+
 
 // ------------------------------------------------------------ host geometry
 // Cell setup of BOX::UPDATE_CELL (src/ClassDefs.C:1940-1971 and callees).
@@ -1784,30 +1787,45 @@ struct FamOut {
   int nkeys = 0;
 };
 
-// Sizes of a call without host reads: capacity of the neighbour list and of
-// each family's cluster list.  The kernels write nothing past them; the
-// device totals tell afterwards whether the call fitted.
-struct Caps {
-  int nnb = 0;
-  int ncl[3] = {0, 0, 0};
-};
-
 struct Batch {
   std::vector<DevFrame> hf;
+  int nframes = 0;
   int natoms = 0;
   long long rows = 0;
+  int nbins_total = 0;         // cell-list bins over all frames
+  double mx = 0.0;             // list cutoff (largest family cutoff)
   DevFrame *d_frames = nullptr;
+  int *d_af = nullptr;         // frame of each atom
   int *d_type = nullptr;
+  double *d_xyz = nullptr, *d_wrap = nullptr;
   FamOut fo[3];
   TotalPtrs tot{};             // device totals: neighbour entries, clusters per family
 };
+
+// Zero n ints / doubles with a kernel (not hipMemsetAsync, which is kept out
+// of captured graphs).
+__global__ void k_zero_i(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_zero_d(double *p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+static void zero_i(Ctx &c, int *p, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_i, dim3((n + 255) / 256), dim3(256), 0, c.stream, p, n);
+}
+static void zero_d(Ctx &c, double *p, size_t n) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, p, n);
+}
 
 // Upload a batch of frames, wrap atoms, build neighbour lists and enumerate
 // + bucket the clusters of every family.  md != 0 selects the force-evaluation
 // pair set (every pair below S_MAXIM, Cheby::Force_all) instead of the
 // least-squares one (S_MINIM < r < S_MAXIM, Cheby::Deriv_2B).
-static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B, bool sticky = true,
-                    const Caps *caps = nullptr) {
+// Host part of a call: frame table (geometry, row offsets, small-cell and
+// cell-list decisions), batch arrays, and their upload.
+static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, bool sticky) {
   std::vector<DevFrame> &hf = B.hf;
   const cg_model &m = c.m;
   if (nframes <= 0) throw Error(CG_ERR_ARG, "no frames");
@@ -1890,6 +1908,29 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   upload(c, d_af, atom_frame.data(), natoms * sizeof(int));
   upload(c, d_type, type.data(), natoms * sizeof(int));
   upload(c, d_xyz, xyz.data(), xyz.size() * sizeof(double));
+  B.nframes = nframes;
+  B.natoms = natoms;
+  B.rows = rows;
+  B.nbins_total = nbins_total;
+  B.mx = mx;
+  B.d_frames = d_frames;
+  B.d_af = d_af;
+  B.d_type = d_type;
+  B.d_xyz = d_xyz;
+  B.d_wrap = d_wrap;
+}
+
+// Device part: wrap, cell lists, neighbour lists, sublists, cluster
+// enumeration and bucketing.  With caps (capacity mode) it reads nothing back
+// and can be captured into a graph; without, it reads the list sizes and the
+// bucket bounds back to the host (exact mode).
+static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
+  const cg_model &m = c.m;
+  const int nframes = B.nframes, natoms = B.natoms, nbins_total = B.nbins_total;
+  const double mx = B.mx;
+  DevFrame *d_frames = B.d_frames;
+  int *d_af = B.d_af, *d_type = B.d_type;
+  double *d_xyz = B.d_xyz, *d_wrap = B.d_wrap;
   const int TB = 256;
   hipLaunchKernelGGL(k_wrap, dim3((natoms + TB - 1) / TB), dim3(TB), 0, c.stream, d_frames, d_af, d_xyz, d_wrap, natoms);
   // ---- cell lists: atoms sorted (stably) by bin, bin starts
@@ -1918,7 +1959,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   int *d_ncount = c.b_ncount.as<int>(natoms + 1);
   int *d_noff = c.b_noff.as<int>(natoms + 1);
   if (!d_ncount || !d_noff) throw Error(CG_ERR_MEMORY, "neighbour counts");
-  CG_HIP(hipMemsetAsync(d_ncount, 0, (natoms + 1) * sizeof(int), c.stream));
+  zero_i(c, d_ncount, natoms + 1);
   hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
                      (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, d_ncount,
                      (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr, (int *)nullptr, 0);
@@ -1993,7 +2034,7 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     E.smin = F.smin; E.smax = F.smax; E.int_pair_map = c.d_int_pair_map;
     E.md = md;
     nitems[w] = w == 0 ? nnb : 4 * natoms;
-    CG_HIP(hipMemsetAsync(E.count + nitems[w], 0, sizeof(int), c.stream));
+    zero_i(c, E.count + nitems[w], 1);
     if (w == 0) hipLaunchKernelGGL((k_enum2<false>), eg, eb, 0, c.stream, E);
     else if (w == 1) hipLaunchKernelGGL((k_enum_wave<3, true>), ag, ab, 0, c.stream, E);
     else hipLaunchKernelGGL((k_enum_wave<4, true>), ag, ab, 0, c.stream, E);
@@ -2080,11 +2121,13 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
     }
   }
   if (!caps) CG_HIP(hipStreamSynchronize(c.stream));                         // host sync 3 (exact mode)
-  CG_HIP(hipEventRecord(c.ev[1], c.stream));
-  B.natoms = natoms;
-  B.rows = rows;
-  B.d_frames = d_frames;
-  B.d_type = d_type;
+  if (!c.capturing) CG_HIP(hipEventRecord(c.ev[1], c.stream));
+}
+
+static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B, bool sticky = true,
+                    const Caps *caps = nullptr) {
+  prepare_host(c, frames, nframes, B, sticky);
+  prepare_device(c, md, B, caps);
 }
 
 void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_host) {
@@ -2233,35 +2276,14 @@ static bool dispatch_force(Ctx &c, int N, dim3 grid, const ForceArgs &a, std::in
   return done;
 }
 
-// Force / energy / virial of a batch of configurations, each standing alone
-// (no small-cell stickiness across them).  F: [3 * sum natoms], E: [nframes],
-// vir: [9 * nframes].
-//
-// Once the context has seen a call, the neighbour and cluster lists are
-// sized from per-atom rates (the largest seen, plus a margin) instead of
-// being read back, the force tasks are built on the device, and the only
-// host synchronisation is the copy of the results.  The device totals come
-// back with them; a call that did not fit is repeated with exact sizes.
-static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F,
-                             double *E, double *vir, bool exact) {
+// Device part of a force call after prepare_device: per-family task lists
+// and force kernels, the call's totals, and the copy of results and totals
+// into the pinned host buffer.
+static void force_device(Ctx &c, const Batch &B, double *d_F, size_t nout) {
   const cg_model &m = c.m;
-  set_coef(c, x_host);
-  int natoms = 0;
-  for (int i = 0; i < nframes; ++i) natoms += frames[i].natoms;
-  Caps caps;
-  const bool capmode = !exact && c.rate_nb > 0.0;
-  if (capmode) {
-    caps.nnb = (int)std::min(2.0e9, natoms * c.rate_nb * 1.25 + 1024.0);
-    for (int w = 0; w < 3; ++w) caps.ncl[w] = (int)std::min(2.0e9, natoms * c.rate_cl[w] * 1.25 + 1024.0);
-  }
-  Batch B;
-  prepare(c, frames, nframes, 1, B, false, capmode ? &caps : nullptr);
-  const size_t nout = 3 * (size_t)natoms + 10 * (size_t)nframes;
-  double *d_F = c.b_force.as<double>(nout + 16);
-  if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
-  double *d_EV = d_F + 3 * (size_t)natoms;
-  CG_HIP(hipMemsetAsync(d_F, 0, nout * sizeof(double), c.stream));
   const int TB = 256;
+  double *d_EV = d_F + 3 * (size_t)B.natoms;
+  zero_d(c, d_F, nout);
   for (int w = 0; w < 3; ++w) {
     const DevFamily &Fm = c.fam[w];
     const FamOut &fo = B.fo[w];
@@ -2280,7 +2302,7 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
                        (const int *)d_toff, nkeys, Fm.ntypes, FORCE_BS, max_tasks, d_tasks);
     ForceArgs a{};
     a.tasks = d_tasks; a.ntasks = d_toff + nkeys;
-    a.frames = B.d_frames; a.wrap = c.b_wrap.as<double>(0); a.refs = fo.refs;
+    a.frames = B.d_frames; a.wrap = B.d_wrap; a.refs = fo.refs;
     a.smin = Fm.smin; a.smax = Fm.smax; a.xavg = Fm.xavg; a.xdiff = Fm.xdiff; a.lambda = Fm.lambda;
     a.coef = c.b_coef[w].as<double>(0); a.tsize = c.coef_size[w];
     a.F = d_F; a.EV = d_EV;
@@ -2298,13 +2320,137 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
     if (!ok) throw Error(CG_ERR_UNSUPPORTED, "force evaluation supports 2/3-body orders up to 16 and 4-body up to 8");
     CG_HIP(hipGetLastError());
   }
-  // results and the call's totals in one copy
-  double *d_tot = d_F + nout;
-  hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, c.stream, B.tot, d_tot);
-  c.h_out.resize(nout + 4);
-  CG_HIP(hipMemcpyAsync(c.h_out.data(), d_F, (nout + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, c.stream, B.tot, d_F + nout);
+  CG_HIP(hipMemcpyAsync(c.h_pin, d_F, (nout + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+
+// Shape of a capacity-mode call: everything its launches depend on besides
+// the contents of the uploaded arrays.
+static std::string shape_key(const Batch &B, const Caps &caps) {
+  std::string k;
+  auto add = [&](long long v) { k += std::to_string(v); k += ','; };
+  add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
+  for (const DevFrame &d : B.hf) {
+    add(d.natoms); add(d.small); add(d.cell); add(d.nbin[0]); add(d.nbin[1]); add(d.nbin[2]);
+  }
+  return k;
+}
+
+static void drop_graphs(Ctx &c) {
+  for (auto &g : c.graphs)
+    if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+  c.graphs.clear();
+}
+
+// Capture prepare_device + force_device of this call into a graph; false
+// (nothing enqueued) when the capture fails, e.g. because a buffer would
+// have to grow.
+static bool capture_call(Ctx &c, Batch &B, const Caps &caps, double *d_F, size_t nout, hipGraphExec_t *exec) {
+  hipGraph_t graph = nullptr;
+  c.capturing = true;
+  t_no_grow = true;
+  bool ok = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    try {
+      prepare_device(c, 1, B, &caps);
+      force_device(c, B, d_F, nout);
+    } catch (const Error &) {
+      ok = false;
+    }
+    if (hipStreamEndCapture(c.stream, &graph) != hipSuccess) ok = false;
+  }
+  c.capturing = false;
+  t_no_grow = false;
+  (void)hipGetLastError();
+  if (ok && graph) ok = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  else ok = false;
+  if (graph) (void)hipGraphDestroy(graph);
+  return ok;
+}
+
+// Force / energy / virial of a batch of configurations, each standing alone
+// (no small-cell stickiness across them).  F: [3 * sum natoms], E: [nframes],
+// vir: [9 * nframes].
+//
+// Once the context has seen a call, the neighbour and cluster lists are
+// sized from per-atom rates (the largest seen, plus a margin) instead of
+// being read back, the force tasks are built on the device, and the only
+// host synchronisation is the copy of the results.  The device totals come
+// back with them; a call that did not fit is repeated with exact sizes.
+// A call of the same shape as the previous one is captured into a HIP graph
+// (kept per shape and replayed for later calls of that shape, after the
+// frame arrays have been uploaded), while the buffers it uses stay in place.
+static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F,
+                             double *E, double *vir, bool exact) {
+  const cg_model &m = c.m;
+  set_coef(c, x_host);
+  int natoms = 0;
+  for (int i = 0; i < nframes; ++i) natoms += frames[i].natoms;
+  Caps caps;
+  const bool capmode = !exact && c.rate_nb > 0.0;
+  if (capmode) {
+    // rounded up to 8 steps per octave, so that the capacities (part of a
+    // graph's shape) stay put while the counts drift from call to call
+    auto cap = [&](double rate) {
+      const double v = std::min(1.9e9, natoms * rate * 1.25 + 1024.0);
+      const double g = std::exp2(std::floor(std::log2(v)) - 3.0);
+      return (int)(std::ceil(v / g) * g);
+    };
+    caps.nnb = cap(c.rate_nb);
+    for (int w = 0; w < 3; ++w) caps.ncl[w] = cap(c.rate_cl[w]);
+    // a call with as many atoms as the last keeps the last capacities while
+    // they leave 5 % headroom over the largest rates seen
+    if (natoms == c.last_caps_natoms) {
+      bool keep = c.last_caps.nnb >= natoms * c.rate_nb * 1.05;
+      for (int w = 0; w < 3; ++w) keep = keep && c.last_caps.ncl[w] >= natoms * c.rate_cl[w] * 1.05;
+      if (keep) caps = c.last_caps;
+    }
+    c.last_caps = caps;
+    c.last_caps_natoms = natoms;
+  }
+  Batch B;
+  prepare_host(c, frames, nframes, B, false);
+  const size_t nout = 3 * (size_t)natoms + 10 * (size_t)nframes;
+  double *d_F = c.b_force.as<double>(nout + 16);
+  if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
+  if (c.h_pin_cap < nout + 4) {
+    if (c.h_pin) (void)hipHostFree(c.h_pin);
+    c.h_pin = nullptr;
+    c.h_pin_cap = 0;
+    ++g_buffer_generation;
+    const size_t n = nout + nout / 4 + 64;
+    CG_HIP(hipHostMalloc((void **)&c.h_pin, n * sizeof(double), hipHostMallocDefault));
+    c.h_pin_cap = n;
+  }
+  bool done = false;
+  if (capmode) {
+    const std::string key = shape_key(B, caps);
+    auto it = c.graphs.find(key);
+    if (it != c.graphs.end() && it->second.gen != g_buffer_generation) {
+      drop_graphs(c);
+      it = c.graphs.end();
+    }
+    if (it != c.graphs.end()) {
+      CG_HIP(hipGraphLaunch(it->second.exec, c.stream));
+      ++c.n_graph;
+      done = true;
+    } else if (key == c.last_key) {
+      hipGraphExec_t exec = nullptr;
+      if (capture_call(c, B, caps, d_F, nout, &exec)) {
+        if (c.graphs.size() >= 16) drop_graphs(c);
+        c.graphs[key] = Ctx::Graph{exec, g_buffer_generation.load()};
+        CG_HIP(hipGraphLaunch(exec, c.stream));
+        done = true;
+      }
+    }
+    c.last_key = key;
+  }
+  if (!done) {
+    prepare_device(c, 1, B, capmode ? &caps : nullptr);
+    force_device(c, B, d_F, nout);
+  }
   CG_HIP(hipStreamSynchronize(c.stream));
-  const double *tot = c.h_out.data() + nout;
+  const double *tot = c.h_pin + nout;
   const double na = std::max(natoms, 1);
   bool fits = true;
   if (capmode) {
@@ -2318,8 +2464,8 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
     force_batch_impl(c, x_host, frames, nframes, F, E, vir, true);
     return;
   }
-  if (F) std::memcpy(F, c.h_out.data(), 3 * (size_t)natoms * sizeof(double));
-  const double *ev = c.h_out.data() + 3 * (size_t)natoms;
+  if (F) std::memcpy(F, c.h_pin, 3 * (size_t)natoms * sizeof(double));
+  const double *ev = c.h_pin + 3 * (size_t)natoms;
   for (int fi = 0; fi < nframes; ++fi) {
     double eo = 0.0;  // per-type energy offsets: atom count x offset coefficient
     if (m.n_offset_cols > 0) {
@@ -2365,6 +2511,10 @@ void ctx_free(Ctx &c) {
                     (void *)F.cperm, (void *)F.sets8})
       if (p) (void)hipFree(p);
   }
+  drop_graphs(c);
+  if (c.h_pin) (void)hipHostFree(c.h_pin);
+  c.h_pin = nullptr;
+  c.h_pin_cap = 0;
   if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);
   for (auto &e : c.ev) if (e) (void)hipEventDestroy(e);
   for (auto &fe : c.fev) for (auto &e : fe) if (e) (void)hipEventDestroy(e);
@@ -2486,10 +2636,11 @@ cg_status cg_copy_probe(cg_ctx *ctx, size_t bytes) {
   }
 }
 
-cg_status cg_force_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats) {
+cg_status cg_force_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats, int64_t *graph_replays) {
   if (!ctx) return CG_ERR_ARG;
   if (calls) *calls = ctx->c.n_force;
   if (repeats) *repeats = ctx->c.n_repeat;
+  if (graph_replays) *graph_replays = ctx->c.n_graph;
   return CG_OK;
 }
 

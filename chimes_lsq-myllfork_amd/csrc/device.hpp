@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -19,14 +21,23 @@ void set_error(const std::string &m);
       throw ::cg::Error(CG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Incremented whenever a grow-only buffer moves: captured HIP graphs hold the
+// buffers' addresses and are dropped when it changes.
+extern std::atomic<uint64_t> g_buffer_generation;
+// Set on a thread while it captures a graph: a buffer that would have to
+// grow then fails instead (allocation is not part of a capture).
+extern thread_local bool t_no_grow;
+
 // Grow-only device buffer.
 struct DBuf {
   void *p = nullptr;
   size_t cap = 0;
   void *get(size_t bytes) {
     if (bytes > cap) {
+      if (t_no_grow) return nullptr;
       if (p) (void)hipFree(p);
       p = nullptr;
+      ++g_buffer_generation;
       size_t n = bytes + bytes / 4 + 256;
       if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; cap = 0; return nullptr; }
       cap = n;
@@ -57,6 +68,14 @@ struct DevFrame {
   int cell;            // neighbour search through fractional-coordinate bins
   int nbin[3];         // bins per cell axis (each at least the list cutoff thick)
   int bin_off;         // first bin of the frame in the batch bin arrays
+};
+
+// Sizes of a call without host reads: capacity of the neighbour list and of
+// each family's cluster list.  The kernels write nothing past them; the
+// device totals tell afterwards whether the call fitted.
+struct Caps {
+  int nnb = 0;
+  int ncl[3] = {0, 0, 0};
 };
 
 // Device copy of one cluster family (3B or 4B) or the 2B pair family.
@@ -108,6 +127,17 @@ struct Ctx {
   double rate_nb = 0.0;                   // largest neighbour entries per atom seen (0: none yet)
   double rate_cl[3] = {0.0, 0.0, 0.0};    // largest clusters per atom seen, per family
   int64_t n_force = 0, n_repeat = 0;      // force calls, and those repeated with exact sizes
+  int64_t n_graph = 0;                    // force calls served by replaying a captured graph
+  // captured force calls by shape (frame sizes, cell lists, list capacities);
+  // a shape is captured the second time in a row it is seen
+  struct Graph { hipGraphExec_t exec = nullptr; uint64_t gen = 0; };
+  std::map<std::string, Graph> graphs;
+  std::string last_key;
+  Caps last_caps;                         // list capacities of the last capacity-mode call
+  int last_caps_natoms = -1;
+  double *h_pin = nullptr;                // pinned host copy of results and totals
+  size_t h_pin_cap = 0;
+  bool capturing = false;
   DBuf b_coef[3];                         // per-family coefficient tensors (force evaluation)
   int coef_size[3] = {0, 0, 0};           // tensor entries per cluster type
   bool coef_valid = false;

@@ -254,13 +254,16 @@ cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f,
 cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, int nframes,
                          double *F, double *E, double *virial9);
 
-/* Force calls made on ctx and how many of them were repeated with exact list
- * sizes.  After its first call a context sizes the neighbour and cluster
- * lists of a force call from the largest per-atom counts it has seen (plus a
- * margin) instead of reading them back; a call whose lists did not fit is
- * repeated once with exact sizes.  No counterpart in the reference (its
- * lists are host vectors). */
-cg_status cg_force_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats);
+/* Force calls made on ctx, how many of them were repeated with exact list
+ * sizes, and how many were served by replaying a captured HIP graph.  After
+ * its first call a context sizes the neighbour and cluster lists of a force
+ * call from the largest per-atom counts it has seen (plus a margin) instead
+ * of reading them back; a call whose lists did not fit is repeated once with
+ * exact sizes.  A call of the same shape (frame sizes, cell lists, list
+ * capacities) as the previous call is captured into a graph and later calls
+ * of that shape replay it.  No counterpart in the reference (its lists are
+ * host vectors). */
+cg_status cg_force_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats, int64_t *graph_replays);
 
 /* 2-body penalty of the force evaluation, A_p (s_min + d_p - r)^3 for
  * r < s_min + d_p (chimes_md defaults d_p = 0.01, A_p = 1e4;

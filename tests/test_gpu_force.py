@@ -115,9 +115,9 @@ def test_force_list_capacity_overflow_repeats_exactly(ch, synth, oracle, tmp_pat
     dev = ch.Device(s)
     dev.force(x, dil)
     F1, E1, V1 = dev.force(x, den)
-    assert dev.force_stats() == (2, 1)
+    assert dev.force_stats() == (2, 1, 0)
     F2, E2, V2 = dev.force(x, den)
-    assert dev.force_stats() == (3, 1)
+    assert dev.force_stats() == (3, 1, 0)
     fresh = ch.Device(s)
     F0, E0, V0 = fresh.force(x, den)
     fresh.close()
@@ -130,3 +130,39 @@ def test_force_list_capacity_overflow_repeats_exactly(ch, synth, oracle, tmp_pat
     assert np.max(np.abs(F0 - Fo)) <= 1e-10 * scale
     assert E0 == pytest.approx(Eo, rel=1e-10, abs=1e-10 * abs(Eo))
     dev.close()
+
+
+def test_force_graph_replay_matches_eager(ch, synth, oracle, tmp_path):
+    """MD-like steps of one configuration (same cell and atom count, moved
+    atoms, coefficients changed half way) are captured into a HIP graph and
+    replayed; every replay equals the result of a context that stays eager
+    (it alternates shapes), and the first one the oracle's."""
+    text, frames, _ = synth.chno(nframes=1, nmin=300, nmax=300, seed=44)
+    s, fr = synth_setup(ch, synth, tmp_path, (text, frames, _))
+    f0 = fr[0]
+    rng = np.random.default_rng(7)
+    other = ch.PyFrame(50, synth.random_sequential(rng, 50, np.diag([12.0] * 3), 0.9),
+                       rng.integers(0, 4, 50).astype(np.int32), np.diag([12.0] * 3))
+    x = rng.normal(size=s.model.ncols)
+    dev, ref = ch.Device(s), ch.Device(s)
+    xyz = np.array(f0.xyz, dtype=np.float64).reshape(-1, 3)
+    for step in range(8):
+        if step == 5:
+            x = x * 0.5 + rng.normal(size=s.model.ncols) * 0.1
+        pos = xyz + rng.normal(0.0, 0.01, xyz.shape)
+        f = ch.PyFrame(f0.natoms, pos, np.array(f0.types, dtype=np.int32), np.array(f0.cell, dtype=np.float64))
+        F, E, V = dev.force(x, f)
+        Fr, Er, Vr = ref.force(x, f)
+        ref.force(x, other)  # the reference context never sees one shape twice in a row
+        scale = np.abs(Fr).max()
+        assert np.max(np.abs(F - Fr)) <= 1e-12 * scale, step
+        assert E == pytest.approx(Er, rel=1e-12, abs=1e-12 * abs(Er)), step
+        assert np.allclose(V, Vr, rtol=1e-12, atol=1e-12 * np.abs(Vr).max()), step
+        if step == 0:
+            Fo, Eo, Vo = oracle.force(s, f, x)
+            assert np.max(np.abs(F - Fo)) <= 1e-10 * scale
+    calls, repeats, replays = dev.force_stats()
+    assert calls == 8 and replays >= 5, (calls, repeats, replays)
+    assert ref.force_stats()[2] == 0
+    dev.close()
+    ref.close()
