@@ -227,6 +227,11 @@ def config_c4(ch, synth, nframes=256):
     dev = ch.Device(s)
     out = {"workload": "C/H/N/O %d frames, 32-2048 atoms, orders 12/6/3" % nframes, "rows": rows,
            "cols": int(s.model.ncols), "atoms": sum(f.natoms for f in fr)}
+    out["fp64_rows_per_s_default_tiles"] = assemble_rate(ch, dev, fr, rows, ch.CG_FP64, steps=1)
+    choice, ms = dev.autotune(fr)
+    cfgs = ["512 threads, widest tile", "256 threads, widest tile", "2 x 256 threads per CU, half tile"]
+    out["autotune"] = {"choice": {"3B": cfgs[choice[1]], "4B": cfgs[choice[2]]},
+                       "kernel_ms": {cfgs[k]: {"3B": float(ms[k][1]), "4B": float(ms[k][2])} for k in range(3)}}
     for prec, tag in [(ch.CG_FP64, "fp64_rows_per_s"), (ch.CG_FP32, "fp32_rows_per_s")]:
         out[tag] = assemble_rate(ch, dev, fr, rows, prec, steps=1)
         ms, ncl = dev.detail()
@@ -353,8 +358,10 @@ def main():
             c3["fp64"]["assembly_rows_per_s"], c3["fp64"]["fit_s"], c3["fp32"]["assembly_rows_per_s"],
             c3["fp32"]["fit_s"], c3["fp32"]["rms_rel_err_vs_fp64"]))
         c4 = config_c4(ch, synth)
-        log("mixed C/H/N/O batch (%d atoms, %d rows x %d cols): FP64 %.3g rows/s, FP32 %.3g rows/s" % (
-            c4["atoms"], c4["rows"], c4["cols"], c4["fp64_rows_per_s"], c4["fp32_rows_per_s"]))
+        log("mixed C/H/N/O batch (%d atoms, %d rows x %d cols): FP64 %.3g rows/s (default tiles %.3g), FP32 %.3g "
+            "rows/s; autotuned %s" % (c4["atoms"], c4["rows"], c4["cols"], c4["fp64_rows_per_s"],
+                                      c4["fp64_rows_per_s_default_tiles"], c4["fp32_rows_per_s"],
+                                      c4["autotune"]["choice"]))
         extra["carbon_600_frames_precision"] = c3
         extra["chno_mixed_batch"] = c4
     cpu = None

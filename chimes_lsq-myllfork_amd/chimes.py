@@ -106,6 +106,7 @@ def lib(path=None):
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
         opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
+        opt("cg_autotune", [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.POINTER(C.c_int), _dp])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
         opt("cg_serve_create", [C.POINTER(Model), C.c_int, _dp, C.c_int, C.c_size_t, C.c_int,
@@ -316,6 +317,16 @@ class Device:
 
     def set_penalty(self, dist, scale):
         check(self.L.cg_set_penalty(self.h, dist, scale), self.h)
+
+    def autotune(self, frames, prec=CG_FP64):
+        """Pick the assembly tile configuration per family by timing (see cg_autotune);
+        returns (choice per family, kernel ms [config][family])."""
+        n = len(frames)
+        cf = (Frame * n)(*[f.c() for f in frames])
+        choice = (C.c_int * 3)()
+        ms = np.zeros(9)
+        check(self.L.cg_autotune(self.h, cf, n, prec, choice, dptr(ms)), self.h)
+        return list(choice), ms.reshape(3, 3)
 
     def force_stats(self):
         """(force calls, calls repeated with exact list sizes, calls served by a graph replay)."""

@@ -1042,8 +1042,13 @@ __host__ __device__ constexpr size_t tile_bytes(int natoms, int w) {
   return (3 * (size_t)natoms + 7 * SE_REP) * (size_t)tile_stride(w) * sizeof(double);
 }
 
+// Instances whose tables need more than ~150 registers run 256 threads and may
+// use more than 256 VGPRs (one wave per SIMD); the others are held to two
+// waves per SIMD so that 256-thread workgroups can share a CU.
+template <int NB, int VW, class T> constexpr bool asm_heavy() { return 2 * (NB * (NB - 1) / 2) * VW * (int)sizeof(T) / 4 > 150; }
+
 template <int NB, int VW, int BASE, class T, int BS>
-__global__ __launch_bounds__(BS) void k_assemble(AsmArgs a) {
+__global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemble(AsmArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
   // 4-body with powers below 5 and 3-body with powers below 9 take the
   // hand-scheduled FP64 set loops for both precisions (FP32 A is then
@@ -1710,6 +1715,14 @@ static void build_family(Ctx &c, int which) {
   F.ncol_type = dalloc_copy(F.h_ncols);
 }
 
+// Tile configurations of the assembly (threads per workgroup, LDS budget of a
+// workgroup's tile): one workgroup of 512 per CU with the widest tile, one of
+// 256 (fewer idle lanes on small (frame, type) buckets), or two of 256 per CU
+// with tiles half as wide.  cg_autotune picks one per family by timing.
+struct TileCfg { int bs; size_t lds; };
+static constexpr TileCfg kTileCfg[] = {{512, 150 * 1024}, {256, 150 * 1024}, {256, 75 * 1024}};
+constexpr int kNumTileCfg = sizeof(kTileCfg) / sizeof(kTileCfg[0]);
+
 // Which kernel instance handles a family.
 template <class T>
 static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, const Task *d_tasks,
@@ -1735,14 +1748,25 @@ static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, con
   dim3 grid((unsigned)tasks.size()), block(512);
   (void)block;
   if (tasks.empty()) return;
-#define CG_LAUNCH(NBv, VWv, BASEv)                                                                          \
+  const int want_bs = which == 0 ? 512 : kTileCfg[c.tile_cfg[which]].bs;
+#define CG_LAUNCH_BS(NBv, VWv, BASEv, BSv)                                                                  \
   do {                                                                                                      \
-    constexpr int BSv = 2 * (NBv * (NBv - 1) / 2) * VWv * (int)sizeof(T) / 4 > 150 ? 256 : 512;           \
     auto kern = k_assemble<NBv, VWv, BASEv, T, BSv>;                                                        \
-    block = dim3(BSv);                                                             \
+    block = dim3(BSv);                                                                                      \
     CG_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes)); \
     hipLaunchKernelGGL(kern, grid, block, lds_bytes, c.stream, a);                                          \
     CG_HIP(hipGetLastError());                                                                              \
+  } while (0)
+  // register-heavy instances run 256 threads whatever the configuration
+#define CG_LAUNCH(NBv, VWv, BASEv)                                                                          \
+  do {                                                                                                      \
+    constexpr bool heavy = asm_heavy<NBv, VWv, T>();                                                      \
+    if constexpr (heavy || NBv == 2) {                                                                      \
+      CG_LAUNCH_BS(NBv, VWv, BASEv, heavy ? 256 : 512);                                                     \
+    } else {                                                                                                \
+      if (want_bs == 256) CG_LAUNCH_BS(NBv, VWv, BASEv, 256);                                               \
+      else CG_LAUNCH_BS(NBv, VWv, BASEv, 512);                                                              \
+    }                                                                                                       \
   } while (0)
   if (which == 0) {
     if (F.max_pow <= 16) CG_LAUNCH(2, 16, 1);
@@ -1761,6 +1785,7 @@ static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, con
     else throw Error(CG_ERR_UNSUPPORTED, "4-body order above 8 is not supported");
   }
 #undef CG_LAUNCH
+#undef CG_LAUNCH_BS
 }
 
 static size_t exclusive_scan(Ctx &c, const int *in, int *out, int n) {
@@ -2162,7 +2187,6 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
   CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
   int maxn = 0;
   for (auto &d : hf) maxn = std::max(maxn, d.natoms);
-  const size_t lds_budget = 150 * 1024;
   bool fam_ran[3] = {false, false, false};
   std::vector<Task> atasks[3];  // host sources of the task uploads, alive until the final sync
   for (int w = 0; w < 3; ++w) c.ncl[w] = fo[w].ncl;
@@ -2170,6 +2194,7 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     const DevFamily &F = c.fam[w];
     if (F.ntypes == 0 || F.ncols == 0 || fo[w].ncl == 0) continue;
     std::vector<Task> tasks;
+    const size_t lds_budget = kTileCfg[w == 0 ? 0 : c.tile_cfg[w]].lds;
     for (int fi = 0; fi < nframes; ++fi) {
       int wmax = 64;
       while (wmax > 0 && tile_bytes(hf[fi].natoms, wmax) > lds_budget) --wmax;
@@ -2557,6 +2582,37 @@ cg_status cg_assemble(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec
     cg::assemble(ctx->c, frames, nframes, prec, b_host);
     if (A_dev) *A_dev = ctx->c.A;
     if (rows_out) *rows_out = ctx->c.A_rows;
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, int *choice3, double *ms_table) {
+  if (!ctx || !frames || nframes <= 0) { cg::set_error("cg_autotune: bad argument"); return CG_ERR_ARG; }
+  if (prec != CG_FP64 && prec != CG_FP32) { cg::set_error("bad precision"); return CG_ERR_ARG; }
+  try {
+    cg::Ctx &c = ctx->c;
+    CG_HIP(hipSetDevice(c.device));
+    double ms[cg::kNumTileCfg][3];
+    for (int w = 0; w < 3; ++w) c.tile_cfg[w] = 0;
+    cg::assemble(c, frames, nframes, prec, nullptr);  // warm-up (buffers, code)
+    for (int k = 0; k < cg::kNumTileCfg; ++k) {
+      for (int w = 0; w < 3; ++w) c.tile_cfg[w] = k;
+      cg::assemble(c, frames, nframes, prec, nullptr);
+      for (int w = 0; w < 3; ++w) ms[k][w] = c.ms_fam[w];
+    }
+    for (int w = 0; w < 3; ++w) {
+      int best = 0;  // the 2-body kernel has one configuration
+      for (int k = 1; k < cg::kNumTileCfg && w > 0; ++k)
+        if (ms[k][w] < ms[best][w]) best = k;
+      c.tile_cfg[w] = best;
+      if (choice3) choice3[w] = best;
+    }
+    if (ms_table)
+      for (int k = 0; k < cg::kNumTileCfg; ++k)
+        for (int w = 0; w < 3; ++w) ms_table[3 * k + w] = ms[k][w];
     return CG_OK;
   } catch (const cg::Error &e) {
     cg::set_error(e.what());

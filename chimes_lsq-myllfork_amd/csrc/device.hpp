@@ -153,6 +153,7 @@ struct Ctx {
   hipEvent_t fev[3][2] = {};   // around each family's assembly launch
   float ms_fam[3] = {0, 0, 0};
   long long ncl[3] = {0, 0, 0};  // clusters per family in the last batch
+  int tile_cfg[3] = {0, 0, 0};   // assembly tile configuration per family (set by cg_autotune)
 };
 
 // Defined in assemble.hip.

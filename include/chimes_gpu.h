@@ -184,6 +184,18 @@ cg_status cg_assemble(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec
                       const void **A_dev, double *b_host, int64_t *rows_out);
 /* Copy the last assembled A (rows [r0, r0+nrows)) to host doubles. */
 cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host);
+
+/* Tile autotuner of the 3- and 4-body assembly: assembles the given frames
+ * once per tile configuration (0: 512-thread workgroups with the widest LDS
+ * tile; 1: 256 threads, widest tile; 2: 256 threads, tiles half as wide so
+ * that two workgroups share a CU), times each family's kernel and keeps the
+ * fastest configuration per family for later cg_assemble calls on ctx.
+ * choice3 (may be NULL) receives the choice per family (2-body always 0);
+ * ms_table (may be NULL) the [3 configurations][3 families] kernel times.
+ * The last assembly left in ctx is that of the last configuration tried.
+ * No counterpart in the reference (a CPU code without tiles). */
+cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, int *choice3,
+                      double *ms_table);
 /* Kernel timing of the last cg_assemble (ms, measured with HIP events on the
  * context stream): neighbour+enumeration, assembly kernels, total. */
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, double *ms_total);

@@ -139,3 +139,27 @@ def test_empty_and_ragged_frames(ch, synth, tmp_path):
     n_iso = s.frame_rows(3, 2)
     assert np.all(A[r0:r0 + 9, : s.model.ncols_short] == 0)
     dev.close()
+
+
+def test_autotune_configurations_give_the_same_A(ch, synth, oracle, tmp_path):
+    """Every tile configuration the autotuner tries assembles the same A as the
+    default one (and as the oracle); the chosen one is then used."""
+    gen = synth.chno(nframes=6, nmin=40, nmax=500, seed=45)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    dev = ch.Device(s)
+    A0, b0 = dev.assemble(fr)
+    choice, ms = dev.autotune(fr)
+    assert all(0 <= k < 3 for k in choice) and choice[0] == 0
+    assert np.all(ms[:, 1:] > 0)
+    # the autotuner leaves the assembly of its last configuration (two
+    # 256-thread workgroups per CU, half-width tiles)
+    rows = A0.shape[0]
+    A2 = np.zeros_like(A0)
+    ch.check(dev.L.cg_copy_A(dev.h, 0, rows, ch.dptr(A2)), dev.h)
+    A1, b1 = dev.assemble(fr)
+    assert colscaled_error(A2, A0) < 1e-12
+    assert colscaled_error(A1, A0) < 1e-12
+    assert np.array_equal(b1, b0)
+    Ao = oracle.assemble(s, fr)
+    assert colscaled_error(A1, Ao) < 1e-10
+    dev.close()
