@@ -85,6 +85,10 @@ def lib(path=None):
         L.cg_write_open.argtypes = [C.c_void_p, C.c_char_p]
         L.cg_write_frame.argtypes = [C.c_void_p, C.POINTER(Frame), _dp, C.c_int]
         L.cg_write_close.argtypes = [C.c_void_p]
+        L.cg_write_open_opts.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.cg_write_rank.argtypes = [C.c_void_p, C.c_int]
+        L.cg_frame_order.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
         L.cg_write_header.argtypes = [C.c_void_p, C.c_char_p]
         def opt(name, argtypes):
             if hasattr(L, name):
@@ -222,6 +226,28 @@ class Setup:
         finally:
             self.L.cg_traj_close(t)
         return out
+
+    def frame_order(self, nranks):
+        """(frames, rank of each) in the order the reference's nranks MPI ranks process them."""
+        n = self.nframes
+        order = (C.c_int * max(n, 1))()
+        rank = (C.c_int * max(n, 1))()
+        cnt = C.c_int()
+        check(self.L.cg_frame_order(self.h, nranks, order, rank, C.byref(cnt)))
+        return list(order[:cnt.value]), list(rank[:cnt.value])
+
+    def write_text(self, outdir, frames, ranks=None, rows=None):
+        """Write b/b-labeled/natoms/dim (and A, from rows [per frame], if given) through the
+        library's writers, frame k belonging to ranks[k]."""
+        check(self.L.cg_write_open_opts(self.h, outdir.encode(), 0 if rows is not None else 1))
+        for k, f in enumerate(frames):
+            if ranks is not None:
+                check(self.L.cg_write_rank(self.h, int(ranks[k])))
+            cf = f.c()
+            nr = self.frame_rows(f.natoms, f.index)
+            r = None if rows is None else np.ascontiguousarray(rows[k], dtype=np.float64)
+            check(self.L.cg_write_frame(self.h, C.byref(cf), dptr(r) if r is not None else None, nr))
+        check(self.L.cg_write_close(self.h))
 
     def write_header(self, outdir):
         check(self.L.cg_write_header(self.h, outdir.encode()))

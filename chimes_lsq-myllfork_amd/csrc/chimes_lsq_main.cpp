@@ -3,10 +3,14 @@
 // that `chimes_lsq.py` performs (src/chimes_lsq.py:169-672).
 //
 //   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--gram fp64|bf16x3]
-//                  [--refine K] [--batch FRAMES] [--no-text]
+//                  [--refine K] [--batch FRAMES] [--ranks N] [--no-A] [--no-text]
 //
 // Files written in the working directory: A.txt b.txt b-labeled.txt natoms.txt
 // dim.txt params.header ff_groups.map; with --fit also params.txt force.txt.
+// --ranks N lays the frames out as the reference's N MPI ranks would process
+// them (contiguous blocks, or round robin with SKPFRMS), rows in rank order;
+// with SPLITFI true each rank's rows go to A.<rank>.txt with dim.<rank>.txt.
+// --no-A skips the A text (b, natoms and dim files are still written).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -32,17 +36,19 @@ int die(const char *what, cg_status st, const void *ctx = nullptr) {
 int main(int argc, char **argv) {
   if (argc < 2) {
     std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--gram fp64|bf16x3] [--refine K] "
-                         "[--batch FRAMES] [--no-text]\n");
+                         "[--batch FRAMES] [--ranks N] [--no-A] [--no-text]\n");
     return 1;
   }
   const char *infile = argv[1];
-  bool do_fit = false, text = true;
+  bool do_fit = false, text = true, write_A = true;
   double eps = 1.0e-5;
-  int prec = CG_FP64, refine = 3, batch = 0, gram = CG_GRAM_FP64;
+  int prec = CG_FP64, refine = 3, batch = 0, gram = CG_GRAM_FP64, nranks = 1;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "--fit") do_fit = true;
     else if (a == "--no-text") text = false;
+    else if (a == "--no-A") write_A = false;
+    else if (a == "--ranks" && i + 1 < argc) nranks = std::atoi(argv[++i]);
     else if (a == "--eps" && i + 1 < argc) eps = std::atof(argv[++i]);
     else if (a == "--refine" && i + 1 < argc) refine = std::atoi(argv[++i]);
     else if (a == "--batch" && i + 1 < argc) batch = std::atoi(argv[++i]);
@@ -85,17 +91,24 @@ int main(int argc, char **argv) {
     o.f.type = o.type.data();
   }
   cg_traj_close(traj);
+  // frames in the order the reference's ranks process them
+  if (nranks < 1) { std::fprintf(stderr, "--ranks must be >= 1\n"); return 1; }
+  std::vector<int> order(nframes), rank_of(nframes);
+  int nproc = 0;
+  if ((st = cg_frame_order(setup, nranks, order.data(), rank_of.data(), &nproc))) return die("frame order", st);
   cg_ctx *ctx = nullptr;
   if ((st = cg_create(m, 0, &ctx))) return die("creating device context", st);
-  if (text && (st = cg_write_open(setup, dir.c_str()))) return die("opening A/b files", st);
-  std::vector<cg_frame> cf(nframes);
-  for (int i = 0; i < nframes; ++i) cf[i] = frames[i].f;
-  if (do_fit || batch <= 0) batch = nframes;
+  if (text && (st = cg_write_open_opts(setup, dir.c_str(), write_A ? 0 : CG_WRITE_NO_A)))
+    return die("opening A/b files", st);
+  std::vector<cg_frame> cf(nproc);
+  for (int i = 0; i < nproc; ++i) cf[i] = frames[order[i]].f;
+  const int nf_run = nproc;
+  if (do_fit || batch <= 0) batch = std::max(nf_run, 1);
   double ms_asm = 0;
   int64_t total_rows = 0;
   std::vector<double> b_all;
-  for (int f0 = 0; f0 < nframes; f0 += batch) {
-    int nb = std::min(batch, nframes - f0);
+  for (int f0 = 0; f0 < nf_run; f0 += batch) {
+    int nb = std::min(batch, nf_run - f0);
     int64_t rows = 0;
     for (int i = 0; i < nb; ++i) rows += cg_model_frame_rows(m, cf[f0 + i].natoms, cf[f0 + i].index);
     std::vector<double> b(rows);
@@ -106,12 +119,14 @@ int main(int argc, char **argv) {
     ms_asm += tt;
     b_all.insert(b_all.end(), b.begin(), b.end());
     if (text) {
-      std::vector<double> host((size_t)rows * m->ncols);
-      if ((st = cg_copy_A(ctx, 0, rows, host.data()))) return die("copying A", st, ctx);
+      std::vector<double> host(write_A ? (size_t)rows * m->ncols : 0);
+      if (write_A && (st = cg_copy_A(ctx, 0, rows, host.data()))) return die("copying A", st, ctx);
       int64_t r = 0;
       for (int i = 0; i < nb; ++i) {
         int nr = (int)cg_model_frame_rows(m, cf[f0 + i].natoms, cf[f0 + i].index);
-        if ((st = cg_write_frame(setup, &cf[f0 + i], host.data() + r * m->ncols, nr))) return die("writing A", st);
+        if ((st = cg_write_rank(setup, rank_of[f0 + i]))) return die("writing A", st);
+        if ((st = cg_write_frame(setup, &cf[f0 + i], write_A ? host.data() + r * m->ncols : nullptr, nr)))
+          return die("writing A", st);
         r += nr;
       }
     }

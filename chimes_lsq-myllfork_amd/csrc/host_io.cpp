@@ -7,6 +7,7 @@
 // (src/chimes_lsq.C:562-716), FCUT::print_header (src/Fcut.C:169-178) and
 // CLUSTER[_LIST]::print_header / print_special (src/Cluster.C:423-508,1636-1697).
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <ctime>
 #include <cstdio>
@@ -189,26 +190,71 @@ cg_status cg_traj_next(cg_traj *t, cg_frame *f) {
 }
 
 // ------------------------------------------------------------ A/b text files
-cg_status cg_write_open(const cg_setup *cs, const char *outdir) {
+cg_status cg_write_open_opts(const cg_setup *cs, const char *outdir, int flags) {
   Setup &s = *cs->s;
   s.outdir = outdir ? outdir : ".";
-  s.fA.open(s.outdir + "/A.txt");
+  s.write_A = !(flags & CG_WRITE_NO_A);
+  s.cur_rank = 0;
+  s.rank_open = false;
+  s.rank_rows.assign(std::max(s.nranks, 1), 0);
+  s.abuf.clear();
+  if (s.write_A && !s.split_files) {
+    s.fA.open(s.outdir + "/A.txt");
+    if (!s.fA) { g_err = "Could not open A.txt in " + s.outdir; return CG_ERR_INPUT; }
+  }
   s.fb.open(s.outdir + "/b.txt");
   s.fbl.open(s.outdir + "/b-labeled.txt");
   s.fna.open(s.outdir + "/natoms.txt");
-  if (!s.fA || !s.fb || !s.fbl || !s.fna) { g_err = "Could not open output files in " + s.outdir; return CG_ERR_INPUT; }
-  s.fA.precision(16); s.fA << std::scientific;
+  if (!s.fb || !s.fbl || !s.fna) { g_err = "Could not open output files in " + s.outdir; return CG_ERR_INPUT; }
   s.fb.precision(16); s.fb << std::scientific;
   s.data_count = 0;
   return CG_OK;
 }
 
+cg_status cg_write_open(const cg_setup *cs, const char *outdir) { return cg_write_open_opts(cs, outdir, 0); }
+
 namespace {
-void ones(const Setup &s, std::ostream &o, bool energy_row, const std::vector<int> &count) {
+std::string rank_file(const Setup &s, const char *stem, int rank) {
+  char name[32];
+  std::snprintf(name, sizeof name, "%s.%04d.txt", stem, rank);
+  return s.outdir + "/" + name;
+}
+
+void flush_A(Setup &s) {
+  if (!s.abuf.empty()) {
+    s.fA.write(s.abuf.data(), (std::streamsize)s.abuf.size());
+    s.abuf.clear();
+  }
+}
+
+// "%.16e" of x (what the reference's `fileA << x` prints with precision 16
+// and std::scientific), appended to the A buffer
+void put_e16(std::string &out, double x) {
+  char tmp[40];
+  const auto r = std::to_chars(tmp, tmp + sizeof tmp, x, std::chars_format::scientific, 16);
+  out.append(tmp, r.ptr);
+}
+}  // namespace
+
+cg_status cg_write_rank(const cg_setup *cs, int rank) {
+  Setup &s = *cs->s;
+  if (rank < 0 || rank < s.cur_rank) { g_err = "cg_write_rank: ranks must not decrease"; return CG_ERR_ARG; }
+  if ((int)s.rank_rows.size() <= rank) s.rank_rows.resize(rank + 1, 0);
+  if (s.split_files && s.write_A && rank != s.cur_rank && s.rank_open) {
+    flush_A(s);
+    s.fA.close();
+    s.rank_open = false;
+  }
+  s.cur_rank = rank;
+  return CG_OK;
+}
+
+namespace {
+void ones(const Setup &s, std::string &o, bool energy_row, const std::vector<int> &count) {
   if (!s.fit_ener) return;
   for (int i = 0; i < s.ntypes; ++i) {
-    if (!energy_row) o << " " << "0.0";
-    else o << " " << count[i];
+    if (!energy_row) o += " 0.0";
+    else { o += ' '; o += std::to_string(count[i]); }
   }
 }
 }  // namespace
@@ -219,6 +265,13 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
   const int nc = m.ncols, ns = m.ncols_short;
   int64_t expect = cg_model_frame_rows(&m, f->natoms, f->index);
   if (nrows != expect) { g_err = "row count mismatch in cg_write_frame"; return CG_ERR_ARG; }
+  if (s.write_A && !rows) { g_err = "cg_write_frame: rows are NULL"; return CG_ERR_ARG; }
+  if (s.write_A && s.split_files && !s.rank_open) {
+    s.fA.open(rank_file(s, "A", s.cur_rank));
+    if (!s.fA) { g_err = "Could not open " + rank_file(s, "A", s.cur_rank); return CG_ERR_INPUT; }
+    s.rank_open = true;
+  }
+  if ((int)s.rank_rows.size() <= s.cur_rank) s.rank_rows.resize(s.cur_rank + 1, 0);
   // which input file the frame came from (for the b-labeled flags)
   size_t file = 0;
   int check = 0;
@@ -234,11 +287,14 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
   for (int c : count) total += c;
   int r = 0;
   auto row_out = [&](const char *sep, bool energy_row) {
-    const double *a = rows + (int64_t)r * nc;
-    for (int n = 0; n < ns; ++n) s.fA << a[n] << sep;
-    ones(s, s.fA, energy_row, count);
+    if (s.write_A) {
+      const double *a = rows + (int64_t)r * nc;
+      for (int n = 0; n < ns; ++n) { put_e16(s.abuf, a[n]); s.abuf += sep; }
+      ones(s, s.abuf, energy_row, count);
+      s.abuf += '\n';
+      if (s.abuf.size() > (1u << 20)) flush_A(s);
+    }
     s.fna << total << "\n";
-    s.fA << "\n";
     ++r;
   };
   for (int a = 0; a < f->natoms; ++a) {
@@ -272,16 +328,63 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
     }
     s.data_count += 3;
   }
-  if (!s.fA.good() || !s.fb.good()) { g_err = "Error in A/b file"; return CG_ERR_INPUT; }
+  s.rank_rows[s.cur_rank] += r;
+  if ((s.write_A && !s.fA.good()) || !s.fb.good()) { g_err = "Error in A/b file"; return CG_ERR_INPUT; }
   return CG_OK;
 }
 
 cg_status cg_write_close(const cg_setup *cs) {
   Setup &s = *cs->s;
-  s.fA.close(); s.fb.close(); s.fbl.close(); s.fna.close();
+  if (s.write_A) {
+    flush_A(s);
+    if (s.fA.is_open()) s.fA.close();
+  }
+  s.rank_open = false;
+  s.fb.close(); s.fbl.close(); s.fna.close();
+  if (s.split_files) {
+    // per rank with rows: columns, first and last row (global), total rows
+    int64_t start = 0;
+    for (size_t r = 0; r < s.rank_rows.size(); ++r) {
+      const int64_t n = s.rank_rows[r];
+      if (n > 0) {
+        std::ofstream dim(rank_file(s, "dim", (int)r));
+        if (!dim) { g_err = "Could not open " + rank_file(s, "dim", (int)r); return CG_ERR_INPUT; }
+        dim << s.ncols << " " << start << " " << start + n - 1 << " " << s.data_count << "\n";
+      }
+      start += n;
+    }
+    return CG_OK;
+  }
   std::ofstream dim(s.outdir + "/dim.txt");
   if (!dim) { g_err = "Could not open dim.txt"; return CG_ERR_INPUT; }
   dim << s.ncols << " " << s.data_count << std::endl;
+  return CG_OK;
+}
+
+cg_status cg_frame_order(const cg_setup *cs, int nranks, int *order, int *rank, int *n_out) {
+  if (!cs || nranks < 1 || !order || !rank || !n_out) { g_err = "cg_frame_order: bad argument"; return CG_ERR_ARG; }
+  Setup &s = *cs->s;
+  const int nf = s.nframes;
+  s.nranks = nranks;
+  int n = 0;
+  for (int r = 0; r < nranks; ++r) {
+    // divide_atoms: ranks take ceil-divided contiguous blocks
+    int i0 = 1, i1 = 0;
+    if (nf > 0) {
+      const int used = std::min(nranks, nf);
+      i0 = (int)std::ceil((double)r * nf / used);
+      if (r > nf) { i0 = nf + 1; i1 = nf - 1; }
+      else if (r == used - 1) i1 = nf - 1;
+      else i1 = std::min(nf - 1, (int)std::ceil((double)(r + 1) * nf / used) - 1);
+    }
+    for (int i = 0; i < nf; ++i) {
+      bool take;
+      if (s.skip_frames <= 0) take = i >= i0 && i <= i1;
+      else take = (i + r) % nranks == 0 && ((i + r) / nranks) % s.skip_frames == 0;
+      if (take) { order[n] = i; rank[n] = r; ++n; }
+    }
+  }
+  *n_out = n;
   return CG_OK;
 }
 

@@ -580,8 +580,6 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
     if (s->use_partial_charges) throw Error(CG_ERR_UNSUPPORTED, "fixed non-zero charges (Ewald subtraction) are not supported");
     if (s->wrap && s->nlayers > 0) s->wrap = false;  // RUN_SANITY_LSQ
     if (s->nlayers < 1) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS 0 (no ghost atoms) is not supported");
-    if (s->split_files) throw Error(CG_ERR_UNSUPPORTED, "SPLITFI true is not supported");
-    if (s->skip_frames > 1) throw Error(CG_ERR_UNSUPPORTED, "SKPFRMS > 1 is not supported");
 
     // clusters (build_clusters, src/chimes_lsq.C:533-560)
     s->trips.order = s->o3;

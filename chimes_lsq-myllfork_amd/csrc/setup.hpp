@@ -106,6 +106,12 @@ struct Setup {
   std::ofstream fA, fb, fbl, fna;
   int64_t data_count = 0;
   std::string outdir;
+  bool write_A = true;               // A text written (cg_write_open_opts)
+  int nranks = 1;                    // ranks of the frame distribution (cg_frame_order)
+  int cur_rank = 0;                  // rank of the frames being written
+  bool rank_open = false;            // A.<rank>.txt of cur_rank open (SPLITFI)
+  std::vector<int64_t> rank_rows;    // rows written per rank
+  std::string abuf;                  // formatted A text awaiting a write
 };
 
 void set_error(const std::string &m);  // thread-local message for cg_last_error

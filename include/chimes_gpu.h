@@ -161,10 +161,27 @@ cg_status cg_traj_next(cg_traj *t, cg_frame *f);
 void cg_traj_close(cg_traj *t);
 
 /* Text writers (byte layout of the reference's files).  rows: row-major
- * [nrows*ncols] for the frame in reference row order. */
+ * [nrows*ncols] for the frame in reference row order.
+ * With SPLITFI true the A rows of each rank go to A.<rank>.txt and
+ * cg_write_close writes dim.<rank>.txt ("cols first_row last_row total_rows")
+ * for every rank with rows (A_Matrix.C:626-720); otherwise A.txt and dim.txt.
+ * b.txt, b-labeled.txt and natoms.txt are single files in rank order, as the
+ * reference's concatenation leaves them. */
 cg_status cg_write_open(const cg_setup *s, const char *outdir);
+enum { CG_WRITE_NO_A = 1 };  /* flags of cg_write_open_opts: skip the A text */
+cg_status cg_write_open_opts(const cg_setup *s, const char *outdir, int flags);
+/* Following frames belong to `rank` (ranks in increasing order). */
+cg_status cg_write_rank(const cg_setup *s, int rank);
 cg_status cg_write_frame(const cg_setup *s, const cg_frame *f, const double *rows, int nrows);
 cg_status cg_write_close(const cg_setup *s);
+
+/* Which frames the reference's `nranks` MPI ranks process and in what order
+ * (src/chimes_lsq.C:263,305-319, divide_atoms in src/functions.C:182-222):
+ * contiguous blocks, or with SKPFRMS >= 1 round robin (rank r takes frame i
+ * when (i + r) % nranks == 0 and every SKPFRMS-th such frame).  order
+ * [nframes] receives the processed frames rank by rank, rank [nframes] the
+ * rank of each, n_out their number. */
+cg_status cg_frame_order(const cg_setup *s, int nranks, int *order, int *rank, int *n_out);
 cg_status cg_write_header(const cg_setup *s, const char *outdir); /* params.header + ff_groups.map */
 
 /* ------------------------------------------------------- device context */

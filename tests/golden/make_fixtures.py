@@ -14,6 +14,11 @@ expected outputs), trimmed to the frames the tests need:
                            12 golden A rows (12x1308).
   lsq/fitstrs/             last frame (96 atoms, diagonal stress rows) and the
                            176 golden A rows that survive in A.txt.ab.
+  lsq/tatb/                SPLITFI + SKPFRMS 1 run over 36 ranks (36 frames,
+                           C/H/O/N, orders 8/4): input, the per-rank dim files
+                           and processed frames (frames.NNNN.log), the
+                           concatenated b, b-labeled and natoms files, and
+                           params.header / ff_groups.map (gzip).
   md/<test>/               force-evaluation cases of test_suite-md without
                            Coulomb terms: params.txt (gzip), the first frame
                            of the run's coordinate file and the reference's
@@ -62,6 +67,20 @@ def tail_rows(path):
     return np.array([[float(x) for x in l.split()] for l in lines])
 
 
+def gz(src, dst):
+    """Deterministic gzip copy (no name, mtime 0)."""
+    with open(dst, "wb") as raw:
+        with gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0, filename="") as g:
+            g.write(open(src, "rb").read())
+
+
+def setup_of(d):
+    """The fm_setup.in the golden outputs were produced from: correct_output's copy
+    when the test keeps one (tatb's top-level file has a different 4-body order)."""
+    co = os.path.join(d, "correct_output", "fm_setup.in")
+    return co if os.path.exists(co) else os.path.join(d, "fm_setup.in")
+
+
 def set_key(text, key, value):
     return re.sub(r"(# %s #[^\n]*\n)[^\n]*\n" % key, lambda m: m.group(1) + "\t%s\n" % value, text, count=1)
 
@@ -72,7 +91,7 @@ def main():
         d = os.path.join(LSQ, t)
         if not os.path.exists(os.path.join(d, "correct_output", "params.header")):
             continue
-        ent = {"fm_setup": open(os.path.join(d, "fm_setup.in")).read()}
+        ent = {"fm_setup": open(setup_of(d)).read()}
         if os.path.exists(os.path.join(d, "traj_list.dat")):
             ent["traj_list"] = open(os.path.join(d, "traj_list.dat")).read()
         for fn in ["params.header", "ff_groups.map"]:
@@ -122,6 +141,23 @@ def main():
         open(os.path.join(out, "fm_setup.in"), "w").write(text)
         rows = tail_rows(os.path.join(src, "correct_output", apiece))
         np.savez_compressed(os.path.join(out, "golden.npz"), A=rows, row0=frame_rows - rows.shape[0])
+    # tatb: split A files over 36 ranks, round-robin frame order
+    src = os.path.join(LSQ, "tatb")
+    out = os.path.join(HERE, "lsq", "tatb")
+    os.makedirs(out, exist_ok=True)
+    co = os.path.join(src, "correct_output")
+    open(os.path.join(out, "fm_setup.in"), "w").write(open(setup_of(src)).read())
+    gz(os.path.join(src, "SCAN_all.xyzf"), os.path.join(out, "SCAN_all.xyzf.gz"))
+    for fn in ["b.txt", "b-labeled.txt", "natoms.txt", "params.header", "ff_groups.map"]:
+        gz(os.path.join(co, fn), os.path.join(out, fn + ".gz"))
+    meta = {"nranks": 36, "dims": {}, "frames": {}}
+    for r in range(36):
+        d = os.path.join(co, "dim.%04d.txt" % r)
+        if os.path.exists(d):
+            meta["dims"][str(r)] = open(d).read()
+        log = open(os.path.join(co, "frames.%04d.log" % r)).read()
+        meta["frames"][str(r)] = [int(v) for v in re.findall(r"Processing frame (\d+)", log)]
+    json.dump(meta, open(os.path.join(out, "split.json"), "w"), indent=1, sort_keys=True)
     # force-evaluation cases: params.txt + first coordinate frame + forceout.txt
     md = os.path.join(REF, "test_suite-md")
     for t in MD_TESTS:

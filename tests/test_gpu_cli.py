@@ -50,3 +50,49 @@ def test_cli_nonorth2_end_to_end(tmp_path):
             assert u == pytest.approx(v, rel=1e-4, abs=1e-9 * 358.7), (a, b)
     y = np.loadtxt(tmp_path / "force.txt")
     assert np.max(np.abs(y - g["force"]) / np.maximum(np.abs(g["force"]), 1e-6)) < 1e-5
+
+
+def test_cli_split_files_over_ranks(tmp_path):
+    """SPLITFI true with SKPFRMS 1 over 3 ranks: the per-rank A.<rank>.txt
+    concatenate to the A.txt of the same run without SPLITFI, the dim files
+    tile the rows, and b.txt is the same; frames are in round-robin rank order
+    (rank 0: frames 0, 3, 6, 9)."""
+    d = golden_dir("nonorth2")
+    text = open(os.path.join(d, "fm_setup.in")).read()
+    text = text.rstrip("\n").replace("# ENDFILE #", "") + "\n# SPLITFI #\n\ttrue\n# SKPFRMS #\n\t1\n\n# ENDFILE #\n"
+    runs = {}
+    for split in (True, False):
+        w = tmp_path / ("split" if split else "whole")
+        w.mkdir()
+        shutil.copy(os.path.join(d, "input.xyzf"), w / "input.xyzf")
+        open(w / "fm_setup.in", "w").write(text if split else text.replace("\ttrue\n# SKPFRMS #", "\tfalse\n# SKPFRMS #"))
+        exe = os.path.join(PKG, "chimes_lsq_gpu")
+        r = subprocess.run([exe, "fm_setup.in", "--ranks", "3"], cwd=str(w), capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs[split] = w
+    s, whole = runs[True], runs[False]
+    assert not os.path.exists(s / "A.txt") and not os.path.exists(s / "dim.txt")
+    parts = [open(s / ("A.%04d.txt" % r)).read() for r in range(3)]
+    joined, ref = "".join(parts), open(whole / "A.txt").read()
+    # same layout; values equal to rounding (LDS accumulation order differs from run to run)
+    num = r"[-+]?\d+\.\d+e[-+]\d+"
+    assert re.sub(num, "#", joined) == re.sub(num, "#", ref)
+    a = np.array([float(v) for v in re.findall(num, joined)])
+    b = np.array([float(v) for v in re.findall(num, ref)])
+    assert a.shape == b.shape and np.max(np.abs(a - b)) <= 1e-13 * np.abs(b).max()
+    total = int(open(whole / "dim.txt").read().split()[1])
+    ncols = int(open(whole / "dim.txt").read().split()[0])
+    start = 0
+    for r in range(3):
+        n = parts[r].count("\n")
+        assert open(s / ("dim.%04d.txt" % r)).read() == "%d %d %d %d\n" % (ncols, start, start + n - 1, total)
+        start += n
+    assert start == total
+    assert open(s / "b.txt").read() == open(whole / "b.txt").read()
+    # rank order: frames 0 3 6 9 | 2 5 8 | 1 4 7 (frame i to rank r when (i + r) % 3 == 0)
+    g = np.load(os.path.join(d, "golden.npz"))
+    rows = len(g["b"]) // 10
+    order = [0, 3, 6, 9, 2, 5, 8, 1, 4, 7]
+    want = np.concatenate([g["b"][rows * i: rows * (i + 1)] for i in order])
+    assert np.array_equal(np.loadtxt(s / "b.txt"), want)
