@@ -110,6 +110,8 @@ def lib(path=None):
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
         opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
+        opt("cg_set_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.c_int])
+        opt("cg_setup_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
         opt("cg_autotune", [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.POINTER(C.c_int), _dp])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
@@ -227,6 +229,14 @@ class Setup:
             self.L.cg_traj_close(t)
         return out
 
+    def fit_columns(self):
+        """Columns of A a fit keeps under the hierarchical 1B/2B exclusions."""
+        n = self.model.ncols
+        keep = (C.c_int * n)()
+        nk = C.c_int()
+        check(self.L.cg_setup_fit_columns(self.h, keep, C.byref(nk)))
+        return list(keep[:nk.value])
+
     def frame_order(self, nranks):
         """(frames, rank of each) in the order the reference's nranks MPI ranks process them."""
         n = self.nframes
@@ -343,6 +353,12 @@ class Device:
 
     def set_penalty(self, dist, scale):
         check(self.L.cg_set_penalty(self.h, dist, scale), self.h)
+
+    def set_fit_columns(self, keep=None):
+        """Fit only the given (increasing) columns, the others held at zero; None: all."""
+        keep = [] if keep is None else list(keep)
+        arr = (C.c_int * max(len(keep), 1))(*keep)
+        check(self.L.cg_set_fit_columns(self.h, arr, len(keep)), self.h)
 
     def autotune(self, frames, prec=CG_FP64):
         """Pick the assembly tile configuration per family by timing (see cg_autotune);

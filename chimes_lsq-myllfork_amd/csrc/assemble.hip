@@ -2527,7 +2527,7 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 void ctx_free(Ctx &c) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
-                  &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1], &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
+                  &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1], &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
                   &c.b_tmp})
     b->release();
   for (auto &F : c.fam) {
@@ -2615,6 +2615,7 @@ cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec
         for (int w = 0; w < 3; ++w) ms_table[3 * k + w] = ms[k][w];
     return CG_OK;
   } catch (const cg::Error &e) {
+    for (int w = 0; w < 3; ++w) ctx->c.tile_cfg[w] = 0;  // back to the default tiles
     cg::set_error(e.what());
     return e.code;
   }

@@ -138,10 +138,18 @@ int main(int argc, char **argv) {
               (long long)total_rows, m->ncols, ms_asm, total_rows / (ms_asm * 1e-3));
   if (do_fit) {
     std::vector<double> x(m->ncols), y(total_rows);
+    // hierarchical exclusions: fit the kept columns; params.txt takes the
+    // kept coefficients in order, as chimes_lsq.py reads the reduced A.txt
+    std::vector<int> keep(m->ncols);
+    int nkeep = 0;
+    if ((st = cg_setup_fit_columns(setup, keep.data(), &nkeep))) return die("fit columns", st);
+    if ((st = cg_set_fit_columns(ctx, keep.data(), nkeep < m->ncols ? nkeep : 0))) return die("fit columns", st, ctx);
     cg_fit_result res{};
     if ((st = cg_fit(ctx, b_all.data(), eps, refine, gram, x.data(), &res))) return die("fitting", st, ctx);
     if ((st = cg_matvec(ctx, x.data(), y.data()))) return die("computing A x", st, ctx);
-    if ((st = cg_write_params(setup, dir.c_str(), x.data(), m->ncols, total_rows, &res, y.data(), b_all.data())))
+    std::vector<double> xk(nkeep);
+    for (int i = 0; i < nkeep; ++i) xk[i] = x[keep[i]];
+    if ((st = cg_write_params(setup, dir.c_str(), xk.data(), nkeep, total_rows, &res, y.data(), b_all.data())))
       return die("writing params.txt", st);
     std::printf("fit: RMS %.6e, %d of %d variables kept, gram %.3f ms, solve %.3f ms, refine %.3f ms\n", res.rms,
                 res.nvars, m->ncols, res.ms_gram, res.ms_solve, res.ms_refine);

@@ -122,6 +122,8 @@ struct Ctx {
   DBuf b_tcount[3];                       // force tasks: chunks per bucket and their offsets
   DBuf b_A, b_scan;                       // design matrix; hipcub scratch
   DBuf b_x;                               // fit workspace
+  DBuf b_fitkeep;                         // kept columns of a fit (device copy)
+  std::vector<int> fit_keep;              // columns a fit keeps (empty: all)
   DBuf b_force;                           // force evaluation: F, energy/virial, call totals
   std::vector<double> h_out;              // host copy of the force results and totals
   double rate_nb = 0.0;                   // largest neighbour entries per atom seen (0: none yet)

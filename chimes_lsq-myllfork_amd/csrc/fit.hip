@@ -228,6 +228,25 @@ __global__ void k_scale(double *v, const double *w, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] *= w[i];
 }
+// Kept rows/columns of the Gram (column-major) and kept entries of a vector
+// (fits over a column subset: hierarchical exclusions).
+__global__ void k_gather_sym(const double *G, int n, const int *keep, int nk, double *Gk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < nk && j < nk) Gk[(size_t)j * nk + i] = G[(size_t)keep[j] * n + keep[i]];
+}
+__global__ void k_gather(const double *v, const int *keep, int nk, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nk) out[i] = v[keep[i]];
+}
+__global__ void k_zero(double *out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 0.0;
+}
+__global__ void k_scatter_vals(const double *v, const int *keep, int nk, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nk) out[keep[i]] = v[i];
+}
+
 __global__ void k_axpy(double *x, const double *d, int n, double alpha) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += alpha * d[i];
@@ -337,7 +356,7 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, int mode, double 
   // buffers: G [n*n], part [S*n*n], b [m], r [m], x, c, t, dx, w, W, E [n], info
   long long S, kchunk;
   gram_plan(m, n, S, kchunk);
-  double *G = c.b_x.as<double>((size_t)n * n + (size_t)S * n * n + 2 * (size_t)m + 8 * (size_t)n + 64);
+  double *G = c.b_x.as<double>((size_t)n * n + (size_t)S * n * n + 2 * (size_t)m + 10 * (size_t)n + 128);
   if (!G) throw Error(CG_ERR_MEMORY, "fit buffers");
   double *part = G + (size_t)n * n;
   double *d_b = part + (size_t)S * n * n;
@@ -347,34 +366,64 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, int mode, double 
   CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
   CG_HIP(hipEventRecord(e0, c.stream));
   gram(c, mode, G, part, S, kchunk);
+  // a fit over a column subset works on the kept rows/columns of G (gathered
+  // into the Gram's split-K scratch, free after the reduction) and keeps the
+  // other coefficients at zero
+  const bool masked = !c.fit_keep.empty() && (int)c.fit_keep.size() < n;
+  const int ne = masked ? (int)c.fit_keep.size() : n;
+  double *Ge = G;
+  double *d_g = d_E + n + 16, *d_g2 = d_g + n;
+  int *d_keep = nullptr;
+  if (masked) {
+    d_keep = c.b_fitkeep.as<int>(ne);
+    if (!d_keep) throw Error(CG_ERR_MEMORY, "fit column list");
+    CG_HIP(hipMemcpyAsync(d_keep, c.fit_keep.data(), ne * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    Ge = part;
+    hipLaunchKernelGGL(k_gather_sym, dim3((ne + 255) / 256, ne), dim3(256), 0, c.stream, G, n, (const int *)d_keep,
+                       ne, Ge);
+  }
   CG_HIP(hipEventRecord(e1, c.stream));
   // eigen-decomposition G = V diag(W) V^T (V column-major in G; G is symmetric)
   rocblas_int *d_info = reinterpret_cast<rocblas_int *>(d_E + n);
-  if (rocsolver_dsyevd(fs.h, rocblas_evect_original, rocblas_fill_upper, n, G, n, d_W, d_E, d_info) !=
+  if (rocsolver_dsyevd(fs.h, rocblas_evect_original, rocblas_fill_upper, ne, Ge, ne, d_W, d_E, d_info) !=
       rocblas_status_success)
     throw Error(CG_ERR_NUMERIC, "rocsolver_dsyevd failed");
-  std::vector<double> W(n);
+  std::vector<double> W(ne);
   int info = 0;
-  CG_HIP(hipMemcpyAsync(W.data(), d_W, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(W.data(), d_W, ne * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   CG_HIP(hipStreamSynchronize(c.stream));
   if (info != 0) throw Error(CG_ERR_NUMERIC, "eigensolver did not converge (info=" + std::to_string(info) + ")");
   double lmax = 0;
   for (double v : W) lmax = std::max(lmax, std::fabs(v));
   const double thr = eps * eps * lmax;
-  std::vector<double> winv(n, 0.0);
+  std::vector<double> winv(ne, 0.0);
   int nvars = 0;
-  for (int i = 0; i < n; ++i)
+  for (int i = 0; i < ne; ++i)
     if (W[i] > thr) { winv[i] = 1.0 / W[i]; ++nvars; }
-  CG_HIP(hipMemcpyAsync(d_w, winv.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  // P v = V W+ V^T v
+  CG_HIP(hipMemcpyAsync(d_w, winv.data(), ne * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  // P v = V W+ V^T v (on the kept entries; zero elsewhere)
   auto project = [&](const double *in, double *out) {
     const double one = 1.0, zero = 0.0;
-    if (rocblas_dgemv(fs.h, rocblas_operation_transpose, n, n, &one, G, n, in, 1, &zero, d_t, 1) != rocblas_status_success)
+    const double *src = in;
+    double *dst = out;
+    if (masked) {
+      hipLaunchKernelGGL(k_gather, dim3((ne + 255) / 256), dim3(256), 0, c.stream, in, (const int *)d_keep, ne, d_g);
+      src = d_g;
+      dst = d_g2;
+    }
+    if (rocblas_dgemv(fs.h, rocblas_operation_transpose, ne, ne, &one, Ge, ne, src, 1, &zero, d_t, 1) !=
+        rocblas_status_success)
       throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
-    hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_t, d_w, n);
-    if (rocblas_dgemv(fs.h, rocblas_operation_none, n, n, &one, G, n, d_t, 1, &zero, out, 1) != rocblas_status_success)
+    hipLaunchKernelGGL(k_scale, dim3((ne + 255) / 256), dim3(256), 0, c.stream, d_t, d_w, ne);
+    if (rocblas_dgemv(fs.h, rocblas_operation_none, ne, ne, &one, Ge, ne, d_t, 1, &zero, dst, 1) !=
+        rocblas_status_success)
       throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
+    if (masked) {
+      hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, c.stream, out, n);
+      hipLaunchKernelGGL(k_scatter_vals, dim3((ne + 255) / 256), dim3(256), 0, c.stream, d_g2, (const int *)d_keep,
+                         ne, out);
+    }
   };
   auto resid = [&](const double *x) {
     if (c.A_prec == CG_FP64) residual(c, (const double *)c.A, m, n, x, d_b, d_r);
@@ -483,6 +532,18 @@ cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters
     cg::set_error(e.what());
     return e.code;
   }
+}
+
+cg_status cg_set_fit_columns(cg_ctx *ctx, const int *keep, int nkeep) {
+  if (!ctx || nkeep < 0 || (nkeep > 0 && !keep)) { cg::set_error("cg_set_fit_columns: bad argument"); return CG_ERR_ARG; }
+  const int n = ctx->c.m.ncols;
+  for (int i = 0; i < nkeep; ++i)
+    if (keep[i] < 0 || keep[i] >= n || (i > 0 && keep[i] <= keep[i - 1])) {
+      cg::set_error("cg_set_fit_columns: columns must be increasing and below ncols");
+      return CG_ERR_ARG;
+    }
+  ctx->c.fit_keep.assign(keep, keep + nkeep);
+  return CG_OK;
 }
 
 cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y) {

@@ -250,9 +250,17 @@ cg_status cg_write_rank(const cg_setup *cs, int rank) {
 }
 
 namespace {
+// 2-body column n of an excluded pair type (A_MAT::skip_2b, src/A_Matrix.C:192-208)
+bool skip_2b(const Setup &s, int n) {
+  for (int q : s.excl_2b)
+    if (n >= q * s.o2 && n < (q + 1) * s.o2) return true;
+  return false;
+}
+
 void ones(const Setup &s, std::string &o, bool energy_row, const std::vector<int> &count) {
   if (!s.fit_ener) return;
   for (int i = 0; i < s.ntypes; ++i) {
+    if (std::find(s.excl_1b.begin(), s.excl_1b.end(), i) != s.excl_1b.end()) continue;  // add_col_of_ones
     if (!energy_row) o += " 0.0";
     else { o += ' '; o += std::to_string(count[i]); }
   }
@@ -289,7 +297,11 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
   auto row_out = [&](const char *sep, bool energy_row) {
     if (s.write_A) {
       const double *a = rows + (int64_t)r * nc;
-      for (int n = 0; n < ns; ++n) { put_e16(s.abuf, a[n]); s.abuf += sep; }
+      for (int n = 0; n < ns; ++n) {
+        if (!s.excl_2b.empty() && skip_2b(s, n)) continue;
+        put_e16(s.abuf, a[n]);
+        s.abuf += sep;
+      }
       ones(s, s.abuf, energy_row, count);
       s.abuf += '\n';
       if (s.abuf.size() > (1u << 20)) flush_A(s);
@@ -349,7 +361,7 @@ cg_status cg_write_close(const cg_setup *cs) {
       if (n > 0) {
         std::ofstream dim(rank_file(s, "dim", (int)r));
         if (!dim) { g_err = "Could not open " + rank_file(s, "dim", (int)r); return CG_ERR_INPUT; }
-        dim << s.ncols << " " << start << " " << start + n - 1 << " " << s.data_count << "\n";
+        dim << s.ncols - s.n_excl_lt3b << " " << start << " " << start + n - 1 << " " << s.data_count << "\n";
       }
       start += n;
     }
@@ -357,7 +369,22 @@ cg_status cg_write_close(const cg_setup *cs) {
   }
   std::ofstream dim(s.outdir + "/dim.txt");
   if (!dim) { g_err = "Could not open dim.txt"; return CG_ERR_INPUT; }
-  dim << s.ncols << " " << s.data_count << std::endl;
+  dim << s.ncols - s.n_excl_lt3b << " " << s.data_count << std::endl;
+  return CG_OK;
+}
+
+cg_status cg_setup_fit_columns(const cg_setup *cs, int *keep, int *nkeep) {
+  if (!cs || !keep || !nkeep) { g_err = "cg_setup_fit_columns: bad argument"; return CG_ERR_ARG; }
+  const Setup &s = *cs->s;
+  const cg_model &m = s.model;
+  int n = 0;
+  for (int c = 0; c < m.ncols; ++c) {
+    bool drop = false;
+    if (c < m.ncols_short) drop = !s.excl_2b.empty() && c < m.n2b_cols && skip_2b(s, c);
+    else drop = std::find(s.excl_1b.begin(), s.excl_1b.end(), c - m.ncols_short) != s.excl_1b.end();
+    if (!drop) keep[n++] = c;
+  }
+  *nkeep = n;
   return CG_OK;
 }
 
@@ -469,8 +496,22 @@ std::string header_text(const Setup &s) {
   h << "FITCOUL: false" << std::endl;
   h << "USE3BCH: " << (s.o3 > 0 ? "true" : "false") << std::endl;
   h << "USE4BCH: " << (s.o4 > 0 ? "true" : "false") << std::endl;
-  h << "EXCLD1B: false" << std::endl;
-  h << "EXCLD2B: false" << std::endl;
+  // EXCLD lines: excluded type / pair indices, each followed by a space
+  // (src/chimes_lsq.C:592-610)
+  if (!s.excl_1b.empty()) {
+    h << "EXCLD1B: ";
+    for (int t : s.excl_1b) h << t << " ";
+    h << std::endl;
+  } else {
+    h << "EXCLD1B: false" << std::endl;
+  }
+  if (!s.excl_2b.empty()) {
+    h << "EXCLD2B: ";
+    for (int q : s.excl_2b) h << q << " ";
+    h << std::endl;
+  } else {
+    h << "EXCLD2B: false" << std::endl;
+  }
   h << std::endl << "PAIRTYP: " << "CHEBYSHEV" << " ";
   h << " " << s.o2 << " " << s.o3 << " " << s.o4 << " " << double(s.cheby_low) << " " << double(s.cheby_high) << std::endl;
   h << std::endl << "ATOM TYPES: " << s.ntypes << std::endl << std::endl;
@@ -565,10 +606,21 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   auto tok = [](const std::string &l) { return split_ws(l); };
   int trips_line = 0, quads_line = 0;
   long total_trips = 0, total_quads = 0;
+  std::vector<int> excl2, excl1;  // EXCLD2B / EXCLD1B lists of the header
   for (size_t i = 0; i < hf.size(); ++i) {
     P(hf[i]);
     auto t = tok(hf[i]);
     bool brk = false;
+    if (hf[i].find("EXCLD2B") != std::string::npos) {
+      excl2.clear();
+      if (!(t.size() == 2 && t[1] == "false"))
+        for (size_t q = 1; q < t.size(); ++q) excl2.push_back(std::stoi(t[q]));
+    }
+    if (hf[i].find("EXCLD1B") != std::string::npos) {
+      excl1.clear();
+      if (!(t.size() == 2 && t[1] == "false"))
+        for (size_t q = 1; q < t.size(); ++q) excl1.push_back(std::stoi(t[q]));
+    }
     if (t.size() > 3 && t[2] == "TRIPLETS:") {
       total_trips = std::stol(t[3]);
       trips_line = (int)i;
@@ -600,11 +652,19 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   for (int i = 0; i < total_pairs; ++i) {
     auto t = tok(hf[pairs_line + 2 + i + 1]);
     P("PAIRTYPE PARAMS: " + std::to_string(i) + " " + t[1] + " " + t[2] + "\n");
-    for (int j = 0; j < snum2; ++j) {
-      std::snprintf(buf, sizeof buf, "%3d %21.13e", j, x[effective_pair * snum2 + j]);
-      P(buf);
+    if (std::find(excl2.begin(), excl2.end(), i) != excl2.end()) {
+      // excluded pair type: zeros, no entry of x
+      for (int j = 0; j < snum2; ++j) {
+        std::snprintf(buf, sizeof buf, "%3d %21.13e", j, 0.0);
+        P(buf);
+      }
+    } else {
+      for (int j = 0; j < snum2; ++j) {
+        std::snprintf(buf, sizeof buf, "%3d %21.13e", j, x[effective_pair * snum2 + j]);
+        P(buf);
+      }
+      ++effective_pair;
     }
-    ++effective_pair;
     P(" ");
   }
   int counted_trip = 0, counted_quad = 0;
@@ -674,11 +734,15 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   for (auto &l : split_lines(map_text(s))) P(l);
   P("");
   int total = effective_pair * snum2 + counted_trip + counted_quad;
-  if (total != nx && nx != total + ntypes) throw Error(CG_ERR_ARG, "Error in counting parameters");
-  if (nx == total + ntypes) {
+  const int n_offsets = ntypes - (int)excl1.size();
+  if (total != nx && nx != total + n_offsets) throw Error(CG_ERR_ARG, "Error in counting parameters");
+  if (nx == total + n_offsets) {
     P("NO ENERGY OFFSETS:  " + std::to_string(ntypes));
+    int eff = 0;
     for (int i = 0; i < ntypes; ++i) {
-      std::snprintf(buf, sizeof buf, "ENERGY OFFSET %d %21.13e", i + 1, x[total + i]);
+      const bool ex = std::find(excl1.begin(), excl1.end(), i) != excl1.end();
+      std::snprintf(buf, sizeof buf, "ENERGY OFFSET %d %21.13e", i + 1, ex ? 0.0 : x[total + eff]);
+      if (!ex) ++eff;
       P(buf);
     }
   }

@@ -65,8 +65,6 @@ Setup *read_params(const std::string &path) {
   if (flag("USECOUL:") || flag("FITCOUL:"))
     throw Error(CG_ERR_UNSUPPORTED, "Coulomb (Ewald) terms in " + path + " are not supported");
   if (flag("USEPOVR:")) throw Error(CG_ERR_UNSUPPORTED, "over-coordination terms in " + path + " are not supported");
-  if (flag("EXCLD1B:") || flag("EXCLD2B:"))
-    throw Error(CG_ERR_UNSUPPORTED, "hierarchical 1B/2B exclusions in " + path + " are not supported");
 
   // ---- header
   size_t i = P.find("PAIRTYP:");

@@ -502,12 +502,34 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
               (s->type_names[j] == p.a1 && s->type_names[i] == p.a2))
             s->pair_map.insert({s->type_names[i] + s->type_names[j], (int)q});
         }
-    // EXCLUDE (src/input.C:753-874)
-    for (size_t i = 0; i < N; ++i) {
-      std::string h = L.head3(i);
-      if (h == "EXCLUDE 1B INTERACTION:" || h == "EXCLUDE 2B INTERACTION:") {
-        if (s->hierarchical) throw Error(CG_ERR_UNSUPPORTED, "hierarchical 1B/2B exclusions are not supported");
-      }
+    // EXCLUDE (src/input.C:753-874): 1- and 2-body lists only in a
+    // hierarchical fit, the first list of each kind
+    if (s->hierarchical) {
+      for (size_t i = 0; i < N; ++i)
+        if (L.width(i) >= 4 && L.head3(i) == "EXCLUDE 1B INTERACTION:") {
+          const int ne = to_int(L.at(i, 3));
+          for (int j = 1; j <= ne; ++j)
+            for (int t = 0; t < nt; ++t)
+              if (L.at(i + j, 0) == s->type_names[t]) {
+                s->excl_1b.push_back(t);
+                s->n_excl_lt3b += 1;
+              }
+          break;
+        }
+      for (size_t i = 0; i < N; ++i)
+        if (L.width(i) >= 4 && L.head3(i) == "EXCLUDE 2B INTERACTION:") {
+          const int ne = to_int(L.at(i, 3));
+          for (int j = 1; j <= ne; ++j)
+            for (size_t q = 0; q < s->pairs.size(); ++q) {
+              const auto &p = s->pairs[q];
+              const std::string a = L.at(i + j, 0), b = L.at(i + j, 1);
+              if ((p.a1 == a && p.a2 == b) || (p.a1 == b && p.a2 == a)) {
+                s->excl_2b.push_back((int)q);
+                s->n_excl_lt3b += s->o2;
+              }
+            }
+          break;
+        }
     }
     for (int fam = 3; fam <= 4; ++fam) {
       auto &F = fam == 3 ? s->trips : s->quads;

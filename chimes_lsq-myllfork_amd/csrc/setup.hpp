@@ -63,6 +63,11 @@ struct Setup {
   std::vector<int> infile_frames;
   std::vector<std::string> force_flags, stress_flags, energy_flags;
   bool wrap = false, split_files = false, hierarchical = false;
+  // hierarchical fit (HIERARC true): excluded atom types (energy-offset
+  // columns) and pair types (2-body columns), in input order, and the number
+  // of columns they remove (N_EXCL_LT_3B, src/input.C:760-840)
+  std::vector<int> excl_1b, excl_2b;
+  int n_excl_lt3b = 0;
   int nframes = 0, nlayers = 0, skip_frames = 0;
   bool fitcoul = false, fit_ener = false, use_partial_charges = false;
   int fit_stress = CG_STRESS_NONE, nstress = -1, nener = -1;

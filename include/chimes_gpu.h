@@ -263,6 +263,15 @@ typedef struct cg_fit_result {
 } cg_fit_result;
 cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, int gram_mode,
                  double *x, cg_fit_result *res);
+/* Fit only the columns keep[0..nkeep) (increasing column indices) and hold the
+ * others at zero; nkeep = 0 restores fits over every column.  Used for the
+ * hierarchical fit's excluded 1-body and 2-body columns, which chimes_lsq
+ * leaves out of A.txt (src/A_Matrix.C:155-208). */
+cg_status cg_set_fit_columns(cg_ctx *ctx, const int *keep, int nkeep);
+/* Columns of A that a fit keeps under the setup's hierarchical exclusions
+ * (EXCLUDE 1B/2B INTERACTION with HIERARC true; src/input.C:760-840): keep
+ * [ncols] receives them in increasing order, nkeep their number. */
+cg_status cg_setup_fit_columns(const cg_setup *s, int *keep, int *nkeep);
 /* y = A x for the last assembled A (host output). */
 cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y);
 cg_status cg_write_params(const cg_setup *s, /* params.txt + force.txt */ const char *outdir, const double *x,

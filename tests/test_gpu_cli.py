@@ -96,3 +96,38 @@ def test_cli_split_files_over_ranks(tmp_path):
     order = [0, 3, 6, 9, 2, 5, 8, 1, 4, 7]
     want = np.concatenate([g["b"][rows * i: rows * (i + 1)] for i in order])
     assert np.array_equal(np.loadtxt(s / "b.txt"), want)
+
+
+def test_cli_hierarchical_fit(ch, synth, oracle, tmp_path):
+    """HIERARC with EXCLUDE 1B (H) and 2B (O-H): A.txt is the oracle's A without
+    those columns, the fit is the thin-SVD fit of that reduced A, and params.txt
+    prints zeros for the excluded pair and energy offset."""
+    text, frames, names = synth.water(nframes=6, nmol=27, o2=8, o3=4)
+    text = text.replace("# ENDFILE #", "# HIERARC #\n\ttrue\nEXCLUDE 1B INTERACTION: 1\nH\n"
+                                       "EXCLUDE 2B INTERACTION: 1\nO H\n\n# ENDFILE #")
+    open(tmp_path / "fm_setup.in", "w").write(text)
+    synth.write_xyzf(str(tmp_path / "input.xyzf"), frames, names)
+    exe = os.path.join(PKG, "chimes_lsq_gpu")
+    r = subprocess.run([exe, "fm_setup.in", "--fit"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    s = ch.Setup(str(tmp_path / "fm_setup.in"))
+    fr = s.frames(str(tmp_path))
+    keep = s.fit_columns()
+    Ar = oracle.assemble(s, fr)[:, keep]
+    A = np.loadtxt(tmp_path / "A.txt")
+    assert A.shape == Ar.shape and colscaled_error(A, Ar) < 1e-10
+    b = np.loadtxt(tmp_path / "b.txt")
+    U, S, Vt = np.linalg.svd(Ar, full_matrices=False)
+    k = S > 1e-5 * S.max()
+    x = Vt[k].T @ ((U[:, k].T @ b) / S[k])
+    y = Ar @ x
+    yg = np.loadtxt(tmp_path / "force.txt")
+    assert np.max(np.abs(yg - y)) <= 2e-6 * np.abs(y).max()
+    txt = open(tmp_path / "params.txt").read()
+    assert "EXCLD1B: 1 \nEXCLD2B: 2 \n" in txt
+    blk = re.search(r"PAIRTYPE PARAMS: 2 O H\n\n((?:\s*\d+ +\S+\n)+)", txt).group(1)
+    assert all(float(l.split()[1]) == 0.0 for l in blk.strip().split("\n"))
+    assert re.search(r"ENERGY OFFSET 2 +(\S+)", txt).group(1) == "0.0000000000000e+00"
+    rms = float(re.search(r"! RMS force error\s*=\s*(\S+)", txt).group(1))
+    # params.txt prints the RMS with 5 significant digits (%11.4e)
+    assert rms == pytest.approx(np.sqrt(np.mean((b - y) ** 2)), rel=1e-4)
