@@ -41,7 +41,7 @@ class Model(C.Structure):
                 ("smooth_dist", C.c_double), ("nlayers", C.c_int), ("small_mode", C.c_int),
                 ("fit_stress", C.c_int), ("fit_energy", C.c_int), ("nstress", C.c_int),
                 ("nener", C.c_int), ("n2b_cols", C.c_int), ("ncols_short", C.c_int),
-                ("n_offset_cols", C.c_int), ("ncols", C.c_int), ("max_cutoff_2b", C.c_double),
+                ("n_charge_cols", C.c_int), ("n_offset_cols", C.c_int), ("ncols", C.c_int), ("max_cutoff_2b", C.c_double),
                 ("c3", ClusterSet), ("c4", ClusterSet)]
 
 
@@ -115,6 +115,9 @@ def lib(path=None):
         opt("cg_autotune", [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.POINTER(C.c_int), _dp])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
+        opt("cg_append_rows", [C.c_void_p, _dp, C.c_int])
+        opt("cg_ewald_params", [C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)])
+        opt("cg_setup_constraint_rows", [C.c_void_p, _dp, _dp, C.POINTER(C.c_int)])
         opt("cg_serve_create", [C.POINTER(Model), C.c_int, _dp, C.c_int, C.c_size_t, C.c_int,
                                 C.POINTER(C.c_void_p)])
         opt("cg_serve_submit", [C.c_void_p, C.POINTER(Frame), C.POINTER(C.c_int64)])
@@ -123,6 +126,14 @@ def lib(path=None):
         opt("cg_serve_destroy", [C.c_void_p])
         _lib = L
     return _lib
+
+
+def ewald_params(natoms, cell):
+    """(alpha, rc, kmax) of a frame's Ewald columns."""
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    a, r, k = C.c_double(), C.c_double(), C.c_int()
+    check(lib().cg_ewald_params(int(natoms), dptr(cell), C.byref(a), C.byref(r), C.byref(k)))
+    return a.value, r.value, k.value
 
 
 def check(st, ctx=None):
@@ -237,6 +248,16 @@ class Setup:
         check(self.L.cg_setup_fit_columns(self.h, keep, C.byref(nk)))
         return list(keep[:nk.value])
 
+    def constraint_rows(self):
+        """FITCOUL charge-constraint rows and their b values ([n x ncols], [n])."""
+        n = C.c_int()
+        check(self.L.cg_setup_constraint_rows(self.h, None, None, C.byref(n)))
+        rows = np.zeros((n.value, self.model.ncols))
+        b = np.zeros(n.value)
+        if n.value:
+            check(self.L.cg_setup_constraint_rows(self.h, dptr(rows), dptr(b), C.byref(n)))
+        return rows, b
+
     def frame_order(self, nranks):
         """(frames, rank of each) in the order the reference's nranks MPI ranks process them."""
         n = self.nframes
@@ -313,6 +334,11 @@ class Device:
         n = (C.c_int64 * 3)()
         check(self.L.cg_last_detail(self.h, ms, n), self.h)
         return list(ms), list(n)
+
+    def append_rows(self, rows):
+        """Append host rows (e.g. charge constraints) after the last assembled A."""
+        r = np.ascontiguousarray(rows, dtype=np.float64)
+        check(self.L.cg_append_rows(self.h, dptr(r), r.shape[0]), self.h)
 
     def fit(self, b, eps=1e-5, refine=2, gram=CG_GRAM_FP64):
         x = np.zeros(self.setup.model.ncols)

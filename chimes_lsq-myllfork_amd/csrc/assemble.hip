@@ -2230,6 +2230,7 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     else
       hipLaunchKernelGGL(k_offsets<double>, dim3(nframes), dim3(256), 0, c.stream, d_frames, d_type, nframes, m.ntypes, m.ncols_short, (double *)A, ld);
   }
+  if (m.n_charge_cols > 0) ewald_columns(c, frames, hf, B.d_af, d_type, B.d_wrap, B.natoms, A, ld, prec);
   CG_HIP(hipEventRecord(c.ev[2], c.stream));
   CG_HIP(hipEventSynchronize(c.ev[2]));
   CG_HIP(hipGetLastError());
@@ -2505,6 +2506,8 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
 
 void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
                  double *vir) {
+  if (c.m.n_charge_cols > 0)
+    throw Error(CG_ERR_UNSUPPORTED, "force evaluation of a model with Ewald charge columns is not supported");
   ++c.n_force;
   force_batch_impl(c, x_host, frames, nframes, F, E, vir, false);
 }
@@ -2528,7 +2531,7 @@ void ctx_free(Ctx &c) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
                   &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1], &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
-                  &c.b_tmp})
+                  &c.b_tmp, &c.b_ewald})
     b->release();
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
@@ -2634,6 +2637,44 @@ cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host) {
       CG_HIP(hipMemcpy(tmp.data(), (float *)c.A + r0 * c.A_ld, n * sizeof(float), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
     }
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_append_rows(cg_ctx *ctx, const double *rows, int nrows) {
+  if (!ctx || !ctx->c.A) { cg::set_error("nothing assembled"); return CG_ERR_ARG; }
+  if (nrows < 0 || (nrows > 0 && !rows)) { cg::set_error("cg_append_rows: bad rows"); return CG_ERR_ARG; }
+  if (nrows == 0) return CG_OK;
+  auto &c = ctx->c;
+  try {
+    CG_HIP(hipSetDevice(c.device));
+    const size_t esz = c.A_prec == CG_FP32 ? sizeof(float) : sizeof(double);
+    const size_t old_bytes = (size_t)c.A_rows * c.A_ld * esz;
+    const size_t add = (size_t)nrows * c.A_ld;
+    if (old_bytes + add * esz > c.b_A.cap) {
+      // grow, keeping the assembled rows
+      void *p = nullptr;
+      const size_t n = old_bytes + add * esz;
+      if (hipMalloc(&p, n + n / 4 + 256) != hipSuccess) throw cg::Error(CG_ERR_MEMORY, "A does not fit in device memory");
+      CG_HIP(hipMemcpyAsync(p, c.A, old_bytes, hipMemcpyDeviceToDevice, c.stream));
+      CG_HIP(hipStreamSynchronize(c.stream));
+      c.b_A.release();
+      c.b_A.p = p;
+      c.b_A.cap = n + n / 4 + 256;
+      ++cg::g_buffer_generation;
+      c.A = p;
+    }
+    char *dst = static_cast<char *>(c.A) + old_bytes;
+    if (c.A_prec == CG_FP32) {
+      std::vector<float> tmp(rows, rows + add);
+      CG_HIP(hipMemcpy(dst, tmp.data(), add * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      CG_HIP(hipMemcpy(dst, rows, add * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c.A_rows += nrows;
     return CG_OK;
   } catch (const cg::Error &e) {
     cg::set_error(e.what());

@@ -137,6 +137,16 @@ int main(int argc, char **argv) {
   std::printf("...matrix population complete: %lld rows x %d columns, %.3f ms on the GPU (%.3e rows/s)\n",
               (long long)total_rows, m->ncols, ms_asm, total_rows / (ms_asm * 1e-3));
   if (do_fit) {
+    // FITCOUL charge-constraint rows follow the last frame's rows
+    int ncons = 0;
+    if ((st = cg_setup_constraint_rows(setup, nullptr, nullptr, &ncons))) return die("constraint rows", st);
+    if (ncons > 0) {
+      std::vector<double> cr((size_t)ncons * m->ncols), cb(ncons);
+      if ((st = cg_setup_constraint_rows(setup, cr.data(), cb.data(), &ncons))) return die("constraint rows", st);
+      if ((st = cg_append_rows(ctx, cr.data(), ncons))) return die("constraint rows", st, ctx);
+      b_all.insert(b_all.end(), cb.begin(), cb.end());
+      total_rows += ncons;
+    }
     std::vector<double> x(m->ncols), y(total_rows);
     // hierarchical exclusions: fit the kept columns; params.txt takes the
     // kept coefficients in order, as chimes_lsq.py reads the reduced A.txt

@@ -145,6 +145,8 @@ struct Ctx {
   bool coef_valid = false;
   std::vector<double> h_x_last;           // coefficients the tensors were built from
   DBuf b_tmp;                             // matvec output
+  DBuf b_ewald;                           // Ewald frame table, k-vectors, structure factors
+  std::vector<char> h_ewald;              // host source of the Ewald table upload
   void *A = nullptr;          // last assembled A (device)
   int A_prec = CG_FP64;
   int64_t A_rows = 0;
@@ -163,6 +165,12 @@ void ctx_init(Ctx &c, const cg_model *m, int device);
 void ctx_free(Ctx &c);
 void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
                  double *vir);
+
+// Defined in ewald.hip: Ewald parameters of a frame (optimal_ewald_params) and
+// the FITCOUL charge columns of an assembled batch.
+void ewald_params(int natoms, const double L[3], double *alpha, double *rc, int *kmax);
+void ewald_columns(Ctx &c, const cg_frame *frames, const std::vector<DevFrame> &hf, const int *d_af,
+                   const int *d_type, const double *d_wrap, int natoms_total, void *A, long long ld, int prec);
 
 }  // namespace cg
 

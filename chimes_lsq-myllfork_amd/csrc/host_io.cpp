@@ -198,6 +198,7 @@ cg_status cg_write_open_opts(const cg_setup *cs, const char *outdir, int flags) 
   s.rank_open = false;
   s.rank_rows.assign(std::max(s.nranks, 1), 0);
   s.abuf.clear();
+  s.cons_written = false;
   if (s.write_A && !s.split_files) {
     s.fA.open(s.outdir + "/A.txt");
     if (!s.fA) { g_err = "Could not open A.txt in " + s.outdir; return CG_ERR_INPUT; }
@@ -236,6 +237,41 @@ void put_e16(std::string &out, double x) {
 }
 }  // namespace
 
+namespace {
+// FITCOUL charge-constraint rows (A_MAT::PRINT_CONSTRAINTS,
+// src/A_Matrix.C:567-624): zeros for the short-range columns, then the
+// weights in pair-index order; b gets the constraint's value.  They go to
+// `rank`'s part of the output: the first rank without frames in split mode,
+// else the last rank.
+void write_constraints(Setup &s, int rank) {
+  if (s.charge_cons.empty() || s.cons_written) return;
+  s.cons_written = true;
+  const int np = (int)s.pairs.size();
+  std::string t;
+  for (const auto &cc : s.charge_cons) {
+    for (int n = 0; n < s.nshort; ++n) t += "0.0 ";
+    for (int j = 0; j < np; ++j)
+      for (size_t k = 0; k < cc.pair_idx.size(); ++k)
+        if (cc.pair_idx[k] == j) {
+          put_e16(t, cc.weight[k]);
+          t += ' ';
+        }
+    t += '\n';
+    s.fb << cc.force << "\n";
+    s.data_count += 1;
+  }
+  if ((int)s.rank_rows.size() <= rank) s.rank_rows.resize(rank + 1, 0);
+  s.rank_rows[rank] += (int64_t)s.charge_cons.size();
+  if (!s.write_A) return;
+  if (!s.split_files || (rank == s.cur_rank && s.rank_open)) {
+    s.abuf += t;
+    return;
+  }
+  std::ofstream f(rank_file(s, "A", rank));  // a rank without frames: a new file
+  f << t;
+}
+}  // namespace
+
 cg_status cg_write_rank(const cg_setup *cs, int rank) {
   Setup &s = *cs->s;
   if (rank < 0 || rank < s.cur_rank) { g_err = "cg_write_rank: ranks must not decrease"; return CG_ERR_ARG; }
@@ -245,6 +281,10 @@ cg_status cg_write_rank(const cg_setup *cs, int rank) {
     s.fA.close();
     s.rank_open = false;
   }
+  // ranks passed over without frames: the first takes the constraint rows
+  if (s.split_files)
+    for (int j = s.cur_rank; j < rank; ++j)
+      if (s.rank_rows[j] == 0) { write_constraints(s, j); break; }
   s.cur_rank = rank;
   return CG_OK;
 }
@@ -294,7 +334,7 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
   int total = 0;
   for (int c : count) total += c;
   int r = 0;
-  auto row_out = [&](const char *sep, bool energy_row) {
+  auto row_out = [&](const char *sep, bool energy_row, bool force_row) {
     if (s.write_A) {
       const double *a = rows + (int64_t)r * nc;
       for (int n = 0; n < ns; ++n) {
@@ -302,6 +342,12 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
         put_e16(s.abuf, a[n]);
         s.abuf += sep;
       }
+      // FITCOUL: the Ewald columns, on force rows only (src/A_Matrix.C:249-252)
+      if (force_row)
+        for (int q = 0; q < m.n_charge_cols; ++q) {
+          put_e16(s.abuf, a[ns + q]);
+          s.abuf += sep;
+        }
       ones(s, s.abuf, energy_row, count);
       s.abuf += '\n';
       if (s.abuf.size() > (1u << 20)) flush_A(s);
@@ -310,7 +356,7 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
     ++r;
   };
   for (int a = 0; a < f->natoms; ++a) {
-    for (int c = 0; c < 3; ++c) row_out("   ", false);
+    for (int c = 0; c < 3; ++c) row_out("   ", false, true);
     for (int c = 0; c < 3; ++c) s.fb << f->forces[3 * a + c] << "\n";
     s.data_count += 3;
     const std::string &nm = s.type_names[f->type[a]];
@@ -318,13 +364,13 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
   }
   bool st = s.fit_stress != CG_STRESS_NONE && (s.nstress < 0 || f->index < s.nstress);
   if (st && s.fit_stress == CG_STRESS_DIAG) {
-    for (int c = 0; c < 3; ++c) row_out(" ", false);
+    for (int c = 0; c < 3; ++c) row_out(" ", false, false);
     const char *lab[3] = {"s_xx ", "s_yy ", "s_zz "};
     for (int c = 0; c < 3; ++c) s.fb << f->stress[c] / kGPa << "\n";
     s.data_count += 3;
     for (int c = 0; c < 3; ++c) s.fbl << s.stress_flags[file] << lab[c] << f->stress[c] / kGPa << "\n";
   } else if (st && s.fit_stress == CG_STRESS_ALL) {
-    for (int c = 0; c < 9; ++c) row_out(" ", false);
+    for (int c = 0; c < 9; ++c) row_out(" ", false, false);
     // xx xy xz / yx yy yz / zx zy zz from (xx yy zz xy xz yz)
     const int idx[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
     const char *lab[9] = {"s_xx ", "s_xy ", "s_xz ", "s_yx ", "s_yy ", "s_yz ", "s_zx ", "s_zy ", "s_zz "};
@@ -333,7 +379,7 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
     for (int c = 0; c < 9; ++c) s.fbl << s.stress_flags[file] << lab[c] << f->stress[idx[c]] / kGPa << "\n";
   }
   if (s.fit_ener && (s.nener < 0 || f->index < s.nener)) {
-    for (int c = 0; c < 3; ++c) row_out(" ", true);
+    for (int c = 0; c < 3; ++c) row_out(" ", true, false);
     for (int c = 0; c < 3; ++c) {
       s.fb << f->energy << "\n";
       s.fbl << s.energy_flags[file] << "+1 " << f->energy << "\n";
@@ -347,6 +393,14 @@ cg_status cg_write_frame(const cg_setup *cs, const cg_frame *f, const double *ro
 
 cg_status cg_write_close(const cg_setup *cs) {
   Setup &s = *cs->s;
+  if (!s.charge_cons.empty() && !s.cons_written) {
+    int pr = std::max(s.nranks, 1) - 1;
+    if (s.split_files)
+      for (int j = s.cur_rank; j < std::max(s.nranks, 1); ++j)
+        if (j >= (int)s.rank_rows.size() || s.rank_rows[j] == 0) { pr = j; break; }
+    if (!s.split_files) pr = s.cur_rank;  // one A.txt: the rows go last
+    write_constraints(s, pr);
+  }
   if (s.write_A) {
     flush_A(s);
     if (s.fA.is_open()) s.fA.close();
@@ -370,6 +424,23 @@ cg_status cg_write_close(const cg_setup *cs) {
   std::ofstream dim(s.outdir + "/dim.txt");
   if (!dim) { g_err = "Could not open dim.txt"; return CG_ERR_INPUT; }
   dim << s.ncols - s.n_excl_lt3b << " " << s.data_count << std::endl;
+  return CG_OK;
+}
+
+cg_status cg_setup_constraint_rows(const cg_setup *cs, double *rows, double *b, int *nrows) {
+  if (!cs || !nrows) { g_err = "cg_setup_constraint_rows: bad argument"; return CG_ERR_ARG; }
+  const Setup &s = *cs->s;
+  const cg_model &m = s.model;
+  *nrows = (int)s.charge_cons.size();
+  for (size_t i = 0; i < s.charge_cons.size(); ++i) {
+    const auto &cc = s.charge_cons[i];
+    if (rows) {
+      double *r = rows + i * (size_t)m.ncols;
+      std::fill(r, r + m.ncols, 0.0);
+      for (size_t k = 0; k < cc.pair_idx.size(); ++k) r[m.ncols_short + cc.pair_idx[k]] = cc.weight[k];
+    }
+    if (b) b[i] = cc.force;
+  }
   return CG_OK;
 }
 
@@ -492,8 +563,8 @@ static void print_family(std::ostream &h, const ClusterFamily &F, int natoms, in
 
 std::string header_text(const Setup &s) {
   std::ostringstream h;
-  h << "USECOUL: false" << std::endl;
-  h << "FITCOUL: false" << std::endl;
+  h << "USECOUL: " << (s.fitcoul || s.use_partial_charges ? "true" : "false") << std::endl;
+  h << "FITCOUL: " << (s.fitcoul ? "true" : "false") << std::endl;
   h << "USE3BCH: " << (s.o3 > 0 ? "true" : "false") << std::endl;
   h << "USE4BCH: " << (s.o4 > 0 ? "true" : "false") << std::endl;
   // EXCLD lines: excluded type / pair indices, each followed by a space
@@ -516,8 +587,13 @@ std::string header_text(const Setup &s) {
   h << " " << s.o2 << " " << s.o3 << " " << s.o4 << " " << double(s.cheby_low) << " " << double(s.cheby_high) << std::endl;
   h << std::endl << "ATOM TYPES: " << s.ntypes << std::endl << std::endl;
   h << "# TYPEIDX #\t# ATM_TYP #\t# ATMCHRG #\t# ATMMASS #" << std::endl;
-  for (int i = 0; i < s.ntypes; ++i)
-    h << i << "\t\t" << s.pairs[i].a1 << "\t\t" << s.pairs[i].chg1 << "\t\t" << s.pairs[i].mass1 << std::endl;
+  // FITCOUL prints the charge sign of the input, else the fixed charge
+  for (int i = 0; i < s.ntypes; ++i) {
+    h << i << "\t\t" << s.pairs[i].a1 << "\t\t";
+    if (s.fitcoul) h << s.pairs[i].chrgsgn;
+    else h << s.pairs[i].chg1;
+    h << "\t\t" << s.pairs[i].mass1 << std::endl;
+  }
   h << std::endl << "ATOM PAIRS: " << s.pairs.size() << std::endl << std::endl;
   h << "# PAIRIDX #\t# ATM_TY1 #\t# ATM_TY1 #\t# S_MINIM #\t# S_MAXIM #\t# CHBDIST #\t# MORSE_LAMBDA #" << std::endl;
   for (const auto &p : s.pairs) {
@@ -648,7 +724,9 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   int ntypes = std::stoi(tok(hf[9])[2]);
   int pairs_line = 9 + 2 + ntypes + 2;
   int total_pairs = std::stoi(tok(hf[pairs_line])[2]);
-  int effective_pair = 0;
+  int effective_pair = 0, counted_coul = 0;
+  const bool fitcoul = tok(hf[1]).size() > 1 && tok(hf[1])[1] == "true";
+  if (fitcoul && !excl2.empty()) throw Error(CG_ERR_ARG, "ERROR: cannot use hierarchical learning with charge fitting.");
   for (int i = 0; i < total_pairs; ++i) {
     auto t = tok(hf[pairs_line + 2 + i + 1]);
     P("PAIRTYPE PARAMS: " + std::to_string(i) + " " + t[1] + " " + t[2] + "\n");
@@ -664,6 +742,13 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
         P(buf);
       }
       ++effective_pair;
+    }
+    if (fitcoul) {
+      // q_A x q_B: the pair's Ewald column, after every short-range column
+      std::snprintf(buf, sizeof buf, "q_%s x q_%s %21.13e", t[1].c_str(), t[2].c_str(),
+                    x[total_pairs * snum2 + s.n3b + s.n4b + i]);
+      P(buf);
+      ++counted_coul;
     }
     P(" ");
   }
@@ -733,7 +818,7 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   P("");
   for (auto &l : split_lines(map_text(s))) P(l);
   P("");
-  int total = effective_pair * snum2 + counted_trip + counted_quad;
+  int total = effective_pair * snum2 + counted_trip + counted_quad + counted_coul;
   const int n_offsets = ntypes - (int)excl1.size();
   if (total != nx && nx != total + n_offsets) throw Error(CG_ERR_ARG, "Error in counting parameters");
   if (nx == total + n_offsets) {

@@ -564,10 +564,26 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       for (int j = 0; j < nt; ++j)
         s->int_pair_map[s->type_idx[i] * nt + s->type_idx[j]] = s->pair_map.at(s->type_names[i] + s->type_names[j]);
     for (auto &p : s->pairs) cheby_params(p.smin, p.smax, p.lambda, s->transform, p.xmin, p.xmax, p.xdiff, p.xavg);
-    for (size_t i = 0; i < N; ++i)
-      if (L.keyword(i, "CHGCONS")) {
-        if (s->fitcoul) throw Error(CG_ERR_UNSUPPORTED, "charge constraints (FITCOUL) are not supported");
-      }
+    // CHARGE CONSTRAINTS: npairs-1 lines of npairs pair names, npairs
+    // weights and the b value (last token), src/input.C:1148-1200
+    if (s->fitcoul)
+      for (size_t i = 0; i < N; ++i)
+        if (L.width(i) >= 2 && L.v[i][0] == "CHARGE" && L.v[i][1] == "CONSTRAINTS:") {
+          const int np = (int)s->pairs.size();
+          for (int n = 0; n < np - 1; ++n) {
+            Setup::ChargeConstraint cc;
+            const size_t li = i + 1 + n;
+            for (int j = 0; j < np; ++j) {
+              auto it = s->pair_map.find(L.at(li, j));
+              if (it == s->pair_map.end()) throw Error(CG_ERR_INPUT, "unknown pair in CHARGE CONSTRAINTS: " + L.at(li, j));
+              cc.pair_idx.push_back(it->second);
+            }
+            for (int j = 0; j < np; ++j) cc.weight.push_back(to_double(L.at(li, np + j)));
+            cc.force = to_double(L.at(li, L.width(li) - 1));
+            s->charge_cons.push_back(cc);
+          }
+          break;
+        }
     // SPECIAL nB S_MINIM / S_MAXIM
     for (int fam = 3; fam <= 4; ++fam) {
       auto &F = fam == 3 ? s->trips : s->quads;
@@ -598,7 +614,15 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       s->fcut_line = joined;
     }
     parse_fcut(*s);
-    if (s->fitcoul) throw Error(CG_ERR_UNSUPPORTED, "FITCOUL true (Ewald charge columns) is not supported");
+    // the reference writes the Ewald columns on force rows only
+    // (src/A_Matrix.C:240-290) and stops on stress fits with Ewald
+    // (src/input.C:1336-1337); chimes_lsq.py stops on hierarchical charge fits
+    if (s->fitcoul && s->fit_stress != CG_STRESS_NONE)
+      throw Error(CG_ERR_INPUT, "ERROR: Inclusion of stress tensors currently not compatible with use of ZCalc_Ewald_Deriv.");
+    if (s->fitcoul && s->fit_ener)
+      throw Error(CG_ERR_UNSUPPORTED, "FITCOUL true with FITENER: the reference's energy rows carry no Ewald columns");
+    if (s->fitcoul && (!s->excl_1b.empty() || !s->excl_2b.empty()))
+      throw Error(CG_ERR_UNSUPPORTED, "ERROR: cannot use hierarchical learning with charge fitting.");
     if (s->use_partial_charges) throw Error(CG_ERR_UNSUPPORTED, "fixed non-zero charges (Ewald subtraction) are not supported");
     if (s->wrap && s->nlayers > 0) s->wrap = false;  // RUN_SANITY_LSQ
     if (s->nlayers < 1) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS 0 (no ghost atoms) is not supported");
@@ -700,8 +724,9 @@ void flatten(Setup &s) {
   s.nshort = s.n2b + s.n3b + s.n4b;
   m.n2b_cols = s.n2b;
   m.ncols_short = s.nshort;
+  m.n_charge_cols = s.fitcoul ? (int)s.pairs.size() : 0;
   m.n_offset_cols = s.fit_ener ? s.ntypes : 0;
-  m.ncols = s.nshort + m.n_offset_cols;
+  m.ncols = s.nshort + m.n_charge_cols + m.n_offset_cols;
   s.ncols = m.ncols;
   m.max_cutoff_2b = s.max_cutoff_2b;
   flatten_family(s.trips, s.n2b, s.f3, m.c3);

@@ -69,6 +69,16 @@ struct Setup {
   std::vector<int> excl_1b, excl_2b;
   int n_excl_lt3b = 0;
   int nframes = 0, nlayers = 0, skip_frames = 0;
+  // FITCOUL: one Ewald column per pair type, and the CHARGE CONSTRAINTS
+  // rows appended after the last frame (src/input.C:1148-1200): per row the
+  // pair index and weight of each listed pair, and the row's b value
+  struct ChargeConstraint {
+    std::vector<int> pair_idx;
+    std::vector<double> weight;
+    double force = 0.0;
+  };
+  std::vector<ChargeConstraint> charge_cons;
+  bool cons_written = false;  // text output: constraint rows already written
   bool fitcoul = false, fit_ener = false, use_partial_charges = false;
   int fit_stress = CG_STRESS_NONE, nstress = -1, nener = -1;
   std::string ff_type;

@@ -118,6 +118,8 @@ typedef struct cg_model {
   int nstress, nener;         /* -1: every frame                           */
   int n2b_cols;               /* TOT_SNUM                                  */
   int ncols_short;            /* TOT_SHORT_RANGE                           */
+  int n_charge_cols;          /* FITCOUL Ewald columns (0 or npairs), right
+                                 after the short-range columns             */
   int n_offset_cols;          /* energy-offset columns (0 or ntypes)       */
   int ncols;                  /* columns of A (TOT_ALL_PARAMS)             */
   double max_cutoff_2b;       /* NEIGHBORS::MAX_CUTOFF                     */
@@ -201,6 +203,16 @@ cg_status cg_assemble(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec
                       const void **A_dev, double *b_host, int64_t *rows_out);
 /* Copy the last assembled A (rows [r0, r0+nrows)) to host doubles. */
 cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host);
+/* Append nrows host rows (row-major, ncols doubles each) after the last
+ * assembled A, so that cg_gram / cg_fit / cg_matvec include them: the
+ * FITCOUL charge-constraint rows that the reference prints after the last
+ * frame (A_MAT::PRINT_CONSTRAINTS, src/A_Matrix.C:567-624). */
+cg_status cg_append_rows(cg_ctx *ctx, const double *rows, int nrows);
+/* Ewald parameters the charge columns of a frame use (optimal_ewald_params,
+ * src/Ewald.C:428-475, accuracy 1e-6): splitting parameter alpha (1/A),
+ * real-space cutoff rc (A) and integer k cutoff kmax; cell as cg_frame.cell
+ * (orthorhombic: its diagonal is read). */
+cg_status cg_ewald_params(int natoms, const double *cell, double *alpha, double *rc, int *kmax);
 
 /* Tile autotuner of the 3- and 4-body assembly: assembles the given frames
  * once per tile configuration (0: 512-thread workgroups with the widest LDS
@@ -272,6 +284,12 @@ cg_status cg_set_fit_columns(cg_ctx *ctx, const int *keep, int nkeep);
  * (EXCLUDE 1B/2B INTERACTION with HIERARC true; src/input.C:760-840): keep
  * [ncols] receives them in increasing order, nkeep their number. */
 cg_status cg_setup_fit_columns(const cg_setup *s, int *keep, int *nkeep);
+/* FITCOUL charge-constraint rows (CHARGE CONSTRAINTS:, src/input.C:1148-1200;
+ * A_MAT::PRINT_CONSTRAINTS, src/A_Matrix.C:567-624): *nrows receives their
+ * count (npairs - 1 with FITCOUL, else 0); rows (may be NULL) receives them as
+ * [nrows x ncols] doubles (zeros but the Ewald columns, which hold the
+ * weights in pair-index order) and b (may be NULL) their b values. */
+cg_status cg_setup_constraint_rows(const cg_setup *s, double *rows, double *b, int *nrows);
 /* y = A x for the last assembled A (host output). */
 cg_status cg_matvec(cg_ctx *ctx, const double *x, double *y);
 cg_status cg_write_params(const cg_setup *s, /* params.txt + force.txt */ const char *outdir, const double *x,

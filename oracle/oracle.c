@@ -17,6 +17,8 @@
  *   cutoff             FCUT::get_fcut, PROCEED src/Fcut.C:17-72,125-140
  *   assembly           Cheby::Deriv_2B/3B/4B  src/Cheby.C:709-1729
  *   row layout         A_MAT::PRINT_FRAME      src/A_Matrix.C:210-567
+ *   Ewald columns      ZCalc_Ewald_Deriv, optimal_ewald_params, generate_trig,
+ *                      add_sines              src/Ewald.C:428-825
  *   force evaluation   Cheby::Force_all/Force_3B/Force_4B src/Cheby.C:1731-2675
  *
  * Parity pins for each function are listed in tests/test_oracle.py.
@@ -28,6 +30,7 @@
 #include "../include/chimes_gpu.h"
 
 #define PI_REF 3.14159265359 /* src/functions.h:47 */
+#define PI_M 3.14159265358979323846 /* M_PI of <cmath> */
 
 typedef struct {
   double H[9], Hi[9];           /* HMAT, INVR_HMAT (row-major as reference) */
@@ -463,6 +466,152 @@ static void deriv_nb(const cg_model *m, const cg_cluster_set *cs, const sys_t *s
   }
 }
 
+/* ---------------------------------------------------------------- Ewald */
+/* optimal_ewald_params (src/Ewald.C:428-475). */
+static void ewald_opt(double accuracy, int nat, const box_t *b, double *alpha, double *rc, int *kc) {
+  double effort_ratio = 5.5, balance_factor = 1.1, accuracy_factor = 0.8;
+  double min_boxdim = b->ax;
+  if (b->by < min_boxdim) min_boxdim = b->by;
+  if (b->cz < min_boxdim) min_boxdim = b->cz;
+  double V = b->ax * b->by * b->cz;
+  double p = -log(accuracy) * accuracy_factor;
+  *alpha = sqrt(PI_M) * pow(effort_ratio * nat / (V * V), 1.0 / 6.0);
+  *rc = balance_factor * sqrt(p) / *alpha;
+  if (*rc > 0.5 * min_boxdim) {
+    *rc = 0.5 * min_boxdim;
+    *alpha = sqrt(p) / *rc;
+  }
+  double rkc = 2.0 * *alpha * sqrt(p) / balance_factor;
+  double lavg = pow(V, 1.0 / 3.0);
+  *kc = (int)ceil(rkc * lavg / (2.0 * PI_M));
+}
+
+/* generate_trig (src/Ewald.C:797-825): sin/cos of k * 2 pi r / L for
+ * k = 0..kmax by angle addition. */
+static void gen_trig(const double *rvec, const box_t *b, int kmax, double (*S)[3], double (*C)[3]) {
+  const double L[3] = {b->ax, b->by, b->cz};
+  for (int d = 0; d < 3; ++d) {
+    S[0][d] = 0.0;
+    C[0][d] = 1.0;
+    S[1][d] = sin(2.0 * PI_M * rvec[d] / L[d]);
+    C[1][d] = cos(2.0 * PI_M * rvec[d] / L[d]);
+  }
+  for (int k = 2; k <= kmax; ++k)
+    for (int d = 0; d < 3; ++d) {
+      S[k][d] = S[k - 1][d] * C[1][d] + C[k - 1][d] * S[1][d];
+      C[k][d] = C[k - 1][d] * C[1][d] - S[k - 1][d] * S[1][d];
+    }
+}
+
+/* add_sines (src/Ewald.C:752-795): sin(kx tx + ky ty + kz tz). */
+static double add_sines(int kx, int ky, int kz, double (*S)[3], double (*C)[3]) {
+  int k[3] = {kx, ky, kz};
+  double sk[3], ck[3];
+  for (int d = 0; d < 3; ++d) {
+    if (k[d] >= 0) { sk[d] = S[k[d]][d]; ck[d] = C[k[d]][d]; }
+    else { sk[d] = -S[-k[d]][d]; ck[d] = C[-k[d]][d]; }
+  }
+  double sinyz = sk[1] * ck[2] + ck[1] * sk[2];
+  double cosyz = ck[1] * ck[2] - sk[1] * sk[2];
+  return sk[0] * cosyz + ck[0] * sinyz;
+}
+
+/* ZCalc_Ewald_Deriv (src/Ewald.C:477-750): per pair type p and atom a the
+ * Ewald force per unit charge product, CH[(p * n + a) * 3 + c].  Real space
+ * over the Ewald list (half list, or the unordered one with PERM_SCALE[2] in
+ * small cells; ghost coordinates), k space over atom pairs a2 < a1 of the
+ * unwrapped coordinates with the reference's k-vector enumeration. */
+static int ewald_deriv(const cg_model *m, const cg_frame *f, const sys_t *s, double *CH) {
+  const box_t *b = &s->box;
+  if (!b->ortho) return -2;
+  const int n = s->n, np = m->npairs;
+  const double PI = 3.14159265359; /* src/Ewald.C:489 */
+  double alpha, r_cut;
+  int kmax;
+  ewald_opt(1.0e-6, n, b, &alpha, &r_cut, &kmax);
+  const double alphasq = alpha * alpha, Volume = b->vol;
+  memset(CH, 0, sizeof(double) * (size_t)np * n * 3);
+  /* k vectors (src/Ewald.C:554-598) */
+  int cap = (kmax + 1) * (kmax + 1) * (kmax + 1), totk = 0;
+  int *KV = (int *)malloc(sizeof(int) * 3 * cap);
+  double *Kfac = (double *)malloc(sizeof(double) * cap);
+  for (int kx = 0; kx <= kmax; kx++)
+    for (int ky = 0; ky <= kmax; ky++)
+      for (int kz = 0; kz <= kmax; kz++) {
+        int ksq = kx * kx + ky * ky + kz * kz;
+        if (ksq != 0 && ksq < 50) {
+          double RK[3] = {(2.0 * PI / b->ax) * kx, (2.0 * PI / b->by) * ky, (2.0 * PI / b->cz) * kz};
+          double rksq = RK[0] * RK[0] + RK[1] * RK[1] + RK[2] * RK[2];
+          KV[3 * totk] = kx; KV[3 * totk + 1] = ky; KV[3 * totk + 2] = kz;
+          Kfac[totk] = exp(-rksq / (4.0 * alphasq)) / rksq;
+          totk++;
+        }
+      }
+  const double perm_scale = s->ps[2];
+  const double pad = 0.3; /* RCUT_PADDING */
+  /* real space (src/Ewald.C:608-651) */
+  for (int a1 = 0; a1 < n; a1++)
+    for (int a2 = 0; a2 < s->nall; a2++) {
+      if (a2 == a1) continue;
+      if (!s->small && a1 > s->parent[a2]) continue;
+      double rvec[3];
+      double rlen = dist(s, a1, a2, rvec);
+      if (!(rlen < r_cut + pad)) continue; /* LIST_EWALD */
+      int fa2 = s->parent[a2];
+      int ip = m->int_pair_map[s->type[a1] * m->ntypes + s->type[fa2]];
+      if (rlen < r_cut) {
+        double tempd = ((-2.0 / sqrt(PI)) * exp(-1 * alpha * alpha * rlen * rlen) * rlen * alpha - erfc(alpha * rlen)) /
+                       (rlen * rlen);
+        tempd *= -1.0;
+        tempd *= 2.0;
+        tempd *= 0.5 / rlen;
+        tempd *= perm_scale;
+        for (int c = 0; c < 3; ++c) {
+          CH[((size_t)ip * n + a1) * 3 + c] += tempd * rvec[c];
+          CH[((size_t)ip * n + fa2) * 3 + c] -= tempd * rvec[c];
+        }
+      }
+    }
+  /* k space (src/Ewald.C:653-738) */
+  double (*S)[3] = (double (*)[3])malloc(sizeof(double) * 3 * (kmax + 1));
+  double (*C)[3] = (double (*)[3])malloc(sizeof(double) * 3 * (kmax + 1));
+  const double pre = 0.5 * (4 * PI / Volume);
+  for (int a1 = 0; a1 < n; a1++)
+    for (int a2 = 0; a2 < a1; a2++) {
+      int ip = m->int_pair_map[f->type[a1] * m->ntypes + f->type[a2]];
+      double rvec[3];
+      for (int c = 0; c < 3; ++c) rvec[c] = f->xyz[3 * a2 + c] - f->xyz[3 * a1 + c];
+      gen_trig(rvec, b, kmax, S, C);
+      for (int ik = 0; ik < totk; ik++) {
+        int kx = KV[3 * ik], ky = KV[3 * ik + 1], kz = KV[3 * ik + 2];
+        double RK[3] = {(2.0 * PI / b->ax) * kx, (2.0 * PI / b->by) * ky, (2.0 * PI / b->cz) * kz};
+        double t1 = add_sines(kx, ky, kz, S, C) * 2.0 * Kfac[ik] * pre, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+        if (ky > 0) {
+          t2 = add_sines(kx, -ky, kz, S, C) * 2.0 * Kfac[ik] * pre;
+          if (kz > 0) t3 = add_sines(kx, -ky, -kz, S, C) * 2.0 * Kfac[ik] * pre;
+        }
+        if (kz > 0) t4 = add_sines(kx, ky, -kz, S, C) * 2.0 * Kfac[ik] * pre;
+        if (kx > 0) { t1 *= 2.0; t2 *= 2.0; t3 *= 2.0; t4 *= 2.0; }
+        double v[3] = {(t1 + t2 + t3 + t4) * RK[0], (t1 - t2 - t3 + t4) * RK[1], (t1 + t2 - t3 - t4) * RK[2]};
+        for (int c = 0; c < 3; ++c) {
+          CH[((size_t)ip * n + a1) * 3 + c] += v[c];
+          CH[((size_t)ip * n + a2) * 3 + c] -= v[c];
+        }
+      }
+    }
+  for (size_t i = 0; i < (size_t)np * n * 3; ++i) CH[i] *= -1.0;
+  free(S); free(C); free(KV); free(Kfac);
+  return 0;
+}
+
+/* Ewald parameters of a frame (for tests). */
+int orc_ewald_params(int natoms, const double *cell, double *alpha, double *rc, int *kmax) {
+  box_t b;
+  if (box_setup(cell, &b)) return -1;
+  ewald_opt(1.0e-6, natoms, &b, alpha, rc, kmax);
+  return 0;
+}
+
 /* Rows of one frame in reference order; returns number of rows written.
  * out: [rows x ncols] row-major.  small_state is the sticky SMALL flag. */
 int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, double *out) {
@@ -490,14 +639,26 @@ int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, d
   }
   int counts[CG_MAX_ATOM_TYPES] = {0};
   for (int a = 0; a < s.n; ++a) counts[f->type[a]]++;
+  /* FITCOUL: Ewald columns after the short-range ones, force rows only */
+  const int nq = m->n_charge_cols;
+  double *CH = NULL;
+  if (nq > 0) {
+    CH = (double *)malloc(sizeof(double) * (size_t)m->npairs * s.n * 3 + 8);
+    if (ewald_deriv(m, f, &s, CH)) {
+      free(CH); free(A.F); free(A.S); free(A.E); sys_free(&s);
+      return -2;
+    }
+  }
   int row = 0;
   for (int a = 0; a < s.n; ++a)
     for (int c = 0; c < 3; ++c) {
       double *o = out + (size_t)row * nc;
       for (int p = 0; p < P; ++p) o[p] = A.F[((size_t)a * P + p) * 3 + c];
-      for (int t = 0; t < m->n_offset_cols; ++t) o[P + t] = 0.0;
+      for (int q = 0; q < nq; ++q) o[P + q] = CH[((size_t)q * s.n + a) * 3 + c];
+      for (int t = 0; t < m->n_offset_cols; ++t) o[P + nq + t] = 0.0;
       ++row;
     }
+  free(CH);
   if (fit_stress) {
     static const int diag[3] = {0, 1, 2};
     static const int all[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
@@ -506,7 +667,8 @@ int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, d
     for (int k = 0; k < nr; ++k) {
       double *o = out + (size_t)row * nc;
       for (int p = 0; p < P; ++p) o[p] = A.S[p * 6 + sel[k]];
-      for (int t = 0; t < m->n_offset_cols; ++t) o[P + t] = 0.0;
+      for (int q = 0; q < nq; ++q) o[P + q] = 0.0;
+      for (int t = 0; t < m->n_offset_cols; ++t) o[P + nq + t] = 0.0;
       ++row;
     }
   }
@@ -514,7 +676,8 @@ int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, d
     for (int k = 0; k < 3; ++k) {
       double *o = out + (size_t)row * nc;
       for (int p = 0; p < P; ++p) o[p] = A.E[p];
-      for (int t = 0; t < m->n_offset_cols; ++t) o[P + t] = counts[t];
+      for (int q = 0; q < nq; ++q) o[P + q] = 0.0;
+      for (int t = 0; t < m->n_offset_cols; ++t) o[P + nq + t] = counts[t];
       ++row;
     }
   }

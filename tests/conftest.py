@@ -138,3 +138,37 @@ def md_case(ch, name, tmp_path):
     ref = np.array([float(v) for v in open(os.path.join(d, "forceout.txt")).read().split()])
     assert ref.size == 3 * n
     return s, f, ref
+
+
+WATER = os.path.join(GOLDEN, "lsq", "water")
+
+
+def water_setup(ch, tmp_path, name, nframes=None):
+    """A FITCOUL water case of the reference test suite in tmp_path: its
+    fm_setup.in (NFRAMES optionally lowered) and the shared trajectory."""
+    import gzip
+    import re
+    text = open(os.path.join(WATER, name, "fm_setup.in")).read()
+    if nframes is not None:
+        text = re.sub(r"(# NFRAMES #[^\n]*\n)[^\n]*\n", lambda m: m.group(1) + "\t%d\n" % nframes, text, count=1)
+    open(os.path.join(str(tmp_path), "fm_setup.in"), "w").write(text)
+    with gzip.open(os.path.join(WATER, "input.xyzf.gz"), "rb") as g:
+        open(os.path.join(str(tmp_path), "input.xyzf"), "wb").write(g.read())
+    return ch.Setup(os.path.join(str(tmp_path), "fm_setup.in"))
+
+
+def fitcoul_text(text, signs, constraints):
+    """A generated fm_setup.in turned into a FITCOUL one: charge signs in the
+    ATMCHRG column, no stress / energy rows, CHARGE CONSTRAINTS lines."""
+    import re
+    text = text.replace("# FITCOUL #\n\tfalse", "# FITCOUL #\n\ttrue")
+    text = text.replace("# FITSTRS #\n\ttrue", "# FITSTRS #\n\tfalse").replace("# FITENER #\n\ttrue", "# FITENER #\n\tfalse")
+    lines = text.split("\n")
+    k = lines.index("# TYPEIDX #\t# ATM_TYP #\t# ATMCHRG #\t# ATMMASS #")
+    for i, sg in enumerate(signs):
+        parts = lines[k + 1 + i].split("\t\t")
+        parts[2] = sg
+        lines[k + 1 + i] = "\t\t".join(parts)
+    e = lines.index("# FCUTTYP #")
+    lines[e:e] = ["CHARGE CONSTRAINTS:"] + constraints + [""]
+    return "\n".join(lines)

@@ -19,6 +19,13 @@ expected outputs), trimmed to the frames the tests need:
                            and processed frames (frames.NNNN.log), the
                            concatenated b, b-labeled and natoms files, and
                            params.header / ff_groups.map (gzip).
+  lsq/water/               FITCOUL cases (O/H, Ewald charge columns and charge-
+                           constraint rows): the shared trajectory (gzip), per
+                           case its fm_setup.in; h2o-4bcheby and -small the 154
+                           golden A rows that survive in A.txt.ab (the last 152
+                           force rows and the 2 constraint rows), h2o-4bcheby
+                           its b and force.txt; skip_frames (36 ranks, round
+                           robin) its b.txt (gzip) and dim.txt.
   md/<test>/               force-evaluation cases of test_suite-md without
                            Coulomb terms: params.txt (gzip), the first frame
                            of the run's coordinate file and the reference's
@@ -42,7 +49,11 @@ MD_TESTS = ["verify-lsq-forces-special3b", "verify-lsq-forces-special4b", "verif
 
 HEADER_TESTS = ["nonorth2", "test_4atoms", "test_4atoms.2", "fitstrs", "fitstrs3b", "h2o-invr", "special3b",
                 "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2", "stress-and-ener-2b3",
-                "stress-and-ener-4b", "stress-and-ener-4b2", "two-traj-files", "one-file-three-trajs"]
+                "stress-and-ener-4b", "stress-and-ener-4b2", "two-traj-files", "one-file-three-trajs",
+                "h2o-2bcheby", "h2o-3bcheby", "h2o-3bcheby2", "h2o-3bcheby3", "h2o-4bcheby", "h2o-4bcheby-small"]
+
+# FITCOUL (Ewald charge columns) cases; they share one 99-frame water trajectory
+WATER_TESTS = ["h2o-2bcheby", "h2o-4bcheby", "h2o-4bcheby-small", "skip_frames"]
 
 
 def sha(path):
@@ -158,6 +169,27 @@ def main():
         log = open(os.path.join(co, "frames.%04d.log" % r)).read()
         meta["frames"][str(r)] = [int(v) for v in re.findall(r"Processing frame (\d+)", log)]
     json.dump(meta, open(os.path.join(out, "split.json"), "w"), indent=1, sort_keys=True)
+    # FITCOUL water cases
+    wat = os.path.join(HERE, "lsq", "water")
+    os.makedirs(wat, exist_ok=True)
+    gz(os.path.join(LSQ, "h2o-4bcheby", "input.xyzf"), os.path.join(wat, "input.xyzf.gz"))
+    for t in WATER_TESTS:
+        src = os.path.join(LSQ, t)
+        assert open(os.path.join(src, "input.xyzf"), "rb").read() == open(os.path.join(LSQ, "h2o-4bcheby", "input.xyzf"), "rb").read()
+        out = os.path.join(wat, t)
+        os.makedirs(out, exist_ok=True)
+        open(os.path.join(out, "fm_setup.in"), "w").write(open(setup_of(src)).read())
+        co = os.path.join(src, "correct_output")
+        if os.path.exists(os.path.join(co, "A.txt.ab")):
+            rows = tail_rows(os.path.join(co, "A.txt.ab"))
+            nb = sum(1 for l in open(os.path.join(co, "b.txt")) if l.strip())
+            extra = {}
+            if t == "h2o-4bcheby":
+                extra = {"b": np.loadtxt(os.path.join(co, "b.txt")), "force": np.loadtxt(os.path.join(co, "force.txt"))}
+            np.savez_compressed(os.path.join(out, "golden.npz"), A=rows, row0=nb - rows.shape[0], **extra)
+        if t == "skip_frames":
+            gz(os.path.join(co, "b.txt"), os.path.join(out, "b.txt.gz"))
+            open(os.path.join(out, "dim.txt"), "w").write(open(os.path.join(co, "dim.txt")).read())
     # force-evaluation cases: params.txt + first coordinate frame + forceout.txt
     md = os.path.join(REF, "test_suite-md")
     for t in MD_TESTS:

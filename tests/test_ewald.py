@@ -1,0 +1,187 @@
+"""FITCOUL Ewald charge columns (ZCalc_Ewald_Deriv, src/Ewald.C:477-750) and
+charge-constraint rows (src/A_Matrix.C:567-624), CPU side: the oracle's
+restatement against the reference's golden A rows of h2o-4bcheby (half
+neighbour lists) and h2o-4bcheby-small (unordered lists, PERM_SCALE), against
+minus the gradient of the Ewald pair energy by finite differences, and the
+text output (b.txt, dim.txt, constraint rows) against the reference's
+36-rank skip_frames run."""
+import ctypes
+import gzip
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import WATER, colscaled_error, fitcoul_text, synth_setup, water_setup
+
+
+def _orc_params(oracle, natoms, cell):
+    a, r, k = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    assert oracle.L.orc_ewald_params(natoms, cell.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     ctypes.byref(a), ctypes.byref(r), ctypes.byref(k)) == 0
+    return a.value, r.value, k.value
+
+
+@pytest.mark.parametrize("name", ["h2o-4bcheby", "h2o-4bcheby-small"])
+def test_oracle_matches_golden_rows(ch, oracle, tmp_path, name):
+    """Frame 32 of the reference run: its last 152 force rows (416 Chebyshev
+    columns + 3 Ewald columns) and the two constraint rows after it."""
+    s = water_setup(ch, tmp_path, name)
+    g = np.load(os.path.join(WATER, name, "golden.npz"))
+    G = g["A"]
+    assert G.shape == (154, s.model.ncols) and s.model.n_charge_cols == 3
+    fr = s.frames(str(tmp_path))
+    A = oracle.assemble(s, [fr[-1]])
+    assert colscaled_error(A[-152:], G[:152]) < 1e-10
+    # columns of pairs without the atom's type are -0.0, as printed
+    P = s.model.ncols_short
+    assert np.array_equal(np.signbit(A[-152:, P:]), np.signbit(G[:152, P:]))
+    rows, b = s.constraint_rows()
+    assert np.array_equal(rows, G[152:]) and np.array_equal(b, [0.0, 0.0])
+
+
+def test_ewald_parameters(ch, oracle):
+    """optimal_ewald_params: the library and the oracle agree; in the golden
+    water box (96 atoms, 8.609 A) the cutoff is capped at half the cell and
+    alpha follows from it; in a large cell the uncapped Fincham values hold."""
+    L = 8.6091309
+    a, rc, k = ch.ewald_params(96, np.diag([L, L, L]))
+    assert (a, rc, k) == _orc_params(oracle, 96, np.diag([L, L, L]))
+    p = -math.log(1e-6) * 0.8
+    assert rc == pytest.approx(L / 2, rel=1e-15) and a == pytest.approx(math.sqrt(p) / rc, rel=1e-14)
+    assert k == math.ceil(2 * a * math.sqrt(p) / 1.1 * L / (2 * math.pi))
+    cell = np.diag([40.0, 42.0, 44.0])
+    a, rc, k = ch.ewald_params(5000, cell)
+    assert (a, rc, k) == _orc_params(oracle, 5000, cell)
+    V = 40.0 * 42.0 * 44.0
+    a0 = math.sqrt(math.pi) * (5.5 * 5000 / V ** 2) ** (1 / 6)
+    assert a == pytest.approx(a0, rel=1e-14) and rc == pytest.approx(1.1 * math.sqrt(p) / a0, rel=1e-14)
+    assert rc < 20.0
+
+
+def _ewald_energy(pos, types, L, alpha, rc, kmax, pair_of):
+    """Ewald pair energy per unit charge product, summed per pair type:
+    erfc(a r)/r over minimum-image pairs within rc, plus
+    (4 pi / V) sum_k exp(-k^2/4a^2)/k^2 cos(k.r_ij) over the k set of the
+    reference (|k_x|,|k_y|,|k_z| <= kmax, 0 < |n|^2 < 50)."""
+    n = len(types)
+    E = np.zeros(max(pair_of.values()) + 1)
+    V = L[0] * L[1] * L[2]
+    ks = []
+    for kx in range(-kmax, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            for kz in range(-kmax, kmax + 1):
+                q = kx * kx + ky * ky + kz * kz
+                if 0 < q < 50:
+                    ks.append((kx, ky, kz))
+    K = np.array(ks, dtype=float) * (2 * math.pi / np.asarray(L))
+    ksq = (K ** 2).sum(1)
+    kf = np.exp(-ksq / (4 * alpha ** 2)) / ksq
+    for i in range(n):
+        for j in range(i):
+            d = pos[j] - pos[i]
+            dm = d - L * np.rint(d / L)
+            r = np.sqrt((dm ** 2).sum())
+            p = pair_of[(types[i], types[j])]
+            if r < rc:
+                E[p] += math.erfc(alpha * r) / r
+            E[p] += (4 * math.pi / V) * (kf * np.cos(K @ d)).sum()
+    return E
+
+
+def test_oracle_columns_are_minus_energy_gradient(ch, synth, oracle, tmp_path):
+    """Each Ewald column of a force row is -dE_p/dr (central differences) of the
+    pair type's Ewald energy: pins the signs, the half-space k factor 2, the
+    real-space prefactor and the pair-type mapping (3 types)."""
+    rng = np.random.default_rng(7)
+    L = np.array([7.3, 8.1, 8.9])
+    n = 14
+    pos = rng.uniform(0, 1, (n, 3)) * L
+    types = np.array([i % 3 for i in range(n)], dtype=np.int32)
+    text = synth.fm_setup(["O", "H", "N"], [16.0, 1.0, 14.0], 4, 0, 0, 0.5, 3.0, 1.4)
+    text = fitcoul_text(text, ["-", "+", "+"], ["OO HH NN OH ON HN 1 1 1 1 1 1 0.0"] * 5)
+    fr = dict(natoms=n, xyz=pos.reshape(-1), types=types, cell=np.diag(L).reshape(-1), forces=np.zeros(3 * n),
+              stress=np.zeros(6), energy=0.0, index=0)
+    s, frs = synth_setup(ch, synth, tmp_path, (text, [fr], ["O", "H", "N"]))
+    m = s.model
+    assert m.n_charge_cols == 6 and m.ncols == m.ncols_short + 6
+    A = oracle.assemble(s, frs)
+    alpha, rc, kmax = ch.ewald_params(n, np.diag(L))
+    ipm = [m.int_pair_map[i] for i in range(9)]
+    pair_of = {(a, b): ipm[a * 3 + b] for a in range(3) for b in range(3)}
+    h = 1e-5
+    P = m.ncols_short
+    for i in (0, 1, 5):
+        for c in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, c] += h
+            pm[i, c] -= h
+            g = -(_ewald_energy(pp, types, L, alpha, rc, kmax, pair_of) -
+                  _ewald_energy(pm, types, L, alpha, rc, kmax, pair_of)) / (2 * h)
+            got = A[3 * i + c, P:P + 6]
+            assert np.allclose(got, g, rtol=1e-6, atol=1e-7 * np.abs(A[:, P:]).max()), (i, c, got, g)
+    # Newton's third law per column
+    for c in range(3):
+        assert np.all(np.abs(A[c::3, P:].sum(axis=0)) < 1e-11 * np.abs(A[:, P:]).sum(axis=0))
+
+
+def test_constraint_rows_and_b_of_the_36_rank_run(ch, tmp_path):
+    """skip_frames (FITCOUL, SKPFRMS 1, 36 ranks, 33 frames): b.txt byte for byte
+    (frames in rank order, then the two constraint values) and dim.txt (43
+    columns, 9506 rows with the constraint rows)."""
+    s = water_setup(ch, tmp_path, "skip_frames")
+    frames = s.frames(str(tmp_path))
+    order, rank = s.frame_order(36)
+    out = tmp_path / "out"
+    out.mkdir()
+    s.write_text(str(out), [frames[i] for i in order], rank)
+    with gzip.open(os.path.join(WATER, "skip_frames", "b.txt.gz"), "rb") as g:
+        assert open(out / "b.txt", "rb").read() == g.read()
+    assert open(out / "dim.txt").read() == open(os.path.join(WATER, "skip_frames", "dim.txt")).read()
+
+
+def test_constraint_rows_in_A_text(ch, oracle, tmp_path):
+    """The A text of a FITCOUL run: force rows carry the Ewald columns, the
+    constraint rows follow the last frame as the reference prints them
+    ("0.0 " per short-range column, then the weights in pair-index order)."""
+    s = water_setup(ch, tmp_path, "h2o-4bcheby", nframes=2)
+    frames = s.frames(str(tmp_path))
+    A = oracle.assemble(s, frames)
+    rows = [A[:288], A[288:]]
+    out = tmp_path / "out"
+    out.mkdir()
+    s.write_text(str(out), frames, rows=rows)
+    lines = open(out / "A.txt").read().split("\n")
+    assert lines[-1] == "" and len(lines) == 576 + 2 + 1
+    P = s.model.ncols_short
+    assert lines[-3] == "0.0 " * P + "1.0000000000000000e+04 4.0000000000000000e+04 4.0000000000000000e+04 "
+    assert lines[-2] == "0.0 " * P + "1.0000000000000000e+04 -4.0000000000000000e+04 0.0000000000000000e+00 "
+    first = lines[0].split("   ")
+    assert len(first) == s.model.ncols + 1 and first[-1] == ""
+    got = np.array([[float(v) for v in l.split()] for l in lines[:576]])
+    assert np.array_equal(got, A)
+    assert open(out / "dim.txt").read() == "%d %d\n" % (s.model.ncols, 578)
+    b = open(out / "b.txt").read().split("\n")
+    assert b[-3:] == ["0.0000000000000000e+00", "0.0000000000000000e+00", ""]
+
+
+def test_fitcoul_refusals(ch, synth, tmp_path):
+    """Stress fits with Ewald columns stop as in the reference (RUN_SANITY_LSQ);
+    energy rows (which the reference prints without the Ewald columns) and
+    hierarchical charge fits are refused."""
+    base = synth.fm_setup(["O", "H"], [16.0, 1.0], 4, 0, 0, 0.5, 3.0, 1.4)
+    good = fitcoul_text(base, ["-", "+"], ["OO HH OH 1 1 1 0.0", "OO OH HH 1 2 3 0.5"])
+    for bad in [good.replace("# FITSTRS #\n\tfalse", "# FITSTRS #\n\ttrue"),
+                good.replace("# FITENER #\n\tfalse", "# FITENER #\n\ttrue"),
+                good.replace("# ENDFILE #", "# HIERARC #\n\ttrue\nEXCLUDE 2B INTERACTION: 1\nO H\n\n# ENDFILE #")]:
+        open(tmp_path / "fm_setup.in", "w").write(bad)
+        with pytest.raises(RuntimeError):
+            ch.Setup(str(tmp_path / "fm_setup.in"))
+    open(tmp_path / "fm_setup.in", "w").write(good)
+    s = ch.Setup(str(tmp_path / "fm_setup.in"))
+    rows, b = s.constraint_rows()
+    # npairs - 1 rows; weights land in pair-index order (OO 0, HH 1, OH 2)
+    assert rows.shape == (2, s.model.ncols) and list(rows[0, -3:]) == [1.0, 1.0, 1.0]
+    assert list(rows[1, -3:]) == [1.0, 3.0, 2.0] and list(b) == [0.0, 0.5]

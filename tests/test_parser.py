@@ -95,8 +95,13 @@ def test_error_1000_line_cap(ch, tmp_path):
         ch.Setup(_write(tmp_path, text))
 
 
-def test_error_fitcoul_unsupported(ch, synth, tmp_path):
+def test_error_fitcoul_with_stress_and_fixed_charges(ch, synth, tmp_path):
+    """FITCOUL with stress rows stops as RUN_SANITY_LSQ does (src/input.C:1336-1337);
+    fixed non-zero charges (Ewald forces subtracted from b) are not supported."""
     text = synth.fm_setup(["C"], [12.0], 8, 0, 0, 1.0, 4.0, 1.4).replace("# FITCOUL #\n\tfalse", "# FITCOUL #\n\ttrue")
+    with pytest.raises(RuntimeError, match="not compatible with use of ZCalc_Ewald_Deriv"):
+        ch.Setup(_write(tmp_path, text))
+    text = synth.fm_setup(["C"], [12.0], 8, 0, 0, 1.0, 4.0, 1.4).replace("\t\t0.0\t\t12", "\t\t0.41\t\t12")
     with pytest.raises(RuntimeError, match="UNSUPPORTED"):
         ch.Setup(_write(tmp_path, text))
 
