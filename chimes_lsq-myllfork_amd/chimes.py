@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libchimes_gpu.so")
 
 CG_FP64, CG_FP32 = 0, 1
 CG_GRAM_FP64, CG_GRAM_BF16X3 = 0, 1
+CG_FIT_QR = 2
 STATUS = {0: "CG_OK", 1: "CG_ERR_ARG", 2: "CG_ERR_INPUT", 3: "CG_ERR_UNSUPPORTED",
           4: "CG_ERR_DEVICE", 5: "CG_ERR_MEMORY", 6: "CG_ERR_NUMERIC", 7: "CG_ERR_EOF"}
 

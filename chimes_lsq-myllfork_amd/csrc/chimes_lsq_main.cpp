@@ -2,7 +2,7 @@
 // with the design matrix assembled on the GPU, optionally followed by the fit
 // that `chimes_lsq.py` performs (src/chimes_lsq.py:169-672).
 //
-//   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--gram fp64|bf16x3]
+//   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--gram qr|fp64|bf16x3]
 //                  [--refine K] [--batch FRAMES] [--ranks N] [--no-A] [--no-text]
 //
 // Files written in the working directory: A.txt b.txt b-labeled.txt natoms.txt
@@ -35,14 +35,14 @@ int die(const char *what, cg_status st, const void *ctx = nullptr) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--gram fp64|bf16x3] [--refine K] "
+    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--gram qr|fp64|bf16x3] [--refine K] "
                          "[--batch FRAMES] [--ranks N] [--no-A] [--no-text]\n");
     return 1;
   }
   const char *infile = argv[1];
   bool do_fit = false, text = true, write_A = true;
   double eps = 1.0e-5;
-  int prec = CG_FP64, refine = 3, batch = 0, gram = CG_GRAM_FP64, nranks = 1;
+  int prec = CG_FP64, refine = 3, batch = 0, gram = CG_FIT_QR, nranks = 1;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "--fit") do_fit = true;
@@ -57,7 +57,7 @@ int main(int argc, char **argv) {
       prec = p == "fp32" ? CG_FP32 : CG_FP64;
     } else if (a == "--gram" && i + 1 < argc) {
       std::string p = argv[++i];
-      gram = p == "bf16x3" ? CG_GRAM_BF16X3 : CG_GRAM_FP64;
+      gram = p == "bf16x3" ? CG_GRAM_BF16X3 : p == "fp64" ? CG_GRAM_FP64 : CG_FIT_QR;
     } else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 1;

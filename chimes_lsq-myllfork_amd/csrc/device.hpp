@@ -123,6 +123,7 @@ struct Ctx {
   DBuf b_A, b_scan;                       // design matrix; hipcub scratch
   DBuf b_x;                               // fit workspace
   DBuf b_fitkeep;                         // kept columns of a fit (device copy)
+  DBuf b_qr;                              // QR fit: FP64 copy of A, factors, SVD
   std::vector<int> fit_keep;              // columns a fit keeps (empty: all)
   DBuf b_force;                           // force evaluation: F, energy/virial, call totals
   std::vector<double> h_out;              // host copy of the force results and totals

@@ -14,6 +14,9 @@
 //                    v_mfma_f32_16x16x32_bf16 with FP32 accumulation per
 //                    K-slab and FP64 accumulation across slabs.
 // Residual and A^T r are always FP64.
+//
+//   CG_FIT_QR      : no Gram matrix; LQ factorization of A and SVD of its
+//                    triangular factor (fit_qr below).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -247,6 +250,23 @@ __global__ void k_scatter_vals(const double *v, const int *keep, int nk, double 
   if (i < nk) out[keep[i]] = v[i];
 }
 
+// FP64 copy of the fitted columns of A: row-major m x ne, i.e. the
+// column-major ne x m matrix A^T that the LQ factorization takes.
+template <class T>
+__global__ void k_copy_cols(const T *__restrict__ A, long long ld, long long m, int ne, const int *__restrict__ keep,
+                            double *__restrict__ W) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= m * ne) return;
+  const long long r = idx / ne;
+  const int j = (int)(idx - r * ne);
+  W[idx] = (double)A[r * ld + (keep ? keep[j] : j)];
+}
+// Lower triangle of the leading ne x ne block of a column-major matrix.
+__global__ void k_lower(const double *W, int ne, double *L) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < ne) L[(size_t)j * ne + i] = i >= j ? W[(size_t)j * ne + i] : 0.0;
+}
+
 __global__ void k_axpy(double *x, const double *d, int n, double alpha) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += alpha * d[i];
@@ -317,12 +337,123 @@ static void gram_plan(long long m, int n, long long &S, long long &kchunk) {
 }
 
 static void check_mode(int mode) {
-  if (mode != CG_GRAM_FP64 && mode != CG_GRAM_BF16X3) throw Error(CG_ERR_ARG, "unknown Gram mode");
+  if (mode != CG_GRAM_FP64 && mode != CG_GRAM_BF16X3 && mode != CG_FIT_QR) throw Error(CG_ERR_ARG, "unknown fit mode");
+}
+
+// CG_FIT_QR: the truncated SVD solution of chimes_lsq.py from an SVD of A
+// itself, to working precision (no Gram matrix, whose eigenvectors resolve
+// singular directions near the cut eps * s_max only to about
+// 1e-16 s_max^2 / (s_i^2 - s_j^2)).  The row-major A is the column-major
+// ne x m matrix A^T = L Q (LQ factorization, rocsolver_dgelqf), so
+// A = Q^T L^T; with the SVD L = U S V^T (rocsolver_dgesvd of the ne x ne
+// factor) A = (Q^T V) S U^T and
+//   x = sum_{s_i > eps s_max} U[:, i] (V^T (Q b))_i / s_i,
+// Q b applied from the stored reflectors (rocsolver_dormlq).
+static void fit_qr(Ctx &c, const double *b_host, double eps, double *x_host, cg_fit_result *res) {
+  const long long m = c.A_rows;
+  const int n = c.A_ld;
+  FitState &fs = fit_state(c);
+  const bool masked = !c.fit_keep.empty() && (int)c.fit_keep.size() < n;
+  const int ne = masked ? (int)c.fit_keep.size() : n;
+  const size_t nn = (size_t)ne * ne;
+  double *W = c.b_qr.as<double>((size_t)m * ne + 3 * (size_t)m + 3 * nn + 6 * (size_t)ne + (size_t)n + 64);
+  if (!W) throw Error(CG_ERR_MEMORY, "QR fit buffers");
+  double *d_b = W + (size_t)m * ne, *d_q = d_b + m, *d_r = d_q + m;
+  double *L = d_r + m, *U = L + nn, *VT = U + nn;
+  double *S = VT + nn, *E = S + ne, *tau = E + ne, *t = tau + ne, *winv = t + ne, *xk = winv + ne, *d_x = xk + ne;
+  rocblas_int *d_info = reinterpret_cast<rocblas_int *>(d_x + n);
+  double *d_z = d_x + n + 8;
+  int *d_keep = nullptr;
+  if (masked) {
+    d_keep = c.b_fitkeep.as<int>(ne);
+    if (!d_keep) throw Error(CG_ERR_MEMORY, "fit column list");
+    CG_HIP(hipMemcpyAsync(d_keep, c.fit_keep.data(), ne * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  }
+  hipEvent_t e0, e1, e2;
+  CG_HIP(hipEventCreate(&e0)); CG_HIP(hipEventCreate(&e1)); CG_HIP(hipEventCreate(&e2));
+  CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CG_HIP(hipMemcpyAsync(d_q, d_b, m * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  CG_HIP(hipEventRecord(e0, c.stream));
+  const long long tot = m * ne;
+  const unsigned grid = (unsigned)((tot + 255) / 256);
+  if (c.A_prec == CG_FP64)
+    hipLaunchKernelGGL(k_copy_cols<double>, dim3(grid), dim3(256), 0, c.stream, (const double *)c.A, (long long)n, m, ne,
+                       (const int *)d_keep, W);
+  else
+    hipLaunchKernelGGL(k_copy_cols<float>, dim3(grid), dim3(256), 0, c.stream, (const float *)c.A, (long long)n, m, ne,
+                       (const int *)d_keep, W);
+  CG_HIP(hipGetLastError());
+  if (rocsolver_dgelqf(fs.h, ne, (rocblas_int)m, W, ne, tau) != rocblas_status_success)
+    throw Error(CG_ERR_NUMERIC, "rocsolver_dgelqf failed");
+  if (rocsolver_dormlq(fs.h, rocblas_side_left, rocblas_operation_none, (rocblas_int)m, 1, ne, W, ne, tau, d_q,
+                       (rocblas_int)m) != rocblas_status_success)
+    throw Error(CG_ERR_NUMERIC, "rocsolver_dormlq failed");
+  hipLaunchKernelGGL(k_lower, dim3((ne + 255) / 256, ne), dim3(256), 0, c.stream, W, ne, L);
+  CG_HIP(hipEventRecord(e1, c.stream));
+  if (rocsolver_dgesvd(fs.h, rocblas_svect_all, rocblas_svect_all, ne, ne, L, ne, S, U, ne, VT, ne, E,
+                       rocblas_outofplace, d_info) != rocblas_status_success)
+    throw Error(CG_ERR_NUMERIC, "rocsolver_dgesvd failed");
+  std::vector<double> sv(ne);
+  int info = 0;
+  CG_HIP(hipMemcpyAsync(sv.data(), S, ne * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  if (info != 0) throw Error(CG_ERR_NUMERIC, "SVD did not converge (info=" + std::to_string(info) + ")");
+  double smax = 0;
+  for (double v : sv) smax = std::max(smax, std::fabs(v));
+  const double thr = eps * smax;
+  std::vector<double> wi(ne, 0.0);
+  int nvars = 0;
+  for (int i = 0; i < ne; ++i)
+    if (std::fabs(sv[i]) > thr) { wi[i] = 1.0 / sv[i]; ++nvars; }
+  CG_HIP(hipMemcpyAsync(winv, wi.data(), ne * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  // t = S+ V^T (Q b)[0:ne];  x = U t
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemv(fs.h, rocblas_operation_none, ne, ne, &one, VT, ne, d_q, 1, &zero, t, 1) != rocblas_status_success)
+    throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
+  hipLaunchKernelGGL(k_scale, dim3((ne + 255) / 256), dim3(256), 0, c.stream, t, winv, ne);
+  if (rocblas_dgemv(fs.h, rocblas_operation_none, ne, ne, &one, U, ne, t, 1, &zero, xk, 1) != rocblas_status_success)
+    throw Error(CG_ERR_DEVICE, "rocblas_dgemv");
+  if (masked) {
+    hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_x, n);
+    hipLaunchKernelGGL(k_scatter_vals, dim3((ne + 255) / 256), dim3(256), 0, c.stream, xk, (const int *)d_keep, ne, d_x);
+  } else {
+    CG_HIP(hipMemcpyAsync(d_x, xk, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  }
+  CG_HIP(hipEventRecord(e2, c.stream));
+  // residual over every row (the reference's RMS and BIC use b - A x)
+  if (c.A_prec == CG_FP64) residual(c, (const double *)c.A, m, n, d_x, d_b, d_r);
+  else residual(c, (const float *)c.A, m, n, d_x, d_b, d_r);
+  double Z = 0;
+  if (rocblas_set_pointer_mode(fs.h, rocblas_pointer_mode_device) != rocblas_status_success ||
+      rocblas_ddot(fs.h, (rocblas_int)m, d_r, 1, d_r, 1, d_z) != rocblas_status_success)
+    throw Error(CG_ERR_DEVICE, "rocblas_ddot");
+  rocblas_set_pointer_mode(fs.h, rocblas_pointer_mode_host);
+  CG_HIP(hipMemcpyAsync(&Z, d_z, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipMemcpyAsync(x_host, d_x, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CG_HIP(hipStreamSynchronize(c.stream));
+  double mx = 0;
+  for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(x_host[i]));
+  float tq = 0, ts = 0;
+  CG_HIP(hipEventElapsedTime(&tq, e0, e1));
+  CG_HIP(hipEventElapsedTime(&ts, e1, e2));
+  for (auto e : {e0, e1, e2}) (void)hipEventDestroy(e);
+  res->eps = eps;
+  res->rms = std::sqrt(Z / (double)m);
+  res->max_abs_x = mx;
+  res->nvars = nvars;
+  res->eps_abs = thr;
+  res->bic = (double)m * std::log(Z / (double)m) + (double)nvars * std::log((double)m);
+  res->ms_gram = tq;
+  res->ms_solve = ts;
+  res->ms_refine = 0.f;
+  res->iters = 0;
 }
 
 void gram_host(Ctx &c, int mode, const double *b_host, double *G_host, double *Atb_host) {
   if (!c.A) throw Error(CG_ERR_ARG, "nothing assembled");
   check_mode(mode);
+  if (mode == CG_FIT_QR) throw Error(CG_ERR_ARG, "cg_gram takes a Gram mode");
   const long long m = c.A_rows;
   const int n = c.A_ld;
   long long S, kchunk;
@@ -350,6 +481,11 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, int mode, double 
   if (!(eps > 0.0)) throw Error(CG_ERR_ARG, "eps must be positive");
   const long long m = c.A_rows;
   const int n = c.A_ld;
+  const int nfit = !c.fit_keep.empty() && (int)c.fit_keep.size() < n ? (int)c.fit_keep.size() : n;
+  if (mode == CG_FIT_QR) {
+    if (m >= nfit) return fit_qr(c, b_host, eps, x_host, res);
+    mode = CG_GRAM_FP64;  // fewer rows than columns: the Gram path
+  }
   FitState &fs = fit_state(c);
   hipEvent_t e0, e1, e2, e3;
   CG_HIP(hipEventCreate(&e0)); CG_HIP(hipEventCreate(&e1)); CG_HIP(hipEventCreate(&e2)); CG_HIP(hipEventCreate(&e3));

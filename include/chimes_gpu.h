@@ -248,6 +248,9 @@ cg_status cg_synchronize(cg_ctx *ctx);
  *                    Relative error ~1e-6 of max|G|: too coarse to resolve
  *                    singular values below ~1e-3 sigma_max. */
 enum { CG_GRAM_FP64 = 0, CG_GRAM_BF16X3 = 1 };
+/* Fit mode of cg_fit without a Gram matrix: LQ factorization of A and the SVD
+ * of its triangular factor (rocSOLVER), the SVD of A to working precision. */
+enum { CG_FIT_QR = 2 };
 
 /* No reference counterpart as an entry point (chimes_lsq.py forms no Gram
  * matrix); exposed so callers can run their own solvers on G and A^T b.
@@ -262,6 +265,11 @@ cg_status cg_gram(cg_ctx *ctx, int gram_mode, const double *b_host, double *G, d
  * x += V Lambda^+ V^T A^T (b - A x) with the residual in FP64.  Refinement
  * stops early when the residual norm stops decreasing (res->iters).
  * eps: relative singular-value cut (chimes_lsq.py --eps), refine: iterations.
+ * gram_mode CG_FIT_QR instead takes the SVD of A itself (A^T = L Q, then
+ * L = U S V^T; x = U S+ V^T (Q b)): singular directions near the cut are then
+ * resolved to ~1e-16 sigma_max / gap rather than ~1e-16 sigma_max^2 / gap^2,
+ * at the cost of an FP64 copy of A (needs rows >= columns, else the FP64
+ * Gram path runs); refine is unused there.
  * Outputs: x[ncols], rms (over all rows), nvars kept, eps*sigma_max. */
 typedef struct cg_fit_result {
   double eps;           /* relative cut used (input)                       */
