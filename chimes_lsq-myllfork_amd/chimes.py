@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libchimes_gpu.so")
 CG_FP64, CG_FP32 = 0, 1
 CG_GRAM_FP64, CG_GRAM_BF16X3 = 0, 1
 CG_FIT_QR = 2
+CG_ALG_SVD, CG_ALG_RIDGE = 0, 1
 STATUS = {0: "CG_OK", 1: "CG_ERR_ARG", 2: "CG_ERR_INPUT", 3: "CG_ERR_UNSUPPORTED",
           4: "CG_ERR_DEVICE", 5: "CG_ERR_MEMORY", 6: "CG_ERR_NUMERIC", 7: "CG_ERR_EOF"}
 
@@ -55,7 +56,8 @@ class Frame(C.Structure):
 class FitResult(C.Structure):
     _fields_ = [("eps", C.c_double), ("rms", C.c_double), ("max_abs_x", C.c_double), ("nvars", C.c_int),
                 ("eps_abs", C.c_double), ("bic", C.c_double), ("ms_gram", C.c_double),
-                ("ms_solve", C.c_double), ("ms_refine", C.c_double), ("iters", C.c_int)]
+                ("ms_solve", C.c_double), ("ms_refine", C.c_double), ("iters", C.c_int),
+                ("algorithm", C.c_int), ("alpha", C.c_double), ("weights_file", C.c_char * 256)]
 
 
 _lib = None
@@ -112,6 +114,8 @@ def lib(path=None):
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
         opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
         opt("cg_set_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.c_int])
+        opt("cg_set_fit_weights", [C.c_void_p, _dp, C.c_int64])
+        opt("cg_fit_ridge", [C.c_void_p, _dp, C.c_double, _dp, C.POINTER(FitResult)])
         opt("cg_setup_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
         opt("cg_autotune", [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.POINTER(C.c_int), _dp])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
@@ -346,6 +350,21 @@ class Device:
         res = FitResult()
         b = np.ascontiguousarray(b, dtype=np.float64)
         check(self.L.cg_fit(self.h, dptr(b), eps, refine, gram, dptr(x), C.byref(res)), self.h)
+        return x, res
+
+    def set_weights(self, w):
+        """Row weights of the following CG_FIT_QR fits (None clears)."""
+        if w is None:
+            check(self.L.cg_set_fit_weights(self.h, None, 0), self.h)
+        else:
+            self._w = np.ascontiguousarray(w, dtype=np.float64)
+            check(self.L.cg_set_fit_weights(self.h, dptr(self._w), self._w.size), self.h)
+
+    def fit_ridge(self, b, alpha):
+        x = np.zeros(self.setup.model.ncols)
+        res = FitResult()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        check(self.L.cg_fit_ridge(self.h, dptr(b), alpha, dptr(x), C.byref(res)), self.h)
         return x, res
 
     def gram(self, b=None, mode=CG_GRAM_FP64):

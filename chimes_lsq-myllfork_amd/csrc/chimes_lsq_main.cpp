@@ -2,7 +2,8 @@
 // with the design matrix assembled on the GPU, optionally followed by the fit
 // that `chimes_lsq.py` performs (src/chimes_lsq.py:169-672).
 //
-//   chimes_lsq_gpu fm_setup.in [--fit] [--eps E] [--prec fp64|fp32] [--gram qr|fp64|bf16x3]
+//   chimes_lsq_gpu fm_setup.in [--fit] [--algorithm svd|ridge] [--eps E] [--alpha A] [--weights FILE]
+//                  [--prec fp64|fp32] [--gram qr|fp64|bf16x3]
 //                  [--refine K] [--batch FRAMES] [--ranks N] [--no-A] [--no-text]
 //
 // Files written in the working directory: A.txt b.txt b-labeled.txt natoms.txt
@@ -35,13 +36,15 @@ int die(const char *what, cg_status st, const void *ctx = nullptr) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--eps E] [--prec fp64|fp32] [--gram qr|fp64|bf16x3] [--refine K] "
+    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--algorithm svd|ridge] [--eps E] [--alpha A] [--weights FILE] "
+                         "[--prec fp64|fp32] [--gram qr|fp64|bf16x3] [--refine K] "
                          "[--batch FRAMES] [--ranks N] [--no-A] [--no-text]\n");
     return 1;
   }
   const char *infile = argv[1];
   bool do_fit = false, text = true, write_A = true;
-  double eps = 1.0e-5;
+  double eps = 1.0e-5, alpha = 1.0e-4;
+  std::string algorithm = "svd", weights;
   int prec = CG_FP64, refine = 3, batch = 0, gram = CG_FIT_QR, nranks = 1;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
@@ -50,6 +53,9 @@ int main(int argc, char **argv) {
     else if (a == "--no-A") write_A = false;
     else if (a == "--ranks" && i + 1 < argc) nranks = std::atoi(argv[++i]);
     else if (a == "--eps" && i + 1 < argc) eps = std::atof(argv[++i]);
+    else if (a == "--alpha" && i + 1 < argc) alpha = std::atof(argv[++i]);
+    else if (a == "--algorithm" && i + 1 < argc) algorithm = argv[++i];
+    else if (a == "--weights" && i + 1 < argc) weights = argv[++i];
     else if (a == "--refine" && i + 1 < argc) refine = std::atoi(argv[++i]);
     else if (a == "--batch" && i + 1 < argc) batch = std::atoi(argv[++i]);
     else if (a == "--prec" && i + 1 < argc) {
@@ -155,7 +161,27 @@ int main(int argc, char **argv) {
     if ((st = cg_setup_fit_columns(setup, keep.data(), &nkeep))) return die("fit columns", st);
     if ((st = cg_set_fit_columns(ctx, keep.data(), nkeep < m->ncols ? nkeep : 0))) return die("fit columns", st, ctx);
     cg_fit_result res{};
-    if ((st = cg_fit(ctx, b_all.data(), eps, refine, gram, x.data(), &res))) return die("fitting", st, ctx);
+    if (!weights.empty() && algorithm == "svd") {
+      // one weight per row of b (src/chimes_lsq.py:75-129)
+      std::vector<double> w;
+      std::FILE *wf = std::fopen(weights.c_str(), "r");
+      if (!wf) { std::fprintf(stderr, "cannot read %s\n", weights.c_str()); return 1; }
+      double v;
+      while (std::fscanf(wf, "%lf", &v) == 1) w.push_back(v);
+      std::fclose(wf);
+      if ((int64_t)w.size() != total_rows) { std::printf("Wrong number of lines in WEIGHTS file\n"); return 1; }
+      if ((st = cg_set_fit_weights(ctx, w.data(), (int64_t)w.size()))) return die("weights", st, ctx);
+      gram = CG_FIT_QR;
+    }
+    if (algorithm == "ridge") {
+      if ((st = cg_fit_ridge(ctx, b_all.data(), alpha, x.data(), &res))) return die("fitting", st, ctx);
+    } else if (algorithm == "svd") {
+      if ((st = cg_fit(ctx, b_all.data(), eps, refine, gram, x.data(), &res))) return die("fitting", st, ctx);
+    } else {
+      std::printf("Unrecognized fitting algorithm\n");
+      return 1;
+    }
+    if (!weights.empty()) std::snprintf(res.weights_file, sizeof res.weights_file, "%s", weights.c_str());
     if ((st = cg_matvec(ctx, x.data(), y.data()))) return die("computing A x", st, ctx);
     std::vector<double> xk(nkeep);
     for (int i = 0; i < nkeep; ++i) xk[i] = x[keep[i]];

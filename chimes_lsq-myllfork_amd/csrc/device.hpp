@@ -124,6 +124,8 @@ struct Ctx {
   DBuf b_x;                               // fit workspace
   DBuf b_fitkeep;                         // kept columns of a fit (device copy)
   DBuf b_qr;                              // QR fit: FP64 copy of A, factors, SVD
+  DBuf b_fitw;                            // row weights of a fit (device copy)
+  std::vector<double> fit_w;              // row weights (empty: none)
   std::vector<int> fit_keep;              // columns a fit keeps (empty: all)
   DBuf b_force;                           // force evaluation: F, energy/virial, call totals
   std::vector<double> h_out;              // host copy of the force results and totals

@@ -254,12 +254,17 @@ __global__ void k_scatter_vals(const double *v, const int *keep, int nk, double 
 // column-major ne x m matrix A^T that the LQ factorization takes.
 template <class T>
 __global__ void k_copy_cols(const T *__restrict__ A, long long ld, long long m, int ne, const int *__restrict__ keep,
-                            double *__restrict__ W) {
+                            const double *__restrict__ w, double *__restrict__ W) {
   const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= m * ne) return;
   const long long r = idx / ne;
   const int j = (int)(idx - r * ne);
-  W[idx] = (double)A[r * ld + (keep ? keep[j] : j)];
+  const double v = (double)A[r * ld + (keep ? keep[j] : j)];
+  W[idx] = w ? w[r] * v : v;
+}
+__global__ void k_mul(double *v, const double *w, long long m) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < m) v[i] *= w[i];
 }
 // Lower triangle of the leading ne x ne block of a column-major matrix.
 __global__ void k_lower(const double *W, int ne, double *L) {
@@ -349,7 +354,13 @@ static void check_mode(int mode) {
 // factor) A = (Q^T V) S U^T and
 //   x = sum_{s_i > eps s_max} U[:, i] (V^T (Q b))_i / s_i,
 // Q b applied from the stored reflectors (rocsolver_dormlq).
-static void fit_qr(Ctx &c, const double *b_host, double eps, double *x_host, cg_fit_result *res) {
+//
+// Row weights (chimes_lsq.py --weights, src/chimes_lsq.py:75-150) scale the
+// rows of A and b for the solve; the RMS, BIC and A x use the unweighted
+// rows, as the script does.  ridge >= 0 gives sklearn's Ridge without
+// intercept (src/chimes_lsq.py:216-225) from the same SVD:
+//   x = sum_i s_i / (s_i^2 + ridge) (V^T (Q b))_i U[:, i], every column kept.
+static void fit_qr(Ctx &c, const double *b_host, double eps, double ridge, double *x_host, cg_fit_result *res) {
   const long long m = c.A_rows;
   const int n = c.A_ld;
   FitState &fs = fit_state(c);
@@ -371,17 +382,27 @@ static void fit_qr(Ctx &c, const double *b_host, double eps, double *x_host, cg_
   }
   hipEvent_t e0, e1, e2;
   CG_HIP(hipEventCreate(&e0)); CG_HIP(hipEventCreate(&e1)); CG_HIP(hipEventCreate(&e2));
+  const bool weighted = !c.fit_w.empty() && ridge < 0.0;
+  double *d_w = nullptr;
+  if (!c.fit_w.empty() && (long long)c.fit_w.size() != m)
+    throw Error(CG_ERR_ARG, "Wrong number of lines in WEIGHTS file");
+  if (weighted) {
+    d_w = c.b_fitw.as<double>(m);
+    if (!d_w) throw Error(CG_ERR_MEMORY, "fit weights");
+    CG_HIP(hipMemcpyAsync(d_w, c.fit_w.data(), m * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  }
   CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
   CG_HIP(hipMemcpyAsync(d_q, d_b, m * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  if (weighted) hipLaunchKernelGGL(k_mul, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, d_q, d_w, m);
   CG_HIP(hipEventRecord(e0, c.stream));
   const long long tot = m * ne;
   const unsigned grid = (unsigned)((tot + 255) / 256);
   if (c.A_prec == CG_FP64)
     hipLaunchKernelGGL(k_copy_cols<double>, dim3(grid), dim3(256), 0, c.stream, (const double *)c.A, (long long)n, m, ne,
-                       (const int *)d_keep, W);
+                       (const int *)d_keep, d_w, W);
   else
     hipLaunchKernelGGL(k_copy_cols<float>, dim3(grid), dim3(256), 0, c.stream, (const float *)c.A, (long long)n, m, ne,
-                       (const int *)d_keep, W);
+                       (const int *)d_keep, d_w, W);
   CG_HIP(hipGetLastError());
   if (rocsolver_dgelqf(fs.h, ne, (rocblas_int)m, W, ne, tau) != rocblas_status_success)
     throw Error(CG_ERR_NUMERIC, "rocsolver_dgelqf failed");
@@ -404,8 +425,13 @@ static void fit_qr(Ctx &c, const double *b_host, double eps, double *x_host, cg_
   const double thr = eps * smax;
   std::vector<double> wi(ne, 0.0);
   int nvars = 0;
-  for (int i = 0; i < ne; ++i)
-    if (std::fabs(sv[i]) > thr) { wi[i] = 1.0 / sv[i]; ++nvars; }
+  if (ridge >= 0.0) {
+    for (int i = 0; i < ne; ++i) wi[i] = sv[i] / (sv[i] * sv[i] + ridge);
+    nvars = ne;
+  } else {
+    for (int i = 0; i < ne; ++i)
+      if (std::fabs(sv[i]) > thr) { wi[i] = 1.0 / sv[i]; ++nvars; }
+  }
   CG_HIP(hipMemcpyAsync(winv, wi.data(), ne * sizeof(double), hipMemcpyHostToDevice, c.stream));
   // t = S+ V^T (Q b)[0:ne];  x = U t
   const double one = 1.0, zero = 0.0;
@@ -448,6 +474,8 @@ static void fit_qr(Ctx &c, const double *b_host, double eps, double *x_host, cg_
   res->ms_solve = ts;
   res->ms_refine = 0.f;
   res->iters = 0;
+  res->algorithm = ridge >= 0.0 ? CG_ALG_RIDGE : CG_ALG_SVD;
+  res->alpha = ridge >= 0.0 ? ridge : 0.0;
 }
 
 void gram_host(Ctx &c, int mode, const double *b_host, double *G_host, double *Atb_host) {
@@ -483,9 +511,10 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, int mode, double 
   const int n = c.A_ld;
   const int nfit = !c.fit_keep.empty() && (int)c.fit_keep.size() < n ? (int)c.fit_keep.size() : n;
   if (mode == CG_FIT_QR) {
-    if (m >= nfit) return fit_qr(c, b_host, eps, x_host, res);
+    if (m >= nfit) return fit_qr(c, b_host, eps, -1.0, x_host, res);
     mode = CG_GRAM_FP64;  // fewer rows than columns: the Gram path
   }
+  if (!c.fit_w.empty()) throw Error(CG_ERR_UNSUPPORTED, "row weights need the CG_FIT_QR fit");
   FitState &fs = fit_state(c);
   hipEvent_t e0, e1, e2, e3;
   CG_HIP(hipEventCreate(&e0)); CG_HIP(hipEventCreate(&e1)); CG_HIP(hipEventCreate(&e2)); CG_HIP(hipEventCreate(&e3));
@@ -624,6 +653,19 @@ void fit(Ctx &c, const double *b_host, double eps, int refine, int mode, double 
   res->ms_solve = ts;
   res->ms_refine = tr;
   res->iters = iters;
+  res->algorithm = CG_ALG_SVD;
+  res->alpha = 0.0;
+}
+
+void fit_ridge(Ctx &c, const double *b_host, double alpha, double *x_host, cg_fit_result *res) {
+  if (!c.A) throw Error(CG_ERR_ARG, "nothing assembled");
+  if (!(alpha >= 0.0)) throw Error(CG_ERR_ARG, "ridge alpha must be >= 0");
+  const int n = c.A_ld;
+  const int nfit = !c.fit_keep.empty() && (int)c.fit_keep.size() < n ? (int)c.fit_keep.size() : n;
+  if (c.A_rows < nfit) throw Error(CG_ERR_UNSUPPORTED, "ridge fit needs at least as many rows as columns");
+  fit_qr(c, b_host, 1.0, alpha, x_host, res);
+  res->eps = 0.0;
+  res->eps_abs = 0.0;
 }
 
 void matvec(Ctx &c, const double *x_host, double *y_host) {
@@ -668,6 +710,25 @@ cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters
     cg::set_error(e.what());
     return e.code;
   }
+}
+
+cg_status cg_fit_ridge(cg_ctx *ctx, const double *b_host, double alpha, double *x, cg_fit_result *res) {
+  if (!ctx || !b_host || !x || !res) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::fit_ridge(ctx->c, b_host, alpha, x, res);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_set_fit_weights(cg_ctx *ctx, const double *w, int64_t nrows) {
+  if (!ctx || nrows < 0 || (nrows > 0 && !w)) { cg::set_error("cg_set_fit_weights: bad argument"); return CG_ERR_ARG; }
+  if (w) ctx->c.fit_w.assign(w, w + nrows);
+  else ctx->c.fit_w.clear();
+  return CG_OK;
 }
 
 cg_status cg_set_fit_columns(cg_ctx *ctx, const int *keep, int nkeep) {

@@ -670,13 +670,20 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   P("!");
   P("! Number of variables            =  " + std::to_string(nx));
   P("! Number of equations            =  " + std::to_string(nlines));
-  P("! svd algorithm used");
-  P(fx("! eps (= args.eps*dmax)          =  %11.4e", r.eps_abs));
-  P(fx("! SVD regularization factor      = %11.4e", r.eps));
+  if (r.algorithm == CG_ALG_RIDGE) {  // src/chimes_lsq.py:216-225
+    P("! ridge regression used");
+    P(fx("! Ridge alpha = %11.4e", r.alpha));
+  } else {
+    P("! svd algorithm used");
+    P(fx("! eps (= args.eps*dmax)          =  %11.4e", r.eps_abs));
+    P(fx("! SVD regularization factor      = %11.4e", r.eps));
+  }
   P(fx("! RMS force error                = %11.4e", r.rms));
   P(fx("! max abs variable               = %11.4e", r.max_abs_x));
   P("! number of fitting vars         =  " + std::to_string(r.nvars));
   P(fx("! Bayesian Information Criterion = %11.4e", r.bic));
+  const std::string wf(r.weights_file, strnlen(r.weights_file, sizeof r.weights_file));
+  if (!wf.empty()) P("! Using weighting file:             " + wf);  // src/chimes_lsq.py:307-308
   P("!");
   std::vector<std::string> hf = split_lines(header_text(s));
   auto tok = [](const std::string &l) { return split_ws(l); };

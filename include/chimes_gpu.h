@@ -280,7 +280,11 @@ typedef struct cg_fit_result {
   double bic;
   double ms_gram, ms_solve, ms_refine;
   int iters;            /* refinement iterations applied                   */
+  int algorithm;        /* CG_ALG_SVD or CG_ALG_RIDGE (set by the fit)     */
+  double alpha;         /* ridge alpha                                     */
+  char weights_file[256]; /* row-weight file named in params.txt ("" none) */
 } cg_fit_result;
+enum { CG_ALG_SVD = 0, CG_ALG_RIDGE = 1 };
 cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, int gram_mode,
                  double *x, cg_fit_result *res);
 /* Fit only the columns keep[0..nkeep) (increasing column indices) and hold the
@@ -288,6 +292,17 @@ cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters
  * hierarchical fit's excluded 1-body and 2-body columns, which chimes_lsq
  * leaves out of A.txt (src/A_Matrix.C:155-208). */
 cg_status cg_set_fit_columns(cg_ctx *ctx, const int *keep, int nkeep);
+/* Row weights of the following CG_FIT_QR fits (chimes_lsq.py --weights,
+ * src/chimes_lsq.py:75-150): rows of A and b are scaled by w for the solve,
+ * the RMS / BIC / A x use the unweighted rows.  nrows must equal the rows of
+ * A at fit time ("Wrong number of lines in WEIGHTS file" otherwise); NULL
+ * clears.  Ridge fits ignore them, as the script's ridge branch does. */
+cg_status cg_set_fit_weights(cg_ctx *ctx, const double *w, int64_t nrows);
+/* Ridge regression without intercept (chimes_lsq.py --algorithm=ridge, i.e.
+ * sklearn Ridge(alpha, fit_intercept=False), src/chimes_lsq.py:216-225):
+ * x = argmin |A x - b|^2 + alpha |x|^2, from the SVD of A (LQ factor, as
+ * CG_FIT_QR): x = sum_i s_i / (s_i^2 + alpha) (u_i . b) v_i. */
+cg_status cg_fit_ridge(cg_ctx *ctx, const double *b_host, double alpha, double *x, cg_fit_result *res);
 /* Columns of A that a fit keeps under the setup's hierarchical exclusions
  * (EXCLUDE 1B/2B INTERACTION with HIERARC true; src/input.C:760-840): keep
  * [ncols] receives them in increasing order, nkeep their number. */

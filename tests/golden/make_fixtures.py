@@ -25,7 +25,9 @@ expected outputs), trimmed to the frames the tests need:
                            golden A rows that survive in A.txt.ab (the last 152
                            force rows and the 2 constraint rows), h2o-4bcheby
                            its b and force.txt; skip_frames (36 ranks, round
-                           robin) its b.txt (gzip) and dim.txt.
+                           robin) its b.txt (gzip) and dim.txt; lsq2 (same
+                           setup) the row weights and the params.txt of the
+                           weighted SVD, eps 1e-4 and ridge 1e-3 fits.
   md/<test>/               force-evaluation cases of test_suite-md without
                            Coulomb terms: params.txt (gzip), the first frame
                            of the run's coordinate file and the reference's
@@ -190,6 +192,15 @@ def main():
         if t == "skip_frames":
             gz(os.path.join(co, "b.txt"), os.path.join(out, "b.txt.gz"))
             open(os.path.join(out, "dim.txt"), "w").write(open(os.path.join(co, "dim.txt")).read())
+    # lsq2 (the h2o-4bcheby setup): chimes_lsq.py runs with row weights, another
+    # eps and ridge regression (its Makefile), their params.txt and the weights
+    src = os.path.join(LSQ, "lsq2")
+    assert open(os.path.join(src, "fm_setup.in")).read() == open(os.path.join(LSQ, "h2o-4bcheby", "fm_setup.in")).read()
+    out = os.path.join(wat, "lsq2")
+    os.makedirs(out, exist_ok=True)
+    gz(os.path.join(src, "weights.txt"), os.path.join(out, "weights.txt.gz"))
+    for fn in ["params.svd.weight.txt", "params.svd.1.0e-04.txt", "params.ridge.1.0e-03.txt"]:
+        gz(os.path.join(src, "correct_output", fn), os.path.join(out, fn + ".gz"))
     # force-evaluation cases: params.txt + first coordinate frame + forceout.txt
     md = os.path.join(REF, "test_suite-md")
     for t in MD_TESTS:

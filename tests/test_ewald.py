@@ -185,3 +185,80 @@ def test_fitcoul_refusals(ch, synth, tmp_path):
     # npairs - 1 rows; weights land in pair-index order (OO 0, HH 1, OH 2)
     assert rows.shape == (2, s.model.ncols) and list(rows[0, -3:]) == [1.0, 1.0, 1.0]
     assert list(rows[1, -3:]) == [1.0, 3.0, 2.0] and list(b) == [0.0, 0.5]
+
+
+# ------------------------------------------------------------ lsq2 printer
+LSQ2 = os.path.join(WATER, "lsq2")
+_NUM = r"[-+]?\d+\.\d+e[-+]\d+"
+
+
+def _lsq2_text(fn):
+    with gzip.open(os.path.join(LSQ2, fn + ".gz"), "rt") as g:
+        return g.read()
+
+
+def _body(lines):
+    k = next(i for i, l in enumerate(lines) if l.startswith("USECOUL:"))
+    return lines[k:]
+
+
+def x_from_params(ch, s, text, tmp_path):
+    """The coefficient vector a params.txt prints: the printer's output for
+    x = 1..ncols names the column of every printed number."""
+    import re
+    n = s.model.ncols
+    probe = np.arange(1, n + 1, dtype=float)
+    res = ch.FitResult()
+    d = tmp_path / "probe"
+    d.mkdir(exist_ok=True)
+    ch.check(ch.lib().cg_write_params(s.h, str(d).encode(), ch.dptr(probe), n, 1, ctypes.byref(res), None, None))
+    lp = _body(open(d / "params.txt").read().split("\n"))
+    lg = _body(text.split("\n"))
+    assert len(lp) == len(lg)
+    x = np.full(n, np.nan)
+    for a, b in zip(lp, lg):
+        for u, v in zip(re.findall(_NUM, a), re.findall(_NUM, b)):
+            x[int(round(float(u))) - 1] = float(v)
+    assert not np.isnan(x).any()
+    return x
+
+
+def _summary(text):
+    import re
+    vals = {}
+    for l in text.split("\n")[:16]:
+        if l.startswith("! ") and "=" in l:
+            k, v = l[2:].rsplit("=", 1)
+            vals[k.strip()] = v.strip()
+    return vals
+
+
+@pytest.mark.parametrize("fn,alg", [("params.svd.weight.txt", "svd"), ("params.svd.1.0e-04.txt", "svd"),
+                                    ("params.ridge.1.0e-03.txt", "ridge")])
+def test_params_printer_weights_and_ridge(ch, tmp_path, fn, alg):
+    """params.txt of chimes_lsq.py's weighted SVD, eps 1e-4 SVD and ridge runs
+    (test_suite-lsq/lsq2) byte for byte from their coefficients: the ridge
+    summary lines and the weighting-file line."""
+    s = water_setup(ch, tmp_path, "h2o-4bcheby")  # lsq2 runs the same fm_setup.in
+    text = _lsq2_text(fn)
+    x = x_from_params(ch, s, text, tmp_path)
+    v = _summary(text)
+    res = ch.FitResult()
+    res.rms = float(v["RMS force error"])
+    res.max_abs_x = float(v["max abs variable"])
+    res.nvars = int(v["number of fitting vars"])
+    res.bic = float(v["Bayesian Information Criterion"])
+    if alg == "ridge":
+        res.algorithm = ch.CG_ALG_RIDGE
+        res.alpha = float(v["Ridge alpha"])
+    else:
+        res.eps_abs = float(v["eps (= args.eps*dmax)"])
+        res.eps = float(v["SVD regularization factor"])
+    if "weight" in fn:
+        res.weights_file = b"weights.txt"
+    out = tmp_path / "out"
+    out.mkdir()
+    neq = int(v["Number of equations"])
+    ch.check(ch.lib().cg_write_params(s.h, str(out).encode(), ch.dptr(x), len(x), neq, ctypes.byref(res), None, None))
+    ours = open(out / "params.txt").read().split("\n")[1:]
+    assert ours == text.split("\n")[1:]

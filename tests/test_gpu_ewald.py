@@ -160,3 +160,42 @@ def test_cli_h2o_4bcheby_end_to_end(ch, tmp_path):
     rel = np.max(np.abs(y - g["force"]) / np.maximum(np.abs(g["force"]), 1e-3 * np.abs(g["force"]).max()))
     print("force.txt max rel diff %.2e" % rel)
     assert rel < 1e-6
+
+
+@pytest.mark.parametrize("fn,args,xtol", [
+    ("params.svd.weight.txt", ["--weights", "weights.txt"], 1e-8),
+    ("params.svd.1.0e-04.txt", ["--eps", "1.0e-04"], 1e-8),
+    # sklearn's Ridge solves (A^T A + alpha I) x = A^T b by Cholesky: relative
+    # error up to ~1e-16 s_max^2 / alpha = 4e-4 here (s_max 6.5e4, alpha 1e-3)
+    ("params.ridge.1.0e-03.txt", ["--algorithm", "ridge", "--alpha", "1.0e-03"], 1e-3)])
+def test_cli_lsq2_weights_eps_ridge(ch, tmp_path, fn, args, xtol):
+    """chimes_lsq.py's weighted SVD, eps 1e-4 SVD and ridge runs of the
+    reference's lsq2 case (h2o-4bcheby setup) through the drop-in CLI: the
+    summary lines to their printed digits and every coefficient against the
+    reference's (within xtol of max|x|)."""
+    import ctypes
+    import gzip
+    from test_ewald import _lsq2_text, _summary, x_from_params
+    print("case", fn, flush=True)
+    s = water_setup(ch, tmp_path, "h2o-4bcheby")
+    with gzip.open(os.path.join(WATER, "lsq2", "weights.txt.gz"), "rb") as g:
+        open(tmp_path / "weights.txt", "wb").write(g.read())
+    exe = os.path.join(PKG, "chimes_lsq_gpu")
+    r = subprocess.run([exe, "fm_setup.in", "--fit", "--no-A"] + args, cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    gold = _lsq2_text(fn)
+    ours = open(tmp_path / "params.txt").read()
+    vg, vo = _summary(gold), _summary(ours)
+    assert vg.keys() == vo.keys()
+    for k in vg:
+        if k == "number of fitting vars" or k.startswith("Number"):
+            assert vo[k] == vg[k], k
+        else:
+            assert float(vo[k]) == pytest.approx(float(vg[k]), rel=1e-4), k
+    assert [l for l in ours.split("\n") if "weighting" in l] == [l for l in gold.split("\n") if "weighting" in l]
+    xg = x_from_params(ch, s, gold, tmp_path)
+    xo = x_from_params(ch, s, ours, tmp_path)
+    d = np.abs(xo - xg).max() / np.abs(xg).max()
+    print("max |x - x_ref| / max|x_ref| = %.2e" % d)
+    assert d <= xtol
