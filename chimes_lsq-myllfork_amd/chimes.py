@@ -44,7 +44,7 @@ class Model(C.Structure):
                 ("fit_stress", C.c_int), ("fit_energy", C.c_int), ("nstress", C.c_int),
                 ("nener", C.c_int), ("n2b_cols", C.c_int), ("ncols_short", C.c_int),
                 ("n_charge_cols", C.c_int), ("n_offset_cols", C.c_int), ("ncols", C.c_int), ("max_cutoff_2b", C.c_double),
-                ("c3", ClusterSet), ("c4", ClusterSet)]
+                ("c3", ClusterSet), ("c4", ClusterSet), ("charge_sign", C.POINTER(C.c_int))]
 
 
 class Frame(C.Structure):

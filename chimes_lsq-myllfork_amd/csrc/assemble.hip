@@ -1814,6 +1814,7 @@ struct FamOut {
 
 struct Batch {
   std::vector<DevFrame> hf;
+  const cg_frame *frames = nullptr;  // host frames of the call
   int nframes = 0;
   int natoms = 0;
   long long rows = 0;
@@ -1852,6 +1853,7 @@ static void zero_d(Ctx &c, double *p, size_t n) {
 // cell-list decisions), batch arrays, and their upload.
 static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, bool sticky) {
   std::vector<DevFrame> &hf = B.hf;
+  B.frames = frames;
   const cg_model &m = c.m;
   if (nframes <= 0) throw Error(CG_ERR_ARG, "no frames");
   // ---- host frame table
@@ -2284,6 +2286,7 @@ static void set_coef(Ctx &c, const double *x) {
     CG_HIP(hipMemcpy(d, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice));
     c.coef_size[w] = (int)ts;
   }
+  if (m.n_charge_cols > 0) ewald_type_charges(m, x, c.q_type);
   c.h_x_last.assign(x, x + m.ncols);
   c.coef_valid = true;
 }
@@ -2346,6 +2349,8 @@ static void force_device(Ctx &c, const Batch &B, double *d_F, size_t nout) {
     if (!ok) throw Error(CG_ERR_UNSUPPORTED, "force evaluation supports 2/3-body orders up to 16 and 4-body up to 8");
     CG_HIP(hipGetLastError());
   }
+  // Ewald terms of a model fitted with charges (chimes_md's ZCalc_Ewald)
+  if (m.n_charge_cols > 0) ewald_md(c, B.frames, B.hf, B.d_af, B.d_type, B.d_wrap, B.natoms, d_F, d_EV);
   hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, c.stream, B.tot, d_F + nout);
   CG_HIP(hipMemcpyAsync(c.h_pin, d_F, (nout + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
 }
@@ -2449,7 +2454,8 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
     c.h_pin_cap = n;
   }
   bool done = false;
-  if (capmode) {
+  // calls with Ewald terms upload per-call k tables and stay out of graphs
+  if (capmode && m.n_charge_cols == 0) {
     const std::string key = shape_key(B, caps);
     auto it = c.graphs.find(key);
     if (it != c.graphs.end() && it->second.gen != g_buffer_generation) {
@@ -2506,8 +2512,6 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
 
 void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
                  double *vir) {
-  if (c.m.n_charge_cols > 0)
-    throw Error(CG_ERR_UNSUPPORTED, "force evaluation of a model with Ewald charge columns is not supported");
   ++c.n_force;
   force_batch_impl(c, x_host, frames, nframes, F, E, vir, false);
 }

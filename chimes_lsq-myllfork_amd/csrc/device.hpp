@@ -150,6 +150,7 @@ struct Ctx {
   DBuf b_tmp;                             // matvec output
   DBuf b_ewald;                           // Ewald frame table, k-vectors, structure factors
   std::vector<char> h_ewald;              // host source of the Ewald table upload
+  std::vector<double> q_type;             // force evaluation with Ewald terms: charge per atom type (e)
   void *A = nullptr;          // last assembled A (device)
   int A_prec = CG_FP64;
   int64_t A_rows = 0;
@@ -174,6 +175,11 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
 void ewald_params(int natoms, const double L[3], double *alpha, double *rc, int *kmax);
 void ewald_columns(Ctx &c, const cg_frame *frames, const std::vector<DevFrame> &hf, const int *d_af,
                    const int *d_type, const double *d_wrap, int natoms_total, void *A, long long ld, int prec);
+// Charges per atom type from the Ewald columns of x, and the Ewald forces,
+// energies and virials of a force call added to its device results.
+void ewald_type_charges(const cg_model &m, const double *x, std::vector<double> &qt);
+void ewald_md(Ctx &c, const cg_frame *frames, const std::vector<DevFrame> &hf, const int *d_af, const int *d_type,
+              const double *d_wrap, int natoms_total, double *d_F, double *d_EV);
 
 }  // namespace cg
 

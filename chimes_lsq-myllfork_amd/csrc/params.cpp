@@ -62,8 +62,12 @@ Setup *read_params(const std::string &path) {
     auto t = tokenize(after_colon(P.lines[i]));
     return !t.empty() && (t[0] == "true" || t[0] == "TRUE" || t[0] == "True");
   };
-  if (flag("USECOUL:") || flag("FITCOUL:"))
-    throw Error(CG_ERR_UNSUPPORTED, "Coulomb (Ewald) terms in " + path + " are not supported");
+  // FITCOUL: charge products q_A x q_B after each pair's 2-body block
+  // (chimes_md.C:2009-2016); fixed charges (USECOUL without FITCOUL) are not
+  // supported
+  const bool fitcoul = flag("FITCOUL:");
+  if (flag("USECOUL:") && !fitcoul)
+    throw Error(CG_ERR_UNSUPPORTED, "fixed Coulomb charges in " + path + " are not supported");
   if (flag("USEPOVR:")) throw Error(CG_ERR_UNSUPPORTED, "over-coordination terms in " + path + " are not supported");
 
   // ---- header
@@ -148,12 +152,14 @@ Setup *read_params(const std::string &path) {
 
   // ---- equivalent fm_setup.in keywords
   std::ostringstream fm;
-  fm << "# TRJFILE #\n\tnone.xyzf\n# NFRAMES #\n\t1\n# NLAYERS #\n\t1\n# FITCOUL #\n\tfalse\n"
+  fm << "# TRJFILE #\n\tnone.xyzf\n# NFRAMES #\n\t1\n# NLAYERS #\n\t1\n# FITCOUL #\n\t" << (fitcoul ? "true" : "false") << "\n"
      << "# FITSTRS #\n\tfalse\n# FITENER #\n\t" << (offsets.empty() ? "false" : "true") << "\n"
      << "# PAIRTYP #\n\tCHEBYSHEV " << o2 << " " << o3 << " " << o4 << " " << lo << " " << hi << "\n"
      << "# CHBTYPE #\n\t" << pairs[0][5] << "\n"
      << "# NATMTYP #\n\t" << nt << "\n# TYPEIDX # # ATM_TYP # # ATMCHRG # # ATMMASS #\n";
-  for (int q = 0; q < nt; ++q) fm << "\t" << q + 1 << " " << types[q][1] << " 0 " << types[q][3] << "\n";
+  for (int q = 0; q < nt; ++q)
+    fm << "\t" << q + 1 << " " << types[q][1] << " " << (fitcoul ? types[q][2] : std::string("0")) << " " << types[q][3]
+       << "\n";
   fm << "# PAIRIDX # # ATM_TY1 # # ATM_TY1 # # S_MINIM # # S_MAXIM # # CHBDIST # # MORSE_LAMBDA #\n";
   for (auto &p : pairs)
     fm << "\t" << p[0] << " " << p[1] << " " << p[2] << " " << p[3] << " " << p[4] << " " << p[5] << " " << p[6]
@@ -210,6 +216,14 @@ Setup *read_params(const std::string &path) {
         ++got;
       }
       if (got != m.order2b) bad(path, "truncated PAIRTYPE PARAMS block " + std::to_string(p));
+      if (fitcoul) {
+        // "q_A x q_B value", the pair's Ewald column
+        size_t q = k + 1;
+        while (q < P.lines.size() && !starts(P.lines[q], "q_")) ++q;
+        auto t = q < P.lines.size() ? tokenize(P.lines[q]) : std::vector<std::string>();
+        if (t.size() != 4 || t[1] != "x") bad(path, "missing charge product after PAIRTYPE PARAMS block " + std::to_string(p));
+        put(m.ncols_short + p, to_double(t[3]), P.lines[k]);
+      }
       ++k;
     }
     // cluster blocks

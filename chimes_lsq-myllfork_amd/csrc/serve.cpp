@@ -180,10 +180,6 @@ cg_status cg_serve_create(const cg_model *m, int device, const double *x, int nw
     return CG_ERR_ARG;
   }
   *out = nullptr;
-  if (m->n_charge_cols > 0) {
-    cg::set_error("cg_serve_create: force evaluation of a model with Ewald charge columns is not supported");
-    return CG_ERR_UNSUPPORTED;
-  }
   auto *sv = new cg_server;
   Server &s = sv->s;
   s.m = m;

@@ -729,6 +729,10 @@ void flatten(Setup &s) {
   m.ncols = s.nshort + m.n_charge_cols + m.n_offset_cols;
   s.ncols = m.ncols;
   m.max_cutoff_2b = s.max_cutoff_2b;
+  s.charge_sign.clear();
+  if (s.fitcoul)
+    for (int t = 0; t < s.ntypes; ++t) s.charge_sign.push_back(s.pairs[t].chrgsgn == "+" ? 1 : -1);
+  m.charge_sign = s.charge_sign.empty() ? nullptr : s.charge_sign.data();
   flatten_family(s.trips, s.n2b, s.f3, m.c3);
   flatten_family(s.quads, s.n2b + s.n3b, s.f4, m.c4);
   pair_types(s.trips, s.f3.pair_type);

@@ -78,6 +78,7 @@ struct Setup {
     double force = 0.0;
   };
   std::vector<ChargeConstraint> charge_cons;
+  std::vector<int> charge_sign;  // FITCOUL: +1 for "+", else -1 (chimes_md.C:1448-1463)
   bool cons_written = false;  // text output: constraint rows already written
   bool fitcoul = false, fit_ener = false, use_partial_charges = false;
   int fit_stress = CG_STRESS_NONE, nstress = -1, nener = -1;

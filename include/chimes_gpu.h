@@ -124,6 +124,8 @@ typedef struct cg_model {
   int ncols;                  /* columns of A (TOT_ALL_PARAMS)             */
   double max_cutoff_2b;       /* NEIGHBORS::MAX_CUTOFF                     */
   cg_cluster_set c3, c4;
+  const int *charge_sign;     /* [ntypes] FITCOUL: +1 for "+" in ATMCHRG,
+                                 else -1 (chimes_md's reading), or NULL    */
 } cg_model;
 
 /* -------------------------------------------------------------- frames */

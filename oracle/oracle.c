@@ -604,6 +604,107 @@ static int ewald_deriv(const cg_model *m, const cg_frame *f, const sys_t *s, dou
   return 0;
 }
 
+/* chimes_md's Ewald forces, energy and virial (ZCalc_Ewald, Ewald_K_Space_New,
+ * src/Ewald.C:28-426) of a model fitted with charges: per-type charges from
+ * the charge products in x (src/chimes_md.C:2044-2078), the mean charge taken
+ * off every atom (check_charges, src/functions.C:996-1050).  Adds to F, *e and
+ * sv (xx yy zz xy xz yz).  Returns -3 for inconsistent pair charges, -4 for
+ * too large a total charge, -2 for a non-orthorhombic cell. */
+static int ewald_md(const cg_model *m, const cg_frame *f, const sys_t *s, const double *x, double *F, double *e,
+                    double *sv) {
+  const double ke = 332.0637157615209; /* src/functions.h:40 */
+  const box_t *b = &s->box;
+  if (!b->ortho) return -2;
+  const int n = s->n, nt = m->ntypes, P = m->ncols_short;
+  double qt[CG_MAX_ATOM_TYPES];
+  for (int j = 0; j < nt; ++j) {
+    double Q = x[P + m->int_pair_map[j * nt + j]] / ke;
+    qt[j] = sqrt(fabs(Q)) * (m->charge_sign ? m->charge_sign[j] : 1);
+  }
+  for (int i = 0; i < m->npairs; ++i)
+    if (fabs(qt[m->pair_t1[i]] * qt[m->pair_t2[i]] - x[P + i] / ke) > 1.0e-4) return -3;
+  double *q = (double *)malloc(sizeof(double) * (n + 1));
+  double total = 0.0;
+  for (int a = 0; a < n; ++a) { q[a] = qt[f->type[a]]; total += q[a]; }
+  if (total > 1.0e-3 * n) { free(q); return -4; }
+  for (int a = 0; a < n; ++a) q[a] -= total / n;
+  double alpha, r_cut;
+  int k_cut;
+  ewald_opt(1.0e-6, n, b, &alpha, &r_cut, &k_cut);
+  const double alphasq = alpha * alpha, SQRT_PI = sqrt(PI_M);
+  const double perm_scale = s->ps[2], pad = 0.3;
+  double *T = (double *)calloc((size_t)3 * n + 3, sizeof(double));
+  double U = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
+  /* real space over the Ewald list (src/Ewald.C:345-403) */
+  for (int a1 = 0; a1 < n; a1++)
+    for (int a2 = 0; a2 < s->nall; a2++) {
+      if (a2 == a1) continue;
+      if (!s->small && a1 > s->parent[a2]) continue;
+      double R[3];
+      double rlen = dist(s, a1, a2, R);
+      if (!(rlen < r_cut + pad)) continue;
+      int fa2 = s->parent[a2];
+      if (rlen < r_cut) {
+        double erfc_val = erfc(alpha * rlen);
+        double tempy = erfc_val / rlen;
+        double tempd = ((-2.0 / SQRT_PI) * exp(-1.0 * alphasq * rlen * rlen) * rlen * alpha - erfc_val) /
+                       (rlen * rlen * rlen);
+        double tempx = -tempd * q[a1] * q[fa2];
+        tempx *= ke;
+        tempx *= perm_scale;
+        tempy *= perm_scale;
+        for (int c = 0; c < 3; ++c) {
+          T[3 * a1 + c] += R[c] * tempx;
+          T[3 * fa2 + c] -= R[c] * tempx;
+        }
+        W[0] += tempx * R[0] * R[0]; W[1] += tempx * R[1] * R[1]; W[2] += tempx * R[2] * R[2];
+        W[3] += tempx * R[0] * R[1]; W[4] += tempx * R[0] * R[2]; W[5] += tempx * R[1] * R[2];
+        U += ke * q[a1] * q[fa2] * tempy;
+      }
+    }
+  /* k space (src/Ewald.C:28-240) */
+  const double Volume = b->ax * b->by * b->cz;
+  double tempk = 0.0;
+  double *sn = (double *)malloc(sizeof(double) * (n + 1)), *cs = (double *)malloc(sizeof(double) * (n + 1));
+  for (int kx = -k_cut; kx <= k_cut; kx++)
+    for (int ky = -k_cut; ky <= k_cut; ky++)
+      for (int kz = -k_cut; kz <= k_cut; kz++) {
+        int ksq = kx * kx + ky * ky + kz * kz;
+        if (ksq == 0 || ksq >= k_cut * k_cut) continue;
+        double K[3] = {(2.0 * PI_M / b->ax) * kx, (2.0 * PI_M / b->by) * ky, (2.0 * PI_M / b->cz) * kz};
+        double rksq = K[0] * K[0] + K[1] * K[1] + K[2] * K[2];
+        double kfac = exp(-rksq / (4.0 * alphasq)) / rksq;
+        double tc = 0.0, ts = 0.0;
+        for (int a = 0; a < n; a++) {
+          double kdotr = f->xyz[3 * a] * K[0] + f->xyz[3 * a + 1] * K[1] + f->xyz[3 * a + 2] * K[2];
+          sn[a] = sin(kdotr);
+          cs[a] = cos(kdotr);
+          tc += q[a] * cs[a];
+          ts += q[a] * sn[a];
+        }
+        double kweight = kfac * (tc * tc + ts * ts);
+        tempk += kweight;
+        double pre = (2 * PI_M * ke / Volume) * kweight;
+        W[0] += pre * (1.0 - 2.0 * K[0] * K[0] / rksq - 0.5 * K[0] * K[0] / alphasq);
+        W[1] += pre * (1.0 - 2.0 * K[1] * K[1] / rksq - 0.5 * K[1] * K[1] / alphasq);
+        W[2] += pre * (1.0 - 2.0 * K[2] * K[2] / rksq - 0.5 * K[2] * K[2] / alphasq);
+        W[3] += pre * (-2.0 * K[0] * K[1] / rksq - 0.5 * K[0] * K[1] / alphasq);
+        W[4] += pre * (-2.0 * K[0] * K[2] / rksq - 0.5 * K[0] * K[2] / alphasq);
+        W[5] += pre * (-2.0 * K[1] * K[2] / rksq - 0.5 * K[1] * K[2] / alphasq);
+        for (int a = 0; a < n; a++) {
+          double tempd = -4.0 * PI_M * kfac * ke * q[a] * (tc * sn[a] - ts * cs[a]) / Volume;
+          for (int c = 0; c < 3; ++c) T[3 * a + c] += K[c] * tempd;
+        }
+      }
+  U += ke * (2 * PI_M / Volume) * tempk;
+  for (int a = 0; a < n; a++) U -= ke * alpha / sqrt(PI_M) * q[a] * q[a];
+  for (int i = 0; i < 3 * n; ++i) F[i] -= T[i];
+  *e += U;
+  for (int c = 0; c < 6; ++c) sv[c] += W[c];
+  free(q); free(T); free(sn); free(cs);
+  return 0;
+}
+
 /* Ewald parameters of a frame (for tests). */
 int orc_ewald_params(int natoms, const double *cell, double *alpha, double *rc, int *kmax) {
   box_t b;
@@ -808,7 +909,12 @@ int orc_force(const cg_model *m, const cg_frame *f, const double *x, double pena
    * src/functions.C:843-852, src/input.C:2097); they are the energy-offset
    * columns of x, after the short-range ones */
   if (m->n_offset_cols > 0)
-    for (int a = 0; a < f->natoms; ++a) e += x[m->ncols_short + f->type[a]];
+    for (int a = 0; a < f->natoms; ++a) e += x[m->ncols_short + m->n_charge_cols + f->type[a]];
+  /* Ewald terms of a model fitted with charges */
+  if (m->n_charge_cols > 0) {
+    int rc = ewald_md(m, f, &s, x, F, &e, sv);
+    if (rc) { free(A.F); free(A.S); free(A.E); sys_free(&s); return rc; }
+  }
   *E = e;
   /* xx xy xz / yx yy yz / zx zy zz */
   vir9[0] = sv[0]; vir9[1] = sv[3]; vir9[2] = sv[4];

@@ -117,7 +117,8 @@ def md_dir(name):
     return os.path.join(GOLDEN, "lsq", "..", "md", name)
 
 
-MD_CASES = ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b", "stress-and-ener-4b2"]
+MD_CASES = ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b", "stress-and-ener-4b2", "h2o-3bcheby",
+            "h2o-4bcheby"]
 
 
 def md_case(ch, name, tmp_path):

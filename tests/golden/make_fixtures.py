@@ -28,8 +28,9 @@ expected outputs), trimmed to the frames the tests need:
                            robin) its b.txt (gzip) and dim.txt; lsq2 (same
                            setup) the row weights and the params.txt of the
                            weighted SVD, eps 1e-4 and ridge 1e-3 fits.
-  md/<test>/               force-evaluation cases of test_suite-md without
-                           Coulomb terms: params.txt (gzip), the first frame
+  md/<test>/               force-evaluation cases of test_suite-md (two of
+                           them with fitted charges: Ewald terms): params.txt
+                           (gzip), the first frame
                            of the run's coordinate file and the reference's
                            forceout.txt (forces, kcal/mol/A, 14 digits).
 """
@@ -47,7 +48,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LSQ = os.path.join(REF, "test_suite-lsq")
 
 MD_TESTS = ["verify-lsq-forces-special3b", "verify-lsq-forces-special4b", "verify-lsq-forces-stress-and-ener-2b1",
-            "verify-lsq-forces-stress-and-ener-4b", "verify-lsq-forces-stress-and-ener-4b2"]
+            "verify-lsq-forces-stress-and-ener-4b", "verify-lsq-forces-stress-and-ener-4b2",
+            "verify-lsq-forces-h2o-3bcheby", "verify-lsq-forces-h2o-4bcheby"]
 
 HEADER_TESTS = ["nonorth2", "test_4atoms", "test_4atoms.2", "fitstrs", "fitstrs3b", "h2o-invr", "special3b",
                 "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2", "stress-and-ener-2b3",

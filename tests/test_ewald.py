@@ -262,3 +262,93 @@ def test_params_printer_weights_and_ridge(ch, tmp_path, fn, alg):
     ch.check(ch.lib().cg_write_params(s.h, str(out).encode(), ch.dptr(x), len(x), neq, ctypes.byref(res), None, None))
     ours = open(out / "params.txt").read().split("\n")[1:]
     assert ours == text.split("\n")[1:]
+
+
+# ------------------------------------------------------- chimes_md's Ewald
+KE = 332.0637157615209
+
+
+def _md_energy(pos, q, H, alpha, rc, kmax):
+    """Ewald energy (kcal/mol) of point charges q (e) at pos in the cell with
+    column vectors H: erfc(a r)/r over minimum-image pairs within rc, the
+    reciprocal sum over n != 0 with |n|^2 < kmax^2 (k = 2 pi H^-T n), minus the
+    self term -- the textbook Ewald sum at fixed alpha, rc and n-set."""
+    n = len(q)
+    Hi = np.linalg.inv(H)
+    E = 0.0
+    for i in range(n):
+        for j in range(i + 1, n):
+            d = pos[j] - pos[i]
+            s = Hi @ d
+            d = H @ (s - np.rint(s))
+            r = np.sqrt(d @ d)
+            if r < rc:
+                E += KE * q[i] * q[j] * math.erfc(alpha * r) / r
+    V = abs(np.linalg.det(H))
+    ns = [(a, b, c) for a in range(-kmax, kmax + 1) for b in range(-kmax, kmax + 1) for c in range(-kmax, kmax + 1)
+          if 0 < a * a + b * b + c * c < kmax * kmax]
+    K = 2 * math.pi * np.array(ns, dtype=float) @ Hi  # rows: k = 2 pi H^-T n
+    k2 = (K ** 2).sum(1)
+    Sk = (q[None, :] * np.exp(1j * (K @ pos.T))).sum(1)
+    E += 2 * math.pi * KE / V * (np.exp(-k2 / (4 * alpha ** 2)) / k2 * np.abs(Sk) ** 2).sum()
+    E -= KE * alpha / math.sqrt(math.pi) * (q ** 2).sum()
+    return E
+
+
+@pytest.mark.parametrize("nO,nH", [(4, 8), (5, 8)])
+def test_md_ewald_energy_forces_virial(ch, synth, oracle, tmp_path, nO, nH):
+    """The oracle's chimes_md Ewald terms (only the charge products of x set):
+    the energy equals the textbook Ewald sum with the same parameters, forces
+    are minus its gradient, and the virial is minus its strain derivative
+    (diagonal and shear components, the sheared cell handled by the same sum).
+    The (5, 8) case has a net charge of -0.8 e: every atom's charge is shifted
+    by the mean, as check_charges does."""
+    rng = np.random.default_rng(3 + nO)
+    L = np.array([7.1, 7.6, 8.3])
+    n = nO + nH
+    pos = []
+    while len(pos) < n:
+        p = rng.uniform(0, 1, 3) * L
+        if all(np.linalg.norm((p - o) - L * np.rint((p - o) / L)) > 1.0 for o in pos):
+            pos.append(p)
+    pos = np.array(pos)
+    types = np.array([0] * nO + [1] * nH, dtype=np.int32)
+    text = synth.fm_setup(["O", "H"], [16.0, 1.0], 4, 0, 0, 0.5, 3.0, 1.4)
+    text = fitcoul_text(text, ["-", "+"], ["OO HH OH 1 1 1 0.0", "OO HH OH 1 -1 0 0.0"])
+    fr = dict(natoms=n, xyz=pos.reshape(-1), types=types, cell=np.diag(L).reshape(-1), forces=np.zeros(3 * n),
+              stress=np.zeros(6), energy=0.0, index=0)
+    s, frs = synth_setup(ch, synth, tmp_path, (text, [fr], ["O", "H"]))
+    m = s.model
+    qO, qH = -0.8, 0.4
+    x = np.zeros(m.ncols)
+    P = m.ncols_short
+    for p in range(m.npairs):
+        qa = qO if m.pair_t1[p] == 0 else qH
+        qb = qO if m.pair_t2[p] == 0 else qH
+        x[P + p] = KE * qa * qb
+    F, E, V = oracle.force(s, frs[0], x)
+    q = np.where(types == 0, qO, qH).astype(float)
+    q -= q.sum() / n
+    alpha, rc, kmax = ch.ewald_params(n, np.diag(L))
+    H = np.diag(L)
+    E0 = _md_energy(pos, q, H, alpha, rc, kmax)
+    assert E == pytest.approx(E0, rel=1e-12)
+    h = 1e-5
+    for k in range(0, 3 * n, 5):
+        pp, pm = pos.copy(), pos.copy()
+        pp.flat[k] += h
+        pm.flat[k] -= h
+        g = -(_md_energy(pp, q, H, alpha, rc, kmax) - _md_energy(pm, q, H, alpha, rc, kmax)) / (2 * h)
+        assert F[k] == pytest.approx(g, rel=1e-6, abs=1e-7 * np.abs(F).max())
+    W = V.reshape(3, 3)
+    for a, b in [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]:
+        eps = np.zeros((3, 3))
+        eps[a, b] += 0.5
+        eps[b, a] += 0.5
+        dE = []
+        for sg in (1, -1):
+            D = np.eye(3) + sg * h * eps
+            dE.append(_md_energy(pos @ D.T, q, D @ H, alpha, rc, kmax))
+        w = -(dE[0] - dE[1]) / (2 * h)
+        assert W[a, b] == pytest.approx(w, rel=1e-6, abs=1e-7 * np.abs(W).max()), (a, b)
+        assert W[a, b] == W[b, a]

@@ -199,3 +199,34 @@ def test_cli_lsq2_weights_eps_ridge(ch, tmp_path, fn, args, xtol):
     d = np.abs(xo - xg).max() / np.abs(xg).max()
     print("max |x - x_ref| / max|x_ref| = %.2e" % d)
     assert d <= xtol
+
+
+@pytest.mark.parametrize("case", ["h2o-3bcheby", "h2o-4bcheby"])
+def test_md_ewald_force_energy_virial_vs_oracle(ch, oracle, tmp_path, case):
+    """Force calls on a model fitted with charges (chimes_md's Ewald terms):
+    forces, energy and virial against the oracle (1e-10), alone and batched
+    with the same configuration shifted and a smaller 48-atom cut (net charge
+    0), repeated calls (no graph capture for these models)."""
+    from conftest import md_case
+    s, f, ref = md_case(ch, case, tmp_path)
+    x, pd, ps = s.coeffs()
+    dev = ch.Device(s)
+    dev.set_penalty(pd, ps)
+    F, E, V = dev.force(x, f)
+    Fo, Eo, Vo = oracle.force(s, f, x, penalty_scale=ps, penalty_dist=pd)
+    assert np.max(np.abs(F - Fo)) <= 1e-10 * np.abs(Fo).max()
+    assert E == pytest.approx(Eo, rel=1e-10)
+    assert np.allclose(V, Vo, rtol=1e-10, atol=1e-10 * np.abs(Vo).max())
+    f2 = ch.PyFrame(f.natoms, f.xyz + 0.31, f.types, f.cell)
+    sel = np.concatenate([np.where(f.types == 0)[0][:16], np.where(f.types == 1)[0][:32]])
+    f3 = ch.PyFrame(48, f.xyz.reshape(-1, 3)[sel].reshape(-1), f.types[sel], np.diag([9.0, 9.5, 10.0]))
+    Fb, Eb, Vb = dev.force_batch(x, [f, f2, f3])
+    for i, fr in enumerate([f, f2, f3]):
+        Fo, Eo, Vo = oracle.force(s, fr, x, penalty_scale=ps, penalty_dist=pd)
+        assert np.max(np.abs(Fb[i] - Fo)) <= 1e-10 * np.abs(Fo).max(), i
+        assert Eb[i] == pytest.approx(Eo, rel=1e-10), i
+        assert np.allclose(Vb[i], Vo, rtol=1e-10, atol=1e-10 * np.abs(Vo).max()), i
+    for _ in range(3):
+        F2, E2, V2 = dev.force(x, f)
+        assert np.max(np.abs(F2 - F)) <= 1e-12 * np.abs(F).max() and E2 == pytest.approx(E, rel=1e-12)
+    dev.close()

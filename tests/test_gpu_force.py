@@ -74,9 +74,11 @@ def test_force_batch_matches_single_calls(ch, synth, oracle, tmp_path):
 
 
 @pytest.mark.parametrize("case", ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b",
-                                  "stress-and-ener-4b2"])
+                                  "stress-and-ener-4b2", "h2o-3bcheby", "h2o-4bcheby"])
 def test_force_matches_reference_forceout(ch, tmp_path, case):
-    """GPU forces with a reference params.txt reproduce chimes_md's forceout.txt."""
+    """GPU forces with a reference params.txt reproduce chimes_md's forceout.txt
+    (the h2o cases with fitted charges: Ewald terms included)."""
+    print("case", case, flush=True)
     from conftest import md_case
     s, f, ref = md_case(ch, case, tmp_path)
     x, pd, ps = s.coeffs()
