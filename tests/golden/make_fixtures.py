@@ -28,6 +28,12 @@ expected outputs), trimmed to the frames the tests need:
                            robin) its b.txt (gzip) and dim.txt; lsq2 (same
                            setup) the row weights and the params.txt of the
                            weighted SVD, eps 1e-4 and ridge 1e-3 fits.
+  lsq/cases/<test>/        end-to-end runs of 19 LSQ tests: fm_setup.in, the
+                           trajectory (gzip, the frames the run reads; the
+                           water cases use lsq/water/input.xyzf.gz), force.txt
+                           (gzip), the sha256 of b.txt, the fitted coefficients
+                           (test_suite_params.txt) and the fit summary of
+                           params.txt (meta.json).
   md/<test>/               force-evaluation cases of test_suite-md (two of
                            them with fitted charges: Ewald terms): params.txt
                            (gzip), the first frame
@@ -55,6 +61,13 @@ HEADER_TESTS = ["nonorth2", "test_4atoms", "test_4atoms.2", "fitstrs", "fitstrs3
                 "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2", "stress-and-ener-2b3",
                 "stress-and-ener-4b", "stress-and-ener-4b2", "two-traj-files", "one-file-three-trajs",
                 "h2o-2bcheby", "h2o-3bcheby", "h2o-3bcheby2", "h2o-3bcheby3", "h2o-4bcheby", "h2o-4bcheby-small"]
+
+# end-to-end cases: chimes_lsq + chimes_lsq.py runs whose inputs are kept
+# (trimmed to the frames the run reads), with their b.txt digest and force.txt
+E2E_TESTS = ["fitstrs", "fitstrs3b", "special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2",
+             "stress-and-ener-2b3", "stress-and-ener-4b", "stress-and-ener-4b2", "one-file-three-trajs",
+             "test_4atoms", "test_4atoms.2", "h2o-invr", "two-traj-files", "h2o-2bcheby", "h2o-3bcheby",
+             "h2o-2bcheby2", "h2o-3bcheby2", "h2o-3bcheby3"]
 
 # FITCOUL (Ewald charge columns) cases; they share one 99-frame water trajectory
 WATER_TESTS = ["h2o-2bcheby", "h2o-4bcheby", "h2o-4bcheby-small", "skip_frames"]
@@ -203,6 +216,42 @@ def main():
     gz(os.path.join(src, "weights.txt"), os.path.join(out, "weights.txt.gz"))
     for fn in ["params.svd.weight.txt", "params.svd.1.0e-04.txt", "params.ridge.1.0e-03.txt"]:
         gz(os.path.join(src, "correct_output", fn), os.path.join(out, fn + ".gz"))
+    # end-to-end cases
+    water_md5 = hashlib.sha256(open(os.path.join(LSQ, "h2o-4bcheby", "input.xyzf"), "rb").read()).hexdigest()
+    for t in E2E_TESTS:
+        src = os.path.join(LSQ, t)
+        out = os.path.join(HERE, "lsq", "cases", t)
+        os.makedirs(out, exist_ok=True)
+        co = os.path.join(src, "correct_output")
+        text = open(setup_of(src)).read()
+        open(os.path.join(out, "fm_setup.in"), "w").write(text)
+        nframes = int(re.search(r"# NFRAMES #[^\n]*\n\s*(\d+)", text).group(1))
+        trj = re.search(r"# TRJFILE #[^\n]*\n\s*(\S+)", text).group(1)
+        meta = {"b_sha256": sha(os.path.join(co, "b.txt"))[0], "b_bytes": sha(os.path.join(co, "b.txt"))[1],
+                "trajectories": [],
+                "x": [float(l.split()[1]) for l in open(os.path.join(co, "test_suite_params.txt")) if l.strip()],
+                "summary": [l for l in open(os.path.join(co, "params.txt")).read().split("\n")[1:13]
+                            if l.startswith("!")]}
+        if trj == "MULTI":
+            lst = re.search(r"# TRJFILE #[^\n]*\n\s*MULTI\s+(\S+)", text).group(1)
+            open(os.path.join(out, lst), "w").write(open(os.path.join(src, lst)).read())
+            files = [l.split()[1] for l in open(os.path.join(src, lst)).read().split("\n")[1:] if l.strip()]
+        else:
+            files = [trj]
+        for fn in files:
+            path = os.path.join(src, fn)
+            if hashlib.sha256(open(path, "rb").read()).hexdigest() == water_md5:
+                meta["trajectories"].append({"name": fn, "water": True})
+                continue
+            frs = frames_of(path)
+            keep = frs[:nframes] if trj != "MULTI" else frs
+            tmp = os.path.join(out, fn + ".tmp")
+            open(tmp, "w").write("".join(keep))
+            gz(tmp, os.path.join(out, fn + ".gz"))
+            os.remove(tmp)
+            meta["trajectories"].append({"name": fn, "water": False})
+        gz(os.path.join(co, "force.txt"), os.path.join(out, "force.txt.gz"))
+        json.dump(meta, open(os.path.join(out, "meta.json"), "w"), indent=1, sort_keys=True)
     # force-evaluation cases: params.txt + first coordinate frame + forceout.txt
     md = os.path.join(REF, "test_suite-md")
     for t in MD_TESTS:
