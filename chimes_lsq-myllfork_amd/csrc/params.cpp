@@ -103,7 +103,14 @@ Setup *read_params(const std::string &path) {
   for (size_t k = i + 1; (int)pairs.size() < np && k < P.lines.size(); ++k) {
     auto t = tokenize(P.lines[k]);
     if (t.empty()) continue;
-    if (t.size() < 7) bad(path, "bad atom pair line: " + P.lines[k]);
+    if (t.size() < 6) bad(path, "bad atom pair line: " + P.lines[k]);
+    // older files carry an S_DELTA column before CHBDIST: a number where the
+    // transform name is expected is skipped (chimes_md.C:1846-1858)
+    char *end = nullptr;
+    std::strtod(t[5].c_str(), &end);
+    if (end && *end == '\0' && !t[5].empty()) t.erase(t.begin() + 5);
+    if (t.size() < 6 || (t[5] == "MORSE" && t.size() < 7)) bad(path, "bad atom pair line: " + P.lines[k]);
+    if (t.size() < 7) t.push_back("0");
     pairs.push_back(t);
   }
   if ((int)pairs.size() != np) bad(path, "truncated PAIRIDX table");

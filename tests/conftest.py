@@ -118,7 +118,7 @@ def md_dir(name):
 
 
 MD_CASES = ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b", "stress-and-ener-4b2", "h2o-3bcheby",
-            "h2o-4bcheby"]
+            "h2o-4bcheby", "run-ti-nonortho", "run-ti-ortho"]
 
 
 def md_case(ch, name, tmp_path):
@@ -132,10 +132,12 @@ def md_case(ch, name, tmp_path):
     names = s.type_names()
     lines = open(os.path.join(d, "frame.xyzf")).read().split("\n")
     n = int(lines[0].split()[0])
-    box = [float(v) for v in lines[1].split()[:3]]
+    t = lines[1].split()
+    # box line: three lengths, or NON_ORTHO and the cell vectors a, b, c
+    cell = np.array([float(v) for v in t[1:10]]) if t[0] == "NON_ORTHO" else np.diag([float(v) for v in t[:3]])
     xyz = np.array([[float(v) for v in lines[2 + i].split()[1:4]] for i in range(n)])
     typ = np.array([names.index(lines[2 + i].split()[0]) for i in range(n)], dtype=np.int32)
-    f = ch.PyFrame(n, xyz, typ, np.diag(box))
+    f = ch.PyFrame(n, xyz, typ, cell)
     ref = np.array([float(v) for v in open(os.path.join(d, "forceout.txt")).read().split()])
     assert ref.size == 3 * n
     return s, f, ref
@@ -173,3 +175,19 @@ def fitcoul_text(text, signs, constraints):
     e = lines.index("# FCUTTYP #")
     lines[e:e] = ["CHARGE CONSTRAINTS:"] + constraints + [""]
     return "\n".join(lines)
+
+
+def check_md_printout(meta, natoms, cell, E, V):
+    """Energy, pressure and stress a one-step chimes_md run printed for its
+    initial configuration at zero temperature: Vtot/N to its 7 decimals, P
+    (GPa, with the run's kinetic term of Ktot/N at most 3e-7 kcal/mol) within
+    2e-7 GPa, the diagonal stresses to their 4 decimals."""
+    if "vtot_per_atom" not in meta:
+        return
+    assert round(E / natoms, 7) == pytest.approx(meta["vtot_per_atom"], abs=1e-12)
+    gpa = 6.9476955  # kcal/mol/A^3 per GPa, as the stress rows of A
+    vol = abs(np.linalg.det(np.asarray(cell).reshape(3, 3)))
+    W = np.asarray(V).reshape(3, 3)
+    assert np.trace(W) / (3 * vol) * gpa == pytest.approx(meta["pressure_gpa"], abs=2e-7)
+    for k, a in (("xx", 0), ("yy", 1), ("zz", 2)):
+        assert round(W[a, a] / vol * gpa, 4) == pytest.approx(meta["sigma_gpa"][k], abs=1e-9)

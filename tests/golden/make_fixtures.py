@@ -62,6 +62,13 @@ HEADER_TESTS = ["nonorth2", "test_4atoms", "test_4atoms.2", "fitstrs", "fitstrs3
                 "stress-and-ener-4b", "stress-and-ener-4b2", "two-traj-files", "one-file-three-trajs",
                 "h2o-2bcheby", "h2o-3bcheby", "h2o-3bcheby2", "h2o-3bcheby3", "h2o-4bcheby", "h2o-4bcheby-small"]
 
+# chimes_md runs of one step whose forceout.txt holds the forces of the
+# initial configuration, with the run's parameter file (older layout with an
+# S_DELTA column, Tersoff cutoffs; one triclinic cell).  Runs of many steps
+# print forces after moving the atoms; carbon-penalty's file lists power sets
+# of an older cluster enumeration.
+MD_RUNS = ["ti-nonortho", "ti-ortho"]
+
 # end-to-end cases: chimes_lsq + chimes_lsq.py runs whose inputs are kept
 # (trimmed to the frames the run reads), with their b.txt digest and force.txt
 E2E_TESTS = ["fitstrs", "fitstrs3b", "special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-2b2",
@@ -284,6 +291,27 @@ def main():
                     meta["forcetxt_energy"] = y[k]
                     break
         json.dump(meta, open(os.path.join(out, "expected.json"), "w"), indent=1, sort_keys=True)
+    for t in MD_RUNS:
+        src = os.path.join(md, t)
+        out = os.path.join(HERE, "md", "run-" + t)
+        os.makedirs(out, exist_ok=True)
+        run = open(os.path.join(src, "run_md.in")).read()
+        crd = re.search(r"# CRDFILE #[^\n]*\n\s*(\S+)", run).group(1)
+        prm = re.search(r"# PRMFILE #[^\n]*\n\s*(\S+)", run).group(1)
+        first = frames_of(os.path.join(src, crd))[0]
+        open(os.path.join(out, "frame.xyzf"), "w").write(first)
+        gz(os.path.join(src, prm), os.path.join(out, "params.txt.gz"))
+        n = int(first.split()[0])
+        vals = open(os.path.join(src, "correct_output", "forceout.txt")).read().split("\n")
+        open(os.path.join(out, "forceout.txt"), "w").write("\n".join(vals[:3 * n]) + "\n")
+        # the step line printed at time 0 (zero temperature: potential terms only)
+        # and the stress tensor the run printed
+        log = open(os.path.join(src, "correct_output", "run_md.out")).read()
+        st = re.search(r"\n\s+1\s+0\.00\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)", log)
+        sig = {k: float(v) for k, v in re.findall(r"sigma_(\w\w): (\S+) GPa", log)}
+        json.dump({"abs_energy_error_per_atom": None, "ktot_per_atom": float(st.group(1)),
+                   "vtot_per_atom": float(st.group(2)), "pressure_gpa": float(st.group(5)), "sigma_gpa": sig},
+                  open(os.path.join(out, "expected.json"), "w"), indent=1, sort_keys=True)
     print("fixtures written to", HERE)
 
 

@@ -74,7 +74,8 @@ def test_force_batch_matches_single_calls(ch, synth, oracle, tmp_path):
 
 
 @pytest.mark.parametrize("case", ["special3b", "special4b", "stress-and-ener-2b1", "stress-and-ener-4b",
-                                  "stress-and-ener-4b2", "h2o-3bcheby", "h2o-4bcheby"])
+                                  "stress-and-ener-4b2", "h2o-3bcheby", "h2o-4bcheby", "run-ti-nonortho",
+                                  "run-ti-ortho"])
 def test_force_matches_reference_forceout(ch, tmp_path, case):
     """GPU forces with a reference params.txt reproduce chimes_md's forceout.txt
     (the h2o cases with fitted charges: Ewald terms included)."""
@@ -86,7 +87,7 @@ def test_force_matches_reference_forceout(ch, tmp_path, case):
     dev.set_penalty(pd, ps)
     F, E, V = dev.force(x, f)
     dev.close()
-    assert np.max(np.abs(F - ref)) <= 1e-10 * np.abs(ref).max()
+    assert np.max(np.abs(F - ref)) <= 1e-10 * max(np.abs(ref).max(), 1.0)
     # energy incl. per-atom offsets, as the reference run checked it: |E - E_cmp| / N
     # to 6 decimals, E_cmp = energy row of its force.txt
     import json
@@ -96,6 +97,8 @@ def test_force_matches_reference_forceout(ch, tmp_path, case):
     if meta["abs_energy_error_per_atom"] is not None:
         assert round(abs(E - meta["forcetxt_energy"]) / f.natoms, 6) == pytest.approx(
             meta["abs_energy_error_per_atom"], abs=1e-12)
+    from conftest import check_md_printout
+    check_md_printout(meta, f.natoms, f.cell, E, V)
 
 
 def test_force_list_capacity_overflow_repeats_exactly(ch, synth, oracle, tmp_path):

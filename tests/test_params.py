@@ -29,7 +29,9 @@ def test_params_reader_oracle_forces(ch, oracle, tmp_path, case):
     s, f, ref = md_case(ch, case, tmp_path)
     x, pd, ps = s.coeffs()
     F, E, V = oracle.force(s, f, x, penalty_scale=ps, penalty_dist=pd)
-    assert np.max(np.abs(F - ref)) <= 1e-10 * np.abs(ref).max()
+    # relative to the largest force, floor 1 kcal/mol/A (the TiO2 crystals'
+    # net forces cancel to ~1e-7)
+    assert np.max(np.abs(F - ref)) <= 1e-10 * max(np.abs(ref).max(), 1.0)
     # energy (with the per-atom offsets): the reference run printed
     # |E - E_cmp| / N to 6 decimals, E_cmp being the energy row of its force.txt
     import json
@@ -38,6 +40,8 @@ def test_params_reader_oracle_forces(ch, oracle, tmp_path, case):
     if meta["abs_energy_error_per_atom"] is not None:
         err = abs(E - meta["forcetxt_energy"]) / f.natoms
         assert round(err, 6) == pytest.approx(meta["abs_energy_error_per_atom"], abs=1e-12)
+    from conftest import check_md_printout
+    check_md_printout(meta, f.natoms, f.cell, E, V)
 
 
 def test_params_reader_refuses_coulomb(ch, tmp_path):
