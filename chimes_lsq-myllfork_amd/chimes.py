@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libchimes_gpu.so")
 CG_FP64, CG_FP32 = 0, 1
 CG_GRAM_FP64, CG_GRAM_BF16X3 = 0, 1
 CG_FIT_QR = 2
-CG_ALG_SVD, CG_ALG_RIDGE = 0, 1
+CG_ALG_SVD, CG_ALG_RIDGE, CG_ALG_LASSOLARS = 0, 1, 2
 STATUS = {0: "CG_OK", 1: "CG_ERR_ARG", 2: "CG_ERR_INPUT", 3: "CG_ERR_UNSUPPORTED",
           4: "CG_ERR_DEVICE", 5: "CG_ERR_MEMORY", 6: "CG_ERR_NUMERIC", 7: "CG_ERR_EOF"}
 
@@ -116,6 +116,7 @@ def lib(path=None):
         opt("cg_set_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.c_int])
         opt("cg_set_fit_weights", [C.c_void_p, _dp, C.c_int64])
         opt("cg_fit_ridge", [C.c_void_p, _dp, C.c_double, _dp, C.POINTER(FitResult)])
+        opt("cg_fit_lassolars", [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)])
         opt("cg_setup_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
         opt("cg_autotune", [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.POINTER(C.c_int), _dp])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
@@ -365,6 +366,13 @@ class Device:
         res = FitResult()
         b = np.ascontiguousarray(b, dtype=np.float64)
         check(self.L.cg_fit_ridge(self.h, dptr(b), alpha, dptr(x), C.byref(res)), self.h)
+        return x, res
+
+    def fit_lassolars(self, b, alpha, max_iter=100000):
+        x = np.zeros(self.setup.model.ncols)
+        res = FitResult()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        check(self.L.cg_fit_lassolars(self.h, dptr(b), alpha, max_iter, dptr(x), C.byref(res)), self.h)
         return x, res
 
     def gram(self, b=None, mode=CG_GRAM_FP64):

@@ -2535,7 +2535,7 @@ void ctx_free(Ctx &c) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
                   &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1], &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
-                  &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw})
+                  &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars})
     b->release();
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,

@@ -2,7 +2,7 @@
 // with the design matrix assembled on the GPU, optionally followed by the fit
 // that `chimes_lsq.py` performs (src/chimes_lsq.py:169-672).
 //
-//   chimes_lsq_gpu fm_setup.in [--fit] [--algorithm svd|ridge] [--eps E] [--alpha A] [--weights FILE]
+//   chimes_lsq_gpu fm_setup.in [--fit] [--algorithm svd|ridge|lassolars] [--eps E] [--alpha A] [--weights FILE]
 //                  [--prec fp64|fp32] [--gram qr|fp64|bf16x3]
 //                  [--refine K] [--batch FRAMES] [--ranks N] [--no-A] [--no-text]
 //
@@ -36,7 +36,7 @@ int die(const char *what, cg_status st, const void *ctx = nullptr) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--algorithm svd|ridge] [--eps E] [--alpha A] [--weights FILE] "
+    std::fprintf(stderr, "Usage: chimes_lsq_gpu input_file [--fit] [--algorithm svd|ridge|lassolars] [--eps E] [--alpha A] [--weights FILE] "
                          "[--prec fp64|fp32] [--gram qr|fp64|bf16x3] [--refine K] "
                          "[--batch FRAMES] [--ranks N] [--no-A] [--no-text]\n");
     return 1;
@@ -173,8 +173,14 @@ int main(int argc, char **argv) {
       if ((st = cg_set_fit_weights(ctx, w.data(), (int64_t)w.size()))) return die("weights", st, ctx);
       gram = CG_FIT_QR;
     }
+    if (!weights.empty() && algorithm == "lassolars") {
+      std::fprintf(stderr, "--weights with --algorithm lassolars is not supported\n");
+      return 1;
+    }
     if (algorithm == "ridge") {
       if ((st = cg_fit_ridge(ctx, b_all.data(), alpha, x.data(), &res))) return die("fitting", st, ctx);
+    } else if (algorithm == "lassolars") {
+      if ((st = cg_fit_lassolars(ctx, b_all.data(), alpha, 100000, x.data(), &res))) return die("fitting", st, ctx);
     } else if (algorithm == "svd") {
       if ((st = cg_fit(ctx, b_all.data(), eps, refine, gram, x.data(), &res))) return die("fitting", st, ctx);
     } else {

@@ -125,6 +125,7 @@ struct Ctx {
   DBuf b_fitkeep;                         // kept columns of a fit (device copy)
   DBuf b_qr;                              // QR fit: FP64 copy of A, factors, SVD
   DBuf b_fitw;                            // row weights of a fit (device copy)
+  DBuf b_lars;                            // LassoLars: Cholesky factor, coefficients
   std::vector<double> fit_w;              // row weights (empty: none)
   std::vector<int> fit_keep;              // columns a fit keeps (empty: all)
   DBuf b_force;                           // force evaluation: F, energy/virial, call totals
@@ -169,6 +170,10 @@ void ctx_init(Ctx &c, const cg_model *m, int device);
 void ctx_free(Ctx &c);
 void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F, double *E,
                  double *vir);
+
+// Defined in fit.hip / lars.hip.
+void matvec(Ctx &c, const double *x_host, double *y_host);
+void gram_and_atb(Ctx &c, const double *b_host, double **G, double **Atb);
 
 // Defined in ewald.hip: Ewald parameters of a frame (optimal_ewald_params) and
 // the FITCOUL charge columns of an assembled batch.

@@ -668,6 +668,27 @@ void fit_ridge(Ctx &c, const double *b_host, double alpha, double *x_host, cg_fi
   res->eps_abs = 0.0;
 }
 
+// G = A^T A (FP64 Gram) and A^T b of the last assembled A, left on the device
+// in the fit workspace (for the LassoLars solver).
+void gram_and_atb(Ctx &c, const double *b_host, double **G_out, double **Atb_out) {
+  if (!c.A) throw Error(CG_ERR_ARG, "nothing assembled");
+  const long long m = c.A_rows;
+  const int n = c.A_ld;
+  long long S, kchunk;
+  gram_plan(m, n, S, kchunk);
+  double *G = c.b_x.as<double>((size_t)n * n + (size_t)S * n * n + (size_t)m + (size_t)n + 64);
+  if (!G) throw Error(CG_ERR_MEMORY, "Gram buffers");
+  double *part = G + (size_t)n * n;
+  double *d_b = part + (size_t)S * n * n;
+  double *d_c = d_b + m;
+  gram(c, CG_GRAM_FP64, G, part, S, kchunk);
+  CG_HIP(hipMemcpyAsync(d_b, b_host, m * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (c.A_prec == CG_FP64) atv(c, (const double *)c.A, m, n, d_b, d_c);
+  else atv(c, (const float *)c.A, m, n, d_b, d_c);
+  *G_out = G;
+  *Atb_out = d_c;
+}
+
 void matvec(Ctx &c, const double *x_host, double *y_host) {
   const long long m = c.A_rows;
   const int n = c.A_ld;

@@ -673,6 +673,9 @@ std::string params_text(const Setup &s, const double *x, int nx, long long nline
   if (r.algorithm == CG_ALG_RIDGE) {  // src/chimes_lsq.py:216-225
     P("! ridge regression used");
     P(fx("! Ridge alpha = %11.4e", r.alpha));
+  } else if (r.algorithm == CG_ALG_LASSOLARS) {  // src/chimes_lsq.py:248-253
+    P("! LARS implementation of LASSO used");
+    P(fx("! LASSO alpha = %11.4e", r.alpha));
   } else {
     P("! svd algorithm used");
     P(fx("! eps (= args.eps*dmax)          =  %11.4e", r.eps_abs));

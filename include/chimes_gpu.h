@@ -286,7 +286,7 @@ typedef struct cg_fit_result {
   double alpha;         /* ridge alpha                                     */
   char weights_file[256]; /* row-weight file named in params.txt ("" none) */
 } cg_fit_result;
-enum { CG_ALG_SVD = 0, CG_ALG_RIDGE = 1 };
+enum { CG_ALG_SVD = 0, CG_ALG_RIDGE = 1, CG_ALG_LASSOLARS = 2 };
 cg_status cg_fit(cg_ctx *ctx, const double *b_host, double eps, int refine_iters, int gram_mode,
                  double *x, cg_fit_result *res);
 /* Fit only the columns keep[0..nkeep) (increasing column indices) and hold the
@@ -305,6 +305,15 @@ cg_status cg_set_fit_weights(cg_ctx *ctx, const double *w, int64_t nrows);
  * x = argmin |A x - b|^2 + alpha |x|^2, from the SVD of A (LQ factor, as
  * CG_FIT_QR): x = sum_i s_i / (s_i^2 + alpha) (u_i . b) v_i. */
 cg_status cg_fit_ridge(cg_ctx *ctx, const double *b_host, double alpha, double *x, cg_fit_result *res);
+/* LASSO by least-angle regression (chimes_lsq.py --algorithm=lassolars:
+ * scikit-learn LassoLars(alpha, fit_intercept=False, fit_path=False,
+ * max_iter), src/chimes_lsq.py:248-264): minimises
+ * |A x - b|^2 / (2 rows) + alpha |x|_1 along the LARS-lasso path on the
+ * device (Gram form: rows > columns, at most 2048 columns; no column subsets
+ * or row weights).  res->nvars counts |x_i| > 1e-5 as the script does;
+ * res->iters the path steps. */
+cg_status cg_fit_lassolars(cg_ctx *ctx, const double *b_host, double alpha, int max_iter, double *x,
+                           cg_fit_result *res);
 /* Columns of A that a fit keeps under the setup's hierarchical exclusions
  * (EXCLUDE 1B/2B INTERACTION with HIERARC true; src/input.C:760-840): keep
  * [ncols] receives them in increasing order, nkeep their number. */

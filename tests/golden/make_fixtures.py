@@ -27,7 +27,8 @@ expected outputs), trimmed to the frames the tests need:
                            its b and force.txt; skip_frames (36 ranks, round
                            robin) its b.txt (gzip) and dim.txt; lsq2 (same
                            setup) the row weights and the params.txt of the
-                           weighted SVD, eps 1e-4 and ridge 1e-3 fits.
+                           weighted SVD, eps 1e-4, ridge 1e-3 and LassoLars
+                           1e-4 fits.
   lsq/cases/<test>/        end-to-end runs of 19 LSQ tests: fm_setup.in, the
                            trajectory (gzip, the frames the run reads; the
                            water cases use lsq/water/input.xyzf.gz), force.txt
@@ -221,7 +222,8 @@ def main():
     out = os.path.join(wat, "lsq2")
     os.makedirs(out, exist_ok=True)
     gz(os.path.join(src, "weights.txt"), os.path.join(out, "weights.txt.gz"))
-    for fn in ["params.svd.weight.txt", "params.svd.1.0e-04.txt", "params.ridge.1.0e-03.txt"]:
+    for fn in ["params.svd.weight.txt", "params.svd.1.0e-04.txt", "params.ridge.1.0e-03.txt",
+               "params.lassolars.1.0e-04.txt"]:
         gz(os.path.join(src, "correct_output", fn), os.path.join(out, fn + ".gz"))
     # end-to-end cases
     water_md5 = hashlib.sha256(open(os.path.join(LSQ, "h2o-4bcheby", "input.xyzf"), "rb").read()).hexdigest()

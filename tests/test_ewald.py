@@ -234,7 +234,8 @@ def _summary(text):
 
 
 @pytest.mark.parametrize("fn,alg", [("params.svd.weight.txt", "svd"), ("params.svd.1.0e-04.txt", "svd"),
-                                    ("params.ridge.1.0e-03.txt", "ridge")])
+                                    ("params.ridge.1.0e-03.txt", "ridge"),
+                                    ("params.lassolars.1.0e-04.txt", "lassolars")])
 def test_params_printer_weights_and_ridge(ch, tmp_path, fn, alg):
     """params.txt of chimes_lsq.py's weighted SVD, eps 1e-4 SVD and ridge runs
     (test_suite-lsq/lsq2) byte for byte from their coefficients: the ridge
@@ -251,6 +252,9 @@ def test_params_printer_weights_and_ridge(ch, tmp_path, fn, alg):
     if alg == "ridge":
         res.algorithm = ch.CG_ALG_RIDGE
         res.alpha = float(v["Ridge alpha"])
+    elif alg == "lassolars":
+        res.algorithm = ch.CG_ALG_LASSOLARS
+        res.alpha = float(v["LASSO alpha"])
     else:
         res.eps_abs = float(v["eps (= args.eps*dmax)"])
         res.eps = float(v["SVD regularization factor"])

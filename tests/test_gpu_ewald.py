@@ -167,7 +167,9 @@ def test_cli_h2o_4bcheby_end_to_end(ch, tmp_path):
     ("params.svd.1.0e-04.txt", ["--eps", "1.0e-04"], 1e-8),
     # sklearn's Ridge solves (A^T A + alpha I) x = A^T b by Cholesky: relative
     # error up to ~1e-16 s_max^2 / alpha = 4e-4 here (s_max 6.5e4, alpha 1e-3)
-    ("params.ridge.1.0e-03.txt", ["--algorithm", "ridge", "--alpha", "1.0e-03"], 1e-3)])
+    ("params.ridge.1.0e-03.txt", ["--algorithm", "ridge", "--alpha", "1.0e-03"], 1e-3),
+    # a LARS path of ~560 steps; scikit-learn 1.7.2 on the oracle's A is within 2.3e-7
+    ("params.lassolars.1.0e-04.txt", ["--algorithm", "lassolars", "--alpha", "1.0e-04"], 1e-5)])
 def test_cli_lsq2_weights_eps_ridge(ch, tmp_path, fn, args, xtol):
     """chimes_lsq.py's weighted SVD, eps 1e-4 SVD and ridge runs of the
     reference's lsq2 case (h2o-4bcheby setup) through the drop-in CLI: the
