@@ -713,6 +713,7 @@ struct AsmArgs {
   const int *col_rel;       // [ntypes] family-relative first column of a type
   const int *col_abs;       // [ntypes] absolute first column of a type
   const double *smin, *smax, *xavg, *xdiff, *lambda;  // [ntypes*NP]
+  const double *recip;                                 // [ntypes*NP*3]: 1/lambda, 1/xdiff, 1/smax
   void *A;
   long long ld;
   int transform, fix_type, fcut_type, stress_mode;
@@ -768,19 +769,22 @@ template <int VW, bool SPLIT> struct PairTab<float, VW, SPLIT> {
 // with set_polys / set_polys_out_of_range / FCUT::get_fcut semantics.
 template <class T, int VW, int BASE, class Tab>
 __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double smin, double smax, double xavg,
-                                           double xdiff, double lambda, Tab &P) {
+                                           const double *recip, Tab &P) {
+  // recip: 1/lambda, 1/xdiff, 1/smax of the pair type (host-computed: the
+  // table is rebuilt for every column tile, and FP64 divisions cost ~10 VALU)
+  const double ilambda = recip[0], ixdiff = recip[1], ismax = recip[2];
   double re = r < smin ? smin : r;
   double x, ex = 0.0, dxdr;
   if (a.transform == CG_MORSE) {
-    ex = exp(-re / lambda);
-    x = (ex - xavg) / xdiff;
-    dxdr = (-ex / lambda) / xdiff;
+    ex = exp(-re * ilambda);
+    x = (ex - xavg) * ixdiff;
+    dxdr = (-ex * ilambda) * ixdiff;
   } else if (a.transform == CG_INVRSE_R) {
-    x = (1.0 / re - xavg) / xdiff;
-    dxdr = -1.0 / (re * re * xdiff);
+    x = (1.0 / re - xavg) * ixdiff;
+    dxdr = -ixdiff / (re * re);
   } else {
-    x = (re - xavg) / xdiff;
-    dxdr = 1.0 / xdiff;
+    x = (re - xavg) * ixdiff;
+    dxdr = ixdiff;
   }
   dxdr *= a.deriv_const;
   double damp = 1.0, shift = 0.0, dscale = 1.0;
@@ -796,9 +800,9 @@ __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double sm
   }
   double fc, fcd;
   if (a.fcut_type == CG_FCUT_CUBIC) {
-    double f0 = 1.0 - r / smax;
+    double f0 = 1.0 - r * ismax;
     fc = f0 * f0 * f0;
-    fcd = f0 * f0 * (-3.0 / smax);
+    fcd = f0 * f0 * (-3.0 * ismax);
   } else {
     double th = smax - a.fcut_offset * smax;
     if (r < th) { fc = 1.0; fcd = 0.0; }
@@ -1131,12 +1135,12 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           const int q = ty * NP + k;
           if constexpr (ASM4) {
             Tab4Pair tp{tab4, k};
-            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
+            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
           } else if constexpr (ASM3) {
             Tab3Pair tp{tab3, k};
-            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tp);
+            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
           } else {
-            pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
+            pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tab[k]);
           }
           ++k;
         }
@@ -1261,6 +1265,7 @@ struct ForceArgs {
   const double *wrap;
   const uint32_t *refs;
   const double *smin, *smax, *xavg, *xdiff, *lambda;
+  const double *recip;  // [ntypes*NP*3]: 1/lambda, 1/xdiff, 1/smax
   const double *coef;    // [ntypes * tsize] coefficient tensors
   int tsize;             // N^NP
   double *F;             // [batch atoms * 3]
@@ -1498,9 +1503,10 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a) {
         for (int y = x + 1; y < NB; ++y) {
           double v0 = p[y][0] - p[x][0], v1 = p[y][1] - p[x][1], v2 = p[y][2] - p[x][2];
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-          rh[k][0] = v0 / r; rh[k][1] = v1 / r; rh[k][2] = v2 / r; rl[k] = r;
+          const double ir = 1.0 / r;
+          rh[k][0] = v0 * ir; rh[k][1] = v1 * ir; rh[k][2] = v2 * ir; rl[k] = r;
           const int q = ty * NP + k;
-          pair_table<double, N, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.xdiff[q], a.lambda[q], tab[k]);
+          pair_table<double, N, BASE>(aa, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tab[k]);
           ++k;
         }
     }
@@ -1709,6 +1715,13 @@ static void build_family(Ctx &c, int which) {
   }
   F.smin = dalloc_copy(smin); F.smax = dalloc_copy(smax);
   F.xavg = dalloc_copy(xavg); F.xdiff = dalloc_copy(xdiff); F.lambda = dalloc_copy(lam);
+  std::vector<double> recip(3 * smin.size());
+  for (size_t q = 0; q < smin.size(); ++q) {
+    recip[3 * q] = 1.0 / lam[q];
+    recip[3 * q + 1] = 1.0 / xdiff[q];
+    recip[3 * q + 2] = 1.0 / smax[q];
+  }
+  F.recip = dalloc_copy(recip);
   std::vector<int> both(col_rel);
   both.insert(both.end(), col_abs.begin(), col_abs.end());
   F.col_offset = dalloc_copy(both);  // [0,ntypes): relative, [ntypes,2*ntypes): absolute
@@ -1738,7 +1751,7 @@ static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, con
   a.colptr = F.colptr;
   a.col_rel = F.col_offset;
   a.col_abs = F.col_offset + F.ntypes;
-  a.smin = F.smin; a.smax = F.smax; a.xavg = F.xavg; a.xdiff = F.xdiff; a.lambda = F.lambda;
+  a.smin = F.smin; a.smax = F.smax; a.xavg = F.xavg; a.xdiff = F.xdiff; a.lambda = F.lambda; a.recip = F.recip;
   a.A = A;
   a.ld = ld;
   a.transform = c.m.transform; a.fix_type = c.m.fix_type; a.fcut_type = c.m.fcut_type;
@@ -2333,6 +2346,7 @@ static void force_device(Ctx &c, const Batch &B, double *d_F, size_t nout) {
     a.tasks = d_tasks; a.ntasks = d_toff + nkeys;
     a.frames = B.d_frames; a.wrap = B.d_wrap; a.refs = fo.refs;
     a.smin = Fm.smin; a.smax = Fm.smax; a.xavg = Fm.xavg; a.xdiff = Fm.xdiff; a.lambda = Fm.lambda;
+    a.recip = Fm.recip;
     a.coef = c.b_coef[w].as<double>(0); a.tsize = c.coef_size[w];
     a.F = d_F; a.EV = d_EV;
     a.transform = m.transform; a.fix_type = m.fix_type; a.fcut_type = m.fcut_type;
@@ -2539,7 +2553,7 @@ void ctx_free(Ctx &c) {
     b->release();
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
-                    (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.cct, (void *)F.csmax,
+                    (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.recip, (void *)F.cct, (void *)F.csmax,
                     (void *)F.cperm, (void *)F.sets8})
       if (p) (void)hipFree(p);
   }

@@ -96,6 +96,7 @@ struct DevFamily {
   uint32_t *perm = nullptr;   // [nperm] packed powers, 4 bits per canonical pair
   int *sets8 = nullptr;       // set records (2 * power per pair): 4-body powers < 5 [nperm + 2][8], 3-body powers < 9 [nperm + 2][4]
   double *smin = nullptr, *smax = nullptr, *xavg = nullptr, *xdiff = nullptr, *lambda = nullptr; // [ntypes*np]
+  double *recip = nullptr;  // [ntypes*np*3]: 1/lambda, 1/xdiff, 1/smax (assembly tables)
   int *cct = nullptr;         // [ntypes_atom^nb] cluster type of a type tuple, -1 if none
   double *csmax = nullptr;    // [ntypes_atom^nb * np] S_MAXIM of each enumerated pair slot
   int *cperm = nullptr;       // [ntypes_atom^nb * nb] canonical slot of each enumerated atom
