@@ -765,6 +765,16 @@ template <int VW, bool SPLIT> struct PairTab<float, VW, SPLIT> {
   __device__ __forceinline__ void get(unsigned i, float &f, float &d) const { f = v[2 * i]; d = v[2 * i + 1]; }
 };
 
+// 1/r for r > 0 from the hardware reciprocal estimate and two Newton steps
+// y <- y + y (1 - r y) (each squares the relative error, so the estimate's
+// ~2^-26 ends below one ulp), in 5 VALU instead of a correctly rounded
+// division's ~12.
+__device__ __forceinline__ double recip_nr(double r) {
+  double y = __builtin_amdgcn_rcp(r);
+  y = fma(y, fma(-r, y, 1.0), y);
+  return fma(y, fma(-r, y, 1.0), y);
+}
+
 // Chebyshev tables of one pair: F[i] = fc*T_{i+BASE}(x), D[i] = d/dr of that,
 // with set_polys / set_polys_out_of_range / FCUT::get_fcut semantics.
 template <class T, int VW, int BASE, class Tab>
@@ -1070,9 +1080,12 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   const DevFrame &f = a.frames[t.frame];
   const int nf = 3 * f.natoms;
   const int W = t.ncols;
-  // tile layout: row r (3 rows per atom, then 7 * SE_REP stress/energy
-  // replica rows) at acc[r * Wp + column]; the odd stride Wp spreads the rows
-  // of different atoms over distinct LDS banks
+  // tile layout: force row r (3 rows per atom) at acc[r * Wp + column], the
+  // odd stride Wp spreading the rows of different atoms over distinct LDS
+  // banks; then SE_REP replicas of the 6 stress and 1 energy sums, replica q
+  // of column c holding its 7 components at se[(q * Wp + c) * 7 + comp]
+  // (component offsets are instruction immediates, and the odd replica
+  // stride 7 * Wp keeps the 16 replicas on distinct banks)
   const int Wp = tile_stride(W);
   const int nacc = (nf + 7 * SE_REP) * Wp;
   for (int i = threadIdx.x; i < nacc; i += BS) acc[i] = 0.0;
@@ -1092,7 +1105,10 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   for (int rnd = 0; rnd < R; ++rnd) {
     const int j = threadIdx.x * R + rnd;
     const bool valid = j < ncl;
-    const int ci = t.c0 + j;
+    // lanes past the bucket's end build the tables of its last cluster (all
+    // registers then hold defined values without a clearing pass) and drop
+    // their sums at the projection
+    const int ci = t.c0 + min(j, ncl - 1);
     // 4-body FP64 with powers below 5 takes the hand-scheduled set loop
     PairTab<T, VW> tab[ASM4 || ASM3 ? 1 : NP];
     Tab4 tab4;
@@ -1102,18 +1118,9 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
     TP dpos[NB][3], sc[NP];
     int atom[NB];
-    if constexpr (ASM4) tab4.zero();
-    else if constexpr (ASM3) tab3.zero();
-    else {
-#pragma unroll
-      for (int k = 0; k < NP; ++k) tab[k].zero();
-    }
-#pragma unroll
-    for (int k = 0; k < NP; ++k) sc[k] = (TP)0;
-#pragma unroll
-    for (int s = 0; s < NB; ++s) { atom[s] = 0; dpos[s][0] = dpos[s][1] = dpos[s][2] = (TP)0; }
-    if (valid) {
+    {
       double p[NB][3];
+      dpos[0][0] = dpos[0][1] = dpos[0][2] = (TP)0;
 #pragma unroll
       for (int s = 0; s < NB; ++s) {
         uint32_t r = a.refs[(size_t)NB * ci + s];
@@ -1131,7 +1138,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
         for (int y = x + 1; y < NB; ++y) {
           double v0 = p[y][0] - p[x][0], v1 = p[y][1] - p[x][1], v2 = p[y][2] - p[x][2];
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-          sc[k] = (TP)((double)ps / r);
+          sc[k] = (TP)((double)ps * recip_nr(r));
           const int q = ty * NP + k;
           if constexpr (ASM4) {
             Tab4Pair tp{tab4, k};
@@ -1147,6 +1154,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     }
     for (int cc = 0; cc < W; ++cc) {
       const int col = colbase + cc;
+      double *se_col = se + (size_t)(rep * Wp + cc) * 7;
       TP e = (TP)0, g[NP];
 #pragma unroll
       for (int k = 0; k < NP; ++k) g[k] = (T)0;
@@ -1214,10 +1222,10 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
             TP sv = 0;
 #pragma unroll
             for (int s = 1; s < NB; ++s) sv += F[s][ia] * dpos[s][ib];
-            lds_add(&se[(comp * SE_REP + rep) * Wp + cc], (double)sv);
+            lds_add(&se_col[comp], (double)sv);
           }
         }
-        lds_add(&se[(6 * SE_REP + rep) * Wp + cc], (double)(ps * e));
+        lds_add(&se_col[6], (double)(ps * e));
       }
     }
   }
@@ -1240,7 +1248,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     else { comp = 6; scale = 1.0; }
     double s = 0;
 #pragma unroll
-    for (int q = 0; q < SE_REP; ++q) s += se[(comp * SE_REP + q) * Wp + cc];
+    for (int q = 0; q < SE_REP; ++q) s += se[(q * Wp + cc) * 7 + comp];
     Aout[(r0 + nf + k) * a.ld + cabs + cc] = (T)(s * (double)scale);
   }
 }
