@@ -1080,9 +1080,10 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   const DevFrame &f = a.frames[t.frame];
   const int nf = 3 * f.natoms;
   const int W = t.ncols;
-  // tile layout: force row r (3 rows per atom) at acc[r * Wp + column], the
-  // odd stride Wp spreading the rows of different atoms over distinct LDS
-  // banks; then SE_REP replicas of the 6 stress and 1 energy sums, replica q
+  // tile layout: force component c of atom i in column w at
+  // acc[(i * Wp + w) * 3 + c] (components are instruction immediates; the odd
+  // atom stride 3 * Wp spreads different atoms over distinct LDS banks);
+  // then SE_REP replicas of the 6 stress and 1 energy sums, replica q
   // of column c holding its 7 components at se[(q * Wp + c) * 7 + comp]
   // (component offsets are instruction immediates, and the odd replica
   // stride 7 * Wp keeps the 16 replicas on distinct banks)
@@ -1206,10 +1207,10 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
         }
 #pragma unroll
         for (int s = 0; s < NB; ++s) {
-          double *dst = acc + (3 * atom[s]) * Wp + cc;
+          double *dst = acc + ((size_t)atom[s] * Wp + cc) * 3;
           lds_add(dst, (double)F[s][0]);
-          lds_add(dst + Wp, (double)F[s][1]);
-          lds_add(dst + 2 * Wp, (double)F[s][2]);
+          lds_add(dst + 1, (double)F[s][1]);
+          lds_add(dst + 2, (double)F[s][2]);
         }
         if (any_stress) {
           const int nsc = full_stress ? 6 : 3;
@@ -1236,7 +1237,8 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   const int cabs = a.col_abs[ty] + t.col0;
   for (int i = threadIdx.x; i < nf * W; i += BS) {
     const int r = i / W, cc = i - r * W;
-    Aout[(r0 + r) * a.ld + cabs + cc] = (T)acc[r * Wp + cc];
+    const int ia = r / 3;
+    Aout[(r0 + r) * a.ld + cabs + cc] = (T)acc[(ia * Wp + cc) * 3 + (r - 3 * ia)];
   }
   const int nsr = f.stress_rows, ner = f.ener_rows;
   const int sel9[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
