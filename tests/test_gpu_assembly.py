@@ -52,7 +52,7 @@ def test_golden_cases(ch, oracle, name):
     dev.close()
 
 
-@pytest.mark.parametrize("case", ["carbon", "water", "chno", "small_cell"])
+@pytest.mark.parametrize("case", ["carbon", "water", "chno", "small_cell", "high_orders"])
 def test_synthetic_cases(ch, synth, oracle, tmp_path, case):
     print("case", case, flush=True)
     if case == "carbon":
@@ -61,6 +61,10 @@ def test_synthetic_cases(ch, synth, oracle, tmp_path, case):
         gen = synth.water(nframes=2, seed=22)
     elif case == "chno":
         gen = synth.chno(nframes=3, nmin=40, nmax=220, seed=23)
+    elif case == "high_orders":
+        # 3-body order 10 and 4-body order 6: the generic table kernels (the
+        # hand-scheduled set loops cover 3-body orders up to 9, 4-body up to 5)
+        gen = synth.carbon(nframes=2, natoms=48, seed=28, o2=12, o3=10, o4=6)
     else:
         # 8 carbon atoms in a 5 A cell: cell extent below the cutoff -> unordered
         # neighbour lists and PERM_SCALE 1/2, 1/6, 1/24 (NEIGHBORS::DO_UPDATE_SMALL)
