@@ -250,6 +250,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-md", action="store_true",
+                    help="skip the MD-style leg (force calls replayed from HIP graphs), e.g. under a kernel tracer")
     ap.add_argument("--no-configs", action="store_true", help="skip the 600-frame and mixed-batch legs")
     ap.add_argument("--counters", action="store_true",
                     help="one FP64 assembly of the workload plus a 1 GiB device copy, for rocprofv3 --pmc runs")
@@ -347,11 +349,13 @@ def main():
         stream = serving_stream(ch, synth)
         log("serving Poisson stream %.0f req/s: p50 %.3f ms p99 %.3f ms, %.3g atoms/s, mean batch %.2f" % (
             stream["qps_offered"], stream["p50_ms"], stream["p99_ms"], stream["atoms_per_s"], stream["mean_batch"]))
-        md = md_steps(ch, synth)
-        log("MD-style steps, 1k atoms: graph replay p50 %.3f ms p99 %.3f ms (%d replays) | eager p50 %.3f ms "
-            "p99 %.3f ms" % (md["graph_p50_ms"], md["graph_p99_ms"], md["graph_replays"], md["eager_p50_ms"],
-                             md["eager_p99_ms"]))
-        extra["serving"] = {"single_1k_atoms": one, "poisson_stream": stream, "md_steps_1k_atoms": md}
+        extra["serving"] = {"single_1k_atoms": one, "poisson_stream": stream}
+        if not args.no_md:
+            md = md_steps(ch, synth)
+            log("MD-style steps, 1k atoms: graph replay p50 %.3f ms p99 %.3f ms (%d replays) | eager p50 %.3f ms "
+                "p99 %.3f ms" % (md["graph_p50_ms"], md["graph_p99_ms"], md["graph_replays"], md["eager_p50_ms"],
+                                 md["eager_p99_ms"]))
+            extra["serving"]["md_steps_1k_atoms"] = md
     if not args.no_configs:
         c3 = config_c3(ch, synth)
         log("600-frame carbon: FP64 %.3g rows/s + fit %.3f s | FP32 %.3g rows/s + fit %.3f s, RMS rel err %.2e" % (
