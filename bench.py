@@ -257,8 +257,8 @@ def main():
                     help="one FP64 assembly of the workload plus a 1 GiB device copy, for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the design-matrix path of this build is single-device "
-              "(multi-GPU frame sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the design-matrix path is single-device "
+              "(sharding frames over GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
     ch = _load("chimes", os.path.join(PKG, "chimes.py"))
     synth = _load("synth", os.path.join(PKG, "synth.py"))

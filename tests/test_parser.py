@@ -27,7 +27,7 @@ def test_header_and_map_byte_exact(ch, tmp_path, name):
 
 
 def test_column_counts_of_benchmark_orders(ch, synth, tmp_path):
-    # Column counts of the benchmark configurations (BASELINE config table):
+    # Column counts of the benchmark configurations:
     # O/H 12 4 0 -> 136, C 16 9 5 -> 1038, C/H/N/O 12 6 3 -> 10504.
     cases = [(synth.water(nframes=1), 136), (synth.carbon(nframes=1, natoms=64), 1038)]
     for (text, _, _), ncols in cases:
