@@ -129,6 +129,7 @@ def lib(path=None):
         opt("cg_serve_submit", [C.c_void_p, C.POINTER(Frame), C.POINTER(C.c_int64)])
         opt("cg_serve_poll", [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp])
         opt("cg_serve_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        opt("cg_serve_hold", [C.c_void_p, C.c_int])
         opt("cg_serve_destroy", [C.c_void_p])
         _lib = L
     return _lib
@@ -480,6 +481,23 @@ class Server:
             return None
         del self.natoms[ticket]
         return F, E.value, V, lat.value
+
+    def hold(self, on=True):
+        """Stop (on) or resume starting batches; queued requests are then admitted together."""
+        check(self.L.cg_serve_hold(self.h, 1 if on else 0))
+
+    def poll_status(self, ticket):
+        """(status code, message, F, E, virial) of a finished request, without raising."""
+        n = self.natoms[ticket]
+        F = np.zeros(3 * n)
+        E = C.c_double()
+        V = np.zeros(9)
+        lat = C.c_double()
+        done = C.c_int()
+        st = self.L.cg_serve_poll(self.h, ticket, 1, C.byref(done), dptr(F), C.byref(E), dptr(V), C.byref(lat))
+        msg = self.L.cg_last_error(None) if st else b""
+        del self.natoms[ticket]
+        return st, (msg or b"").decode(), F, E.value, V
 
     def stats(self):
         b, r = C.c_int64(), C.c_int64()

@@ -98,14 +98,6 @@ static void host_cell(const double *cell, DevFrame &f) {
 }
 
 // ------------------------------------------------------------ device helpers
-__device__ __forceinline__ int ref_atom(uint32_t r) { return int(r & 0xFFFFFu); }
-__device__ __forceinline__ int ref_sx(uint32_t r) { return int((r >> 20) & 7u) - 4; }
-__device__ __forceinline__ int ref_sy(uint32_t r) { return int((r >> 23) & 7u) - 4; }
-__device__ __forceinline__ int ref_sz(uint32_t r) { return int((r >> 26) & 7u) - 4; }
-__device__ __forceinline__ uint32_t make_ref(int a, int sx, int sy, int sz) {
-  return uint32_t(a) | (uint32_t(sx + 4) << 20) | (uint32_t(sy + 4) << 23) | (uint32_t(sz + 4) << 26);
-}
-
 // Cartesian image position: wrapped coordinate + H * shift.
 __device__ __forceinline__ void image_pos(const DevFrame &f, const double *wrap, uint32_t r, double *p) {
   int a = ref_atom(r);
@@ -166,7 +158,7 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
 }
 
 // Neighbour images of atom i within rc: every (j, shift) with shift in
-// [-L, L]^3 (the reference's ghost layers), restricted to shifts that can lie
+// [-L, L]^3 (the reference's ghost layers, L = the frame's layer count), restricted to shifts that can lie
 // within rc along each axis.  Big cells keep j >= i (half list with the
 // reference's a1 <= PARENT[a2] rule); small cells keep every image.
 // Frames at least three list cutoffs thick along every axis search a cell
@@ -178,7 +170,7 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int *atom_frame, const double *wrap,
                                                 const int *type, const int *bin_start, const int *bin_atom,
-                                                int natoms, double rc, int L, int *count, const int *offset,
+                                                int natoms, double rc, int *count, const int *offset,
                                                 uint32_t *list, double4 *nvec, int *nown, int *ntyp, int cap) {
   const int gi = blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
@@ -248,8 +240,8 @@ __global__ __launch_bounds__(256) void k_neigh(const DevFrame *frames, const int
       for (int r = 0; r < 3; ++r) fj[r] = f.Hi[3 * r] * wj[0] + f.Hi[3 * r + 1] * wj[1] + f.Hi[3 * r + 2] * wj[2];
       for (int r = 0; r < 3; ++r) d[r] = fj[r] - fi[r];
       for (int r = 0; r < 3; ++r) {
-        lo[r] = max(-L, (int)ceil(-d[r] - reach[r]));
-        hi[r] = min(L, (int)floor(-d[r] + reach[r]));
+        lo[r] = max(-f.nlayers, (int)ceil(-d[r] - reach[r]));
+        hi[r] = min(f.nlayers, (int)floor(-d[r] + reach[r]));
       }
     }
     // pass 1: count this lane's images; pass 2 (FILL): write them in order
@@ -1901,6 +1893,22 @@ static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, 
     d.stress_rows = (m.fit_stress != CG_STRESS_NONE && (m.nstress < 0 || fr.index < m.nstress))
                         ? (m.fit_stress == CG_STRESS_ALL ? 9 : 3) : 0;
     d.ener_rows = (m.fit_energy && (m.nener < 0 || fr.index < m.nener)) ? 3 : 0;
+    // Image layers: a fit searches the NLAYERS ghost layers of its input, as
+    // the reference does (it only warns when the cutoff exceeds half a
+    // layered cell vector, BOX::IS_RCUT_SAFE, src/ClassDefs.C:2166-2190,2538);
+    // a force evaluation searches as many layers as the cutoff needs, so that
+    // no image within the cutoff is missed in a small cell.
+    if (sticky) {
+      d.nlayers = m.nlayers;
+    } else {
+      const double wmin = std::min(d.width[0], std::min(d.width[1], d.width[2]));
+      d.nlayers = std::max(1, (int)std::ceil(mx / wmin));
+      if (d.nlayers > kMaxLayers)
+        throw Error(CG_ERR_INPUT, "configuration " + std::to_string(i) + ": cell width " + std::to_string(wmin) +
+                                      " A is below cutoff / " + std::to_string(kMaxLayers) +
+                                      " (more image layers than are supported)");
+    }
+    if (d.nlayers < 1 || d.nlayers > kMaxLayers) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS must be 1 to 7");
     // NEIGHBORS small-cell mode is sticky across the frames of a fit; force
     // evaluations treat every configuration on its own
     const int small_here = (m.small_mode || d.min_extent <= mx) ? 1 : 0;
@@ -2011,7 +2019,7 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
   if (!d_ncount || !d_noff) throw Error(CG_ERR_MEMORY, "neighbour counts");
   zero_i(c, d_ncount, natoms + 1);
   hipLaunchKernelGGL(k_neigh<false>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
-                     (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, d_ncount,
+                     (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, d_ncount,
                      (const int *)nullptr, (uint32_t *)nullptr, (double4 *)nullptr, (int *)nullptr, (int *)nullptr, 0);
   exclusive_scan(c, d_ncount, d_noff, natoms);
   // exact mode reads the list size back (host sync); capacity mode sizes the
@@ -2025,7 +2033,7 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
   int *d_ntyp = d_nown + nn1;
   if (!d_nlist || !d_nvec || !d_nown) throw Error(CG_ERR_MEMORY, "neighbour list");
   hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
-                     (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, m.nlayers, (int *)nullptr,
+                     (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, (int *)nullptr,
                      (const int *)d_noff, d_nlist, d_nvec, d_nown, d_ntyp, (int)nn1);
   // ---- per-family sublists (3B / 4B cutoffs below the list cutoff), sized
   // by the full list so that no host read is needed
@@ -2331,7 +2339,15 @@ static bool dispatch_force(Ctx &c, int N, dim3 grid, const ForceArgs &a, std::in
 // Device part of a force call after prepare_device: per-family task lists
 // and force kernels, the call's totals, and the copy of results and totals
 // into the pinned host buffer.
-static void force_device(Ctx &c, const Batch &B, double *d_F, size_t nout) {
+// Host destination of a force call's results: F (3 * atoms doubles) and the
+// rest (energy / virial per frame and the call totals); F == nullptr means
+// the context's pinned buffer receives everything.
+struct HostOut {
+  double *F = nullptr;
+  double *rest = nullptr;
+};
+
+static void force_device(Ctx &c, const Batch &B, double *d_F, size_t nout, const HostOut &ho = HostOut{}) {
   const cg_model &m = c.m;
   const int TB = 256;
   double *d_EV = d_F + 3 * (size_t)B.natoms;
@@ -2376,17 +2392,26 @@ static void force_device(Ctx &c, const Batch &B, double *d_F, size_t nout) {
   // Ewald terms of a model fitted with charges (chimes_md's ZCalc_Ewald)
   if (m.n_charge_cols > 0) ewald_md(c, B.frames, B.hf, B.d_af, B.d_type, B.d_wrap, B.natoms, d_F, d_EV);
   hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, c.stream, B.tot, d_F + nout);
-  CG_HIP(hipMemcpyAsync(c.h_pin, d_F, (nout + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  if (ho.F) {
+    const size_t nf = 3 * (size_t)B.natoms;
+    CG_HIP(hipMemcpyAsync(ho.F, d_F, nf * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CG_HIP(hipMemcpyAsync(ho.rest, d_F + nf, (nout + 4 - nf) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  } else {
+    CG_HIP(hipMemcpyAsync(c.h_pin, d_F, (nout + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  }
 }
 
 // Shape of a capacity-mode call: everything its launches depend on besides
 // the contents of the uploaded arrays.
-static std::string shape_key(const Batch &B, const Caps &caps) {
+static std::string shape_key(const Ctx &c, const Batch &B, const Caps &caps) {
   std::string k;
   auto add = [&](long long v) { k += std::to_string(v); k += ','; };
+  // the penalty settings are kernel arguments of the captured launches
+  add((long long)__builtin_bit_cast(uint64_t, c.penalty_dist));
+  add((long long)__builtin_bit_cast(uint64_t, c.penalty_scale));
   add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
   for (const DevFrame &d : B.hf) {
-    add(d.natoms); add(d.small); add(d.cell); add(d.nbin[0]); add(d.nbin[1]); add(d.nbin[2]);
+    add(d.natoms); add(d.small); add(d.cell); add(d.nbin[0]); add(d.nbin[1]); add(d.nbin[2]); add(d.nlayers);
   }
   return k;
 }
@@ -2436,13 +2461,17 @@ static bool capture_call(Ctx &c, Batch &B, const Caps &caps, double *d_F, size_t
 // (kept per shape and replayed for later calls of that shape, after the
 // frame arrays have been uploaded), while the buffers it uses stay in place.
 static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frames, int nframes, double *F,
-                             double *E, double *vir, bool exact) {
+                             double *E, double *vir, bool exact, double *rest_host = nullptr) {
   const cg_model &m = c.m;
+  // rest_host (allocation-free calls): results go straight to F and to this
+  // host array of 10 * nframes + 4 doubles, with exact list sizes (no pinned
+  // buffer, no capacity estimates, no graphs)
+  const bool noalloc = rest_host != nullptr;
   set_coef(c, x_host);
   int natoms = 0;
   for (int i = 0; i < nframes; ++i) natoms += frames[i].natoms;
   Caps caps;
-  const bool capmode = !exact && c.rate_nb > 0.0;
+  const bool capmode = !noalloc && !exact && c.rate_nb > 0.0;
   if (capmode) {
     // rounded up to 8 steps per octave, so that the capacities (part of a
     // graph's shape) stay put while the counts drift from call to call
@@ -2468,7 +2497,7 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   const size_t nout = 3 * (size_t)natoms + 10 * (size_t)nframes;
   double *d_F = c.b_force.as<double>(nout + 16);
   if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
-  if (c.h_pin_cap < nout + 4) {
+  if (!noalloc && c.h_pin_cap < nout + 4) {
     if (c.h_pin) (void)hipHostFree(c.h_pin);
     c.h_pin = nullptr;
     c.h_pin_cap = 0;
@@ -2480,7 +2509,7 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   bool done = false;
   // calls with Ewald terms upload per-call k tables and stay out of graphs
   if (capmode && m.n_charge_cols == 0) {
-    const std::string key = shape_key(B, caps);
+    const std::string key = shape_key(c, B, caps);
     auto it = c.graphs.find(key);
     if (it != c.graphs.end() && it->second.gen != g_buffer_generation) {
       drop_graphs(c);
@@ -2501,12 +2530,19 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
     }
     c.last_key = key;
   }
+  HostOut ho;
+  if (noalloc) {
+    ho.F = F;
+    ho.rest = rest_host;
+  }
   if (!done) {
     prepare_device(c, 1, B, capmode ? &caps : nullptr);
-    force_device(c, B, d_F, nout);
+    force_device(c, B, d_F, nout, ho);
   }
   CG_HIP(hipStreamSynchronize(c.stream));
-  const double *tot = c.h_pin + nout;
+  // results: F then energy / virial per frame then the call totals
+  const double *res_rest = noalloc ? rest_host : c.h_pin + 3 * (size_t)natoms;
+  const double *tot = res_rest + 10 * (size_t)nframes;
   const double na = std::max(natoms, 1);
   bool fits = true;
   if (capmode) {
@@ -2520,12 +2556,12 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
     force_batch_impl(c, x_host, frames, nframes, F, E, vir, true);
     return;
   }
-  if (F) std::memcpy(F, c.h_pin, 3 * (size_t)natoms * sizeof(double));
-  const double *ev = c.h_pin + 3 * (size_t)natoms;
+  if (F && !noalloc) std::memcpy(F, c.h_pin, 3 * (size_t)natoms * sizeof(double));
+  const double *ev = res_rest;
   for (int fi = 0; fi < nframes; ++fi) {
     double eo = 0.0;  // per-type energy offsets: atom count x offset coefficient
     if (m.n_offset_cols > 0) {
-      std::vector<int> cnt(m.ntypes, 0);
+      int cnt[CG_MAX_ATOM_TYPES] = {};
       for (int a = 0; a < frames[fi].natoms; ++a) ++cnt[frames[fi].type[a]];
       for (int t = 0; t < m.n_offset_cols && t < m.ntypes; ++t) eo += cnt[t] * x_host[m.ncols_short + t];
     }
@@ -2772,8 +2808,14 @@ cg_status cg_force_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats, int64_t 
 
 cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale) {
   if (!ctx || !(dist >= 0.0) || !(scale >= 0.0)) { cg::set_error("bad penalty"); return CG_ERR_ARG; }
-  ctx->c.penalty_dist = dist;
-  ctx->c.penalty_scale = scale;
+  cg::Ctx &c = ctx->c;
+  if (dist != c.penalty_dist || scale != c.penalty_scale) {
+    // captured force calls hold the penalty in their kernel arguments
+    cg::drop_graphs(c);
+    c.last_key.clear();
+  }
+  c.penalty_dist = dist;
+  c.penalty_scale = scale;
   return CG_OK;
 }
 

@@ -68,7 +68,34 @@ struct DevFrame {
   int cell;            // neighbour search through fractional-coordinate bins
   int nbin[3];         // bins per cell axis (each at least the list cutoff thick)
   int bin_off;         // first bin of the frame in the batch bin arrays
+  int nlayers;         // periodic image layers searched along each cell axis
 };
+
+// Neighbour image reference: atom index in the frame (20 bits) and the image
+// shift along each cell axis (4 bits each, biased by 8: shifts -8..+7).
+constexpr int kRefShiftBias = 8;
+constexpr int kMaxLayers = 7;  // largest |shift| an image layer search may need
+__host__ __device__ constexpr uint32_t make_ref(int a, int sx, int sy, int sz) {
+  return uint32_t(a) | (uint32_t(sx + kRefShiftBias) << 20) | (uint32_t(sy + kRefShiftBias) << 24) |
+         (uint32_t(sz + kRefShiftBias) << 28);
+}
+__host__ __device__ constexpr int ref_atom(uint32_t r) { return int(r & 0xFFFFFu); }
+__host__ __device__ constexpr int ref_sx(uint32_t r) { return int((r >> 20) & 15u) - kRefShiftBias; }
+__host__ __device__ constexpr int ref_sy(uint32_t r) { return int((r >> 24) & 15u) - kRefShiftBias; }
+__host__ __device__ constexpr int ref_sz(uint32_t r) { return int((r >> 28) & 15u) - kRefShiftBias; }
+// Every shift a search of kMaxLayers layers produces, with the largest atom
+// index, decodes to itself.
+constexpr bool ref_round_trip() {
+  for (int sx = -kMaxLayers; sx <= kMaxLayers; ++sx)
+    for (int sy = -kMaxLayers; sy <= kMaxLayers; ++sy)
+      for (int sz = -kMaxLayers; sz <= kMaxLayers; ++sz)
+        for (int a : {0, 1, 0xFFFFF}) {
+          const uint32_t r = make_ref(a, sx, sy, sz);
+          if (ref_atom(r) != a || ref_sx(r) != sx || ref_sy(r) != sy || ref_sz(r) != sz) return false;
+        }
+  return true;
+}
+static_assert(ref_round_trip(), "image reference encoding");
 
 // Sizes of a call without host reads: capacity of the neighbour list and of
 // each family's cluster list.  The kernels write nothing past them; the

@@ -74,18 +74,54 @@ struct Server {
   int64_t next_ticket = 1;
   size_t inflight = 0;
   bool stop = false;
+  bool hold = false;  // workers take no new batch while set (cg_serve_hold)
   int64_t n_batches = 0, n_requests = 0;
+  int64_t n_isolated = 0;  // failed batches whose requests were re-run one by one
   std::vector<std::thread> workers;
   std::string init_error;
   int ready = 0;
 
+  // Forces of requests [i0, i1) of a batch in one cg::force_batch pass into
+  // F / E / V (resized); any failure, device or host, becomes a status.
+  cg_status evaluate(cg::Ctx &ctx, const std::vector<std::shared_ptr<Request>> &batch, size_t i0, size_t i1,
+                     std::vector<double> &F, std::vector<double> &E, std::vector<double> &V, std::string &err) {
+    std::vector<cg_frame> frames;
+    size_t tot = 0;
+    for (size_t i = i0; i < i1; ++i) {
+      frames.push_back(batch[i]->f);
+      tot += 3 * (size_t)batch[i]->f.natoms;
+    }
+    try {
+      F.assign(tot, 0.0);
+      E.assign(frames.size(), 0.0);
+      V.assign(9 * frames.size(), 0.0);
+      CG_HIP(hipSetDevice(device));
+      cg::force_batch(ctx, x.data(), frames.data(), (int)frames.size(), F.data(), E.data(), V.data());
+      return CG_OK;
+    } catch (const cg::Error &e) {
+      err = e.what();
+      return e.code;
+    } catch (const std::bad_alloc &) {
+      err = "host memory exhausted";
+      return CG_ERR_MEMORY;
+    } catch (const std::exception &e) {
+      err = e.what();
+      return CG_ERR_DEVICE;
+    }
+  }
+
   void worker() {
     cg::Ctx ctx;
+    std::string ierr;
     try {
       cg::ctx_init(ctx, m, device);
-    } catch (const cg::Error &e) {
+    } catch (const std::exception &e) {
+      ierr = e.what();
+    }
+    if (!ierr.empty()) {
+      try { cg::ctx_free(ctx); } catch (...) {}
       std::lock_guard<std::mutex> lk(mu);
-      init_error = e.what();
+      init_error = ierr;
       ++ready;
       cv_done.notify_all();
       return;
@@ -96,7 +132,6 @@ struct Server {
       cv_done.notify_all();
     }
     std::vector<std::shared_ptr<Request>> batch;
-    std::vector<cg_frame> frames;
     std::vector<double> F, E, V;
     for (;;) {
       size_t bytes = 0;
@@ -104,7 +139,7 @@ struct Server {
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] {
-          return stop || (!queue.empty() && (inflight == 0 || inflight + queue.front()->est_bytes <= budget));
+          return stop || (!hold && !queue.empty() && (inflight == 0 || inflight + queue.front()->est_bytes <= budget));
         });
         if (queue.empty()) break;  // stop requested and nothing left
         int atoms = 0;
@@ -118,42 +153,46 @@ struct Server {
         }
         inflight += bytes;
       }
-      frames.clear();
-      size_t tot = 0;
-      for (auto &r : batch) {
-        frames.push_back(r->f);
-        tot += 3 * (size_t)r->f.natoms;
-      }
-      F.assign(tot, 0.0);
-      E.assign(batch.size(), 0.0);
-      V.assign(9 * batch.size(), 0.0);
-      cg_status st = CG_OK;
+      // one pass over the whole batch; if it fails, each request is
+      // evaluated alone so that only the requests that fail by themselves
+      // report an error
       std::string err;
-      try {
-        CG_HIP(hipSetDevice(device));
-        cg::force_batch(ctx, x.data(), frames.data(), (int)frames.size(), F.data(), E.data(), V.data());
-      } catch (const cg::Error &e) {
-        st = e.code;
-        err = e.what();
-      }
-      const auto now = Clock::now();
-      {
-        std::lock_guard<std::mutex> lk(mu);
+      const cg_status st = evaluate(ctx, batch, 0, batch.size(), F, E, V, err);
+      if (st == CG_OK) {
         size_t o = 0;
         for (size_t i = 0; i < batch.size(); ++i) {
           Request &r = *batch[i];
           const size_t n3 = 3 * (size_t)r.f.natoms;
-          r.status = st;
-          r.err = err;
-          if (st == CG_OK) {
-            r.F.assign(F.begin() + o, F.begin() + o + n3);
-            r.E = E[i];
-            std::memcpy(r.vir, &V[9 * i], sizeof(r.vir));
-          }
+          r.F.assign(F.begin() + o, F.begin() + o + n3);
+          r.E = E[i];
+          std::memcpy(r.vir, &V[9 * i], sizeof(r.vir));
+          r.status = CG_OK;
           o += n3;
-          r.done = true;
-          r.t_done = now;
         }
+      } else if (batch.size() == 1) {
+        batch[0]->status = st;
+        batch[0]->err = err;
+      } else {
+        for (size_t i = 0; i < batch.size(); ++i) {
+          Request &r = *batch[i];
+          std::string e1;
+          r.status = evaluate(ctx, batch, i, i + 1, F, E, V, e1);
+          r.err = e1;
+          if (r.status == CG_OK) {
+            r.F = F;
+            r.E = E[0];
+            std::memcpy(r.vir, V.data(), sizeof(r.vir));
+          }
+        }
+      }
+      const auto now = Clock::now();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto &rp : batch) {
+          rp->done = true;
+          rp->t_done = now;
+        }
+        if (st != CG_OK && batch.size() > 1) ++n_isolated;
         inflight -= bytes;
         ++n_batches;
         n_requests += (int64_t)batch.size();
@@ -277,6 +316,19 @@ cg_status cg_serve_poll(cg_server *sv, int64_t ticket, int wait, int *done, doub
   if (F) std::memcpy(F, r->F.data(), r->F.size() * sizeof(double));
   if (E) *E = r->E;
   if (virial9) std::memcpy(virial9, r->vir, sizeof(r->vir));
+  return CG_OK;
+}
+
+cg_status cg_serve_hold(cg_server *sv, int hold) {
+  if (!sv) {
+    cg::set_error("cg_serve_hold: bad argument");
+    return CG_ERR_ARG;
+  }
+  {
+    std::lock_guard<std::mutex> lk(sv->s.mu);
+    sv->s.hold = hold != 0;
+  }
+  sv->s.cv_work.notify_all();
   return CG_OK;
 }
 

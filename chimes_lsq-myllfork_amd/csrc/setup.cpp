@@ -626,6 +626,8 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
     if (s->use_partial_charges) throw Error(CG_ERR_UNSUPPORTED, "fixed non-zero charges (Ewald subtraction) are not supported");
     if (s->wrap && s->nlayers > 0) s->wrap = false;  // RUN_SANITY_LSQ
     if (s->nlayers < 1) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS 0 (no ghost atoms) is not supported");
+    // image shifts are stored in 4-bit fields (-8..+7) of a neighbour reference
+    if (s->nlayers > 7) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS above 7 is not supported");
 
     // clusters (build_clusters, src/chimes_lsq.C:533-560)
     s->trips.order = s->o3;

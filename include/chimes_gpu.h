@@ -381,9 +381,15 @@ cg_status cg_serve_create(const cg_model *m, int device, const double *x, int nw
 cg_status cg_serve_submit(cg_server *s, const cg_frame *f, int64_t *ticket);
 /* done = 1 when the result is ready (wait != 0 blocks until then); the
  * outputs are written and the ticket released on the call that sees done.
- * latency_ms: submit to completion, measured in the library. */
+ * latency_ms: submit to completion, measured in the library.
+ * A batch whose evaluation fails is evaluated again request by request, so a
+ * request reports an error only when it fails on its own. */
 cg_status cg_serve_poll(cg_server *s, int64_t ticket, int wait, int *done, double *F, double *E,
                         double *virial9, double *latency_ms);
+/* hold != 0: workers start no new batch (requests keep queueing); hold = 0
+ * releases them, and the queued requests are then admitted together under the
+ * usual batch and budget limits.  Lets a caller group submissions. */
+cg_status cg_serve_hold(cg_server *s, int hold);
 cg_status cg_serve_stats(cg_server *s, int64_t *batches, int64_t *requests);
 /* Finishes every queued request, then stops the workers. */
 void cg_serve_destroy(cg_server *s);

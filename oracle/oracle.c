@@ -252,11 +252,37 @@ static double dist(const sys_t *s, int a1, int a2, double *rab) {
   return sqrt(rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2]);
 }
 
-/* small_state: in/out sticky SMALL flag (NEIGHBORS::UPDATE_WITH_BIG). */
-static int sys_build(const cg_model *m, const cg_frame *f, int *small_state, sys_t *s) {
+/* Largest list cutoff of the model (NEIGHBORS::MAX_CUTOFF over the families). */
+static double model_max_cutoff(const cg_model *m) {
+  double mx = m->max_cutoff_2b;
+  if (m->c3.nbody && m->c3.max_cutoff > mx) mx = m->c3.max_cutoff;
+  if (m->c4.nbody && m->c4.max_cutoff > mx) mx = m->c4.max_cutoff;
+  return mx;
+}
+
+/* Ghost layers a force evaluation needs so that every image within the list
+ * cutoff exists: ceil(cutoff / smallest perpendicular cell width), at least
+ * one (chimes_md run with an NLAYERS large enough for its cell). */
+static int force_layers(const cg_model *m, const double *cell) {
+  const double *a = cell, *b = cell + 3, *c = cell + 6;
+  double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+  double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+  double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  double det = fabs(a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2]);
+  double w = det / vmag(bc[0], bc[1], bc[2]);
+  double w2 = det / vmag(ca[0], ca[1], ca[2]), w3 = det / vmag(ab[0], ab[1], ab[2]);
+  if (w2 < w) w = w2;
+  if (w3 < w) w = w3;
+  int L = (int)ceil(model_max_cutoff(m) / w);
+  return L < 1 ? 1 : L;
+}
+
+/* small_state: in/out sticky SMALL flag (NEIGHBORS::UPDATE_WITH_BIG).
+ * layers: ghost layers (0: the model's NLAYERS). */
+static int sys_build(const cg_model *m, const cg_frame *f, int *small_state, sys_t *s, int layers) {
   memset(s, 0, sizeof(*s));
   if (box_setup(f->cell, &s->box)) return -1;
-  int L = m->nlayers;
+  int L = layers > 0 ? layers : m->nlayers;
   int ncell = (2 * L + 1) * (2 * L + 1) * (2 * L + 1);
   s->n = f->natoms;
   s->nall = f->natoms * ncell;
@@ -717,7 +743,7 @@ int orc_ewald_params(int natoms, const double *cell, double *alpha, double *rc, 
  * out: [rows x ncols] row-major.  small_state is the sticky SMALL flag. */
 int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, double *out) {
   sys_t s;
-  if (sys_build(m, f, small_state, &s)) return -1;
+  if (sys_build(m, f, small_state, &s, 0)) return -1;
   const int P = m->ncols_short, nc = m->ncols;
   int fit_stress = (m->fit_stress != CG_STRESS_NONE && (m->nstress < 0 || f->index < m->nstress)) ? m->fit_stress : 0;
   int fit_ener = m->fit_energy && (m->nener < 0 || f->index < m->nener);
@@ -791,7 +817,7 @@ int orc_assemble_frame(const cg_model *m, const cg_frame *f, int *small_state, d
  * used by tests to cross-check enumeration counts. */
 int orc_count_clusters(const cg_model *m, const cg_frame *f, int *small_state, long long *counts3) {
   sys_t s;
-  if (sys_build(m, f, small_state, &s)) return -1;
+  if (sys_build(m, f, small_state, &s, 0)) return -1;
   counts3[0] = counts3[1] = counts3[2] = 0;
   for (int a1 = 0; a1 < s.n; ++a1)
     for (int k = s.o2[a1]; k < s.o2[a1 + 1]; ++k) {
@@ -863,7 +889,7 @@ int orc_force(const cg_model *m, const cg_frame *f, const double *x, double pena
               double *F, double *E, double *vir9) {
   int small = 0;
   sys_t s;
-  if (sys_build(m, f, &small, &s)) return -1;
+  if (sys_build(m, f, &small, &s, force_layers(m, f->cell))) return -1;
   const int P = m->ncols_short;
   acc_t A;
   A.P = P;

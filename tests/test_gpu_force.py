@@ -47,6 +47,64 @@ def test_force_penalty_below_smin(ch, synth, oracle, tmp_path):
     dev.close()
 
 
+def test_force_penalty_change_after_graph_capture(ch, synth, oracle, tmp_path):
+    """Same-shape calls are captured into a graph; a penalty change afterwards
+    is honoured by the next call (equal to a fresh context with that penalty,
+    and to the oracle)."""
+    text = synth.fm_setup(["C"], [12.0], 8, 4, 3, 1.2, 4.0, 1.4, smax3=3.5, smax4=3.0)
+    p = tmp_path / "fm_setup.in"
+    open(p, "w").write(text)
+    s = ch.Setup(str(p))
+    rng = np.random.default_rng(11)
+    cell = np.diag([9.0, 9.0, 9.0])
+    pos = synth.random_sequential(rng, 30, cell, 1.3)
+    pos[1] = pos[0] + np.array([1.15, 0.0, 0.0])  # below s_min
+    f = ch.PyFrame(30, pos, np.zeros(30, dtype=np.int32), cell)
+    x = rng.normal(size=s.model.ncols)
+    dev = ch.Device(s)
+    for _ in range(4):  # exact, capacity-sized, captured, replayed
+        dev.force(x, f)
+    assert dev.force_stats()[2] >= 1  # the fourth call replayed a graph
+    dev.set_penalty(0.05, 2.0e4)
+    F, E, V = dev.force(x, f)
+    fresh = ch.Device(s)
+    fresh.set_penalty(0.05, 2.0e4)
+    F0, E0, V0 = fresh.force(x, f)
+    fresh.close()
+    scale = np.abs(F0).max()
+    assert np.max(np.abs(F - F0)) <= 1e-12 * scale
+    assert E == pytest.approx(E0, rel=1e-12)
+    Fo, Eo, Vo = oracle.force(s, f, x, penalty_scale=2.0e4, penalty_dist=0.05)
+    assert np.max(np.abs(F - Fo)) <= 1e-10 * scale
+    assert E == pytest.approx(Eo, rel=1e-10)
+    # and the old penalty differs (the change is visible at all)
+    Fd, Ed, Vd = oracle.force(s, f, x)
+    assert abs(Ed - Eo) > 1e-6 * abs(Eo)
+    dev.close()
+
+
+def test_force_small_cell_needs_two_layers(ch, synth, oracle, tmp_path):
+    """A configuration whose cell is thinner than the cutoff gets the images
+    of a second layer (the force path searches as many layers as the cutoff
+    needs), like the oracle's force evaluation."""
+    text = synth.fm_setup(["C"], [12.0], 8, 4, 0, 1.0, 4.0, 1.4, smax3=3.0)
+    p = tmp_path / "fm_setup.in"
+    open(p, "w").write(text)
+    s = ch.Setup(str(p))
+    rng = np.random.default_rng(12)
+    cell = np.array([[3.4, 0.0, 0.0], [0.9, 3.6, 0.0], [0.4, -0.6, 3.5]])
+    pos = synth.random_sequential(rng, 3, cell, 1.2)
+    f = ch.PyFrame(3, pos, np.zeros(3, dtype=np.int32), cell)
+    x = rng.normal(size=s.model.ncols)
+    dev = ch.Device(s)
+    F, E, V = dev.force(x, f)
+    dev.close()
+    Fo, Eo, Vo = oracle.force(s, f, x)
+    assert np.allclose(F, Fo, rtol=1e-10, atol=1e-10 * np.abs(Fo).max())
+    assert E == pytest.approx(Eo, rel=1e-10)
+    assert np.allclose(V, Vo, rtol=1e-10, atol=1e-10 * np.abs(Vo).max())
+
+
 def test_force_batch_matches_single_calls(ch, synth, oracle, tmp_path):
     """A batch of configurations of different sizes (one in a small cell)
     gives each configuration the result it gets alone, and the oracle's."""
