@@ -112,6 +112,7 @@ def lib(path=None):
                                 C.POINTER(FitResult), _dp, _dp])
         opt("cg_force", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_force_batch", [C.c_void_p, _dp, C.POINTER(Frame), C.c_int, _dp, _dp, _dp])
+        opt("cg_force_noalloc", [C.c_void_p, _dp, C.POINTER(Frame), _dp, _dp, _dp])
         opt("cg_set_penalty", [C.c_void_p, C.c_double, C.c_double])
         opt("cg_set_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.c_int])
         opt("cg_set_fit_weights", [C.c_void_p, _dp, C.c_int64])
@@ -401,6 +402,17 @@ class Device:
         V = np.zeros(9)
         f = frame.c()
         check(self.L.cg_force(self.h, dptr(x), C.byref(f), dptr(F), C.byref(E), dptr(V)), self.h)
+        return F, E.value, V
+
+    def force_noalloc(self, x, frame):
+        """cg_force_noalloc: the same results as force(), every device buffer taken
+        from the library's fixed per-GPU scratch array."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        F = np.zeros(3 * frame.natoms)
+        E = C.c_double()
+        V = np.zeros(9)
+        f = frame.c()
+        check(self.L.cg_force_noalloc(self.h, dptr(x), C.byref(f), dptr(F), C.byref(E), dptr(V)), self.h)
         return F, E.value, V
 
     def copy_probe(self, nbytes):

@@ -2576,11 +2576,76 @@ void force_batch(Ctx &c, const double *x_host, const cg_frame *frames, int nfram
   force_batch_impl(c, x_host, frames, nframes, F, E, vir, false);
 }
 
+// Scratch of the allocation-free force entry point: one fixed array per
+// GPU (it lives as long as the process).  In host code the array's name is
+// not a device address; its address on each device comes from
+// hipGetSymbolAddress, taken once per device when the first context is
+// created there, and only that address is handed to kernels and copies.
+constexpr size_t kNoAllocScratchBytes = (size_t)CG_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+thread_local Arena *t_arena = nullptr;
+
+// Device address of this device's scratch array (the device must be current).
+static void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(CG_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  CG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error(CG_ERR_DEVICE, "scratch array has no device address");
+  g_scratch_ptr[device] = p;
+}
+
+static void *scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
+// cg_force with every device buffer of the call carved from the scratch
+// array (no hipMalloc / hipHostMalloc / hipFree): coefficient tensors,
+// frame arrays, neighbour and cluster lists, sort scratch, task lists and
+// results.  Results are copied to F and to a stack array.
+void force_noalloc(Ctx &c, const double *x, const cg_frame *f, double *F, double *E, double *vir) {
+  void *base = scratch_ptr(c.device);
+  if (!base) throw Error(CG_ERR_DEVICE, "scratch array not set up on this device");
+  // work of an earlier call on this stream (one that failed half way) may
+  // still use the scratch; the coefficient upload below is not stream-ordered
+  CG_HIP(hipStreamSynchronize(c.stream));
+  Arena arena;
+  arena.base = static_cast<char *>(base);
+  arena.cap = kNoAllocScratchBytes;
+  arena.reset();
+  double rest[10 + 4];
+  // installed for this call only, whichever way it ends; the context's own
+  // coefficient tensors are not the ones in the scratch
+  struct Install {
+    Ctx &c;
+    Install(Ctx &cc, Arena *a) : c(cc) { t_arena = a; c.coef_valid = false; }
+    ~Install() { t_arena = nullptr; c.coef_valid = false; }
+  };
+  try {
+    Install guard(c, &arena);
+    ++c.n_force;
+    force_batch_impl(c, x, f, 1, F, E, vir, true, rest);
+  } catch (const Error &e) {
+    if (arena.overflow)
+      throw Error(CG_ERR_MEMORY, "cg_force_noalloc: a configuration of " + std::to_string(f->natoms) +
+                                     " atoms needs more than the " + std::to_string(CG_NOALLOC_SCRATCH_MIB) +
+                                     " MiB scratch (at least " + std::to_string((arena.wanted + (1 << 20) - 1) >> 20) +
+                                     " MiB); use cg_force");
+    throw;
+  }
+}
+
 void ctx_init(Ctx &c, const cg_model *m, int device) {
   c.model = m;
   c.m = *m;
   c.device = device;
   CG_HIP(hipSetDevice(device));
+  scratch_setup(device);
   CG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (auto &e : c.ev) CG_HIP(hipEventCreate(&e));
   for (auto &fe : c.fev) for (auto &e : fe) CG_HIP(hipEventCreate(&e));
@@ -2764,6 +2829,19 @@ cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f, double *F, d
   try {
     CG_HIP(hipSetDevice(ctx->c.device));
     cg::force_batch(ctx->c, x, f, 1, F, E, virial9);
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_force_noalloc(cg_ctx *ctx, const double *x, const cg_frame *f, double *F, double *E, double *virial9) {
+  if (!ctx || !x || !f || !F) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  if (f->natoms <= 0 || !f->xyz || !f->type) { cg::set_error("cg_force_noalloc: bad frame"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(ctx->c.device));
+    cg::force_noalloc(ctx->c, x, f, F, E, virial9);
     return CG_OK;
   } catch (const cg::Error &e) {
     cg::set_error(e.what());

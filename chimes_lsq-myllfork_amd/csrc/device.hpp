@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <map>
@@ -28,11 +29,48 @@ extern std::atomic<uint64_t> g_buffer_generation;
 // grow then fails instead (allocation is not part of a capture).
 extern thread_local bool t_no_grow;
 
+struct DBuf;
+
+// Bump allocator over a fixed device region (the allocation-free force entry
+// point): while one is installed on a thread, DBuf::get hands out pieces of
+// it instead of calling hipMalloc.  A buffer asked for again during the same
+// call gets the same piece while it fits (DBuf semantics: contents are kept
+// only while the buffer does not grow).  Nothing is freed; reset() starts
+// over.
+struct Arena {
+  static constexpr int kMaxPieces = 96;
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  size_t wanted = 0;        // bytes the call would have needed (overflow report)
+  bool overflow = false;
+  int n = 0;
+  const DBuf *owner[kMaxPieces];
+  void *ptr[kMaxPieces];
+  size_t size[kMaxPieces];
+  void reset() { used = 0; wanted = 0; overflow = false; n = 0; }
+  void *take(const DBuf *b, size_t bytes) {
+    for (int i = n - 1; i >= 0; --i)
+      if (owner[i] == b) {
+        if (bytes <= size[i]) return ptr[i];
+        break;
+      }
+    const size_t off = (used + 255) & ~size_t(255);
+    wanted = std::max(wanted, off + bytes);
+    if (n >= kMaxPieces || off + bytes > cap) { overflow = true; return nullptr; }
+    owner[n] = b; ptr[n] = base + off; size[n] = bytes; ++n;
+    used = off + bytes;
+    return base + off;
+  }
+};
+// Installed by the allocation-free entry point for the duration of its call.
+extern thread_local Arena *t_arena;
+
 // Grow-only device buffer.
 struct DBuf {
   void *p = nullptr;
   size_t cap = 0;
   void *get(size_t bytes) {
+    if (t_arena) return t_arena->take(this, bytes);
     if (bytes > cap) {
       if (t_no_grow) return nullptr;
       if (p) (void)hipFree(p);

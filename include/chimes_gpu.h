@@ -344,6 +344,27 @@ cg_status cg_force(cg_ctx *ctx, const double *x, const cg_frame *f,
 cg_status cg_force_batch(cg_ctx *ctx, const double *x, const cg_frame *frames, int nframes,
                          double *F, double *E, double *virial9);
 
+/* cg_force without any device or pinned-memory allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference: it is
+ * an addition for those callers.  Every
+ * device buffer of the call (coefficient tensors, frame arrays, neighbour and
+ * cluster lists, sort scratch, task lists, results) is carved from one
+ * fixed array of CG_NOALLOC_SCRATCH_MIB MiB per GPU that the library holds
+ * until the process ends; the call uses exact list sizes (host reads of the
+ * sizes), no graphs, and returns when the results are in F, E and virial9.
+ * A configuration whose buffers do not fit is refused with CG_ERR_MEMORY and
+ * a message naming the size it needs; the need grows with atoms x density^3
+ * (measured: 4096-atom C/H/N/O configurations at 0.15-0.2 atoms/A^3 with
+ * orders 12/6/3 fit, 6000 atoms at 0.22 atoms/A^3 do not).  Calls on one GPU
+ * share the array:
+ * calls that overlap in time on the same GPU (two threads, two contexts)
+ * race and must be serialised by the caller; the array keeps the last
+ * call's contents.  Same arguments and results as cg_force. */
+#define CG_NOALLOC_SCRATCH_MIB 256
+cg_status cg_force_noalloc(cg_ctx *ctx, const double *x, const cg_frame *f, double *F, double *E,
+                           double *virial9);
+
 /* Force calls made on ctx, how many of them were repeated with exact list
  * sizes, and how many were served by replaying a captured HIP graph.  After
  * its first call a context sizes the neighbour and cluster lists of a force
