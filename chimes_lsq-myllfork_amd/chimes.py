@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libchimes_gpu.so")
 
 CG_FP64, CG_FP32 = 0, 1
+CG_FIX_ZERO_DERIV, CG_FIX_CONSTANT_DERIV, CG_FIX_SMOOTH = 0, 1, 2
 CG_GRAM_FP64, CG_GRAM_BF16X3 = 0, 1
 CG_FIT_QR = 2
 CG_ALG_SVD, CG_ALG_RIDGE, CG_ALG_LASSOLARS = 0, 1, 2
@@ -77,6 +78,7 @@ def lib(path=None):
         L.cg_params_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
         L.cg_params_coeffs.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.cg_setup_free.argtypes = [C.c_void_p]
+        L.cg_setup_set_chebyfix.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.cg_setup_model.restype = C.POINTER(Model)
         L.cg_setup_model.argtypes = [C.c_void_p]
         L.cg_setup_nframes.argtypes = [C.c_void_p]
@@ -211,6 +213,10 @@ class Setup:
         d, sc = C.c_double(), C.c_double()
         check(self.L.cg_params_coeffs(self.h, dptr(x), C.byref(d), C.byref(sc)))
         return x, d.value, sc.value
+
+    def set_chebyfix(self, fix_type, smooth_dist=0.01):
+        """Out-of-range mode below S_MINIM (chimes_md's CHEBYFIX) for contexts created afterwards."""
+        check(self.L.cg_setup_set_chebyfix(self.h, fix_type, smooth_dist))
 
     def type_names(self):
         raw = C.string_at(self.model.type_names, 8 * self.model.ntypes)

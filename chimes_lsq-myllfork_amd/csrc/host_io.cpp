@@ -88,6 +88,23 @@ cg_status cg_params_coeffs(const cg_setup *s, double *x, double *penalty_dist, d
   return CG_OK;
 }
 
+cg_status cg_setup_set_chebyfix(cg_setup *s, int fix_type, double smooth_dist) {
+  if (!s) { g_err = "null argument"; return CG_ERR_ARG; }
+  if (fix_type != CG_FIX_ZERO_DERIV && fix_type != CG_FIX_CONSTANT_DERIV && fix_type != CG_FIX_SMOOTH) {
+    g_err = "ERROR: CHEBYFIX was not recognized";
+    return CG_ERR_ARG;
+  }
+  if (fix_type == CG_FIX_SMOOTH && !(smooth_dist > 0.0)) { g_err = "CHEBYFIX SMOOTH needs a positive distance"; return CG_ERR_ARG; }
+  Setup &st = *s->s;
+  st.fix_type = fix_type;
+  st.model.fix_type = fix_type;
+  if (fix_type == CG_FIX_SMOOTH) {
+    st.smooth_dist = smooth_dist;
+    st.model.smooth_dist = smooth_dist;
+  }
+  return CG_OK;
+}
+
 void cg_setup_free(cg_setup *s) {
   if (!s) return;
   delete s->s;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <fstream>
 #include <sstream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -258,8 +259,27 @@ Setup *read_params(const std::string &path) {
           if (pt2[q] == "TOTAL:") total = to_int(pt2[q + 1]);
         }
         const int b = cs.perm_offset[t], e = cs.perm_offset[t + 1];
-        if (uniq != cs.n_unique[t] || total != e - b)
+        // UNIQUE: -1 marks a trimmed file (published force fields): the block
+        // lists only the power sets that carry a parameter, with their powers;
+        // chimes_md sums over exactly those sets (CLUSTER::read_ff_params,
+        // src/Cluster.C:511-603).  Each listed set is found in the rebuilt
+        // table by its powers; a column must be listed with every one of its
+        // permutations, as a fit produces them.
+        const bool trimmed = uniq == -1;
+        if (trimmed) {
+          if (total < 0 || total > e - b)
+            bad(path, "cluster type " + std::to_string(t) + ": TOTAL exceeds the rebuilt tables");
+        } else if (uniq != cs.n_unique[t] || total != e - b) {
           bad(path, "cluster type " + std::to_string(t) + ": UNIQUE/TOTAL differ from the rebuilt tables");
+        }
+        std::map<std::vector<int>, int> by_powers;
+        std::vector<int> listed;
+        if (trimmed) {
+          for (int i = b; i < e; ++i)
+            by_powers[std::vector<int>(cs.perm_powers + (size_t)i * cs.npairs,
+                                       cs.perm_powers + (size_t)(i + 1) * cs.npairs)] = i;
+          listed.assign(cs.n_unique[t], 0);
+        }
         size_t q = k + 3;
         while (q < P.lines.size() && P.lines[q].find("----") == std::string::npos) ++q;
         int row = 0;
@@ -268,6 +288,17 @@ Setup *read_params(const std::string &path) {
           if (r.empty()) continue;
           if ((int)r.size() != cs.npairs + 4) bad(path, "bad cluster parameter row: " + P.lines[q]);
           if (to_int(r[0]) != row) bad(path, "cluster parameter rows out of order: " + P.lines[q]);
+          if (trimmed) {
+            std::vector<int> pw(cs.npairs);
+            for (int c = 0; c < cs.npairs; ++c) pw[c] = to_int(r[1 + c]);
+            auto it = by_powers.find(pw);
+            if (it == by_powers.end()) bad(path, "power set not in the rebuilt tables: " + P.lines[q]);
+            const int col = cs.perm_col[it->second];
+            ++listed[col];
+            put(cs.col_offset[t] + col, to_double(r[cs.npairs + 3]), P.lines[k + 1]);
+            ++row;
+            continue;
+          }
           for (int c = 0; c < cs.npairs; ++c)
             if (to_int(r[1 + c]) != cs.perm_powers[(size_t)(b + row) * cs.npairs + c])
               bad(path, "power set differs from the rebuilt tables: " + P.lines[q]);
@@ -277,6 +308,14 @@ Setup *read_params(const std::string &path) {
           ++row;
         }
         if (row != total) bad(path, "truncated cluster parameter block " + std::to_string(t));
+        if (trimmed) {
+          std::vector<int> nperm(cs.n_unique[t], 0);
+          for (int i = b; i < e; ++i) ++nperm[cs.perm_col[i]];
+          for (int col = 0; col < cs.n_unique[t]; ++col)
+            if (listed[col] && listed[col] != nperm[col])
+              bad(path, "cluster type " + std::to_string(t) + " lists " + std::to_string(listed[col]) + " of the " +
+                            std::to_string(nperm[col]) + " permutations of one power set");
+        }
         k = q;
       }
     }

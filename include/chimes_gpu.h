@@ -153,6 +153,11 @@ cg_status cg_params_read(const char *path, cg_setup **out);
 /* Coefficients in A column order (x[ncols]; energy offsets last) and the
  * file's 2-body penalty settings. */
 cg_status cg_params_coeffs(const cg_setup *s, double *x, double *penalty_dist, double *penalty_scale);
+/* Out-of-range treatment of pair distances below S_MINIM (CG_FIX_*, with the
+ * SMOOTH distance): chimes_md's # CHEBYFIX # run option
+ * (INPUT::PARSE_CONTROLS_CHEBYFIX, src/input.C:713-747), which a params.txt
+ * does not carry.  Applies to contexts created afterwards from this setup. */
+cg_status cg_setup_set_chebyfix(cg_setup *s, int fix_type, double smooth_dist);
 const cg_model *cg_setup_model(const cg_setup *s);
 int cg_setup_nframes(const cg_setup *s);
 /* rows of A produced by a frame (3N + stress rows + energy rows) */

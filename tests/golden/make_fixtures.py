@@ -314,6 +314,25 @@ def main():
         json.dump({"abs_energy_error_per_atom": None, "ktot_per_atom": float(st.group(1)),
                    "vtot_per_atom": float(st.group(2)), "pressure_gpa": float(st.group(5)), "sigma_gpa": sig},
                   open(os.path.join(out, "expected.json"), "w"), indent=1, sort_keys=True)
+    # chimes_md's CHEBYFIX case: the published HN3 2+3+4-body force field
+    # (Tersoff cutoffs, special 3B/4B cutoffs, energy offsets, a trimmed
+    # parameter file) with its run's CHEBYFIX setting and first frame.  The run
+    # starts from velocities its own generator draws at 20000 K, so none of its
+    # printed numbers is a function of this frame alone: the case pins the model
+    # and frame for GPU-vs-oracle parity of every out-of-range mode.
+    src = os.path.join(md, "chebyfix")
+    out = os.path.join(HERE, "md", "chebyfix")
+    os.makedirs(out, exist_ok=True)
+    run = open(os.path.join(src, "run_md.in")).read()
+    crd = re.search(r"# CRDFILE #[^\n]*\n\s*(\S+)", run).group(1)
+    prm = re.search(r"# PRMFILE #[^\n]*\n\s*(\S+)", run).group(1)
+    fix = re.search(r"# CHEBYFIX #[^\n]*\n\s*(\S+)\s+(\S+)", run)
+    lines = open(os.path.join(src, crd)).read().split("\n")
+    n = int(lines[0].split()[0])
+    open(os.path.join(out, "frame.xyzf"), "w").write("\n".join(lines[:n + 2]) + "\n")
+    gz(os.path.join(src, prm), os.path.join(out, "params.txt.gz"))
+    json.dump({"chebyfix": fix.group(1), "smooth_distance": float(fix.group(2)), "nlayers": 1},
+              open(os.path.join(out, "expected.json"), "w"), indent=1, sort_keys=True)
     print("fixtures written to", HERE)
 
 

@@ -167,3 +167,39 @@ def test_autotune_configurations_give_the_same_A(ch, synth, oracle, tmp_path):
     Ao = oracle.assemble(s, fr)
     assert colscaled_error(A1, Ao) < 1e-10
     dev.close()
+
+
+def test_full_size_carbon_frame_matches_oracle(ch, synth, oracle, tmp_path):
+    """One frame of the benchmark workload at full size (512 carbon atoms at
+    2.2 g/cc, orders 16/9/5, 1542 rows x 1038 columns) against the oracle, FP64
+    and FP32."""
+    print("512-atom carbon frame, orders 16/9/5", flush=True)
+    s, fr = synth_setup(ch, synth, tmp_path, synth.carbon(nframes=1, natoms=512, seed=102))
+    dev = ch.Device(s)
+    A, b = dev.assemble(fr)
+    A32, _ = dev.assemble(fr, prec=ch.CG_FP32)
+    dev.close()
+    Ao = oracle.assemble(s, fr)
+    assert A.shape == (1542, 1038)
+    err = colscaled_error(A, Ao)
+    print("FP64 colscaled error %.2e" % err)
+    assert err <= TOL64
+    assert fp32_ok(A32, Ao)
+
+
+def test_large_triclinic_chno_frame_matches_oracle(ch, synth, oracle, tmp_path):
+    """A 2048-atom C/H/N/O frame in a triclinic cell (orders 12/6/3, 10 504
+    columns, cell lists) against the oracle."""
+    print("2048-atom C/H/N/O triclinic frame, orders 12/6/3", flush=True)
+    gen = synth.chno(nframes=1, nmin=2048, nmax=2048, seed=104)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    cell = np.asarray(fr[0].cell).reshape(3, 3)
+    assert np.count_nonzero(cell - np.diag(np.diag(cell))) > 0  # triclinic
+    dev = ch.Device(s)
+    A, b = dev.assemble(fr)
+    dev.close()
+    Ao = oracle.assemble(s, fr)
+    assert A.shape == (3 * 2048 + 3 + 3, s.model.ncols)
+    err = colscaled_error(A, Ao)
+    print("FP64 colscaled error %.2e" % err)
+    assert err <= TOL64

@@ -232,3 +232,36 @@ def test_force_rows_are_A_x_and_gradient(ch, synth, oracle, tmp_path):
         Ep = oracle.force(s, ch.PyFrame(f.natoms, xp, f.types, f.cell), x)[1]
         Em = oracle.force(s, ch.PyFrame(f.natoms, xm, f.types, f.cell), x)[1]
         assert F[k] == pytest.approx(-(Ep - Em) / (2 * h), rel=1e-5, abs=1e-6 * np.abs(F).max())
+
+
+@pytest.mark.parametrize("fcut", ["CUBIC", "TERSOFF 0.5"])
+@pytest.mark.parametrize("mode,dist", [("SMOOTH", 0.01), ("SMOOTH", 0.025), ("CONSTANT_DERIV", None),
+                                       ("ZERO_DERIV", None)])
+def test_oracle_force_is_minus_gradient_below_smin(ch, synth, oracle, tmp_path, mode, dist, fcut):
+    """The oracle's forces in every out-of-range mode are -dE/dx of its own
+    energy (central differences) on 2+3+4-body C/H configurations whose 3- and
+    4-body clusters hold pairs below S_MINIM: each mode's extrapolation and its
+    derivative belong together, so a dropped term or a wrong sign in either
+    shows here."""
+    from chebyfix_cases import SMIN, close_frames, fm_setup_text, pairs_below_smin
+    text = fm_setup_text(synth, mode, dist, orders=(8, 5, 4)).replace("# FCUTTYP #\n\tCUBIC", "# FCUTTYP #\n\t" + fcut)
+    p = tmp_path / "fm_setup.in"
+    open(p, "w").write(text)
+    s = ch.Setup(str(p))
+    f = close_frames(ch, synth, n=1, natoms=40, seed=83)[0]
+    assert pairs_below_smin(f, ["C", "H"], SMIN, 3.5) >= 3
+    x = np.random.default_rng(31).normal(size=s.model.ncols)
+    F, E, V = oracle.force(s, f, x, penalty_scale=0.0)
+    xyz = np.array(f.xyz).reshape(-1, 3)
+    atoms = np.argsort(-np.abs(F.reshape(-1, 3)).max(axis=1))[:4]
+    h = 1e-6
+    for a in atoms:
+        for c in range(3):
+            e = []
+            for sgn in (1, -1):
+                y = xyz.copy()
+                y[a, c] += sgn * h
+                g = ch.PyFrame(f.natoms, y, f.types, f.cell)
+                e.append(oracle.force(s, g, x, penalty_scale=0.0)[1])
+            fd = -(e[0] - e[1]) / (2 * h)
+            assert fd == pytest.approx(F[3 * a + c], rel=1e-6, abs=1e-7 * np.abs(F).max())

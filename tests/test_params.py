@@ -55,3 +55,46 @@ def test_params_reader_refuses_coulomb(ch, tmp_path):
     p.write_text(text.replace("   PAIRS: OO OO OO UNIQUE: 7 TOTAL: 20", "   PAIRS: OO OO OO UNIQUE: 8 TOTAL: 20"))
     with pytest.raises(RuntimeError, match="UNIQUE/TOTAL"):
         ch.Setup.from_params(str(p))
+
+
+def test_trimmed_params_file_chebyfix_case(ch, tmp_path):
+    """The reference's published HN3 force field lists only the power sets that
+    carry parameters (UNIQUE: -1); each listed set lands on its column, and a
+    column is listed with all its permutations."""
+    import gzip
+    import os
+    import re
+    from chebyfix_cases import CHEBYFIX_DIR
+    p = os.path.join(str(tmp_path), "params.txt")
+    text = gzip.open(os.path.join(CHEBYFIX_DIR, "params.txt.gz"), "rb").read().decode()
+    open(p, "w").write(text)
+    s = ch.Setup.from_params(p)
+    x, pd, ps = s.coeffs()
+    assert (pd, ps) == (0.1, 1.0e4)
+    m = s.model
+    assert (m.order2b, m.c3.order, m.c4.order) == (12, 8, 4)
+    assert m.fcut_type == 1 and m.fcut_offset == 0.5 and m.n_offset_cols == 2
+    # every parameter of the file is in x (the listed sets of one column share it)
+    vals = set()
+    for blk in re.split(r"(?:TRIPLETTYPE|QUADRUPLETYPE) PARAMS:", text)[1:]:
+        rows = [l.split() for l in blk.split("----------------------------------------------------------------------------")[1].split("\n")]
+        for r in rows:
+            if len(r) in (7, 10):
+                vals.add(float(r[-1]))
+            elif r and not r[0].isdigit():
+                break
+    xs = set(x[m.n2b_cols:m.ncols_short][x[m.n2b_cols:m.ncols_short] != 0].tolist())
+    assert vals - {0.0} == xs
+    # a block that lists only part of a column's permutations is refused
+    lines = text.split("\n")
+    head = next(i for i, l in enumerate(lines) if "UNIQUE: -1 TOTAL: 390" in l)
+    drop = head + 3 + 388  # row 388, one of three permutations of its column
+    assert lines[drop].split()[0] == "388"
+    pcol = lines[drop].split()[-2]
+    assert sum(1 for l in lines[head + 3:drop] if l.split()[-2:-1] == [pcol]) == 2
+    lines[head] = lines[head].replace("TOTAL: 390", "TOTAL: 389")
+    lines[drop + 1] = "388" + lines[drop + 1][3:]  # renumber the row after it
+    del lines[drop]
+    open(p, "w").write("\n".join(lines))
+    with pytest.raises(RuntimeError, match="permutations"):
+        ch.Setup.from_params(p)
