@@ -307,6 +307,8 @@ struct EnumArgs {
   const int *cct;
   const double *csmax;
   const int *cperm;
+  const int *cslot;     // canonical pair slot of each enumerated pair slot (PAIR_INDICES)
+  unsigned long long *slot_min;  // fit statistics: per (cluster type, slot) smallest distance, or null
   const int *int_pair_map;
   int md;               // force-evaluation pair set
 };
@@ -323,7 +325,7 @@ __device__ __forceinline__ double dist3(const double4 &u, const double4 &v) {
 // pair distances in enumeration order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
 template <int NB>
 __device__ __forceinline__ int cluster_type_c(const EnumArgs &a, const int *types, const double *d,
-                                              const int *&perm) {
+                                              const int *&perm, int &cid_out) {
   constexpr int NP = NB * (NB - 1) / 2;
   int cid = 0;
 #pragma unroll
@@ -335,7 +337,23 @@ __device__ __forceinline__ int cluster_type_c(const EnumArgs &a, const int *type
   for (int pf = 0; pf < NP; ++pf)
     if (!(d[pf] < row[pf])) return -1;
   perm = a.cperm + cid * NB;
+  cid_out = cid;
   return ct;
+}
+
+// Smallest distance of each canonical pair slot of a cluster type, as the
+// reference tracks it (MIN_FOUND[PAIR_INDICES[f]] = r_f, src/Cheby.C:1059-1078,
+// 1506-1519): bit patterns of positive doubles order like the values, and the
+// plain read first keeps the atomics to the few clusters that lower a minimum.
+template <int NP>
+__device__ __forceinline__ void record_slot_min(const EnumArgs &a, int ct, int cid, const double *d) {
+  const int *slot = a.cslot + cid * NP;
+#pragma unroll
+  for (int pf = 0; pf < NP; ++pf) {
+    unsigned long long *m = a.slot_min + ct * NP + slot[pf];
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, d[pf]);
+    if (b < __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(m, b);
+  }
 }
 
 // 2-body enumeration (the pair loop of Deriv_2B / Force_all): one thread per
@@ -493,12 +511,14 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
         if (NB == 3) {
           const double d[3] = {v2.w, v3.w, r23};
           const int *perm;
-          const int ct = cluster_type_c<3>(a, types, d, perm);
+          int cid;
+          const int ct = cluster_type_c<3>(a, types, d, perm, cid);
           if (ct >= 0) {
             if (pass == 1 && base + m < a.ccap) {
               const uint32_t rr[3] = {r1, r2, r3};
               for (int x = 0; x < 3; ++x) a.refs[3 * (base + m) + x] = rr[perm[x]];
               a.key[base + m] = kbase + ct;
+              if (a.slot_min) record_slot_min<3>(a, ct, cid, d);
             }
             ++m;
           }
@@ -534,7 +554,8 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
             types[3] = st.typ(k4);
             const double d[6] = {v2.w, v3.w, v4.w, r23, r24, r34};
             const int *perm;
-            const int ct = cluster_type_c<4>(a, types, d, perm);
+            int cid;
+            const int ct = cluster_type_c<4>(a, types, d, perm, cid);
             if (ct < 0) continue;
             if (pass == 0 && adj) {
 #pragma unroll
@@ -545,6 +566,7 @@ __device__ __forceinline__ int enum_atom(const EnumArgs &a, const S &st, int n, 
               const uint32_t rr[4] = {r1, r2, r3, r4};
               for (int x = 0; x < 4; ++x) a.refs[4 * (base + m) + x] = rr[perm[x]];
               a.key[base + m] = kbase + ct;
+              if (a.slot_min) record_slot_min<6>(a, ct, cid, d);
             }
             ++m;
           }
@@ -615,6 +637,65 @@ __global__ __launch_bounds__(256) void k_enum_wave(EnumArgs a) {
     tot = enum_atom<NB, COUNT>(a, StageGlobal{a.nvec + b, a.nlist + b, a.ntyp + b}, n, qa, qb, !f.small, r1, t1,
                                kbase, out, nullptr);
   if (COUNT && lane == 0) a.count[slot] = tot;
+}
+
+// Fit statistics over the neighbour list (padded by the reference's 0.3 A):
+// the smallest distance of each pair type among the pairs of its 2-body list
+// (MIN_FOUND_DIST, src/Cheby.C:769-770), entries closer than cut only.
+__global__ void k_pair_min(const int *type, const int *nown, const int *ntyp, const double4 *nvec, const int *ntot,
+                           int nnb, const int *int_pair_map, int ntypes, double cut, unsigned long long *pmin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnb || k >= *ntot) return;
+  const double r = nvec[k].w;
+  if (!(r < cut)) return;
+  unsigned long long *m = pmin + int_pair_map[type[nown[k]] * ntypes + ntyp[k]];
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, r);
+  if (b < __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(m, b);
+}
+
+// Size of the reference's flat 3- or 4-body interaction list of each frame
+// (NEIGHBORS::UPDATE_3B/4B_INTERACTION, src/ClassDefs.C:1513-1625): from
+// atom i's list at the family cutoff plus padding (half list, every image in
+// small cells), the ordered entry pairs (j, k), j != k, parent(j) <=
+// parent(k) unless small, with r_jk < cut (3-body) or r_jk <= cut and a
+// third entry l != j, k, parent(k), parent(j) <= parent(l) unless small,
+// r_jl < cut and r_kl < cut (4-body).  One wave per atom, lanes over (j, k).
+template <int NB>
+__global__ __launch_bounds__(256) void k_flat_count(const DevFrame *frames, const int *atom_frame, const int *noff,
+                                                    const uint32_t *nlist, const double4 *nvec, int natoms, int nnb,
+                                                    double cut, unsigned long long *per_frame) {
+  const int gi = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (gi >= natoms) return;
+  const int b = min(noff[gi], nnb), n = min(noff[gi + 1], nnb) - b;
+  const int fidx = atom_frame[gi];
+  const bool ordered = !frames[fidx].small;
+  unsigned long long cnt = 0;
+  for (int q = lane; q < n * n; q += 64) {
+    const int j = q / n, k = q - j * n;
+    if (j == k) continue;
+    const double4 vj = nvec[b + j], vk = nvec[b + k];
+    if (!(vj.w < cut) || !(vk.w < cut)) continue;
+    const int pj = ref_atom(nlist[b + j]), pk = ref_atom(nlist[b + k]);
+    if (ordered && pj > pk) continue;
+    const double rjk = dist3(vj, vk);
+    if (NB == 3) {
+      if (rjk < cut) ++cnt;
+      continue;
+    }
+    if (rjk > cut) continue;
+    for (int l = 0; l < n; ++l) {
+      if (l == j || l == k) continue;
+      const double4 vl = nvec[b + l];
+      if (!(vl.w < cut)) continue;
+      const int pl = ref_atom(nlist[b + l]);
+      if (ordered && (pk > pl || pj > pl)) continue;
+      if (dist3(vj, vl) < cut && dist3(vk, vl) < cut) ++cnt;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  if (lane == 0 && cnt) atomicAdd(per_frame + fidx, cnt);
 }
 
 __global__ void k_iota(int *v, int n) {
@@ -1667,7 +1748,7 @@ static void build_family(Ctx &c, int which) {
     const int nt = m.ntypes, nb = cs.nbody, np = cs.npairs;
     int ntup = 1;
     for (int q = 0; q < nb; ++q) ntup *= nt;
-    std::vector<int> cct(ntup, -1), cperm((size_t)ntup * nb, 0);
+    std::vector<int> cct(ntup, -1), cperm((size_t)ntup * nb, 0), cslot((size_t)ntup * np, 0);
     std::vector<double> csmax((size_t)ntup * np, -1.0);
     for (int cid = 0; cid < ntup; ++cid) {
       int id = 0, w = 1, r = cid;
@@ -1685,10 +1766,12 @@ static void build_family(Ctx &c, int which) {
         const int cp = cs.id_pair_map[(size_t)id * np + pf];
         if (cp < 0 || cp >= np) throw Error(CG_ERR_INPUT, "bad cluster pair map");
         csmax[(size_t)cid * np + pf] = cs.s_max[ctp * np + cp];
+        cslot[(size_t)cid * np + pf] = cp;
       }
     }
     F.cct = dalloc_copy(cct);
     F.cperm = dalloc_copy(cperm);
+    F.cslot = dalloc_copy(cslot);
     F.csmax = dalloc_copy(csmax);
   }
   F.colptr = dalloc_copy(colptr);
@@ -1866,6 +1949,10 @@ static void zero_d(Ctx &c, double *p, size_t n) {
 // least-squares one (S_MINIM < r < S_MAXIM, Cheby::Deriv_2B).
 // Host part of a call: frame table (geometry, row offsets, small-cell and
 // cell-list decisions), batch arrays, and their upload.
+// The reference pads its neighbour lists by 0.3 A (NEIGHBORS::RCUT_PADDING,
+// src/ClassDefs.C:16); the fit statistics count over padded lists.
+constexpr double kListPadding = 0.3;
+
 static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, bool sticky) {
   std::vector<DevFrame> &hf = B.hf;
   B.frames = frames;
@@ -1882,6 +1969,8 @@ static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, 
   double mx = m.max_cutoff_2b;
   if (m.c3.nbody) mx = std::max(mx, m.c3.max_cutoff);
   if (m.c4.nbody) mx = std::max(mx, m.c4.max_cutoff);
+  // list cutoff: the largest family cutoff, padded when a fit gathers statistics
+  const double list_rc = mx + ((sticky && c.stats.on) ? kListPadding : 0.0);
   for (int i = 0; i < nframes; ++i) {
     const cg_frame &fr = frames[i];
     if (fr.natoms <= 0 || fr.natoms >= (1 << 20)) throw Error(CG_ERR_ARG, "bad atom count");
@@ -1902,7 +1991,7 @@ static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, 
       d.nlayers = m.nlayers;
     } else {
       const double wmin = std::min(d.width[0], std::min(d.width[1], d.width[2]));
-      d.nlayers = std::max(1, (int)std::ceil(mx / wmin));
+      d.nlayers = std::max(1, (int)std::ceil(list_rc / wmin));
       if (d.nlayers > kMaxLayers)
         throw Error(CG_ERR_INPUT, "configuration " + std::to_string(i) + ": cell width " + std::to_string(wmin) +
                                       " A is below cutoff / " + std::to_string(kMaxLayers) +
@@ -1929,7 +2018,7 @@ static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, 
       int nb[3];
       bool ok = true;
       for (int r = 0; r < 3; ++r) {
-        nb[r] = (int)std::min(256.0, std::floor(d.width[r] / mx));
+        nb[r] = (int)std::min(256.0, std::floor(d.width[r] / list_rc));
         ok = ok && nb[r] >= 3;
       }
       if (ok && (long long)nb[0] * nb[1] * nb[2] < 125) ok = false;
@@ -1970,7 +2059,7 @@ static void prepare_host(Ctx &c, const cg_frame *frames, int nframes, Batch &B, 
   B.natoms = natoms;
   B.rows = rows;
   B.nbins_total = nbins_total;
-  B.mx = mx;
+  B.mx = list_rc;
   B.d_frames = d_frames;
   B.d_af = d_af;
   B.d_type = d_type;
@@ -2035,6 +2124,31 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
   hipLaunchKernelGGL(k_neigh<true>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, d_frames, d_af, d_wrap,
                      (const int *)d_type, d_bin_start, d_bin_atom, natoms, mx, (int *)nullptr,
                      (const int *)d_noff, d_nlist, d_nvec, d_nown, d_ntyp, (int)nn1);
+  // ---- fit statistics over the padded lists: pair minima, flat list sizes
+  if (c.stats.on && !md && nnb > 0) {
+    LsqStats &S = c.stats;
+    hipLaunchKernelGGL(k_pair_min, dim3((nnb + TB - 1) / TB), dim3(TB), 0, c.stream, (const int *)d_type,
+                       (const int *)d_nown, (const int *)d_ntyp, (const double4 *)d_nvec, (const int *)(d_noff + natoms),
+                       nnb, (const int *)c.d_int_pair_map, m.ntypes, m.max_cutoff_2b + kListPadding, S.d_min);
+    if (S.flat_cap < 2 * (size_t)nframes) {
+      if (S.d_flat) (void)hipFree(S.d_flat);
+      S.d_flat = nullptr;
+      S.flat_cap = 0;
+      CG_HIP(hipMalloc(&S.d_flat, 2 * (size_t)nframes * sizeof(long long)));
+      S.flat_cap = 2 * (size_t)nframes;
+    }
+    CG_HIP(hipMemsetAsync(S.d_flat, 0, 2 * (size_t)nframes * sizeof(long long), c.stream));
+    unsigned long long *flat = reinterpret_cast<unsigned long long *>(S.d_flat);
+    if (c.fam[1].ntypes > 0 && c.fam[1].ncols > 0)
+      hipLaunchKernelGGL(k_flat_count<3>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, (const DevFrame *)d_frames,
+                         (const int *)d_af, (const int *)d_noff, (const uint32_t *)d_nlist, (const double4 *)d_nvec,
+                         natoms, nnb, c.fam[1].cutoff + kListPadding, flat);
+    if (c.fam[2].ntypes > 0 && c.fam[2].ncols > 0)
+      hipLaunchKernelGGL(k_flat_count<4>, dim3((natoms + 3) / 4), dim3(256), 0, c.stream, (const DevFrame *)d_frames,
+                         (const int *)d_af, (const int *)d_noff, (const uint32_t *)d_nlist, (const double4 *)d_nvec,
+                         natoms, nnb, c.fam[2].cutoff + kListPadding, flat + nframes);
+    CG_HIP(hipGetLastError());
+  }
   // ---- per-family sublists (3B / 4B cutoffs below the list cutoff), sized
   // by the full list so that no host read is needed
   const uint32_t *fl_list[3] = {d_nlist, d_nlist, d_nlist};
@@ -2088,7 +2202,8 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
     E.cutoff = F.cutoff;
     E.count = d_ccount + w * cslot;
     E.coff = d_coff + w * cslot;
-    E.cct = F.cct; E.csmax = F.csmax; E.cperm = F.cperm;
+    E.cct = F.cct; E.csmax = F.csmax; E.cperm = F.cperm; E.cslot = F.cslot;
+    E.slot_min = (c.stats.on && !md && w > 0) ? c.stats.d_min + (w == 1 ? c.stats.off3 : c.stats.off4) : nullptr;
     E.smin = F.smin; E.smax = F.smax; E.int_pair_map = c.d_int_pair_map;
     E.md = md;
     nitems[w] = w == 0 ? nnb : 4 * natoms;
@@ -2264,6 +2379,26 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
       hipLaunchKernelGGL(k_offsets<double>, dim3(nframes), dim3(256), 0, c.stream, d_frames, d_type, nframes, m.ntypes, m.ncols_short, (double *)A, ld);
   }
   if (m.n_charge_cols > 0) ewald_columns(c, frames, hf, B.d_af, d_type, B.d_wrap, B.natoms, A, ld, prec);
+  if (c.stats.on) {
+    LsqStats &S = c.stats;
+    for (int w = 0; w < 3; ++w) {
+      const DevFamily &F = c.fam[w];
+      if (S.count[w].size() != (size_t)F.ntypes) S.count[w].assign(F.ntypes, 0);
+      if (fo[w].boff.size() < (size_t)nframes * F.ntypes + 1 || fo[w].ncl == 0) continue;
+      for (int fi = 0; fi < nframes; ++fi)
+        for (int t = 0; t < F.ntypes; ++t)
+          S.count[w][t] += fo[w].boff[(size_t)fi * F.ntypes + t + 1] - fo[w].boff[(size_t)fi * F.ntypes + t];
+    }
+    S.flat3.assign(nframes, 0);
+    S.flat4.assign(nframes, 0);
+    if (S.d_flat && B.natoms > 0) {
+      std::vector<long long> h(2 * (size_t)nframes);
+      CG_HIP(hipMemcpyAsync(h.data(), S.d_flat, h.size() * sizeof(long long), hipMemcpyDeviceToHost, c.stream));
+      CG_HIP(hipStreamSynchronize(c.stream));
+      std::copy(h.begin(), h.begin() + nframes, S.flat3.begin());
+      std::copy(h.begin() + nframes, h.end(), S.flat4.begin());
+    }
+  }
   CG_HIP(hipEventRecord(c.ev[2], c.stream));
   CG_HIP(hipEventSynchronize(c.ev[2]));
   CG_HIP(hipGetLastError());
@@ -2665,10 +2800,13 @@ void ctx_free(Ctx &c) {
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
                     (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.recip, (void *)F.cct, (void *)F.csmax,
-                    (void *)F.cperm, (void *)F.sets8})
+                    (void *)F.cperm, (void *)F.cslot, (void *)F.sets8})
       if (p) (void)hipFree(p);
   }
   drop_graphs(c);
+  if (c.stats.d_min) (void)hipFree(c.stats.d_min);
+  if (c.stats.d_flat) (void)hipFree(c.stats.d_flat);
+  c.stats = LsqStats{};
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   c.h_pin = nullptr;
   c.h_pin_cap = 0;
@@ -2894,6 +3032,78 @@ cg_status cg_set_penalty(cg_ctx *ctx, double dist, double scale) {
   }
   c.penalty_dist = dist;
   c.penalty_scale = scale;
+  return CG_OK;
+}
+
+cg_status cg_stats_enable(cg_ctx *ctx, int on) {
+  if (!ctx) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  try {
+    cg::Ctx &c = ctx->c;
+    CG_HIP(hipSetDevice(c.device));
+    cg::LsqStats &S = c.stats;
+    S.on = on != 0;
+    if (!S.on) return CG_OK;
+    const int n2 = c.fam[0].ntypes, n3 = c.fam[1].ntypes * 3, n4 = c.fam[2].ntypes * 6;
+    S.off3 = n2;
+    S.off4 = n2 + n3;
+    if (S.n_min < n2 + n3 + n4 || !S.d_min) {
+      if (S.d_min) (void)hipFree(S.d_min);
+      S.d_min = nullptr;
+      CG_HIP(hipMalloc(&S.d_min, std::max(1, n2 + n3 + n4) * sizeof(unsigned long long)));
+    }
+    S.n_min = n2 + n3 + n4;
+    // every minimum starts at 1e10, the reference's "not found" value
+    std::vector<unsigned long long> init(std::max(1, S.n_min), __builtin_bit_cast(unsigned long long, 1.0e10));
+    CG_HIP(hipMemcpy(S.d_min, init.data(), init.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    for (int w = 0; w < 3; ++w) S.count[w].assign(c.fam[w].ntypes, 0);
+    S.flat3.clear();
+    S.flat4.clear();
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_stats_get(cg_ctx *ctx, cg_lsq_stats *out) {
+  if (!ctx || !out) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  cg::Ctx &c = ctx->c;
+  cg::LsqStats &S = c.stats;
+  if (!S.d_min) { cg::set_error("statistics were not enabled"); return CG_ERR_ARG; }
+  try {
+    CG_HIP(hipSetDevice(c.device));
+    std::vector<unsigned long long> h(std::max(1, S.n_min));
+    CG_HIP(hipStreamSynchronize(c.stream));
+    CG_HIP(hipMemcpy(h.data(), S.d_min, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    auto val = [&](int i) { return __builtin_bit_cast(double, h[i]); };
+    const int n2 = c.fam[0].ntypes, nt3 = c.fam[1].ntypes, nt4 = c.fam[2].ntypes;
+    for (int t = 0; t < n2; ++t) {
+      if (out->pair_min) out->pair_min[t] = val(t);
+      if (out->pair_count) out->pair_count[t] = S.count[0].empty() ? 0 : S.count[0][t];
+    }
+    for (int t = 0; t < nt3; ++t) {
+      for (int k = 0; k < 3; ++k) if (out->trip_min) out->trip_min[3 * t + k] = val(S.off3 + 3 * t + k);
+      if (out->trip_count) out->trip_count[t] = S.count[1].empty() ? 0 : S.count[1][t];
+    }
+    for (int t = 0; t < nt4; ++t) {
+      for (int k = 0; k < 6; ++k) if (out->quad_min) out->quad_min[6 * t + k] = val(S.off4 + 6 * t + k);
+      if (out->quad_count) out->quad_count[t] = S.count[2].empty() ? 0 : S.count[2][t];
+    }
+    return CG_OK;
+  } catch (const cg::Error &e) {
+    cg::set_error(e.what());
+    return e.code;
+  }
+}
+
+cg_status cg_stats_frames(cg_ctx *ctx, int64_t *flat3, int64_t *flat4, int *nframes) {
+  if (!ctx) { cg::set_error("null argument"); return CG_ERR_ARG; }
+  const cg::LsqStats &S = ctx->c.stats;
+  if (nframes) *nframes = (int)S.flat3.size();
+  for (size_t i = 0; i < S.flat3.size(); ++i) {
+    if (flat3) flat3[i] = S.flat3[i];
+    if (flat4) flat4[i] = S.flat4[i];
+  }
   return CG_OK;
 }
 

@@ -7,11 +7,15 @@
 //                  [--refine K] [--batch FRAMES] [--ranks N] [--no-A] [--no-text]
 //
 // Files written in the working directory: A.txt b.txt b-labeled.txt natoms.txt
-// dim.txt params.header ff_groups.map; with --fit also params.txt force.txt.
+// dim.txt params.header ff_groups.map frames.NNNN.log; with --fit also
+// params.txt force.txt.
 // --ranks N lays the frames out as the reference's N MPI ranks would process
 // them (contiguous blocks, or round robin with SKPFRMS), rows in rank order;
 // with SPLITFI true each rank's rows go to A.<rank>.txt with dim.<rank>.txt.
 // --no-A skips the A text (b, natoms and dim files are still written).
+// Standard output is what chimes_lsq prints on rank 0 of an N-rank run (the
+// reference test suite's fm_setup.out); this program's own summary (GPU
+// times, the fit) goes to standard error.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -74,10 +78,7 @@ int main(int argc, char **argv) {
   cg_status st = cg_setup_read(infile, &setup);
   if (st) return die("reading input", st);
   const cg_model *m = cg_setup_model(setup);
-  std::printf("Reading input file... %s\n", infile);
-  std::printf("The number of two-body non-coulomb parameters is: %d\n", m->n2b_cols);
-  if (m->c3.nbody) std::printf("The number of three-body Chebyshev parameters is: %d\n", m->c3.ncols);
-  if (m->c4.nbody) std::printf("The number of four-body  Chebyshev parameters is: %d\n", m->c4.ncols);
+  if ((st = cg_report_setup(setup, stdout))) return die("report", st);
   // read every frame
   std::string dir = ".";
   cg_traj *traj = nullptr;
@@ -104,6 +105,9 @@ int main(int argc, char **argv) {
   if ((st = cg_frame_order(setup, nranks, order.data(), rank_of.data(), &nproc))) return die("frame order", st);
   cg_ctx *ctx = nullptr;
   if ((st = cg_create(m, 0, &ctx))) return die("creating device context", st);
+  if ((st = cg_stats_enable(ctx, 1))) return die("statistics", st, ctx);
+  std::vector<int64_t> flat3(nframes, 0), flat4(nframes, 0);
+  std::vector<double> frame_ms(nframes, 0.0);
   if (text && (st = cg_write_open_opts(setup, dir.c_str(), write_A ? 0 : CG_WRITE_NO_A)))
     return die("opening A/b files", st);
   std::vector<cg_frame> cf(nproc);
@@ -123,6 +127,18 @@ int main(int argc, char **argv) {
     double te, ta, tt;
     cg_last_timing(ctx, &te, &ta, &tt);
     ms_asm += tt;
+    {
+      std::vector<int64_t> f3(nb), f4(nb);
+      int got = 0;
+      if ((st = cg_stats_frames(ctx, f3.data(), f4.data(), &got))) return die("statistics", st, ctx);
+      for (int i = 0; i < nb && i < got; ++i) {
+        flat3[order[f0 + i]] = f3[i];
+        flat4[order[f0 + i]] = f4[i];
+      }
+      // a frame's share of the batch's GPU time, by its rows
+      for (int i = 0; i < nb; ++i)
+        frame_ms[order[f0 + i]] = tt * cg_model_frame_rows(m, cf[f0 + i].natoms, cf[f0 + i].index) / std::max<int64_t>(rows, 1);
+    }
     b_all.insert(b_all.end(), b.begin(), b.end());
     if (text) {
       std::vector<double> host(write_A ? (size_t)rows * m->ncols : 0);
@@ -140,8 +156,42 @@ int main(int argc, char **argv) {
   }
   if (text && (st = cg_write_close(setup))) return die("closing A/b files", st);
   if ((st = cg_write_header(setup, dir.c_str()))) return die("writing params.header", st);
-  std::printf("...matrix population complete: %lld rows x %d columns, %.3f ms on the GPU (%.3e rows/s)\n",
-              (long long)total_rows, m->ncols, ms_asm, total_rows / (ms_asm * 1e-3));
+  {
+    // the reference's standard output, rank 0 of an nranks run: every frame
+    // read in order, rank 0's frames processed, then the statistics
+    std::vector<int> rank_in(nframes, -1);
+    for (int i = 0; i < nproc; ++i) rank_in[order[i]] = rank_of[i];
+    bool first = true;
+    int last0 = -1;
+    int64_t force_rows = 0;
+    for (int i = 0; i < nframes; ++i) {
+      if ((st = cg_report_frame_read(setup, &frames[i].f, stdout))) return die("report", st);
+      if (rank_in[i] >= 0) force_rows += 3LL * frames[i].f.natoms;
+      if (rank_in[i] != 0) continue;
+      if ((st = cg_report_frame_done(setup, &frames[i].f, first ? 1 : 0, nranks, flat3[i], flat4[i], stdout)))
+        return die("report", st);
+      first = false;
+      last0 = i;
+    }
+    std::vector<int> counts(m->ntypes, 0);
+    if (last0 >= 0)
+      for (int a = 0; a < frames[last0].f.natoms; ++a) ++counts[frames[last0].f.type[a]];
+    const int nt3 = m->c3.nbody ? m->c3.ntypes : 0, nt4 = m->c4.nbody ? m->c4.ntypes : 0;
+    std::vector<double> pmin(m->npairs), tmin(3 * (size_t)nt3 + 1), qmin(6 * (size_t)nt4 + 1);
+    std::vector<int64_t> pcnt(m->npairs), tcnt(nt3 + 1), qcnt(nt4 + 1);
+    cg_lsq_stats sts{pmin.data(), pcnt.data(), tmin.data(), tcnt.data(), qmin.data(), qcnt.data()};
+    if ((st = cg_stats_get(ctx, &sts))) return die("statistics", st, ctx);
+    if ((st = cg_report_finish(setup, force_rows, counts.data(), &sts, stdout))) return die("report", st);
+    std::vector<cg_frame> all(nframes);
+    std::vector<double> secs(nframes);
+    for (int i = 0; i < nframes; ++i) {
+      all[i] = frames[i].f;
+      secs[i] = frame_ms[i] * 1e-3;
+    }
+    if ((st = cg_write_frame_logs(setup, dir.c_str(), nranks, all.data(), secs.data()))) return die("frame logs", st);
+  }
+  std::fprintf(stderr, "assembly: %lld rows x %d columns, %.3f ms on the GPU (%.3e rows/s)\n",
+               (long long)total_rows, m->ncols, ms_asm, total_rows / (ms_asm * 1e-3));
   if (do_fit) {
     // FITCOUL charge-constraint rows follow the last frame's rows
     int ncons = 0;
@@ -193,12 +243,12 @@ int main(int argc, char **argv) {
     for (int i = 0; i < nkeep; ++i) xk[i] = x[keep[i]];
     if ((st = cg_write_params(setup, dir.c_str(), xk.data(), nkeep, total_rows, &res, y.data(), b_all.data())))
       return die("writing params.txt", st);
-    std::printf("fit: RMS %.6e, %d of %d variables kept, gram %.3f ms, solve %.3f ms, refine %.3f ms\n", res.rms,
-                res.nvars, m->ncols, res.ms_gram, res.ms_solve, res.ms_refine);
+    std::fprintf(stderr, "fit: RMS %.6e, %d of %d variables kept, gram %.3f ms, solve %.3f ms, refine %.3f ms\n",
+                 res.rms, res.nvars, m->ncols, res.ms_gram, res.ms_solve, res.ms_refine);
   }
   cg_destroy(ctx);
   cg_setup_free(setup);
   std::chrono::duration<double> dt = std::chrono::steady_clock::now() - t0;
-  std::printf("total wall time %.3f s\n", dt.count());
+  std::fprintf(stderr, "total wall time %.3f s\n", dt.count());
   return 0;
 }

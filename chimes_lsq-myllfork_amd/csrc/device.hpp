@@ -165,6 +165,23 @@ struct DevFamily {
   int *cct = nullptr;         // [ntypes_atom^nb] cluster type of a type tuple, -1 if none
   double *csmax = nullptr;    // [ntypes_atom^nb * np] S_MAXIM of each enumerated pair slot
   int *cperm = nullptr;       // [ntypes_atom^nb * nb] canonical slot of each enumerated atom
+  int *cslot = nullptr;       // [ntypes_atom^nb * np] canonical pair slot of each enumerated pair (PAIR_INDICES)
+};
+
+// Statistics of the fit's clusters that chimes_lsq prints at the end of a
+// run (print_bond_stats, src/chimes_lsq.C:474-531, CLUSTER_LIST::
+// print_min_distances, src/Cluster.C:1569-1633) and the per-frame sizes of
+// its flat 3- and 4-body interaction lists (NEIGHBORS::UPDATE_3B/4B_
+// INTERACTION, src/ClassDefs.C:1513-1625).  Minima are kept as the bit
+// patterns of positive doubles (their order is that of the values).
+struct LsqStats {
+  bool on = false;
+  unsigned long long *d_min = nullptr;  // [npairs | ntypes3*3 | ntypes4*6]
+  int n_min = 0, off3 = 0, off4 = 0;
+  std::vector<long long> count[3];       // per pair / cluster type
+  long long *d_flat = nullptr;           // [2 * frames of the last call]: 3-body, 4-body
+  size_t flat_cap = 0;
+  std::vector<long long> flat3, flat4;   // of the last call's frames
 };
 
 struct Ctx {
@@ -229,6 +246,7 @@ struct Ctx {
   float ms_fam[3] = {0, 0, 0};
   long long ncl[3] = {0, 0, 0};  // clusters per family in the last batch
   int tile_cfg[3] = {0, 0, 0};   // assembly tile configuration per family (set by cg_autotune)
+  LsqStats stats;                // fit statistics (cg_stats_enable)
 };
 
 // Defined in assemble.hip.

@@ -44,10 +44,6 @@ std::vector<std::string> split_ws(const std::string &line) {
 }
 }  // namespace
 
-struct cg_setup {
-  Setup *s;
-};
-
 struct cg_traj {
   const Setup *s;
   std::string workdir;

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 
@@ -48,6 +49,8 @@ struct Lines {
     return l[0] + " " + l[1] + " " + l[2];
   }
 };
+
+const char *bool_str(bool b) { return b ? "true" : "false"; }
 
 bool to_bool(const std::string &s) {
   if (s.empty()) throw Error(CG_ERR_INPUT, "expected true/false");
@@ -178,6 +181,14 @@ void slot_maps(const ClusterType &c, const std::vector<int> &types, std::vector<
   for (int i = 0; i < n; ++i) atom_perm[rev[i]] = i;
 }
 
+// CLUSTER_LIST::tuplet_name (src/Cluster.C:1699-1745).
+std::string tuplet_name(int natoms, bool plural, bool caps) {
+  std::string t = natoms == 2 ? "pair" : natoms == 3 ? "triplet" : natoms == 4 ? "quadruplet" : std::to_string(natoms) + "-tuplet";
+  if (plural) t += "s";
+  if (caps) t[0] = (char)std::toupper(t[0]);
+  return t;
+}
+
 bool is_excluded(const ClusterFamily &f, std::vector<std::string> names) {
   std::sort(names.begin(), names.end());
   for (const auto &e : f.exclude)
@@ -206,6 +217,10 @@ void build_types(Setup &s, ClusterFamily &f, int idx, std::vector<int> &tup) {
     if (f.types[i].pair_names == pn) { found = true; map_index = (int)i; break; }
   bool excl = is_excluded(f, names);
   if (!found) {
+    // the reference prints the new entry before filling in its pair names
+    s.echo << "\n\tMade the following " << tuplet_name(n, true, false) << ": " << f.types.size() << " ";
+    for (int m = 0; m < f.npairs; ++m) s.echo << " ";
+    s.echo << "\n\n";
     ClusterType c;
     c.index = (int)f.types.size();
     c.natoms = n;
@@ -227,11 +242,16 @@ void build_types(Setup &s, ClusterFamily &f, int idx, std::vector<int> &tup) {
   if (map_index >= 0) {
     slot_maps(f.types[map_index], tup, f.id_pair_map[id], f.id_atom_perm[id]);
   }
+  s.echo << "\t\tAtom type idxs: ";
+  for (int i = 0; i < n; ++i) s.echo << std::setw(2) << tup[i];
+  s.echo << "  " << tuplet_name(n, false, true) << " name: " << std::setw(12) << all << " Unique "
+         << tuplet_name(n, false, false) << " ID " << id << " index: " << std::setw(4) << map_index << "\n";
 }
 
 // CLUSTER_LIST::process_cutoff_params (src/Cluster.C:771-954).
 void special_cutoffs(Setup &s, ClusterFamily &f, bool is_max) {
   const auto &in = is_max ? f.smax_input : f.smin_input;
+  const char *what = is_max ? "S_MAXIM" : "S_MINIM";
   if (!in.empty()) {
     auto tk = tokenize(in[0]);
     if (tk.size() < 5) throw Error(CG_ERR_INPUT, "Not enough tokens: " + in[0]);
@@ -242,7 +262,9 @@ void special_cutoffs(Setup &s, ClusterFamily &f, bool is_max) {
         for (auto &x : vec) x = v;
         (is_max ? c.special_max : c.special_min) = true;
       }
+      s.echo << "\tNote: Setting all " << f.natoms << "-body " << what << " values to " << v << "\n";
     } else if (tk[3] == "SPECIFIC") {
+      s.echo << "\tNote: Setting specific " << f.natoms << "-body " << what << " values: \n";
       int nspec = to_int(tk[4]);
       for (int i = 0; i < nspec; ++i) {
         auto t = tokenize(in.at(1 + i));
@@ -261,12 +283,61 @@ void special_cutoffs(Setup &s, ClusterFamily &f, bool is_max) {
           }
         }
       }
+      for (const auto &c : f.types) {
+        s.echo << "\t";
+        for (const auto &nm : c.atom_names) s.echo << nm << " ";
+        for (int j = 0; j < f.npairs; ++j) s.echo << c.pair_names[j] << " " << (is_max ? c.smax : c.smin)[j] << " ";
+        s.echo << "\n";
+      }
+      s.echo << "\n";
     }
   }
   if (is_max) {
     double v = -1.0;
     for (auto &c : f.types) for (double x : c.smax) v = std::max(v, x);
     f.max_cutoff = v;
+    s.echo << "     Max " << f.natoms << "-body cutoff = " << v << "\n\n";
+  } else {
+    double v = 1.0e10;
+    for (auto &c : f.types) for (double x : c.smin) v = std::min(v, x);
+    f.min_cutoff = v;
+    s.echo << "     Min " << f.natoms << "-body cutoff = " << v << "\n";
+  }
+}
+
+// CLUSTER_LIST::print / CLUSTER::print (src/Cluster.C:368-426,1529-1556)
+// without the MD parameters: the power-set tables of every cluster type.
+void print_family(Setup &s, const ClusterFamily &f) {
+  s.echo << "\n\tThe following unique " << tuplet_name(f.natoms, true, false) << " of atoms have been identified:\n";
+  if (!f.exclude.empty()) s.echo << "\tNote: The following types have been removed, if present: \n";
+  for (const auto &e : f.exclude) {
+    for (const auto &nm : e) s.echo << "\t\t" << nm;
+    s.echo << "\n";
+  }
+  s.echo << "\n";
+  for (const auto &c : f.types) {
+    s.echo << "    Cluster index       : " << c.index << "\n";
+    s.echo << "    Atoms in the cluster: ";
+    for (const auto &nm : c.atom_names) s.echo << nm << " ";
+    s.echo << "\n    Pairs in the cluster: ";
+    for (const auto &pn : c.pair_names) s.echo << pn << " ";
+    s.echo << "\n";
+    if (c.excluded) {
+      s.echo << "   Interaction is excluded by the user\n";
+    } else {
+      s.echo << "    Number of unique sets of powers: " << c.unique_powers.size() << " (" << c.allowed.size()
+             << " total)...\n";
+      s.echo << "\t\t     index  |  powers  |  equiv index  |  param index  | count \n";
+      s.echo << "\t\t   ----------------------------------------------------\n";
+      for (size_t j = 0; j < c.allowed.size(); ++j) {
+        s.echo << "\t\t      " << std::left << std::setw(6) << j << " ";
+        for (int p : c.allowed[j]) s.echo << " " << std::setw(2) << p << " ";
+        s.echo << "       " << std::setw(8) << c.equiv_idx[j] << " ";
+        s.echo << "       " << std::setw(8) << c.param_idx[j] << " ";
+        s.echo << "       " << std::setw(8) << c.power_count[j] << " " << std::right << "\n";
+      }
+    }
+    s.echo << "\n";
   }
 }
 
@@ -295,6 +366,7 @@ void build_family(Setup &s, ClusterFamily &f) {
   }
   special_cutoffs(s, f, true);
   special_cutoffs(s, f, false);
+  print_family(s, f);
   for (auto &c : f.types) {
     c.xmin.assign(c.npairs, 0); c.xmax = c.xavg = c.xdiff = c.xmin;
     for (int j = 0; j < c.npairs; ++j) {
@@ -365,12 +437,16 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
         if (L.width(i + 1) == 1) {
           s->infiles.push_back(L.at(i + 1, 0));
           s->force_flags.push_back(""); s->stress_flags.push_back(""); s->energy_flags.push_back("");
+          s->echo << "\t# TRJFILE #: " << s->infiles[0] << "\n";
         } else if (L.width(i + 1) == 2) {
+          s->multi_file = L.at(i + 1, 1);
+          s->echo << "\t# TRJFILE #: MULTI " << s->multi_file << "\n";
           std::ifstream mf(s->dir + "/" + L.at(i + 1, 1));
           if (!mf) throw Error(CG_ERR_INPUT, "ERROR: Cannot open MULTI file: " + L.at(i + 1, 1));
           std::getline(mf, line);
           auto t = tokenize(line);
           if (t.size() != 1) throw Error(CG_ERR_INPUT, "ERROR: Expected to read <nfiles>, got: " + line);
+          s->echo << "\t\t...Will read from the following files: \n";
           int nf = to_int(t[0]);
           for (int j = 0; j < nf; ++j) {
             std::getline(mf, line);
@@ -387,9 +463,12 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
             } else {
               s->force_flags.push_back(""); s->stress_flags.push_back(""); s->energy_flags.push_back("");
             }
+            s->echo << "\t\t-   " << s->infiles[j] << " with " << s->infile_frames[j] << " frames.\n";
           }
+          s->echo << "Removing the following files with zero frames:\n";
           for (int j = (int)s->infile_frames.size() - 1; j >= 0; --j)
             if (s->infile_frames[j] < 1) {
+              s->echo << "\t\t-   " << s->infiles[j] << " with " << s->infile_frames[j] << " frames.\n";
               s->infile_frames.erase(s->infile_frames.begin() + j);
               s->infiles.erase(s->infiles.begin() + j);
               s->force_flags.erase(s->force_flags.begin() + j);
@@ -401,15 +480,31 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       }
     }
     long k;
-    if ((k = find_kw("WRAPTRJ")) >= 0 && L.width(k + 1) == 1) s->wrap = to_bool(L.at(k + 1, 0));
-    if ((k = find_kw("SPLITFI")) >= 0) s->split_files = to_bool(L.at(k + 1, 0));
-    if ((k = find_kw("HIERARC")) >= 0) s->hierarchical = to_bool(L.at(k + 1, 0));
-    if ((k = find_kw("NFRAMES")) >= 0) s->nframes = to_int(L.at(k + 1, 0));
+    if ((k = find_kw("WRAPTRJ")) >= 0 && k + 1 < (long)N && L.width(k + 1) == 1) {
+      s->wrap = to_bool(L.at(k + 1, 0));
+      s->echo << "\t# WRAPTRJ #: " << bool_str(s->wrap) << "\n";
+    }
+    if ((k = find_kw("SPLITFI")) >= 0) {
+      s->split_files = to_bool(L.at(k + 1, 0));
+      s->echo << "\t# SPLITFI #: " << bool_str(s->split_files) << "\n";
+    }
+    if ((k = find_kw("HIERARC")) >= 0) {
+      s->hierarchical = to_bool(L.at(k + 1, 0));
+      s->echo << "\t# HIERARC #: " << bool_str(s->hierarchical) << "\n";
+    }
+    if ((k = find_kw("NFRAMES")) >= 0) {
+      s->nframes = to_int(L.at(k + 1, 0));
+      s->echo << "\t# NFRAMES #: " << s->nframes << "\n";
+    }
     if ((k = find_kw("NLAYERS")) >= 0) {
       s->nlayers = to_int(L.at(k + 1, 0));
+      s->echo << "\t# NLAYERS #: " << s->nlayers << "\n";
       if (L.width(k + 1) == 3) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS REPLICATE is not supported");
     }
-    if ((k = find_kw("FITCOUL")) >= 0) s->fitcoul = to_bool(L.at(k + 1, 0));
+    if ((k = find_kw("FITCOUL")) >= 0) {
+      s->fitcoul = to_bool(L.at(k + 1, 0));
+      s->echo << "\t# FITCOUL #: " << bool_str(s->fitcoul) << "\n";
+    }
     if ((k = find_kw("FITSTRS")) >= 0) {
       const std::string &v = L.at(k + 1, 0);
       if (v == "first" || v == "First" || v == "FIRST") {
@@ -419,12 +514,20 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       } else if (v == "firstall" || v == "FirstAll" || v == "FIRSTALL") {
         s->fit_stress = CG_STRESS_ALL; s->nstress = to_int(L.at(k + 1, 1));
       } else s->fit_stress = to_bool(v) ? CG_STRESS_DIAG : CG_STRESS_NONE;
+      s->echo << "\t# FITSTRS #: ";
+      if (s->fit_stress == CG_STRESS_ALL) s->echo << "true ...will fit to all tensor components\n";
+      else if (s->nstress > 0)
+        s->echo << bool_str(s->fit_stress != CG_STRESS_NONE) << " ...will only fit tensors for first " << s->nstress
+                << " frames.\n";
+      else s->echo << bool_str(s->fit_stress != CG_STRESS_NONE) << "\n";
     }
     for (size_t i = 0; i < N; ++i)   // FITENER: last occurrence wins (no break in reference)
       if (L.keyword(i, "FITENER")) {
         const std::string &v = L.at(i + 1, 0);
         if (v == "first" || v == "First" || v == "FIRST") { s->fit_ener = true; s->nener = to_int(L.at(i + 1, 1)); }
         else s->fit_ener = to_bool(v);
+        s->echo << "\t# FITENER #: " << bool_str(s->fit_ener) << "\n";
+        if (s->nener >= 0) s->echo << "    \t\t\t ...will fit energies for first " << s->nener << " frames.\n";
       }
     if ((k = find_kw("PAIRTYP")) >= 0) {
       s->ff_type = L.at(k + 1, 0);
@@ -437,9 +540,19 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       if (s->cheby_low < -1 || s->cheby_low > 1 || s->cheby_high < -1 || s->cheby_high > 1 ||
           s->cheby_high < s->cheby_low)
         throw Error(CG_ERR_INPUT, "ERROR: bad Chebyshev range");
+      s->echo << "\t# PAIRTYP #: " << s->ff_type
+              << " ....NOTE: Forces reported in units of kcal/(mol.A), potential energy in kcal/mol.\n"
+              << "\t             Will use 2-body order: " << s->o2 << "\n"
+              << "\t             Will use 3-body order: " << s->o3 << "\n"
+              << "\t             Will use 4-body order: " << s->o4 << "\n"
+              << "\t             Will transform Chebyshev pair distances to range " << s->cheby_low << " to "
+              << s->cheby_high << "\n";
     } else throw Error(CG_ERR_INPUT, "missing # PAIRTYP #");
     for (size_t i = 0; i < N; ++i)
-      if (L.keyword(i, "CHBTYPE")) s->transform = transform_of(L.at(i + 1, 0));
+      if (L.keyword(i, "CHBTYPE")) {
+        s->transform = transform_of(L.at(i + 1, 0));
+        s->echo << "\t# CHBTYPE #: " << L.at(i + 1, 0) << "\n";
+      }
     for (size_t i = 0; i < N; ++i)
       if (L.keyword(i, "CHEBYFIX")) {
         const std::string &v = L.at(i + 1, 0);
@@ -447,24 +560,38 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
         else if (v == "CONSTANT_DERIV") s->fix_type = CG_FIX_CONSTANT_DERIV;
         else if (v == "SMOOTH") { s->fix_type = CG_FIX_SMOOTH; s->smooth_dist = to_double(L.at(i + 1, 1)); }
         else throw Error(CG_ERR_INPUT, "ERROR: CHEBYFIX was not recognized");
+        s->echo << "\t# CHEBYFIX #: " << v << "\n";
+        if (v == "SMOOTH") s->echo << "\t\tCheby smoothing distance = " << s->smooth_dist << "\n";
       }
     if ((k = find_kw("USENEIG")) >= 0) {
       s->use_neigh = to_bool(L.at(k + 1, 0));
+      s->echo << "\t# USENEIG #: " << bool_str(s->use_neigh) << "\n";
       if (L.width(k + 1) == 2 && s->use_neigh) {
-        if (L.at(k + 1, 1) == "SMALL") s->small_mode = true;
-        else throw Error(CG_ERR_INPUT, "ERROR: Unrecognized # USENEIG # option");
+        if (L.at(k + 1, 1) == "SMALL") {
+          s->small_mode = true;
+          s->echo << "\t\tWill update the neighbor list through the \"small\" method \n";
+        } else throw Error(CG_ERR_INPUT, "ERROR: Unrecognized # USENEIG # option");
       }
     }
-    if ((k = find_kw("SKPFRMS")) >= 0) s->skip_frames = to_int(L.at(k + 1, 0));
+    if ((k = find_kw("SKPFRMS")) >= 0) {
+      s->skip_frames = to_int(L.at(k + 1, 0));
+      if (s->skip_frames >= 1)
+        s->echo << "\t# SKPFRMS #: " << s->skip_frames
+                << " .. will skip through frames in parallel execution (round-robin).\n";
+      else
+        s->echo << "\t# SKPFRMS #: " << s->skip_frames << " ... will process all frames in contiguous order\n";
+    }
 
     // NATMTYP / TYPEIDX
     if ((k = find_kw("NATMTYP")) < 0) throw Error(CG_ERR_INPUT, "missing # NATMTYP #");
     s->ntypes = to_int(L.at(k + 1, 0));
+    s->echo << "\t# NATMTYP #: " << s->ntypes << "\n";
     if (s->ntypes < 1 || s->ntypes > CG_MAX_ATOM_TYPES - 1)
       throw Error(CG_ERR_UNSUPPORTED, "NATMTYP must be 1..9 (cluster keys are base-10 digits)");
     const int nt = s->ntypes;
     s->pairs.resize(nt * (nt + 1) / 2);
     if ((k = find_kw("TYPEIDX")) < 0) throw Error(CG_ERR_INPUT, "missing # TYPEIDX #");
+    s->echo << "\t# TYPEIDX #    # ATM_TYP #    # ATMCHRG #    # ATMMASS #\n";
     double sumq = 0;
     for (int j = 0; j < nt; ++j) {
       auto &p = s->pairs[j];
@@ -482,6 +609,10 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       p.proper = p.a1 + p.a2;
       s->type_names.push_back(p.a1);
       s->type_idx.push_back(ti);
+      s->echo << " \t" << std::setw(15) << std::left << j + 1 << std::setw(15) << std::left << p.a1;
+      if (!s->fitcoul) s->echo << std::setw(15) << std::left << p.chg1;
+      else s->echo << p.chrgsgn << "\t\t";
+      s->echo << std::setw(15) << std::left << p.mass1 << std::right << "\n";
     }
     if (!s->fitcoul && sumq > 0) s->use_partial_charges = true;
     int t = nt;
@@ -502,23 +633,28 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
               (s->type_names[j] == p.a1 && s->type_names[i] == p.a2))
             s->pair_map.insert({s->type_names[i] + s->type_names[j], (int)q});
         }
+    s->echo << "\n\tThe following unique pair types have been identified:\n";
+    for (const auto &p : s->pairs) s->echo << "\t\t" << p.idx << "  " << p.a1 << " " << p.a2 << "\n";
     // EXCLUDE (src/input.C:753-874): 1- and 2-body lists only in a
     // hierarchical fit, the first list of each kind
     if (s->hierarchical) {
       for (size_t i = 0; i < N; ++i)
         if (L.width(i) >= 4 && L.head3(i) == "EXCLUDE 1B INTERACTION:") {
           const int ne = to_int(L.at(i, 3));
+          s->echo << "Number of 1-body interactions to exclude: " << ne << "\n";
           for (int j = 1; j <= ne; ++j)
             for (int t = 0; t < nt; ++t)
               if (L.at(i + j, 0) == s->type_names[t]) {
                 s->excl_1b.push_back(t);
                 s->n_excl_lt3b += 1;
+                s->echo << "\tEXCLUDED: " << t << ": " << s->type_names[t] << "\n";
               }
           break;
         }
       for (size_t i = 0; i < N; ++i)
         if (L.width(i) >= 4 && L.head3(i) == "EXCLUDE 2B INTERACTION:") {
           const int ne = to_int(L.at(i, 3));
+          s->echo << "Number of 2-body interactions to exclude: " << ne << "\n";
           for (int j = 1; j <= ne; ++j)
             for (size_t q = 0; q < s->pairs.size(); ++q) {
               const auto &p = s->pairs[q];
@@ -526,6 +662,7 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
               if ((p.a1 == a && p.a2 == b) || (p.a1 == b && p.a2 == a)) {
                 s->excl_2b.push_back((int)q);
                 s->n_excl_lt3b += s->o2;
+                s->echo << "\tEXCLUDED: " << p.idx << " " << p.a1 << " " << p.a2 << "\n";
               }
             }
           break;
@@ -559,31 +696,52 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       p.lambda = to_double(L.at(k + 1 + i, 6));
       if (p.smax > s->max_cutoff_2b) s->max_cutoff_2b = p.smax;
     }
+    s->echo << "\t# PAIRIDX #     # ATM_TY1 #     # ATM_TY1 #     # S_MINIM #     # S_MAXIM #     # MORSE_LAMBDA #\n";
+    for (const auto &p : s->pairs)
+      s->echo << "\t" << std::left << std::setw(16) << p.idx << std::setw(16) << p.a1 << std::setw(16) << p.a2
+              << std::setw(16) << p.smin << std::setw(16) << p.smax << std::setw(16) << p.lambda << std::right << "\n";
     s->int_pair_map.assign(nt * nt, -1);
     for (int i = 0; i < nt; ++i)
-      for (int j = 0; j < nt; ++j)
-        s->int_pair_map[s->type_idx[i] * nt + s->type_idx[j]] = s->pair_map.at(s->type_names[i] + s->type_names[j]);
+      for (int j = 0; j < nt; ++j) {
+        const int idx = s->type_idx[i] * nt + s->type_idx[j];
+        s->int_pair_map[idx] = s->pair_map.at(s->type_names[i] + s->type_names[j]);
+        // build_int_pair_map (src/functions.C:1055-1095)
+        s->echo << "\t\tAtom type idxs: " << std::setw(2) << i << std::setw(2) << j << " Pair name: " << std::setw(4)
+                << s->type_names[i] + s->type_names[j] << " Explicit pair index: " << std::setw(4) << idx
+                << " Unique pair index: " << std::setw(4) << s->int_pair_map[idx] << "\n";
+      }
     for (auto &p : s->pairs) cheby_params(p.smin, p.smax, p.lambda, s->transform, p.xmin, p.xmax, p.xdiff, p.xavg);
     // CHARGE CONSTRAINTS: npairs-1 lines of npairs pair names, npairs
-    // weights and the b value (last token), src/input.C:1148-1200
-    if (s->fitcoul)
-      for (size_t i = 0; i < N; ++i)
-        if (L.width(i) >= 2 && L.v[i][0] == "CHARGE" && L.v[i][1] == "CONSTRAINTS:") {
-          const int np = (int)s->pairs.size();
-          for (int n = 0; n < np - 1; ++n) {
-            Setup::ChargeConstraint cc;
-            const size_t li = i + 1 + n;
-            for (int j = 0; j < np; ++j) {
-              auto it = s->pair_map.find(L.at(li, j));
-              if (it == s->pair_map.end()) throw Error(CG_ERR_INPUT, "unknown pair in CHARGE CONSTRAINTS: " + L.at(li, j));
-              cc.pair_idx.push_back(it->second);
-            }
-            for (int j = 0; j < np; ++j) cc.weight.push_back(to_double(L.at(li, np + j)));
-            cc.force = to_double(L.at(li, L.width(li) - 1));
-            s->charge_cons.push_back(cc);
+    // weights and the b value (last token), src/input.C:1148-1200; read (and
+    // echoed) whatever FITCOUL says, their rows belong to FITCOUL fits only
+    // (A_MAT::PRINT_CONSTRAINTS, src/A_Matrix.C:606)
+    for (size_t i = 0; i < N; ++i)
+      if (L.width(i) >= 2 && L.v[i][0] == "CHARGE" && L.v[i][1] == "CONSTRAINTS:") {
+        const int np = (int)s->pairs.size();
+        std::vector<Setup::ChargeConstraint> cons;
+        s->echo << "\n\tAttempting to read " << np - 1 << " charge constraints...:\n";
+        for (int n = 0; n < np - 1; ++n) {
+          Setup::ChargeConstraint cc;
+          const size_t li = i + 1 + n;
+          s->echo << "\t\t" << n + 1 << "\t ";
+          for (int j = 0; j < np; ++j) {
+            auto it = s->pair_map.find(L.at(li, j));
+            if (it == s->pair_map.end()) throw Error(CG_ERR_INPUT, "unknown pair in CHARGE CONSTRAINTS: " + L.at(li, j));
+            cc.pair_idx.push_back(it->second);
+            s->echo << L.at(li, j) << " (" << it->second << ") ";
           }
-          break;
+          for (int j = 0; j < np; ++j) {
+            cc.weight.push_back(to_double(L.at(li, np + j)));
+            s->echo << cc.weight.back() << " ";
+          }
+          cc.force = to_double(L.at(li, L.width(li) - 1));
+          s->echo << cc.force << "\n";
+          cons.push_back(cc);
         }
+        s->echo << "\n";
+        if (s->fitcoul) s->charge_cons = cons;
+        break;
+      }
     // SPECIAL nB S_MINIM / S_MAXIM
     for (int fam = 3; fam <= 4; ++fam) {
       auto &F = fam == 3 ? s->trips : s->quads;
@@ -612,6 +770,7 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
       std::string joined;
       for (auto &w : L.v[k + 1]) joined += (joined.empty() ? "" : " ") + w;
       s->fcut_line = joined;
+      s->echo << "# FCUTTYP #: " << s->fcut_line << "      ... for all Chebyshev interactions\n";
     }
     parse_fcut(*s);
     // the reference writes the Ewald columns on force rows only
@@ -624,7 +783,10 @@ Setup *read_setup_from(std::istream &in, const std::string &path) {
     if (s->fitcoul && (!s->excl_1b.empty() || !s->excl_2b.empty()))
       throw Error(CG_ERR_UNSUPPORTED, "ERROR: cannot use hierarchical learning with charge fitting.");
     if (s->use_partial_charges) throw Error(CG_ERR_UNSUPPORTED, "fixed non-zero charges (Ewald subtraction) are not supported");
-    if (s->wrap && s->nlayers > 0) s->wrap = false;  // RUN_SANITY_LSQ
+    if (s->wrap && s->nlayers > 0) {  // RUN_SANITY_LSQ
+      s->wrap = false;
+      s->echo << "WARNING: Coordinate wrapping not supported for ghost atom use. Turning option off\n";
+    }
     if (s->nlayers < 1) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS 0 (no ghost atoms) is not supported");
     // image shifts are stored in 4-bit fields (-8..+7) of a neighbour reference
     if (s->nlayers > 7) throw Error(CG_ERR_UNSUPPORTED, "NLAYERS above 7 is not supported");

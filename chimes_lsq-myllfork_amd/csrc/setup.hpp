@@ -2,6 +2,7 @@
 // 3/4-body cluster power-set tables, flattened into the cg_model POD.
 #pragma once
 #include <map>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -54,10 +55,17 @@ struct ClusterFamily {
   std::vector<std::vector<std::string>> exclude;
   std::vector<std::string> smax_input, smin_input;
   double max_cutoff = 0;
+  double min_cutoff = 0;                     // smallest S_MINIM of any type
 };
 
 struct Setup {
   std::string path, dir;
+  // What chimes_lsq prints while it reads the input and builds the cluster
+  // tables (src/input.C:172-1338, src/Cluster.C:368-426,771-954,1293-1556),
+  // in its order; printed by the stdout report.
+  std::ostringstream echo;
+  std::string multi_file;   // TRJFILE MULTI list file ("" for one file)
+  double report_ewald_cell[3] = {0, 0, 0};  // stdout report: cell of the last Ewald parameter print
   // controls
   std::vector<std::string> infiles;
   std::vector<int> infile_frames;
@@ -144,3 +152,8 @@ void write_header(const Setup &s, const std::string &dir);
 void write_map(const Setup &s, const std::string &dir);
 
 }  // namespace cg
+
+// Handle of the C ABI (include/chimes_gpu.h).
+struct cg_setup {
+  cg::Setup *s;
+};

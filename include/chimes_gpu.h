@@ -29,6 +29,7 @@
 
 #include <stdint.h>
 #include <stddef.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -193,6 +194,34 @@ cg_status cg_write_close(const cg_setup *s);
 cg_status cg_frame_order(const cg_setup *s, int nranks, int *order, int *rank, int *n_out);
 cg_status cg_write_header(const cg_setup *s, const char *outdir); /* params.header + ff_groups.map */
 
+/* ------------------------------------------------ standard output report
+ * The text chimes_lsq prints to standard output (its test suite keeps it as
+ * fm_setup.out) for a run of this setup, written to `out` in pieces the
+ * caller orders as the reference does (src/chimes_lsq.C:160-424):
+ *   cg_report_setup        input echo, cluster tables, parameter counts
+ *                          (INPUT::PARSE_INFILE_LSQ, build_clusters)
+ *   cg_report_frame_read   per frame read, in trajectory order: the cutoff
+ *                          warnings of BOX::IS_RCUT_SAFE (the reference warns
+ *                          and continues) and, for frame 0, the layering report
+ *   cg_report_frame_done   per frame processed by rank 0 of an `nranks` run
+ *                          (first = its first frame): the flat 3/4-body list
+ *                          sizes (cg_stats_frames) and the Ewald parameters
+ *   cg_report_finish       A length (3 x atoms processed), the atom counts per
+ *                          type of rank 0's last frame (FITENER) and the
+ *                          statistics of cg_stats_get (print_bond_stats)
+ * cg_write_frame_logs writes frames.NNNN.log for each of nranks ranks
+ * (src/chimes_lsq.C:283-338,718-823); frames: all frames in file order,
+ * seconds: the time each took (may be NULL). */
+cg_status cg_report_setup(const cg_setup *s, FILE *out);
+cg_status cg_report_frame_read(const cg_setup *s, const cg_frame *f, FILE *out);
+cg_status cg_report_frame_done(const cg_setup *s, const cg_frame *f, int first, int nranks, int64_t flat3,
+                               int64_t flat4, FILE *out);
+struct cg_lsq_stats;
+cg_status cg_report_finish(const cg_setup *s, int64_t force_rows, const int *last_type_counts,
+                           const struct cg_lsq_stats *stats, FILE *out);
+cg_status cg_write_frame_logs(const cg_setup *s, const char *outdir, int nranks, const cg_frame *frames,
+                              const double *seconds);
+
 /* ------------------------------------------------------- device context */
 typedef struct cg_ctx cg_ctx;
 enum { CG_FP64 = 0, CG_FP32 = 1 };
@@ -240,6 +269,29 @@ cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, doub
  * 3-body and 4-body assembly launches (ms, HIP events on the context stream)
  * and the number of pairs / triplets / quadruplets assembled. */
 cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3);
+
+/* Statistics chimes_lsq prints at the end of a run, gathered by the
+ * following cg_assemble calls on ctx (print_bond_stats,
+ * src/chimes_lsq.C:474-531; CLUSTER_LIST::print_min_distances,
+ * src/Cluster.C:1569-1633): per 2-body pair type the smallest distance in its
+ * neighbour list and the number of pairs with S_MINIM < r < S_MAXIM; per
+ * cluster type the smallest distance of each canonical pair slot and the
+ * number of clusters; per frame of the last cg_assemble the sizes of the flat
+ * 3- and 4-body interaction lists it prints (NEIGHBORS::UPDATE_3B/4B_
+ * INTERACTION, src/ClassDefs.C:1513-1625).  Lists are padded by 0.3 A as the
+ * reference's are.  on != 0 resets and starts gathering; minima never found
+ * read 1e10.  Members of cg_lsq_stats may be NULL. */
+typedef struct cg_lsq_stats {
+  double *pair_min;       /* [npairs]          */
+  int64_t *pair_count;    /* [npairs]          */
+  double *trip_min;       /* [c3.ntypes * 3]   */
+  int64_t *trip_count;    /* [c3.ntypes]       */
+  double *quad_min;       /* [c4.ntypes * 6]   */
+  int64_t *quad_count;    /* [c4.ntypes]       */
+} cg_lsq_stats;
+cg_status cg_stats_enable(cg_ctx *ctx, int on);
+cg_status cg_stats_get(cg_ctx *ctx, cg_lsq_stats *out);
+cg_status cg_stats_frames(cg_ctx *ctx, int64_t *flat3, int64_t *flat4, int *nframes);
 
 /* Wait for all work queued on the context's stream (no reference
  * counterpart: the reference is synchronous). */

@@ -29,7 +29,8 @@ expected outputs), trimmed to the frames the tests need:
                            setup) the row weights and the params.txt of the
                            weighted SVD, eps 1e-4, ridge 1e-3 and LassoLars
                            1e-4 fits.
-  lsq/cases/<test>/        end-to-end runs of 19 LSQ tests: fm_setup.in, the
+  lsq/cases/<test>/        end-to-end runs of 19 LSQ tests (and their standard
+                           output, fm_setup.out.gz): fm_setup.in, the
                            trajectory (gzip, the frames the run reads; the
                            water cases use lsq/water/input.xyzf.gz), force.txt
                            (gzip), the sha256 of b.txt, the fitted coefficients
@@ -160,6 +161,7 @@ def main():
     np.savez_compressed(os.path.join(out, "golden.npz"), A=A, b=b, force=y, row0=0)
     for fn in ["params.txt", "test_suite_params.txt", "dim.txt"]:
         open(os.path.join(out, fn), "w").write(open(os.path.join(src, "correct_output", fn)).read())
+    gz(os.path.join(src, "correct_output", "fm_setup.out"), os.path.join(out, "fm_setup.out.gz"))
 
     # last frame of test_4atoms.2 and fitstrs
     for t, traj, apiece, frame_rows in [("test_4atoms.2", "dump2.xyzf", "A.txt.ab", 12),
@@ -212,6 +214,7 @@ def main():
             if t == "h2o-4bcheby":
                 extra = {"b": np.loadtxt(os.path.join(co, "b.txt")), "force": np.loadtxt(os.path.join(co, "force.txt"))}
             np.savez_compressed(os.path.join(out, "golden.npz"), A=rows, row0=nb - rows.shape[0], **extra)
+        gz(os.path.join(co, "fm_setup.out"), os.path.join(out, "fm_setup.out.gz"))
         if t == "skip_frames":
             gz(os.path.join(co, "b.txt"), os.path.join(out, "b.txt.gz"))
             open(os.path.join(out, "dim.txt"), "w").write(open(os.path.join(co, "dim.txt")).read())
@@ -260,6 +263,8 @@ def main():
             os.remove(tmp)
             meta["trajectories"].append({"name": fn, "water": False})
         gz(os.path.join(co, "force.txt"), os.path.join(out, "force.txt.gz"))
+        # the run's standard output (36 MPI ranks, rank 0 printing)
+        gz(os.path.join(co, "fm_setup.out"), os.path.join(out, "fm_setup.out.gz"))
         json.dump(meta, open(os.path.join(out, "meta.json"), "w"), indent=1, sort_keys=True)
     # force-evaluation cases: params.txt + first coordinate frame + forceout.txt
     md = os.path.join(REF, "test_suite-md")

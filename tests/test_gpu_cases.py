@@ -1,6 +1,7 @@
 """The drop-in executable on 19 of the reference's LSQ tests end to end
 (`chimes_lsq_gpu fm_setup.in --fit`, the SVD of chimes_lsq.py with its
-default eps 1e-5): b.txt byte for byte, the fit summary (number of fitted
+default eps 1e-5): its standard output against the run's fm_setup.out under
+compare.pl semantics (as rank 0 of 36, the runs' layout), b.txt byte for byte, the fit summary (number of fitted
 variables equal, printed numbers within 1e-4), every coefficient against the
 reference's test_suite_params.txt within 1e-8 of max|x|, and force.txt
 (A x at 7 digits) within 1e-6 relative (floor 1e-3 of its largest value).
@@ -46,9 +47,18 @@ def test_cli_reference_case(ch, tmp_path, case):
         with gzip.open(src, "rb") as g:
             open(tmp_path / t["name"], "wb").write(g.read())
     exe = os.path.join(PKG, "chimes_lsq_gpu")
-    r = subprocess.run([exe, "fm_setup.in", "--fit", "--no-A"], cwd=str(tmp_path), capture_output=True, text=True,
-                       timeout=300)
+    # the reference runs behind the golden files used 36 MPI ranks
+    r = subprocess.run([exe, "fm_setup.in", "--fit", "--no-A", "--ranks", "36"], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
+    # standard output = the run's fm_setup.out under compare.pl semantics
+    from stdout_compare import STALE_MIN_DISTANCE_DECIMALS, differences, round_min_distances
+    out = r.stdout
+    if case in STALE_MIN_DISTANCE_DECIMALS:
+        out = round_min_distances(out, STALE_MIN_DISTANCE_DECIMALS[case])
+    bad = differences(gzip.open(os.path.join(d, "fm_setup.out.gz"), "rt").read(), out)
+    assert not bad, "\n".join("ref %s: %r\nour %s: %r" % (a[0], a[1], b[0], b[1]) for a, b in bad)
+    assert os.path.exists(tmp_path / "frames.0000.log") and os.path.exists(tmp_path / "frames.0035.log")
     data = open(tmp_path / "b.txt", "rb").read()
     assert (len(data), hashlib.sha256(data).hexdigest()) == (meta["b_bytes"], meta["b_sha256"])
     ours = open(tmp_path / "params.txt").read()

@@ -24,8 +24,13 @@ def test_cli_nonorth2_end_to_end(tmp_path):
     for fn in ["fm_setup.in", "input.xyzf"]:
         shutil.copy(os.path.join(d, fn), tmp_path / fn)
     exe = os.path.join(PKG, "chimes_lsq_gpu")
-    r = subprocess.run([exe, "fm_setup.in", "--fit"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "fm_setup.in", "--fit", "--ranks", "36"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    import gzip
+    from stdout_compare import differences
+    bad = differences(gzip.open(os.path.join(d, "fm_setup.out.gz"), "rt").read(), r.stdout)
+    assert not bad, bad
     g = np.load(os.path.join(d, "golden.npz"))
     A = np.loadtxt(tmp_path / "A.txt")
     assert A.shape == g["A"].shape
@@ -131,3 +136,24 @@ def test_cli_hierarchical_fit(ch, synth, oracle, tmp_path):
     rms = float(re.search(r"! RMS force error\s*=\s*(\S+)", txt).group(1))
     # params.txt prints the RMS with 5 significant digits (%11.4e)
     assert rms == pytest.approx(np.sqrt(np.mean((b - y) ** 2)), rel=1e-4)
+
+
+def test_cli_skip_frames_stdout(tmp_path):
+    """The reference's skip_frames case (FITCOUL water, SKPFRMS 1: round robin
+    over 36 ranks): standard output against its fm_setup.out as rank 0 of 36
+    (compare.pl semantics; that kept file prints the minimum distances with
+    3 decimals)."""
+    import gzip
+    from conftest import WATER
+    from stdout_compare import STALE_MIN_DISTANCE_DECIMALS, differences, round_min_distances
+    d = os.path.join(WATER, "skip_frames")
+    shutil.copy(os.path.join(d, "fm_setup.in"), tmp_path / "fm_setup.in")
+    with gzip.open(os.path.join(WATER, "input.xyzf.gz"), "rb") as g:
+        open(tmp_path / "input.xyzf", "wb").write(g.read())
+    exe = os.path.join(PKG, "chimes_lsq_gpu")
+    r = subprocess.run([exe, "fm_setup.in", "--no-A", "--ranks", "36"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = round_min_distances(r.stdout, STALE_MIN_DISTANCE_DECIMALS["skip_frames"])
+    bad = differences(gzip.open(os.path.join(d, "fm_setup.out.gz"), "rt").read(), out)
+    assert not bad, bad

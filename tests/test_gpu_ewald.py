@@ -116,8 +116,13 @@ def test_cli_h2o_4bcheby_end_to_end(ch, tmp_path):
     with gzip.open(os.path.join(WATER, "input.xyzf.gz"), "rb") as g:
         open(tmp_path / "input.xyzf", "wb").write(g.read())
     exe = os.path.join(PKG, "chimes_lsq_gpu")
-    r = subprocess.run([exe, "fm_setup.in", "--fit"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "fm_setup.in", "--fit", "--ranks", "36"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    # standard output: the run's fm_setup.out (rank 0 of 36) under compare.pl semantics
+    from stdout_compare import differences
+    bad = differences(gzip.open(os.path.join(d, "fm_setup.out.gz"), "rt").read(), r.stdout)
+    assert not bad, bad
     g = np.load(os.path.join(d, "golden.npz"))
     A = np.loadtxt(tmp_path / "A.txt")
     assert A.shape == (9506, 419)

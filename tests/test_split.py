@@ -122,3 +122,31 @@ def test_header_and_map_match_reference(ch, tmp_path):
         assert mine.replace("EXCLD1B: false\nEXCLD2B: false\n", "") == g.read()
     with gzip.open(os.path.join(TATB, "ff_groups.map.gz"), "rb") as g:
         assert open(out / "ff_groups.map", "rb").read() == g.read()
+
+
+def test_frame_logs_per_rank(ch, tmp_path):
+    """frames.NNNN.log of each of the 36 ranks: the frames it processed, as
+    the reference run's logs list them, each followed by its atom counts per
+    type and its time (src/chimes_lsq.C:718-823)."""
+    import ctypes
+    import re
+    import numpy as np
+    s = _tatb(ch, tmp_path)
+    meta = json.load(open(os.path.join(TATB, "split.json")))
+    frames = s.frames(workdir=str(tmp_path))
+    cf = (ch.Frame * len(frames))(*[f.c() for f in frames])
+    secs = np.arange(len(frames), dtype=np.float64) * 0.25
+    L = ch.lib()
+    L.cg_write_frame_logs.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ch.Frame),
+                                      ctypes.POINTER(ctypes.c_double)]
+    ch.check(L.cg_write_frame_logs(s.h, str(tmp_path).encode(), meta["nranks"], cf, ch.dptr(secs)))
+    names = s.type_names()
+    for r in range(meta["nranks"]):
+        lines = open(tmp_path / ("frames.%04d.log" % r)).read().split("\n")
+        got = [int(l.split()[2]) for l in lines if l.startswith("Processing frame")]
+        assert got == meta["frames"][str(r)], r
+        for i in got:
+            k = lines.index("Processing frame %d" % i)
+            cnt = np.bincount(frames[i].types, minlength=len(names))
+            assert lines[k + 1].split() == [w for t, n in zip(names, cnt) for w in (t, str(n))]
+            assert re.match(r"^Finished frame %d in %.1f seconds$" % (i, secs[i]), lines[k + 2]), lines[k + 2]
