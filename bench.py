@@ -74,20 +74,30 @@ def setup_from(ch, text):
 
 
 def cpu_baseline(ch, setup, frames, seconds_hint=25.0):
-    """Time the oracle (CPU FP64 restatement) on the host cores."""
+    """Time the oracle (CPU FP64 restatement) on the host cores: one frame per OpenMP thread
+    on every core this job is given (OMP_NUM_THREADS, else the machine's count, at most 16),
+    then one frame on a single core."""
     orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
     orc.orc_assemble_many.restype = ctypes.c_longlong
     orc.orc_assemble_many.argtypes = [ctypes.POINTER(ch.Model), ctypes.POINTER(ch.Frame), ctypes.c_int, ctypes.c_int]
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    nproc = os.cpu_count() or 1
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, nproc)
     cores = max(1, min(cores, 16, len(frames)))
-    nf = cores  # one frame per thread: ~23 s of work per frame per core
-    cf = (ch.Frame * nf)(*[f.c() for f in frames[:nf]])
-    t0 = time.perf_counter()
-    rows = orc.orc_assemble_many(setup.model_ptr, cf, nf, cores)
-    dt = time.perf_counter() - t0
+
+    def timed(nf, threads):
+        cf = (ch.Frame * nf)(*[f.c() for f in frames[:nf]])
+        t0 = time.perf_counter()
+        rows = orc.orc_assemble_many(setup.model_ptr, cf, nf, threads)
+        return rows, time.perf_counter() - t0
+
+    rows, dt = timed(cores, cores)  # one frame per thread: ~12 s of work per frame per core
+    rows1, dt1 = timed(1, 1)
     return {"value": rows / dt, "unit": "A rows/s", "cores": cores, "kind": "port",
             "sample": "%d of the benchmark's 512-atom carbon frames (%d rows), one frame per OpenMP thread, "
-                      "%.1f s" % (nf, rows, dt)}
+                      "%.1f s" % (cores, rows, dt),
+            "nproc": nproc,
+            "one_core": {"value": rows1 / dt1, "unit": "A rows/s",
+                         "sample": "1 frame (%d rows) on one core, %.1f s" % (rows1, dt1)}}
 
 
 def serving_latency(ch, synth, n_req=40, natoms=1000, seed=105):
@@ -253,8 +263,9 @@ def main():
     ap.add_argument("--no-md", action="store_true",
                     help="skip the MD-style leg (force calls replayed from HIP graphs), e.g. under a kernel tracer")
     ap.add_argument("--no-configs", action="store_true", help="skip the 600-frame and mixed-batch legs")
-    ap.add_argument("--counters", action="store_true",
-                    help="one FP64 assembly of the workload plus a 1 GiB device copy, for rocprofv3 --pmc runs")
+    ap.add_argument("--counters", nargs="?", const="carbon", choices=["carbon", "chno"],
+                    help="one FP64 assembly of the carbon workload (or of the 256-frame mixed C/H/N/O batch) "
+                         "plus a 1 GiB device copy, for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the design-matrix path is single-device "
@@ -262,6 +273,15 @@ def main():
         sys.exit(2)
     ch = _load("chimes", os.path.join(PKG, "chimes.py"))
     synth = _load("synth", os.path.join(PKG, "synth.py"))
+    if args.counters == "chno":
+        text, frames, _ = synth.chno(nframes=256, nmin=32, nmax=2048, seed=104)
+        setup = setup_from(ch, text)
+        dev = ch.Device(setup)
+        dev.assemble(make_frames(ch, frames), prec=ch.CG_FP64, copy=False)
+        dev.copy_probe(COPY_PROBE_BYTES)
+        dev.close()
+        log("counter pass done: mixed-batch assembly + %d-byte copy" % COPY_PROBE_BYTES)
+        return
     log("generating %d carbon frames" % args.frames)
     text, frames, _ = synth.carbon(nframes=args.frames, natoms=512, seed=102)
     setup = setup_from(ch, text)
@@ -372,7 +392,8 @@ def main():
     if not args.no_cpu:
         log("timing the CPU oracle on the host")
         cpu = cpu_baseline(ch, setup, fr)
-        log("CPU: %.1f rows/s on %d cores" % (cpu["value"], cpu["cores"]))
+        log("CPU: %.1f rows/s on %d cores (machine count %d), %.1f rows/s on one core" % (
+            cpu["value"], cpu["cores"], cpu["nproc"], cpu["one_core"]["value"]))
     out = {
         "metric": METRIC, "value": value, "unit": "A rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -382,7 +403,7 @@ def main():
                                "orders 16/9/5, cutoffs 4.5/4.0/3.5 A, %d rows x %d cols" % (
                                    args.frames, rows, setup.model.ncols),
                    "frames": args.frames, "natoms": 512, "rows": rows, "cols": int(setup.model.ncols)},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
                      "pipe": "FP64 VALU (same 78.6 TFLOP/s dense peak as FP64 MFMA)",
                      "kernel": "k_assemble<4,5,0,double> (4-body rows)", "kernel_ms": ms4,
