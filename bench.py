@@ -100,9 +100,10 @@ def cpu_baseline(ch, setup, frames, seconds_hint=25.0):
                          "sample": "1 frame (%d rows) on one core, %.1f s" % (rows1, dt1)}}
 
 
-def serving_latency(ch, synth, n_req=40, natoms=1000, seed=105):
-    """Force/energy/stress latency of 1k-atom C/H/N/O configurations, one call at a time."""
-    text, frames, _ = synth.chno(nframes=n_req, nmin=natoms, nmax=natoms, seed=seed, target_density=0.1)
+def serving_latency(ch, synth, n_req=400, n_conf=16, natoms=1000, seed=105):
+    """Force/energy/stress latency of 1k-atom C/H/N/O configurations, one call at a
+    time (n_conf configurations of one shape, cycled over n_req calls)."""
+    text, frames, _ = synth.chno(nframes=n_conf, nmin=natoms, nmax=natoms, seed=seed, target_density=0.1)
     s = setup_from(ch, text)
     fr = make_frames(ch, frames)
     rng = np.random.default_rng(seed)
@@ -112,16 +113,17 @@ def serving_latency(ch, synth, n_req=40, natoms=1000, seed=105):
         dev.force(x, f)
     lat = []
     t0 = time.perf_counter()
-    for f in fr:
+    for i in range(n_req):
         t1 = time.perf_counter()
-        dev.force(x, f)
+        dev.force(x, fr[i % n_conf])
         lat.append((time.perf_counter() - t1) * 1e3)
     tot = time.perf_counter() - t0
+    replays = dev.force_stats()[2]
     dev.close()
     lat = np.array(lat)
     return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "atoms_per_s": natoms * n_req / tot, "requests": n_req, "natoms": natoms,
-            "orders": "12/6/3", "ncols": int(s.model.ncols)}
+            "atoms_per_s": natoms * n_req / tot, "requests": n_req, "configurations": n_conf,
+            "graph_replays": replays, "natoms": natoms, "orders": "12/6/3", "ncols": int(s.model.ncols)}
 
 
 def md_steps(ch, synth, n_steps=60, natoms=1000, seed=107):
@@ -156,37 +158,66 @@ def md_steps(ch, synth, n_steps=60, natoms=1000, seed=107):
     return out
 
 
-def serving_stream(ch, synth, n_req=300, qps=400.0, seed=106):
-    """Poisson stream of C/H/N/O configurations (64-4096 atoms, log-uniform)
-    through the continuous-batching server (2 workers); latency is measured in
-    the library from submit to completion."""
-    text, frames, _ = synth.chno(nframes=n_req + 8, nmin=64, nmax=4096, seed=seed, target_density=0.1)
+def serving_sweep(ch, synth, qps_points=(250, 500, 1000, 1500, 2000), n_req=1000, workers=4, seed=106):
+    """Poisson streams of C/H/N/O configurations through the continuous-batching
+    server, 1000 requests per offered rate.  Three requests in four are drawn from
+    200 configurations of 64-4096 atoms (log-uniform), every fourth is one of 16
+    1000-atom configurations; latency is measured in the library from submit to
+    completion and reported per size class."""
+    text, frames, _ = synth.chno(nframes=200, nmin=64, nmax=4096, seed=seed, target_density=0.1)
+    _, frames1k, _ = synth.chno(nframes=16, nmin=1000, nmax=1000, seed=seed + 100, target_density=0.1)
     s = setup_from(ch, text)
-    fr = make_frames(ch, frames)
+    mixed = make_frames(ch, frames)
+    onek = make_frames(ch, frames1k)
     rng = np.random.default_rng(seed)
     x = rng.normal(0, 1e-3, s.model.ncols)
-    srv = ch.Server(s, x, workers=2)
-    for t in [srv.submit(f) for f in fr[:8]]:   # warm-up (device buffers, tensors)
+    srv = ch.Server(s, x, workers=workers)
+    # warm-up: every configuration once, so that device buffers have reached
+    # their working sizes before the timed streams
+    for t in [srv.submit(f) for f in mixed + onek]:
         srv.poll(t)
-    fr = fr[8:]
-    b0, r0 = srv.stats()
-    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_req))
-    tickets = []
-    t0 = time.perf_counter()
-    for i, f in enumerate(fr):
-        dt = t0 + arrivals[i] - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        tickets.append(srv.submit(f))
-    lat = np.array([srv.poll(t)[3] for t in tickets])
-    wall = time.perf_counter() - t0
-    batches, requests = srv.stats()
+    classes = (("64-255", 64, 256), ("256-1023", 256, 1024), ("1024-4096", 1024, 4097))
+    points = []
+    for qps in qps_points:
+        reqs = [onek[(i // 4) % len(onek)] if i % 4 == 3 else mixed[int(rng.integers(len(mixed)))]
+                for i in range(n_req)]
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, n_req))
+        c0 = srv.counters()
+        tickets = []
+        t0 = time.perf_counter()
+        for i, f in enumerate(reqs):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(f))
+        lat = np.array([srv.poll(t)[3] for t in tickets])
+        wall = time.perf_counter() - t0
+        c1 = srv.counters()
+        ev = {k: c1[k] - c0[k] for k in c0}
+        n = np.array([f.natoms for f in reqs])
+        is1k = np.arange(n_req) % 4 == 3
+        pt = {"qps_offered": qps, "qps_served": n_req / wall, "requests": n_req,
+              "atoms_per_s": float(n.sum() / wall), "p50_ms": float(np.percentile(lat, 50)),
+              "p99_ms": float(np.percentile(lat, 99)), "mean_batch": ev["requests"] / max(1, ev["batches"]),
+              "events": ev,
+              "drain_ms": (wall - arrivals[-1]) * 1e3,
+              "p99_ms_by_atoms": {}, "one_k": {"p50_ms": float(np.percentile(lat[is1k], 50)),
+                                               "p99_ms": float(np.percentile(lat[is1k], 99)),
+                                               "requests": int(is1k.sum())}}
+        for name, lo, hi in classes:
+            sel = (~is1k) & (n >= lo) & (n < hi)
+            if sel.any():
+                pt["p99_ms_by_atoms"][name] = float(np.percentile(lat[sel], 99))
+        log("serving %5.0f req/s offered: %.0f served, p50 %.2f ms, p99 %.2f ms, 1k-atom p99 %.2f ms, "
+            "mean batch %.2f, events %s" % (qps, pt["qps_served"], pt["p50_ms"], pt["p99_ms"], pt["one_k"]["p99_ms"],
+                                            pt["mean_batch"], ev))
+        points.append(pt)
     srv.close()
-    atoms = sum(f.natoms for f in fr)
-    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "qps_offered": qps, "requests": n_req, "atoms_per_s": atoms / wall,
-            "mean_batch": (requests - r0) / max(1, batches - b0),
-            "natoms": "64-4096 log-uniform", "orders": "12/6/3", "workers": 2}
+    # sustainable: the highest offered rate whose queue drained within 50 ms of
+    # the last arrival and whose 1000-atom p99 stayed below 5 ms
+    ok = [p for p in points if p["drain_ms"] < 50.0 and p["one_k"]["p99_ms"] < 5.0]
+    return {"workers": workers, "natoms": "64-4096 log-uniform + 1000 (every 4th)", "orders": "12/6/3",
+            "sustainable_qps": max((p["qps_offered"] for p in ok), default=None), "points": points}
 
 
 def assemble_rate(ch, dev, fr, rows, prec, steps=2):
@@ -366,10 +397,9 @@ def main():
     if not args.no_serve:
         one = serving_latency(ch, synth)
         log("serving 1k atoms, one call at a time: p50 %.3f ms p99 %.3f ms" % (one["p50_ms"], one["p99_ms"]))
-        stream = serving_stream(ch, synth)
-        log("serving Poisson stream %.0f req/s: p50 %.3f ms p99 %.3f ms, %.3g atoms/s, mean batch %.2f" % (
-            stream["qps_offered"], stream["p50_ms"], stream["p99_ms"], stream["atoms_per_s"], stream["mean_batch"]))
-        extra["serving"] = {"single_1k_atoms": one, "poisson_stream": stream}
+        sweep = serving_sweep(ch, synth)
+        log("serving sweep: sustainable %s req/s" % sweep["sustainable_qps"])
+        extra["serving"] = {"single_1k_atoms": one, "stream_sweep": sweep}
         if not args.no_md:
             md = md_steps(ch, synth)
             log("MD-style steps, 1k atoms: graph replay p50 %.3f ms p99 %.3f ms (%d replays) | eager p50 %.3f ms "

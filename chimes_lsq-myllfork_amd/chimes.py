@@ -132,6 +132,7 @@ def lib(path=None):
         opt("cg_serve_submit", [C.c_void_p, C.POINTER(Frame), C.POINTER(C.c_int64)])
         opt("cg_serve_poll", [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp])
         opt("cg_serve_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        opt("cg_serve_counters", [C.c_void_p, C.POINTER(C.c_int64), C.c_int])
         opt("cg_serve_hold", [C.c_void_p, C.c_int])
         opt("cg_serve_destroy", [C.c_void_p])
         _lib = L
@@ -521,6 +522,15 @@ class Server:
         b, r = C.c_int64(), C.c_int64()
         check(self.L.cg_serve_stats(self.h, C.byref(b), C.byref(r)))
         return b.value, r.value
+
+    COUNTERS = ("batches", "requests", "isolated", "graph_replays", "graph_captures", "exact_repeats",
+                "buffer_moves")
+
+    def counters(self):
+        """Server counters (cg_serve_counters) as a dict."""
+        out = (C.c_int64 * len(self.COUNTERS))()
+        check(self.L.cg_serve_counters(self.h, out, len(self.COUNTERS)))
+        return dict(zip(self.COUNTERS, list(out)))
 
     def close(self):
         if self.h:

@@ -33,8 +33,8 @@
 
 namespace cg {
 
-std::atomic<uint64_t> g_buffer_generation{0};
 thread_local bool t_no_grow = false;
+thread_local uint64_t t_grow_events = 0;
 
 
 // ------------------------------------------------------------ host geometry
@@ -2551,6 +2551,30 @@ static std::string shape_key(const Ctx &c, const Batch &B, const Caps &caps) {
   return k;
 }
 
+constexpr size_t kMaxGraphs = 16;  // captured shapes kept per context
+
+// Every grow-only device buffer of a context.
+template <class Fn> static void for_each_dbuf(Ctx &c, Fn fn) {
+  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
+                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1],
+                  &c.b_fam[2], &c.b_tasks[0], &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1],
+                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
+                  &c.b_coef[2], &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars})
+    fn(*b);
+}
+
+// Addresses of the context's buffers (and pinned result buffer), hashed: a
+// captured graph is replayed only while they are the ones it was captured
+// with.
+static uint64_t buffers_fingerprint(Ctx &c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for_each_dbuf(c, [&](DBuf &b) { mix((uint64_t)(uintptr_t)b.p); mix(b.cap); });
+  mix((uint64_t)(uintptr_t)c.h_pin);
+  mix(c.h_pin_cap);
+  return h;
+}
+
 static void drop_graphs(Ctx &c) {
   for (auto &g : c.graphs)
     if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
@@ -2633,11 +2657,12 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   double *d_F = c.b_force.as<double>(nout + 16);
   if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
   if (!noalloc && c.h_pin_cap < nout + 4) {
+    const size_t old_cap = c.h_pin_cap;
     if (c.h_pin) (void)hipHostFree(c.h_pin);
     c.h_pin = nullptr;
     c.h_pin_cap = 0;
-    ++g_buffer_generation;
-    const size_t n = nout + nout / 4 + 64;
+    ++t_grow_events;
+    const size_t n = std::max(nout + nout / 4 + 64, std::min(2 * old_cap, 2 * nout + 64));
     CG_HIP(hipHostMalloc((void **)&c.h_pin, n * sizeof(double), hipHostMallocDefault));
     c.h_pin_cap = n;
   }
@@ -2645,20 +2670,30 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   // calls with Ewald terms upload per-call k tables and stay out of graphs
   if (capmode && m.n_charge_cols == 0) {
     const std::string key = shape_key(c, B, caps);
+    const uint64_t fp = buffers_fingerprint(c);
     auto it = c.graphs.find(key);
-    if (it != c.graphs.end() && it->second.gen != g_buffer_generation) {
+    if (it != c.graphs.end() && it->second.buffers != fp) {
+      // a buffer moved since the capture: every graph of the context is stale
       drop_graphs(c);
       it = c.graphs.end();
     }
     if (it != c.graphs.end()) {
+      it->second.used = ++c.graph_clock;
       CG_HIP(hipGraphLaunch(it->second.exec, c.stream));
       ++c.n_graph;
       done = true;
     } else if (key == c.last_key) {
       hipGraphExec_t exec = nullptr;
       if (capture_call(c, B, caps, d_F, nout, &exec)) {
-        if (c.graphs.size() >= 16) drop_graphs(c);
-        c.graphs[key] = Ctx::Graph{exec, g_buffer_generation.load()};
+        ++c.n_capture;
+        if (c.graphs.size() >= kMaxGraphs) {
+          auto lru = c.graphs.begin();
+          for (auto g = c.graphs.begin(); g != c.graphs.end(); ++g)
+            if (g->second.used < lru->second.used) lru = g;
+          (void)hipGraphExecDestroy(lru->second.exec);
+          c.graphs.erase(lru);
+        }
+        c.graphs[key] = Ctx::Graph{exec, fp, ++c.graph_clock};
         CG_HIP(hipGraphLaunch(exec, c.stream));
         done = true;
       }
@@ -2792,11 +2827,7 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 }
 
 void ctx_free(Ctx &c) {
-  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
-                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1], &c.b_fam[2], &c.b_tasks[0],
-                  &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1], &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1], &c.b_coef[2],
-                  &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars})
-    b->release();
+  for_each_dbuf(c, [](DBuf &b) { b.release(); });
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
                     (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.recip, (void *)F.cct, (void *)F.csmax,
@@ -2931,7 +2962,6 @@ cg_status cg_append_rows(cg_ctx *ctx, const double *rows, int nrows) {
       c.b_A.release();
       c.b_A.p = p;
       c.b_A.cap = n + n / 4 + 256;
-      ++cg::g_buffer_generation;
       c.A = p;
     }
     char *dst = static_cast<char *>(c.A) + old_bytes;

@@ -22,12 +22,12 @@ void set_error(const std::string &m);
       throw ::cg::Error(CG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Incremented whenever a grow-only buffer moves: captured HIP graphs hold the
-// buffers' addresses and are dropped when it changes.
-extern std::atomic<uint64_t> g_buffer_generation;
 // Set on a thread while it captures a graph: a buffer that would have to
 // grow then fails instead (allocation is not part of a capture).
 extern thread_local bool t_no_grow;
+// Buffer moves (device or pinned) made by this thread: each one is an
+// allocation, and a device free also waits for the device.
+extern thread_local uint64_t t_grow_events;
 
 struct DBuf;
 
@@ -73,10 +73,15 @@ struct DBuf {
     if (t_arena) return t_arena->take(this, bytes);
     if (bytes > cap) {
       if (t_no_grow) return nullptr;
+      // geometric growth (at least twice the old capacity, at most twice the
+      // request): a buffer that follows growing requests moves O(log) times
+      const size_t old = cap;
       if (p) (void)hipFree(p);
       p = nullptr;
-      ++g_buffer_generation;
+      cap = 0;
+      ++t_grow_events;
       size_t n = bytes + bytes / 4 + 256;
+      if (old > 0) n = std::max(n, std::min(2 * old, 2 * bytes + 256));
       if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; cap = 0; return nullptr; }
       cap = n;
     }
@@ -217,9 +222,14 @@ struct Ctx {
   double rate_cl[3] = {0.0, 0.0, 0.0};    // largest clusters per atom seen, per family
   int64_t n_force = 0, n_repeat = 0;      // force calls, and those repeated with exact sizes
   int64_t n_graph = 0;                    // force calls served by replaying a captured graph
+  int64_t n_capture = 0;                  // force calls captured into a new graph
   // captured force calls by shape (frame sizes, cell lists, list capacities);
   // a shape is captured the second time in a row it is seen
-  struct Graph { hipGraphExec_t exec = nullptr; uint64_t gen = 0; };
+  // captured force calls per shape key: the buffer addresses they were
+  // captured with (fingerprint) and the last use (least recently used graph
+  // is evicted when the cache is full)
+  struct Graph { hipGraphExec_t exec = nullptr; uint64_t buffers = 0; uint64_t used = 0; };
+  uint64_t graph_clock = 0;
   std::map<std::string, Graph> graphs;
   std::string last_key;
   Caps last_caps;                         // list capacities of the last capacity-mode call

@@ -77,6 +77,10 @@ struct Server {
   bool hold = false;  // workers take no new batch while set (cg_serve_hold)
   int64_t n_batches = 0, n_requests = 0;
   int64_t n_isolated = 0;  // failed batches whose requests were re-run one by one
+  // summed over the workers' contexts: calls replayed from a graph, calls
+  // captured into a new graph, calls repeated with exact list sizes, and
+  // buffer moves (allocations)
+  int64_t n_replay = 0, n_capture = 0, n_repeat = 0, n_grow = 0;
   std::vector<std::thread> workers;
   std::string init_error;
   int ready = 0;
@@ -156,6 +160,8 @@ struct Server {
       // one pass over the whole batch; if it fails, each request is
       // evaluated alone so that only the requests that fail by themselves
       // report an error
+      const int64_t g0 = ctx.n_graph, c0 = ctx.n_capture, r0 = ctx.n_repeat;
+      const uint64_t a0 = cg::t_grow_events;
       std::string err;
       const cg_status st = evaluate(ctx, batch, 0, batch.size(), F, E, V, err);
       if (st == CG_OK) {
@@ -193,6 +199,10 @@ struct Server {
           rp->t_done = now;
         }
         if (st != CG_OK && batch.size() > 1) ++n_isolated;
+        n_replay += ctx.n_graph - g0;
+        n_capture += ctx.n_capture - c0;
+        n_repeat += ctx.n_repeat - r0;
+        n_grow += (int64_t)(cg::t_grow_events - a0);
         inflight -= bytes;
         ++n_batches;
         n_requests += (int64_t)batch.size();
@@ -337,6 +347,18 @@ cg_status cg_serve_stats(cg_server *sv, int64_t *batches, int64_t *requests) {
   std::lock_guard<std::mutex> lk(sv->s.mu);
   if (batches) *batches = sv->s.n_batches;
   if (requests) *requests = sv->s.n_requests;
+  return CG_OK;
+}
+
+cg_status cg_serve_counters(cg_server *sv, int64_t *out, int n) {
+  if (!sv || !out || n < 0) {
+    cg::set_error("cg_serve_counters: bad argument");
+    return CG_ERR_ARG;
+  }
+  const Server &s = sv->s;
+  std::lock_guard<std::mutex> lk(sv->s.mu);
+  const int64_t v[7] = {s.n_batches, s.n_requests, s.n_isolated, s.n_replay, s.n_capture, s.n_repeat, s.n_grow};
+  for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
   return CG_OK;
 }
 

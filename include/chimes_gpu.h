@@ -469,6 +469,11 @@ cg_status cg_serve_poll(cg_server *s, int64_t ticket, int wait, int *done, doubl
  * usual batch and budget limits.  Lets a caller group submissions. */
 cg_status cg_serve_hold(cg_server *s, int hold);
 cg_status cg_serve_stats(cg_server *s, int64_t *batches, int64_t *requests);
+/* Server counters, the first n of: batches, requests, batches re-run request
+ * by request after a failure, calls replayed from a captured graph, calls
+ * captured into a new graph, calls repeated with exact list sizes, and
+ * device or pinned buffer moves (each an allocation) made by the workers. */
+cg_status cg_serve_counters(cg_server *s, int64_t *out, int n);
 /* Finishes every queued request, then stops the workers. */
 void cg_serve_destroy(cg_server *s);
 
