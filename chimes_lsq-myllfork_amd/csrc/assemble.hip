@@ -1120,6 +1120,182 @@ __device__ __forceinline__ void set_loop3(const Tab3 &t, const int *cp, uint32_t
 }
 #undef CG_SET3
 
+// FP32 tables of the FP32-A path (4-body powers below 5, 3-body below 9):
+// F and D of power p of a pair are one 64-bit entry (F low, D high) at offset
+// 2p of the pair's slot, so the FP64 set records (2 * power) select entries
+// unchanged and one 64-bit move reads both.  4-body: pair k at v[64 + 10k]
+// (k < 3) / v[96 + 10(k-3)]; 3-body: pair k at v[64 + 18k].  The tables sit
+// in v[64:127].
+struct Tab4f {
+  v32u A, B;
+  __device__ __forceinline__ void zero() { A = 0u; B = 0u; }
+};
+struct Tab4fPair {
+  Tab4f &t;
+  int k;
+  __device__ __forceinline__ void set(int i, float f, float d) {
+    const int o = 10 * (k % 3) + 2 * i;
+    if (k < 3) {
+      t.A[o] = __builtin_bit_cast(uint32_t, f); t.A[o + 1] = __builtin_bit_cast(uint32_t, d);
+    } else {
+      t.B[o] = __builtin_bit_cast(uint32_t, f); t.B[o + 1] = __builtin_bit_cast(uint32_t, d);
+    }
+  }
+};
+struct Tab3f {
+  v32u A, B;
+  __device__ __forceinline__ void zero() { A = 0u; B = 0u; }
+};
+struct Tab3fPair {
+  Tab3f &t;
+  int k;
+  __device__ __forceinline__ void put(int o, float x) {
+    if (o >= 32) t.B[o - 32] = __builtin_bit_cast(uint32_t, x);
+    else t.A[o] = __builtin_bit_cast(uint32_t, x);
+  }
+  __device__ __forceinline__ void set(int i, float f, float d) {
+    put(18 * k + 2 * i, f);
+    put(18 * k + 2 * i + 1, d);
+  }
+};
+
+// One allowed 4-body set in FP32, the products of CG_SET4: (F0, D0) and
+// (F5, D5) are copied to v[60:63], the other pairs are read in place.
+#define CG_SET4F(i0, i1, i2, i3, i4, i5)                     \
+  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
+  "v_mov_b64 v[60:61], v[64:65]\n\t"                         \
+  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
+  "v_mov_b64 v[62:63], v[116:117]\n\t"                       \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_mul_f32 %[M], v75, v60\n\t"                             \
+  "v_mul_f32 %[N], v74, v61\n\t"                             \
+  "v_mul_f32 %[L], v74, v60\n\t"                             \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_mul_f32 %[M4], v107, v62\n\t"                           \
+  "v_mul_f32 %[N4], v106, v63\n\t"                           \
+  "v_mul_f32 %[R], v106, v62\n\t"                            \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_mul_f32 %[DL2], v85, %[L]\n\t"                          \
+  "v_mul_f32 %[L], v84, %[L]\n\t"                            \
+  "v_mul_f32 %[M], v84, %[M]\n\t"                            \
+  "v_mul_f32 %[N], v84, %[N]\n\t"                            \
+  "s_set_gpr_idx_idx " #i3 "\n\t"                            \
+  "v_mul_f32 %[DR3], v97, %[R]\n\t"                          \
+  "v_mul_f32 %[R], v96, %[R]\n\t"                            \
+  "v_mul_f32 %[M4], v96, %[M4]\n\t"                          \
+  "v_mul_f32 %[N4], v96, %[N4]\n\t"                          \
+  "s_set_gpr_idx_off\n\t"                                    \
+  "v_fmac_f32 %[E], %[L], %[R]\n\t"                          \
+  "v_fmac_f32 %[G0], %[N], %[R]\n\t"                         \
+  "v_fmac_f32 %[G1], %[M], %[R]\n\t"                         \
+  "v_fmac_f32 %[G2], %[DL2], %[R]\n\t"                       \
+  "v_fmac_f32 %[G3], %[DR3], %[L]\n\t"                       \
+  "v_fmac_f32 %[G4], %[M4], %[L]\n\t"                        \
+  "v_fmac_f32 %[G5], %[N4], %[L]\n\t"
+
+// set_loop4 with FP32 tables and sums (same record stream and read-ahead).
+__device__ __forceinline__ void set_loop4f(const Tab4f &t, const int *cp, uint32_t lo, uint32_t hi, float &E,
+                                           float *G) {
+  float L, M, N, DL2, R, M4, N4, DR3;
+  float G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
+  asm volatile(
+      "s_mov_b32 s59, m0\n\t"
+      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_sub_i32 s58, s41, s40\n\t"
+      "s_cmp_lt_i32 s58, 1\n\t"
+      "s_cbranch_scc1 L_set4f_done_%=\n\t"
+      "s_mov_b32 s41, 0\n\t"
+      "s_lshl_b64 s[40:41], s[40:41], 5\n\t"
+      "s_add_u32 s56, s40, %[lo]\n\t"
+      "s_addc_u32 s57, s41, %[hi]\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_set4f_%=:\n\t"
+      "s_load_dwordx8 s[48:55], s[56:57], 0x20\n\t"
+      CG_SET4F(s40, s41, s42, s43, s44, s45)
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_cbranch_scc1 L_set4f_done_%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x40\n\t"
+      CG_SET4F(s48, s49, s50, s51, s52, s53)
+      "s_add_u32 s56, s56, 0x40\n\t"
+      "s_addc_u32 s57, s57, 0\n\t"
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cbranch_scc0 L_set4f_%=\n"
+      "L_set4f_done_%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, s59\n\t"
+      : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
+        [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N), [DL2] "=&v"(DL2), [R] "=&v"(R), [M4] "=&v"(M4), [N4] "=&v"(N4),
+        [DR3] "=&v"(DR3)
+      : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[64:95]}"(t.A), "{v[96:127]}"(t.B)
+      : "v60", "v61", "v62", "v63", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50",
+        "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "scc");
+  G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
+}
+#undef CG_SET4F
+
+// One allowed 3-body set in FP32 (products of CG_SET3).
+#define CG_SET3F(i0, i1, i2)                                 \
+  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
+  "v_mov_b64 v[60:61], v[64:65]\n\t"                         \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_mul_f32 %[L], v82, v60\n\t"                             \
+  "v_mul_f32 %[M], v83, v60\n\t"                             \
+  "v_mul_f32 %[N], v82, v61\n\t"                             \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_fmac_f32 %[E], v100, %[L]\n\t"                          \
+  "v_fmac_f32 %[G2], v101, %[L]\n\t"                         \
+  "v_fmac_f32 %[G1], v100, %[M]\n\t"                         \
+  "v_fmac_f32 %[G0], v100, %[N]\n\t"                         \
+  "s_set_gpr_idx_off\n\t"
+
+__device__ __forceinline__ void set_loop3f(const Tab3f &t, const int *cp, uint32_t lo, uint32_t hi, float &E,
+                                           float *G) {
+  float L, M, N;
+  float G0 = G[0], G1 = G[1], G2 = G[2];
+  asm volatile(
+      "s_mov_b32 s59, m0\n\t"
+      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_sub_i32 s58, s41, s40\n\t"
+      "s_cmp_lt_i32 s58, 1\n\t"
+      "s_cbranch_scc1 L_set3f_done_%=\n\t"
+      "s_mov_b32 s41, 0\n\t"
+      "s_lshl_b64 s[40:41], s[40:41], 4\n\t"
+      "s_add_u32 s56, s40, %[lo]\n\t"
+      "s_addc_u32 s57, s41, %[hi]\n\t"
+      "s_load_dwordx4 s[40:43], s[56:57], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_set3f_%=:\n\t"
+      "s_load_dwordx4 s[44:47], s[56:57], 0x10\n\t"
+      CG_SET3F(s40, s41, s42)
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_cbranch_scc1 L_set3f_done_%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_load_dwordx4 s[40:43], s[56:57], 0x20\n\t"
+      CG_SET3F(s44, s45, s46)
+      "s_add_u32 s56, s56, 0x20\n\t"
+      "s_addc_u32 s57, s57, 0\n\t"
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cbranch_scc0 L_set3f_%=\n"
+      "L_set3f_done_%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, s59\n\t"
+      : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N)
+      : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[64:95]}"(t.A), "{v[96:127]}"(t.B)
+      : "v60", "v61", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s56", "s57", "s58", "s59", "scc");
+  G[0] = G0; G[1] = G1; G[2] = G2;
+}
+#undef CG_SET3F
+
 template <class T> __device__ __forceinline__ void lds_add(T *p, T v) { unsafeAtomicAdd(p, v); }
 
 // LDS tile geometry of k_assemble (shared with the host task builder).
@@ -1138,11 +1314,13 @@ template <int NB, int VW, int BASE, class T, int BS>
 __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemble(AsmArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
   // 4-body with powers below 5 and 3-body with powers below 9 take the
-  // hand-scheduled FP64 set loops for both precisions (FP32 A is then
-  // rounded once, at the write-out); other kernels compute tables, products
-  // and projections in T
+  // hand-scheduled set loops: FP64 tables and sums for FP64 A, FP32 ones for
+  // FP32 A (F32A); other kernels compute tables, products and projections in T
   constexpr bool ASM4 = NB == 4 && VW <= 5;
   constexpr bool ASM3 = NB == 3 && VW <= 9;
+  constexpr bool F32A = std::is_same<T, float>::value && (ASM4 || ASM3);
+  // the set-loop sums of F32A are widened once per column: positions,
+  // projection and tile stay in FP64
   using TP = typename std::conditional<ASM4 || ASM3, double, T>::type;
   // The tile is accumulated in FP64 for both precisions: LDS ds_add_f64 is
   // fast on gfx950 while ds_add_f32 stalls the LDS pipeline, and the FP32
@@ -1178,6 +1356,9 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   const int R = (ncl + BS - 1) / BS;
   for (int rnd = 0; rnd < R; ++rnd) {
     const int j = threadIdx.x * R + rnd;
+    // a wave whose first lane is past the bucket's end has no cluster in this
+    // round (or any later one): it leaves the loop (there is no barrier in it)
+    if ((int)(threadIdx.x & ~63u) * R + rnd >= ncl) break;
     const bool valid = j < ncl;
     // lanes past the bucket's end build the tables of its last cluster (all
     // registers then hold defined values without a clearing pass) and drop
@@ -1187,6 +1368,8 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     PairTab<T, VW> tab[ASM4 || ASM3 ? 1 : NP];
     Tab4 tab4;
     Tab3 tab3;
+    Tab4f tab4f;
+    Tab3f tab3f;
     // positions relative to the cluster's first atom and, per pair,
     // PERM_SCALE / r: pair vector v = d[y] - d[x] (d[0] = 0), force on x is
     // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
@@ -1214,7 +1397,13 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
           sc[k] = (TP)((double)ps * recip_nr(r));
           const int q = ty * NP + k;
-          if constexpr (ASM4) {
+          if constexpr (ASM4 && F32A) {
+            Tab4fPair tp{tab4f, k};
+            pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+          } else if constexpr (ASM3 && F32A) {
+            Tab3fPair tp{tab3f, k};
+            pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+          } else if constexpr (ASM4) {
             Tab4Pair tp{tab4, k};
             pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
           } else if constexpr (ASM3) {
@@ -1232,7 +1421,18 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
       TP e = (TP)0, g[NP];
 #pragma unroll
       for (int k = 0; k < NP; ++k) g[k] = (T)0;
-      if constexpr (ASM4) {
+      if constexpr (F32A) {
+        float ef = 0.0f, gf[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) gf[k] = 0.0f;
+        if constexpr (ASM4)
+          set_loop4f(tab4f, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), ef, gf);
+        else
+          set_loop3f(tab3f, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), ef, gf);
+        e = ef;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) g[k] = gf[k];
+      } else if constexpr (ASM4) {
         set_loop4(tab4, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
       } else if constexpr (ASM3) {
         set_loop3(tab3, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
