@@ -270,9 +270,12 @@ def config_c4(ch, synth, nframes=256):
            "cols": int(s.model.ncols), "atoms": sum(f.natoms for f in fr)}
     out["fp64_rows_per_s_default_tiles"] = assemble_rate(ch, dev, fr, rows, ch.CG_FP64, steps=1)
     choice, ms = dev.autotune(fr)
-    cfgs = ["512 threads, widest tile", "256 threads, widest tile", "2 x 256 threads per CU, half tile"]
-    out["autotune"] = {"choice": {"3B": cfgs[choice[1]], "4B": cfgs[choice[2]]},
-                       "kernel_ms": {cfgs[k]: {"3B": float(ms[k][1]), "4B": float(ms[k][2])} for k in range(3)}}
+    cfgs, classes = ch.Device.TILE_CONFIGS, ch.Device.TILE_CLASSES
+    out["autotune"] = {
+        "choice": {fam: {classes[q]: cfgs[choice[w][q]] for q in range(len(classes)) if choice[w][q] >= 0}
+                   for w, fam in ((1, "3B"), (2, "4B"))},
+        "kernel_ms": {cfgs[k]: {fam: {classes[q]: float(ms[k][w][q]) for q in range(len(classes)) if choice[w][q] >= 0}
+                                for w, fam in ((1, "3B"), (2, "4B"))} for k in range(len(cfgs))}}
     for prec, tag in [(ch.CG_FP64, "fp64_rows_per_s"), (ch.CG_FP32, "fp32_rows_per_s")]:
         out[tag] = assemble_rate(ch, dev, fr, rows, prec, steps=1)
         ms, ncl = dev.detail()

@@ -122,6 +122,8 @@ def lib(path=None):
         opt("cg_fit_lassolars", [C.c_void_p, _dp, C.c_double, C.c_int, _dp, C.POINTER(FitResult)])
         opt("cg_setup_fit_columns", [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
         opt("cg_autotune", [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.POINTER(C.c_int), _dp])
+        opt("cg_tile_choice", [C.c_void_p, C.POINTER(C.c_int)])
+        opt("cg_tile_set", [C.c_void_p, C.POINTER(C.c_int)])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
         opt("cg_append_rows", [C.c_void_p, _dp, C.c_int])
@@ -434,15 +436,33 @@ class Device:
         arr = (C.c_int * max(len(keep), 1))(*keep)
         check(self.L.cg_set_fit_columns(self.h, arr, len(keep)), self.h)
 
+    TILE_CONFIGS = ("512 threads, widest tile", "256 threads, widest tile", "2 x 256 threads per CU, half tile",
+                    "512 threads, at most 16 columns", "2 x 256 threads per CU, half tile, at most 16 columns")
+    TILE_CLASSES = ("<128 atoms", "128-511 atoms", "512-1023 atoms", ">=1024 atoms")
+
     def autotune(self, frames, prec=CG_FP64):
-        """Pick the assembly tile configuration per family by timing (see cg_autotune);
-        returns (choice per family, kernel ms [config][family])."""
+        """Pick the assembly tile configuration per (family, frame-size class) by timing
+        (see cg_autotune); returns (choice [family][class], -1 for a class without frames,
+        kernel ms [config][family][class])."""
         n = len(frames)
         cf = (Frame * n)(*[f.c() for f in frames])
-        choice = (C.c_int * 3)()
-        ms = np.zeros(9)
+        nk, nc = len(self.TILE_CONFIGS), len(self.TILE_CLASSES)
+        choice = (C.c_int * (3 * nc))()
+        ms = np.zeros(nk * 3 * nc)
         check(self.L.cg_autotune(self.h, cf, n, prec, choice, dptr(ms)), self.h)
-        return list(choice), ms.reshape(3, 3)
+        return np.array(list(choice)).reshape(3, nc), ms.reshape(nk, 3, nc)
+
+    def tile_set(self, choice):
+        """Use the given tile configuration [family][frame-size class]."""
+        flat = [int(v) for v in np.asarray(choice).reshape(-1)]
+        check(self.L.cg_tile_set(self.h, (C.c_int * len(flat))(*flat)), self.h)
+
+    def tile_choice(self):
+        """Tile configuration in use, [family][frame-size class]."""
+        nc = len(self.TILE_CLASSES)
+        choice = (C.c_int * (3 * nc))()
+        check(self.L.cg_tile_choice(self.h, choice), self.h)
+        return np.array(list(choice)).reshape(3, nc)
 
     def force_stats(self):
         """(force calls, calls repeated with exact list sizes, calls served by a graph replay)."""

@@ -2013,18 +2013,42 @@ static void build_family(Ctx &c, int which) {
   F.ncol_type = dalloc_copy(F.h_ncols);
 }
 
-// Tile configurations of the assembly (threads per workgroup, LDS budget of a
-// workgroup's tile): one workgroup of 512 per CU with the widest tile, one of
-// 256 (fewer idle lanes on small (frame, type) buckets), or two of 256 per CU
-// with tiles half as wide.  cg_autotune picks one per family by timing.
-struct TileCfg { int bs; size_t lds; };
-static constexpr TileCfg kTileCfg[] = {{512, 150 * 1024}, {256, 150 * 1024}, {256, 75 * 1024}};
+// Tile configurations of the assembly: threads per workgroup, LDS budget of a
+// workgroup's tile, and the most columns a tile may take.  0: one workgroup
+// of 512 per CU with the widest tile; 1: one of 256 (fewer idle lanes on
+// small (frame, type) buckets); 2: two of 256 per CU, tiles half as wide;
+// 3 and 4: 0 and 2 with at most 16 columns per tile (more, shorter tasks for
+// small frames).  cg_autotune picks one per (family, frame-size class).
+struct TileCfg { int bs; size_t lds; int wcap; };
+static constexpr TileCfg kTileCfg[] = {{512, 150 * 1024, 64}, {256, 150 * 1024, 64}, {256, 75 * 1024, 64},
+                                       {512, 150 * 1024, 16}, {256, 75 * 1024, 16}};
 constexpr int kNumTileCfg = sizeof(kTileCfg) / sizeof(kTileCfg[0]);
+static_assert(kNumTileCfg == CG_TILE_CONFIGS, "tile configurations");
+
+// Frame-size class of the tile configuration: atoms below 128, 512, 1024, above.
+static int tile_class(int natoms) { return natoms < 128 ? 0 : natoms < 512 ? 1 : natoms < 1024 ? 2 : 3; }
+
+// Choices of cg_autotune kept for the life of the process, per model
+// (species, orders, columns), family and frame-size class; contexts created
+// later for the same model start from them.
+static std::mutex g_tile_mu;
+static std::map<std::string, int> g_tile_cache;
+static std::string tile_key(const cg_model &m, int w, int cls) {
+  std::string k;
+  for (int t = 0; t < m.ntypes; ++t) {
+    if (m.type_names) k.append(m.type_names + 8 * t, strnlen(m.type_names + 8 * t, 8));
+    k += ',';
+  }
+  k += std::to_string(m.order2b) + ',' + std::to_string(m.c3.order) + ',' + std::to_string(m.c4.order) + ',' +
+       std::to_string(m.ncols) + '|' + std::to_string(w) + '|' + std::to_string(cls);
+  return k;
+}
 
 // Which kernel instance handles a family.
 template <class T>
-static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, const Task *d_tasks,
-                          const DevFrame *d_frames, const uint32_t *refs, size_t lds_bytes, void *A, long long ld) {
+static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks, const DevFrame *d_frames,
+                          const uint32_t *refs, size_t lds_bytes, void *A, long long ld, int want_bs,
+                          hipStream_t stream) {
   const DevFamily &F = c.fam[which];
   AsmArgs a{};
   a.tasks = d_tasks;
@@ -2043,16 +2067,16 @@ static void launch_family(Ctx &c, int which, const std::vector<Task> &tasks, con
   a.stress_mode = c.m.fit_stress;
   a.deriv_const = c.m.deriv_const; a.smooth_dist = c.m.smooth_dist; a.fcut_offset = c.m.fcut_offset;
   a.nb_scale = F.nb;
-  dim3 grid((unsigned)tasks.size()), block(512);
+  dim3 grid((unsigned)ntasks), block(512);
   (void)block;
-  if (tasks.empty()) return;
-  const int want_bs = which == 0 ? 512 : kTileCfg[c.tile_cfg[which]].bs;
+  if (ntasks == 0) return;
+  if (which == 0) want_bs = 512;
 #define CG_LAUNCH_BS(NBv, VWv, BASEv, BSv)                                                                  \
   do {                                                                                                      \
     auto kern = k_assemble<NBv, VWv, BASEv, T, BSv>;                                                        \
     block = dim3(BSv);                                                                                      \
     CG_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes)); \
-    hipLaunchKernelGGL(kern, grid, block, lds_bytes, c.stream, a);                                          \
+    hipLaunchKernelGGL(kern, grid, block, lds_bytes, stream, a);                                            \
     CG_HIP(hipGetLastError());                                                                              \
   } while (0)
   // register-heavy instances run 256 threads whatever the configuration
@@ -2537,39 +2561,78 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
   for (auto &d : hf) maxn = std::max(maxn, d.natoms);
   bool fam_ran[3] = {false, false, false};
   std::vector<Task> atasks[3];  // host sources of the task uploads, alive until the final sync
+  std::vector<std::pair<int, int>> fam_groups[3];  // first and last size class of each launch
   for (int w = 0; w < 3; ++w) c.ncl[w] = fo[w].ncl;
   for (int w = 0; w < 3; ++w) {
+    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) c.ms_cls[w][cls] = 0.f;
     const DevFamily &F = c.fam[w];
     if (F.ntypes == 0 || F.ncols == 0 || fo[w].ncl == 0) continue;
-    std::vector<Task> tasks;
-    const size_t lds_budget = kTileCfg[w == 0 ? 0 : c.tile_cfg[w]].lds;
-    for (int fi = 0; fi < nframes; ++fi) {
-      int wmax = 64;
-      while (wmax > 0 && tile_bytes(hf[fi].natoms, wmax) > lds_budget) --wmax;
-      if (wmax < 1) throw Error(CG_ERR_UNSUPPORTED, "frame too large for the LDS tile (" + std::to_string(hf[fi].natoms) + " atoms)");
-      for (int t = 0; t < F.ntypes; ++t) {
-        int c0 = fo[w].boff[fi * F.ntypes + t], c1 = fo[w].boff[fi * F.ntypes + t + 1];
-        if (c1 <= c0) continue;
-        int nc = F.h_ncols[t];
-        int ntile = (nc + wmax - 1) / wmax;
-        int wt = (nc + ntile - 1) / ntile;
-        for (int col = 0; col < nc; col += wt) tasks.push_back(Task{fi, t, col, std::min(wt, nc - col), c0, c1});
+    // tasks per frame-size class, with that class's tile configuration;
+    // adjacent classes of one configuration share a launch (each class is
+    // launched alone while the autotuner times them)
+    auto cfg_of = [&](int cls) { return w == 0 ? 0 : c.tile_cfg[w][cls]; };
+    std::vector<Task> &tasks = atasks[w];
+    struct Group { size_t begin, end; int cls0, cls1; };
+    std::vector<Group> groups;
+    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) {
+      const TileCfg &cfg = kTileCfg[cfg_of(cls)];
+      const size_t first = tasks.size();
+      for (int fi = 0; fi < nframes; ++fi) {
+        if (tile_class(hf[fi].natoms) != cls) continue;
+        int wmax = cfg.wcap;
+        while (wmax > 0 && tile_bytes(hf[fi].natoms, wmax) > cfg.lds) --wmax;
+        if (wmax < 1) throw Error(CG_ERR_UNSUPPORTED, "frame too large for the LDS tile (" + std::to_string(hf[fi].natoms) + " atoms)");
+        for (int t = 0; t < F.ntypes; ++t) {
+          int c0 = fo[w].boff[fi * F.ntypes + t], c1 = fo[w].boff[fi * F.ntypes + t + 1];
+          if (c1 <= c0) continue;
+          int nc = F.h_ncols[t];
+          int ntile = (nc + wmax - 1) / wmax;
+          int wt = (nc + ntile - 1) / ntile;
+          for (int col = 0; col < nc; col += wt) tasks.push_back(Task{fi, t, col, std::min(wt, nc - col), c0, c1});
+        }
+      }
+      if (tasks.size() == first) continue;
+      if (!groups.empty() && !c.split_classes && groups.back().cls1 == cls - 1 &&
+          cfg_of(groups.back().cls0) == cfg_of(cls)) {
+        groups.back().end = tasks.size();
+        groups.back().cls1 = cls;
+      } else {
+        groups.push_back(Group{first, tasks.size(), cls, cls});
       }
     }
-    // heavy tasks first
-    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &x, const Task &y) {
-      return (long long)(x.c1 - x.c0) * x.ncols > (long long)(y.c1 - y.c0) * y.ncols;
-    });
+    // heavy tasks first within a launch
+    for (const Group &g : groups)
+      std::stable_sort(tasks.begin() + g.begin, tasks.begin() + g.end, [](const Task &x, const Task &y) {
+        return (long long)(x.c1 - x.c0) * x.ncols > (long long)(y.c1 - y.c0) * y.ncols;
+      });
     Task *d_tasks = c.b_tasks[w].as<Task>(tasks.size());
     if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
-    atasks[w] = std::move(tasks);
-    upload(c, d_tasks, atasks[w].data(), atasks[w].size() * sizeof(Task));
-    size_t lds = 0;
-    for (auto &tk : atasks[w]) lds = std::max(lds, tile_bytes(hf[tk.frame].natoms, tk.ncols));
+    upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(Task));
     CG_HIP(hipEventRecord(c.fev[w][0], c.stream));
-    if (prec == CG_FP32) launch_family<float>(c, w, atasks[w], d_tasks, d_frames, fo[w].refs, lds, A, ld);
-    else launch_family<double>(c, w, atasks[w], d_tasks, d_frames, fo[w].refs, lds, A, ld);
+    // launches of different configurations run side by side (their tails
+    // overlap); the autotuner's timing runs keep them in sequence
+    const bool fork = groups.size() > 1 && !c.split_classes;
+    if (fork) CG_HIP(hipEventRecord(c.fork, c.stream));
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const Group &g = groups[gi];
+      hipStream_t st = fork && gi > 0 ? c.aux[gi - 1] : c.stream;
+      if (st != c.stream) CG_HIP(hipStreamWaitEvent(st, c.fork, 0));
+      size_t lds = 0;
+      for (size_t i = g.begin; i < g.end; ++i) lds = std::max(lds, tile_bytes(hf[tasks[i].frame].natoms, tasks[i].ncols));
+      const int bs = kTileCfg[cfg_of(g.cls0)].bs;
+      CG_HIP(hipEventRecord(c.cev[w][g.cls0][0], st));
+      if (prec == CG_FP32)
+        launch_family<float>(c, w, g.end - g.begin, d_tasks + g.begin, d_frames, fo[w].refs, lds, A, ld, bs, st);
+      else
+        launch_family<double>(c, w, g.end - g.begin, d_tasks + g.begin, d_frames, fo[w].refs, lds, A, ld, bs, st);
+      CG_HIP(hipEventRecord(c.cev[w][g.cls0][1], st));
+      if (st != c.stream) {
+        CG_HIP(hipEventRecord(c.join[gi - 1], st));
+        CG_HIP(hipStreamWaitEvent(c.stream, c.join[gi - 1], 0));
+      }
+    }
     CG_HIP(hipEventRecord(c.fev[w][1], c.stream));
+    for (const Group &g : groups) fam_groups[w].push_back({g.cls0, g.cls1});
     fam_ran[w] = true;
   }
   if (m.n_offset_cols > 0) {
@@ -2608,6 +2671,11 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
   for (int w = 0; w < 3; ++w) {
     c.ms_fam[w] = 0.f;
     if (fam_ran[w]) CG_HIP(hipEventElapsedTime(&c.ms_fam[w], c.fev[w][0], c.fev[w][1]));
+    for (const auto &g : fam_groups[w]) {
+      float ms = 0.f;
+      CG_HIP(hipEventElapsedTime(&ms, c.cev[w][g.first][0], c.cev[w][g.first][1]));
+      for (int cls = g.first; cls <= g.second; ++cls) c.ms_cls[w][cls] = ms;  // shared by a merged launch
+    }
   }
   c.A = A;
   c.A_prec = prec;
@@ -3017,8 +3085,21 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
   CG_HIP(hipSetDevice(device));
   scratch_setup(device);
   CG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (auto &st : c.aux) CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  CG_HIP(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
+  for (auto &e : c.join) CG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto &e : c.ev) CG_HIP(hipEventCreate(&e));
   for (auto &fe : c.fev) for (auto &e : fe) CG_HIP(hipEventCreate(&e));
+  for (auto &fc : c.cev) for (auto &fe : fc) for (auto &e : fe) CG_HIP(hipEventCreate(&e));
+  {
+    // tile choices made by cg_autotune for this model earlier in the process
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    for (int w = 1; w < 3; ++w)
+      for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) {
+        auto it = g_tile_cache.find(tile_key(*m, w, cls));
+        if (it != g_tile_cache.end()) c.tile_cfg[w][cls] = it->second;
+      }
+  }
   std::vector<int> ipm(m->int_pair_map, m->int_pair_map + m->ntypes * m->ntypes);
   c.d_int_pair_map = dalloc_copy(ipm);
   build_family(c, 0);
@@ -3044,6 +3125,10 @@ void ctx_free(Ctx &c) {
   if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);
   for (auto &e : c.ev) if (e) (void)hipEventDestroy(e);
   for (auto &fe : c.fev) for (auto &e : fe) if (e) (void)hipEventDestroy(e);
+  for (auto &fc : c.cev) for (auto &fe : fc) for (auto &e : fe) if (e) (void)hipEventDestroy(e);
+  for (auto &st : c.aux) if (st) (void)hipStreamDestroy(st);
+  if (c.fork) (void)hipEventDestroy(c.fork);
+  for (auto &e : c.join) if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -3090,36 +3175,83 @@ cg_status cg_assemble(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec
   }
 }
 
-cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, int *choice3, double *ms_table) {
+cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, int *choice, double *ms_table) {
   if (!ctx || !frames || nframes <= 0) { cg::set_error("cg_autotune: bad argument"); return CG_ERR_ARG; }
   if (prec != CG_FP64 && prec != CG_FP32) { cg::set_error("bad precision"); return CG_ERR_ARG; }
+  constexpr int NK = cg::kNumTileCfg, NC = CG_TILE_CLASSES;
+  cg::Ctx &c = ctx->c;
+  int keep[3][NC];
+  std::memcpy(keep, c.tile_cfg, sizeof(keep));
   try {
-    cg::Ctx &c = ctx->c;
     CG_HIP(hipSetDevice(c.device));
-    double ms[cg::kNumTileCfg][3];
-    for (int w = 0; w < 3; ++w) c.tile_cfg[w] = 0;
-    cg::assemble(c, frames, nframes, prec, nullptr);  // warm-up (buffers, code)
-    for (int k = 0; k < cg::kNumTileCfg; ++k) {
-      for (int w = 0; w < 3; ++w) c.tile_cfg[w] = k;
+    double ms[NK][3][NC] = {};
+    bool present[NC] = {};
+    for (int i = 0; i < nframes; ++i) present[cg::tile_class(frames[i].natoms)] = true;
+    c.split_classes = true;
+    for (int k = 0; k < NK; ++k) {
+      for (int w = 0; w < 3; ++w)
+        for (int cls = 0; cls < NC; ++cls) c.tile_cfg[w][cls] = w == 0 ? 0 : k;
+      if (k == 0) cg::assemble(c, frames, nframes, prec, nullptr);  // warm-up (buffers, code)
       cg::assemble(c, frames, nframes, prec, nullptr);
-      for (int w = 0; w < 3; ++w) ms[k][w] = c.ms_fam[w];
+      for (int w = 0; w < 3; ++w)
+        for (int cls = 0; cls < NC; ++cls) ms[k][w][cls] = c.ms_cls[w][cls];
     }
+    c.split_classes = false;
+    // fastest configuration per (family, class) from the timing runs
+    int best[3][NC] = {};
+    for (int w = 1; w < 3; ++w)
+      for (int cls = 0; cls < NC; ++cls)
+        for (int k = 1; k < NK; ++k)
+          if (ms[k][w][cls] > 0.0 && ms[k][w][cls] < ms[best[w][cls]][w][cls]) best[w][cls] = k;
+    // a family keeps its per-class choices only if they beat the default tiles
+    // on a whole assembly (one launch per configuration, launches side by side)
+    float whole[2][3];
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int w = 0; w < 3; ++w)
+        for (int cls = 0; cls < NC; ++cls) c.tile_cfg[w][cls] = pass == 0 ? 0 : best[w][cls];
+      cg::assemble(c, frames, nframes, prec, nullptr);
+      for (int w = 0; w < 3; ++w) whole[pass][w] = c.ms_fam[w];
+    }
+    std::lock_guard<std::mutex> lk(cg::g_tile_mu);
     for (int w = 0; w < 3; ++w) {
-      int best = 0;  // the 2-body kernel has one configuration
-      for (int k = 1; k < cg::kNumTileCfg && w > 0; ++k)
-        if (ms[k][w] < ms[best][w]) best = k;
-      c.tile_cfg[w] = best;
-      if (choice3) choice3[w] = best;
+      const bool keep_best = w > 0 && whole[1][w] < whole[0][w];
+      for (int cls = 0; cls < NC; ++cls) {
+        const int k = keep_best ? best[w][cls] : 0;
+        c.tile_cfg[w][cls] = k;
+        if (w > 0 && present[cls] && ms[0][w][cls] > 0.0) cg::g_tile_cache[cg::tile_key(c.m, w, cls)] = k;
+        if (choice) choice[w * NC + cls] = present[cls] ? k : -1;
+      }
     }
     if (ms_table)
-      for (int k = 0; k < cg::kNumTileCfg; ++k)
-        for (int w = 0; w < 3; ++w) ms_table[3 * k + w] = ms[k][w];
+      for (int k = 0; k < NK; ++k)
+        for (int w = 0; w < 3; ++w)
+          for (int cls = 0; cls < NC; ++cls) ms_table[(k * 3 + w) * NC + cls] = ms[k][w][cls];
     return CG_OK;
   } catch (const cg::Error &e) {
-    for (int w = 0; w < 3; ++w) ctx->c.tile_cfg[w] = 0;  // back to the default tiles
+    c.split_classes = false;
+    std::memcpy(c.tile_cfg, keep, sizeof(keep));  // back to the configuration before the call
     cg::set_error(e.what());
     return e.code;
   }
+}
+
+cg_status cg_tile_choice(cg_ctx *ctx, int *choice) {
+  if (!ctx || !choice) { cg::set_error("cg_tile_choice: bad argument"); return CG_ERR_ARG; }
+  for (int w = 0; w < 3; ++w)
+    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) choice[w * CG_TILE_CLASSES + cls] = ctx->c.tile_cfg[w][cls];
+  return CG_OK;
+}
+
+cg_status cg_tile_set(cg_ctx *ctx, const int *choice) {
+  if (!ctx || !choice) { cg::set_error("cg_tile_set: bad argument"); return CG_ERR_ARG; }
+  for (int i = 0; i < 3 * CG_TILE_CLASSES; ++i)
+    if (choice[i] < 0 || choice[i] >= cg::kNumTileCfg || (i < CG_TILE_CLASSES && choice[i] != 0)) {
+      cg::set_error("cg_tile_set: configuration out of range (the 2-body family has only configuration 0)");
+      return CG_ERR_ARG;
+    }
+  for (int w = 0; w < 3; ++w)
+    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) ctx->c.tile_cfg[w][cls] = choice[w * CG_TILE_CLASSES + cls];
+  return CG_OK;
 }
 
 cg_status cg_copy_A(cg_ctx *ctx, int64_t r0, int64_t nrows, double *host) {

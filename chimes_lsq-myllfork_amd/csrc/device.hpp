@@ -192,6 +192,10 @@ struct LsqStats {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // assembly launches of further frame-size classes run on these (default
+  // priority), forked from and joined back to `stream`
+  hipStream_t aux[CG_TILE_CLASSES - 1] = {};
+  hipEvent_t fork = nullptr, join[CG_TILE_CLASSES - 1] = {};
   const cg_model *model = nullptr;
   cg_model m{};               // copy of the POD header
   DevFamily fam[3];           // 0: 2B pairs, 1: 3B, 2: 4B
@@ -252,10 +256,15 @@ struct Ctx {
   std::vector<double> h_b;    // last b
   float ms_enum = 0, ms_asm = 0, ms_total = 0;
   hipEvent_t ev[4] = {};
-  hipEvent_t fev[3][2] = {};   // around each family's assembly launch
+  hipEvent_t fev[3][2] = {};   // around each family's assembly launches
   float ms_fam[3] = {0, 0, 0};
+  // per (family, frame-size class): events around the class's launch and its
+  // time in the last assembly (0 when the class had no task)
+  hipEvent_t cev[3][CG_TILE_CLASSES][2] = {};
+  float ms_cls[3][CG_TILE_CLASSES] = {};
   long long ncl[3] = {0, 0, 0};  // clusters per family in the last batch
-  int tile_cfg[3] = {0, 0, 0};   // assembly tile configuration per family (set by cg_autotune)
+  int tile_cfg[3][CG_TILE_CLASSES] = {};  // assembly tile configuration per (family, frame-size class)
+  bool split_classes = false;             // one launch per frame-size class (timing by cg_autotune)
   LsqStats stats;                // fit statistics (cg_stats_enable)
 };
 

@@ -250,17 +250,32 @@ cg_status cg_append_rows(cg_ctx *ctx, const double *rows, int nrows);
  * (orthorhombic: its diagonal is read). */
 cg_status cg_ewald_params(int natoms, const double *cell, double *alpha, double *rc, int *kmax);
 
-/* Tile autotuner of the 3- and 4-body assembly: assembles the given frames
- * once per tile configuration (0: 512-thread workgroups with the widest LDS
- * tile; 1: 256 threads, widest tile; 2: 256 threads, tiles half as wide so
- * that two workgroups share a CU), times each family's kernel and keeps the
- * fastest configuration per family for later cg_assemble calls on ctx.
- * choice3 (may be NULL) receives the choice per family (2-body always 0);
- * ms_table (may be NULL) the [3 configurations][3 families] kernel times.
- * The last assembly left in ctx is that of the last configuration tried.
+/* Tile autotuner of the 3- and 4-body assembly.  Frames are split in
+ * CG_TILE_CLASSES size classes (fewer than 128, 512, 1024 atoms, and the
+ * rest), each launched separately.  The given frames are assembled once per
+ * tile configuration (0: 512-thread workgroups with the widest LDS tile;
+ * 1: 256 threads, widest tile; 2: 256 threads, tiles half as wide so that two
+ * workgroups share a CU; 3 and 4: 0 and 2 with at most 16 columns per tile),
+ * each (family, class) launch is timed, and the fastest configuration of each
+ * (family, class) is kept (for a family, only if those choices, launched side
+ * by side, assemble faster than the default tiles; otherwise the family keeps
+ * configuration 0) for later cg_assemble calls on ctx and, for the
+ * life of the process, for contexts created later on a model with the same
+ * species, orders and columns.  choice (may be NULL) receives
+ * [3 families][CG_TILE_CLASSES] choices (2-body always 0, -1 for a class
+ * without frames); ms_table (may be NULL) the
+ * [CG_TILE_CONFIGS][3 families][CG_TILE_CLASSES] kernel times in ms.
+ * The last assembly left in ctx is that of the kept choices.
  * No counterpart in the reference (a CPU code without tiles). */
-cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, int *choice3,
+#define CG_TILE_CONFIGS 5
+#define CG_TILE_CLASSES 4
+cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec, int *choice,
                       double *ms_table);
+/* The tile configuration ctx uses now, [3 families][CG_TILE_CLASSES]. */
+cg_status cg_tile_choice(cg_ctx *ctx, int *choice);
+/* Set it (e.g. a choice saved from cg_tile_choice); configurations
+ * 0..CG_TILE_CONFIGS-1, and 0 for the 2-body family. */
+cg_status cg_tile_set(cg_ctx *ctx, const int *choice);
 /* Kernel timing of the last cg_assemble (ms, measured with HIP events on the
  * context stream): neighbour+enumeration, assembly kernels, total. */
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, double *ms_total);

@@ -146,17 +146,25 @@ def test_empty_and_ragged_frames(ch, synth, tmp_path):
 
 
 def test_autotune_configurations_give_the_same_A(ch, synth, oracle, tmp_path):
-    """Every tile configuration the autotuner tries assembles the same A as the
-    default one (and as the oracle); the chosen one is then used."""
-    gen = synth.chno(nframes=6, nmin=40, nmax=500, seed=45)
+    """Every tile configuration the autotuner tries, per frame-size class, assembles
+    the same A as the default one (and as the oracle); the chosen ones are then
+    used, and a context created later for the same model starts from them."""
+    gen = synth.chno(nframes=6, nmin=40, nmax=700, seed=45)
     s, fr = synth_setup(ch, synth, tmp_path, gen)
+    classes = {0 if f.natoms < 128 else 1 if f.natoms < 512 else 2 if f.natoms < 1024 else 3 for f in fr}
     dev = ch.Device(s)
     A0, b0 = dev.assemble(fr)
     choice, ms = dev.autotune(fr)
-    assert all(0 <= k < 3 for k in choice) and choice[0] == 0
-    assert np.all(ms[:, 1:] > 0)
-    # the autotuner leaves the assembly of its last configuration (two
-    # 256-thread workgroups per CU, half-width tiles)
+    nk = len(ch.Device.TILE_CONFIGS)
+    for cls in range(4):
+        if cls in classes:
+            assert choice[0][cls] == 0
+            assert all(0 <= choice[w][cls] < nk for w in (1, 2))
+            assert np.all(ms[:, 1:, cls] > 0)
+        else:
+            assert all(choice[w][cls] == -1 for w in range(3))
+            assert np.all(ms[:, :, cls] == 0)
+    # the autotuner leaves the assembly of the kept choices
     rows = A0.shape[0]
     A2 = np.zeros_like(A0)
     ch.check(dev.L.cg_copy_A(dev.h, 0, rows, ch.dptr(A2)), dev.h)
@@ -166,6 +174,34 @@ def test_autotune_configurations_give_the_same_A(ch, synth, oracle, tmp_path):
     assert np.array_equal(b1, b0)
     Ao = oracle.assemble(s, fr)
     assert colscaled_error(A1, Ao) < 1e-10
+    used = dev.tile_choice()
+    dev.close()
+    dev2 = ch.Device(s)
+    for cls in classes:
+        assert list(dev2.tile_choice()[1:, cls]) == list(used[1:, cls])
+    dev2.close()
+
+
+def test_mixed_tile_configurations_side_by_side(ch, synth, oracle, tmp_path):
+    """Frame-size classes with different tile configurations are launched side by
+    side on separate streams; A equals the default tiles' A and the oracle's, in
+    FP64 and FP32."""
+    gen = synth.chno(nframes=6, nmin=40, nmax=700, seed=46)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    dev = ch.Device(s)
+    A0, b0 = dev.assemble(fr)
+    A032, _ = dev.assemble(fr, prec=ch.CG_FP32)
+    with pytest.raises(RuntimeError):
+        dev.tile_set([[1, 0, 0, 0], [0] * 4, [0] * 4])
+    for mix in ([[0] * 4, [2, 4, 0, 3], [4, 2, 3, 0]], [[0] * 4, [1, 3, 2, 4], [3, 1, 4, 2]]):
+        dev.tile_set(mix)
+        assert dev.tile_choice().tolist() == mix
+        A1, b1 = dev.assemble(fr)
+        assert colscaled_error(A1, A0) < 1e-12
+        assert np.array_equal(b1, b0)
+        A132, _ = dev.assemble(fr, prec=ch.CG_FP32)
+        assert colscaled_error(A132, A032) < 1e-6
+    assert colscaled_error(A1, oracle.assemble(s, fr)) < 1e-10
     dev.close()
 
 
