@@ -5,7 +5,10 @@ energy rows) of 100 frames of 512-atom carbon at 2.2 g/cc with 2+3+4-body
 Chebyshev orders 16/9/5 (A = 154200 rows x 1038 columns, FP64), synthetic
 seeded inputs.  `value` is A rows per second for that whole batch, timed around
 `cg_assemble` (frames handed over, neighbour build, cluster enumeration,
-2/3/4-body assembly, A resident in HBM afterwards).
+2/3/4-body assembly, A resident in HBM afterwards); after the warm-up calls the
+context replays the assembly from a captured HIP graph.  The 4-body kernel's
+time for the roofline comes from the same number of eager steps (HIP events
+around its launch).
 
 Also reported: the FP32 assembly rate; the fit time of the FP64 path, of the
 FP32-A path (FP64-product Gram + FP64 refinement) and of the split-BF16x3 Gram
@@ -55,6 +58,16 @@ def _load(name, path):
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
     return mod
+
+
+GRAPHS = True  # assembly graph capture (bench.py --no-graphs turns it off)
+
+
+def new_device(ch, setup):
+    dev = ch.Device(setup)
+    if not GRAPHS:
+        dev.assembly_graphs(False)
+    return dev
 
 
 def log(*a):
@@ -220,14 +233,28 @@ def serving_sweep(ch, synth, qps_points=(250, 500, 1000, 1500, 2000), n_req=1000
             "sustainable_qps": max((p["qps_offered"] for p in ok), default=None), "points": points}
 
 
-def assemble_rate(ch, dev, fr, rows, prec, steps=2):
-    dev.assemble(fr, prec=prec, copy=False)
+def assemble_rate(ch, dev, fr, rows, prec, steps=2, warmup=3):
+    """Rows/s of repeated assemblies of one batch: after the warm-up calls (exact
+    list sizes, then capacity-sized, then captured) the timed calls replay the
+    captured graph."""
+    for _ in range(warmup):
+        dev.assemble(fr, prec=prec, copy=False)
     dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         dev.assemble(fr, prec=prec, copy=False)
     dev.synchronize()
     return rows / ((time.perf_counter() - t0) / steps)
+
+
+def family_ms(dev, fr, prec):
+    """Per-family kernel times of one eager assembly (a replayed graph carries no
+    per-launch events)."""
+    dev.assembly_graphs(False)
+    dev.assemble(fr, prec=prec, copy=False)
+    ms, _ = dev.detail()
+    dev.assembly_graphs(GRAPHS)
+    return dict(zip(["2B", "3B", "4B"], ms))
 
 
 def config_c3(ch, synth, nframes=600):
@@ -237,7 +264,7 @@ def config_c3(ch, synth, nframes=600):
     s = setup_from(ch, text)
     fr = make_frames(ch, frames)
     rows = sum(s.frame_rows(f.natoms, f.index) for f in fr)
-    dev = ch.Device(s)
+    dev = new_device(ch, s)
     out = {"workload": "carbon %d frames x 512 atoms, orders 16/9/5" % nframes, "rows": rows,
            "cols": int(s.model.ncols)}
     for prec, tag in [(ch.CG_FP64, "fp64"), (ch.CG_FP32, "fp32")]:
@@ -249,11 +276,22 @@ def config_c3(ch, synth, nframes=600):
         tf = time.perf_counter() - t0
         out[tag] = {"assembly_rows_per_s": rate, "fit_s": tf, "rms": r.rms, "nvars": r.nvars,
                     "assembly_plus_fit_rows_per_s": rows / (rows / rate + tf), "x": x}
+    # split-BF16x3 Gram on the FP32 A (FP64 refinement guarded against growth)
+    t0 = time.perf_counter()
+    xb, rb = dev.fit(b, eps=1e-5, refine=4, gram=ch.CG_GRAM_BF16X3)
+    tb = time.perf_counter() - t0
+    out["fp32_bf16x3_gram"] = {"fit_s": tb, "gram_ms": rb.ms_gram, "rms": rb.rms, "nvars": rb.nvars,
+                               "iters": rb.iters,
+                               "assembly_plus_fit_rows_per_s": rows / (rows / out["fp32"]["assembly_rows_per_s"] + tb)}
     x64, x32 = out["fp64"].pop("x"), out["fp32"].pop("x")
-    out["fp32"]["rms_rel_err_vs_fp64"] = abs(out["fp32"]["rms"] - out["fp64"]["rms"]) / out["fp64"]["rms"]
-    out["fp32"]["coef_rel_err_vs_fp64"] = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+    for tag, x in (("fp32", x32), ("fp32_bf16x3_gram", xb)):
+        out[tag]["rms_rel_err_vs_fp64"] = abs(out[tag]["rms"] - out["fp64"]["rms"]) / out["fp64"]["rms"]
+        out[tag]["coef_rel_err_vs_fp64"] = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+        out[tag]["meets_1e-6_rms_target"] = bool(out[tag]["rms_rel_err_vs_fp64"] <= 1e-6)
     out["fp32_over_fp64_rows_per_s"] = (out["fp32"]["assembly_plus_fit_rows_per_s"] /
                                          out["fp64"]["assembly_plus_fit_rows_per_s"])
+    out["fp32_bf16x3_over_fp64_rows_per_s"] = (out["fp32_bf16x3_gram"]["assembly_plus_fit_rows_per_s"] /
+                                                out["fp64"]["assembly_plus_fit_rows_per_s"])
     dev.close()
     return out
 
@@ -265,10 +303,10 @@ def config_c4(ch, synth, nframes=256):
     s = setup_from(ch, text)
     fr = make_frames(ch, frames)
     rows = sum(s.frame_rows(f.natoms, f.index) for f in fr)
-    dev = ch.Device(s)
+    dev = new_device(ch, s)
     out = {"workload": "C/H/N/O %d frames, 32-2048 atoms, orders 12/6/3" % nframes, "rows": rows,
            "cols": int(s.model.ncols), "atoms": sum(f.natoms for f in fr)}
-    out["fp64_rows_per_s_default_tiles"] = assemble_rate(ch, dev, fr, rows, ch.CG_FP64, steps=1)
+    out["fp64_rows_per_s_default_tiles"] = assemble_rate(ch, dev, fr, rows, ch.CG_FP64, steps=2)
     choice, ms = dev.autotune(fr)
     cfgs, classes = ch.Device.TILE_CONFIGS, ch.Device.TILE_CLASSES
     out["autotune"] = {
@@ -277,10 +315,12 @@ def config_c4(ch, synth, nframes=256):
         "kernel_ms": {cfgs[k]: {fam: {classes[q]: float(ms[k][w][q]) for q in range(len(classes)) if choice[w][q] >= 0}
                                 for w, fam in ((1, "3B"), (2, "4B"))} for k in range(len(cfgs))}}
     for prec, tag in [(ch.CG_FP64, "fp64_rows_per_s"), (ch.CG_FP32, "fp32_rows_per_s")]:
-        out[tag] = assemble_rate(ch, dev, fr, rows, prec, steps=1)
-        ms, ncl = dev.detail()
-        out[tag.replace("rows_per_s", "family_ms")] = dict(zip(["2B", "3B", "4B"], ms))
+        out[tag] = assemble_rate(ch, dev, fr, rows, prec, steps=2)
+        out[tag.replace("rows_per_s", "family_ms")] = family_ms(dev, fr, prec)
     out["clusters"] = dict(zip(["pairs", "triplets", "quadruplets"], dev.detail()[1]))
+    calls, repeats, replays, captures = dev.assembly_stats()
+    out["assembly_calls"] = {"calls": calls, "exact_repeats": repeats, "graph_replays": replays,
+                             "graph_captures": captures}
     dev.close()
     return out
 
@@ -297,10 +337,14 @@ def main():
     ap.add_argument("--no-md", action="store_true",
                     help="skip the MD-style leg (force calls replayed from HIP graphs), e.g. under a kernel tracer")
     ap.add_argument("--no-configs", action="store_true", help="skip the 600-frame and mixed-batch legs")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="keep every assembly eager (no graph capture), e.g. under a kernel tracer")
     ap.add_argument("--counters", nargs="?", const="carbon", choices=["carbon", "chno"],
                     help="one FP64 assembly of the carbon workload (or of the 256-frame mixed C/H/N/O batch) "
                          "plus a 1 GiB device copy, for rocprofv3 --pmc runs")
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the design-matrix path is single-device "
               "(sharding frames over GPUs is designed but not built)", file=sys.stderr)
@@ -310,7 +354,7 @@ def main():
     if args.counters == "chno":
         text, frames, _ = synth.chno(nframes=256, nmin=32, nmax=2048, seed=104)
         setup = setup_from(ch, text)
-        dev = ch.Device(setup)
+        dev = new_device(ch, setup)
         dev.assemble(make_frames(ch, frames), prec=ch.CG_FP64, copy=False)
         dev.copy_probe(COPY_PROBE_BYTES)
         dev.close()
@@ -321,7 +365,7 @@ def main():
     setup = setup_from(ch, text)
     fr = make_frames(ch, frames)
     rows = sum(setup.frame_rows(f.natoms, f.index) for f in fr)
-    dev = ch.Device(setup)
+    dev = new_device(ch, setup)
     if args.counters:
         dev.assemble(fr, prec=ch.CG_FP64, copy=False)
         dev.copy_probe(COPY_PROBE_BYTES)
@@ -332,32 +376,45 @@ def main():
     def step(prec):
         dev.assemble(fr, prec=prec, copy=False)
 
-    for _ in range(args.warmup):
+    # warm-up: the first call sizes its lists exactly, the next ones from the
+    # per-atom counts; a repeated shape is captured into a graph and replayed
+    for _ in range(max(args.warmup, 3)):
         step(ch.CG_FP64)
     dev.synchronize()
     t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(ch.CG_FP64)
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    value = rows / (dt / args.steps)
+    calls, repeats, replays, captures = dev.assembly_stats()
+    # the 4-body kernel's launch time: HIP events on the context stream around
+    # its launch in the same number of eager steps (graphs off)
+    dev.assembly_graphs(False)
     k4 = []
     asm = []
+    t0 = time.perf_counter()
     for _ in range(args.steps):
         step(ch.CG_FP64)
         ms, ncl = dev.detail()
         k4.append(ms[2])
         asm.append(dev.timing())
     dev.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = dt / args.steps * 1e3
-    value = rows / (dt / args.steps)
+    ms_step_eager = (time.perf_counter() - t0) / args.steps * 1e3
+    dev.assembly_graphs(GRAPHS)
     ms4 = float(np.mean(k4))
     # algorithmic work of the 4-body kernel: 4 flops per pair per allowed power set
     # (product, prefix/suffix force coefficient, accumulation), 6 pairs per cluster
     nallowed = setup.model.c4.total_allowed
     flops4 = 24.0 * ncl[2] * nallowed
     achieved = flops4 / (ms4 * 1e-3) / 1e12
-    log("FP64: %.3f ms/step, %.4e rows/s, 4-body kernel %.3f ms (%d quads), %.2f TFLOP/s" % (
-        ms_step, value, ms4, ncl[2], achieved))
-    # FP32 assembly
-    for _ in range(args.warmup):
+    log("FP64: %.3f ms/step (%d graph replays), %.4e rows/s; eager %.3f ms/step; 4-body kernel %.3f ms "
+        "(%d quads), %.2f TFLOP/s" % (ms_step, replays, value, ms_step_eager, ms4, ncl[2], achieved))
+    # FP32 assembly (its own graph after the warm-up)
+    for _ in range(max(args.warmup, 3)):
         step(ch.CG_FP32)
+    dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(ch.CG_FP32)
@@ -365,6 +422,9 @@ def main():
     value32 = rows / ((time.perf_counter() - t0) / args.steps)
     log("FP32: %.4e rows/s" % value32)
     extra = {"fp32_rows_per_s": value32, "kernel_ms": {"enum": asm[-1][0], "assembly": asm[-1][1]},
+             "assembly_calls": {"graph_replays": replays, "graph_captures": captures, "exact_repeats": repeats,
+                                "calls": calls},
+             "eager_ms_per_step": ms_step_eager, "eager_rows_per_s": rows / (ms_step_eager * 1e-3),
              "family_ms": dict(zip(["2B", "3B", "4B"], dev.detail()[0])),
              "clusters": dict(zip(["pairs", "triplets", "quadruplets"], ncl))}
     if not args.no_fit:

@@ -125,6 +125,10 @@ def lib(path=None):
         opt("cg_tile_choice", [C.c_void_p, C.POINTER(C.c_int)])
         opt("cg_tile_set", [C.c_void_p, C.POINTER(C.c_int)])
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+        opt("cg_assembly_graphs", [C.c_void_p, C.c_int])
+        opt("cg_assembly_global_above", [C.c_void_p, C.c_int])
+        opt("cg_assembly_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
         opt("cg_append_rows", [C.c_void_p, _dp, C.c_int])
         opt("cg_ewald_params", [C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)])
@@ -463,6 +467,21 @@ class Device:
         choice = (C.c_int * (3 * nc))()
         check(self.L.cg_tile_choice(self.h, choice), self.h)
         return np.array(list(choice)).reshape(3, nc)
+
+    def assembly_graphs(self, on):
+        """Capture repeated-shape assemblies into HIP graphs (on by default) or keep them eager."""
+        check(self.L.cg_assembly_graphs(self.h, 1 if on else 0), self.h)
+
+    def assembly_global_above(self, natoms):
+        """Accumulate frames with more than natoms atoms in A with global atomics (0: only those beyond
+        the LDS tile)."""
+        check(self.L.cg_assembly_global_above(self.h, int(natoms)), self.h)
+
+    def assembly_stats(self):
+        """(assembly calls, calls repeated with exact list sizes, graph replays, graph captures)."""
+        v = [C.c_int64() for _ in range(4)]
+        check(self.L.cg_assembly_stats(self.h, *[C.byref(x) for x in v]), self.h)
+        return tuple(x.value for x in v)
 
     def force_stats(self):
         """(force calls, calls repeated with exact list sizes, calls served by a graph replay)."""

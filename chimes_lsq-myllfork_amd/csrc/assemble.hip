@@ -744,13 +744,6 @@ __global__ void k_task_counts(const int *bounds, int nkeys, int chunk, int *cnt)
 }
 
 // ------------------------------------------------------------ assembly kernel
-struct Task {
-  int frame;
-  int type;
-  int col0;    // first column of the tile, relative to the type
-  int ncols;   // tile width
-  int c0, c1;  // cluster range (bucketed arrays)
-};
 
 __global__ void k_task_fill(const int *bounds, const int *toff, int nkeys, int ntypes, int chunk, int max_tasks,
                             Task *tasks) {
@@ -777,6 +770,10 @@ __global__ void k_totals(TotalPtrs t, double *out) {
 
 struct AsmArgs {
   const Task *tasks;
+  // bucket bounds of the family: clusters of (frame f, type t) are
+  // [bounds[f*ntypes+t], bounds[f*ntypes+t+1]) of refs (null: no clusters)
+  const int *bounds;
+  int ntypes;
   const DevFrame *frames;
   const double *wrap;
   const uint32_t *refs;     // bucketed cluster refs [ncl*NB]
@@ -1310,7 +1307,10 @@ __host__ __device__ constexpr size_t tile_bytes(int natoms, int w) {
 // waves per SIMD so that 256-thread workgroups can share a CU.
 template <int NB, int VW, class T> constexpr bool asm_heavy() { return 2 * (NB * (NB - 1) / 2) * VW * (int)sizeof(T) / 4 > 150; }
 
-template <int NB, int VW, int BASE, class T, int BS>
+// GACC: frames too large for the LDS tile accumulate straight into A (zeroed
+// first) with global atomics: 12 force, the stress rows and the energy rows
+// per (cluster, column).
+template <int NB, int VW, int BASE, class T, int BS, bool GACC = false>
 __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemble(AsmArgs a) {
   constexpr int NP = NB * (NB - 1) / 2;
   // 4-body with powers below 5 and 3-body with powers below 9 take the
@@ -1331,6 +1331,25 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   const DevFrame &f = a.frames[t.frame];
   const int nf = 3 * f.natoms;
   const int W = t.ncols;
+  const int ty = t.type;
+  // cluster range of the task's bucket, read from the device bounds
+  int cl0 = 0, cl1 = 0;
+  if (a.bounds) {
+    cl0 = a.bounds[t.frame * a.ntypes + ty];
+    cl1 = a.bounds[t.frame * a.ntypes + ty + 1];
+  }
+  const int ncl = cl1 - cl0;
+  if (ncl <= 0) {
+    // a bucket without clusters: its tile of A is zero (block-uniform exit)
+    T *Aout = reinterpret_cast<T *>(a.A);
+    const int nrow = nf + f.stress_rows + f.ener_rows;
+    const int cabs = a.col_abs[ty] + t.col0;
+    for (int i = threadIdx.x; i < nrow * W; i += BS) {
+      const int r = i / W, cc = i - r * W;
+      Aout[(f.row_off + r) * a.ld + cabs + cc] = (T)0;
+    }
+    return;
+  }
   // tile layout: force component c of atom i in column w at
   // acc[(i * Wp + w) * 3 + c] (components are instruction immediates; the odd
   // atom stride 3 * Wp spreads different atoms over distinct LDS banks);
@@ -1340,9 +1359,10 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   // stride 7 * Wp keeps the 16 replicas on distinct banks)
   const int Wp = tile_stride(W);
   const int nacc = (nf + 7 * SE_REP) * Wp;
-  for (int i = threadIdx.x; i < nacc; i += BS) acc[i] = 0.0;
-  __syncthreads();
-  const int ty = t.type;
+  if constexpr (!GACC) {
+    for (int i = threadIdx.x; i < nacc; i += BS) acc[i] = 0.0;
+    __syncthreads();
+  }
   const int colbase = a.col_rel[ty] + t.col0;
   const TP ps = (TP)f.perm_scale[a.nb_scale];
   const bool full_stress = a.stress_mode == CG_STRESS_ALL;
@@ -1352,7 +1372,6 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   // thread i takes clusters [i*R, i*R + R) of the bucket: neighbouring lanes
   // hold clusters of different anchor atoms, so their force-row atomics do
   // not collide on one LDS address
-  const int ncl = t.c1 - t.c0;
   const int R = (ncl + BS - 1) / BS;
   for (int rnd = 0; rnd < R; ++rnd) {
     const int j = threadIdx.x * R + rnd;
@@ -1363,7 +1382,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     // lanes past the bucket's end build the tables of its last cluster (all
     // registers then hold defined values without a clearing pass) and drop
     // their sums at the projection
-    const int ci = t.c0 + min(j, ncl - 1);
+    const int ci = cl0 + min(j, ncl - 1);
     // 4-body FP64 with powers below 5 takes the hand-scheduled set loop
     PairTab<T, VW> tab[ASM4 || ASM3 ? 1 : NP];
     Tab4 tab4;
@@ -1478,6 +1497,30 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
               ++k;
             }
         }
+        if constexpr (GACC) {
+          T *Ag = reinterpret_cast<T *>(a.A);
+          const long long colg = a.col_abs[ty] + t.col0 + cc;
+#pragma unroll
+          for (int s = 0; s < NB; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              unsafeAtomicAdd(&Ag[(f.row_off + 3LL * atom[s] + c) * a.ld + colg], (T)F[s][c]);
+          const int nsr = f.stress_rows, ner = f.ener_rows;
+          // stress row k holds component k (3 rows) or sel9[k] (9 rows), x 1/V
+          for (int k = 0; k < nsr; ++k) {
+            const int comp = nsr == 9 ? (k == 0 ? 0 : k == 1 || k == 3 ? 3 : k == 2 || k == 6 ? 4 : k == 4 ? 1
+                                                  : k == 5 || k == 7 ? 5 : 2) : k;
+            const int ia = comp < 3 ? comp : (comp == 5 ? 1 : 0);
+            const int ib = comp < 3 ? comp : (comp == 3 ? 1 : 2);
+            TP sv = 0;
+#pragma unroll
+            for (int s = 1; s < NB; ++s) sv += F[s][ia] * dpos[s][ib];
+            unsafeAtomicAdd(&Ag[(f.row_off + nf + k) * a.ld + colg], (T)((double)sv * f.inv_vol));
+          }
+          for (int k = 0; k < ner; ++k)
+            unsafeAtomicAdd(&Ag[(f.row_off + nf + nsr + k) * a.ld + colg], (T)(double)(ps * e));
+          continue;
+        }
 #pragma unroll
         for (int s = 0; s < NB; ++s) {
           double *dst = acc + ((size_t)atom[s] * Wp + cc) * 3;
@@ -1503,6 +1546,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
       }
     }
   }
+  if constexpr (GACC) return;
   __syncthreads();
   // write the tile: force rows, stress rows (x 1/V), energy rows (x3)
   T *Aout = reinterpret_cast<T *>(a.A);
@@ -1865,11 +1909,13 @@ __global__ void k_offsets(const DevFrame *frames, const int *type, int nframes, 
   __syncthreads();
   for (int i = threadIdx.x; i < f.natoms; i += blockDim.x) atomicAdd(&cnt[type[f.atom_off + i]], 1);
   __syncthreads();
-  if (f.ener_rows == 0) return;
-  long long r = f.row_off + 3LL * f.natoms + f.stress_rows;
-  for (int k = threadIdx.x; k < 3 * ntypes; k += blockDim.x) {
-    int rr = k / ntypes, t = k % ntypes;
-    A[(r + rr) * ld + col0 + t] = (T)cnt[t];
+  // every row of the frame: zero on force and stress rows, the atom counts
+  // on the energy rows
+  const int nrows = 3 * f.natoms + f.stress_rows + f.ener_rows;
+  const int r_ener = 3 * f.natoms + f.stress_rows;
+  for (int k = threadIdx.x; k < nrows * ntypes; k += blockDim.x) {
+    const int rr = k / ntypes, t = k - rr * ntypes;
+    A[(f.row_off + rr) * ld + col0 + t] = rr >= r_ener ? (T)cnt[t] : (T)0;
   }
 }
 
@@ -2047,11 +2093,13 @@ static std::string tile_key(const cg_model &m, int w, int cls) {
 // Which kernel instance handles a family.
 template <class T>
 static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks, const DevFrame *d_frames,
-                          const uint32_t *refs, size_t lds_bytes, void *A, long long ld, int want_bs,
-                          hipStream_t stream) {
+                          const uint32_t *refs, const int *bounds, size_t lds_bytes, void *A, long long ld,
+                          int want_bs, hipStream_t stream, bool gacc = false) {
   const DevFamily &F = c.fam[which];
   AsmArgs a{};
   a.tasks = d_tasks;
+  a.bounds = bounds;
+  a.ntypes = F.ntypes;
   a.frames = d_frames;
   a.wrap = c.b_wrap.as<double>(0);
   a.refs = refs;
@@ -2079,11 +2127,16 @@ static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks,
     hipLaunchKernelGGL(kern, grid, block, lds_bytes, stream, a);                                            \
     CG_HIP(hipGetLastError());                                                                              \
   } while (0)
-  // register-heavy instances run 256 threads whatever the configuration
+  // register-heavy instances run 256 threads whatever the configuration;
+  // global accumulation (frames beyond the LDS tile) runs 256 threads
 #define CG_LAUNCH(NBv, VWv, BASEv)                                                                          \
   do {                                                                                                      \
     constexpr bool heavy = asm_heavy<NBv, VWv, T>();                                                      \
-    if constexpr (heavy || NBv == 2) {                                                                      \
+    if (gacc) {                                                                                             \
+      auto kern = k_assemble<NBv, VWv, BASEv, T, 256, true>;                                                \
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, a);                                              \
+      CG_HIP(hipGetLastError());                                                                            \
+    } else if constexpr (heavy || NBv == 2) {                                                               \
       CG_LAUNCH_BS(NBv, VWv, BASEv, heavy ? 256 : 512);                                                     \
     } else {                                                                                                \
       if (want_bs == 256) CG_LAUNCH_BS(NBv, VWv, BASEv, 256);                                               \
@@ -2527,16 +2580,229 @@ static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &
   prepare_device(c, md, B, caps);
 }
 
-void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_host) {
+constexpr size_t kMaxGraphs = 16;  // captured shapes kept per context
+
+// List capacities of a capacity-mode call from the largest per-atom rates the
+// context has seen (x1.25), rounded up to 8 steps per octave so that the
+// capacities (part of a graph's shape) stay put while the counts drift from
+// call to call; a call with as many atoms as the last one of its kind keeps
+// the last capacities while they leave 5 % headroom over those rates.
+static Caps plan_caps(Ctx &c, Ctx::CapPlan &plan, int natoms) {
+  auto cap = [&](double rate) {
+    const double v = std::min(1.9e9, natoms * rate * 1.25 + 1024.0);
+    const double g = std::exp2(std::floor(std::log2(v)) - 3.0);
+    return (int)(std::ceil(v / g) * g);
+  };
+  Caps caps;
+  caps.nnb = cap(c.rate_nb);
+  for (int w = 0; w < 3; ++w) caps.ncl[w] = cap(c.rate_cl[w]);
+  if (natoms == plan.natoms) {
+    bool keep = plan.last.nnb >= natoms * c.rate_nb * 1.05;
+    for (int w = 0; w < 3; ++w) keep = keep && plan.last.ncl[w] >= natoms * c.rate_cl[w] * 1.05;
+    if (keep) caps = plan.last;
+  }
+  plan.last = caps;
+  plan.natoms = natoms;
+  return caps;
+}
+
+// Every grow-only device buffer of a context.
+template <class Fn> static void for_each_dbuf(Ctx &c, Fn fn) {
+  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
+                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1],
+                  &c.b_fam[2], &c.b_tasks[0], &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1],
+                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
+                  &c.b_coef[2], &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars, &c.b_atot, &c.b_atasks[0], &c.b_atasks[1], &c.b_atasks[2]})
+    fn(*b);
+}
+
+// Addresses of the context's buffers (and pinned result buffer), hashed: a
+// captured graph is replayed only while they are the ones it was captured
+// with.
+static uint64_t buffers_fingerprint(Ctx &c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for_each_dbuf(c, [&](DBuf &b) { mix((uint64_t)(uintptr_t)b.p); mix(b.cap); });
+  mix((uint64_t)(uintptr_t)c.h_pin);
+  mix(c.h_pin_cap);
+  return h;
+}
+
+static void drop_graphs(Ctx &c) {
+  for (auto &g : c.graphs)
+    if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+  c.graphs.clear();
+}
+
+// Capture what body enqueues on the context stream into a graph; false
+// (nothing enqueued) when the capture fails, e.g. because a buffer would
+// have to grow.
+template <class Body> static bool capture_graph(Ctx &c, Body body, hipGraphExec_t *exec) {
+  hipGraph_t graph = nullptr;
+  c.capturing = true;
+  t_no_grow = true;
+  bool ok = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    try {
+      body();
+    } catch (const Error &) {
+      ok = false;
+    }
+    if (hipStreamEndCapture(c.stream, &graph) != hipSuccess) ok = false;
+  }
+  c.capturing = false;
+  t_no_grow = false;
+  (void)hipGetLastError();
+  if (ok && graph) ok = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  else ok = false;
+  if (graph) (void)hipGraphDestroy(graph);
+  return ok;
+}
+
+// Keep a new graph under key (evicting the least recently used one when the
+// cache is full).
+static void keep_graph(Ctx &c, const std::string &key, hipGraphExec_t exec, uint64_t fp) {
+  if (c.graphs.size() >= kMaxGraphs) {
+    auto lru = c.graphs.begin();
+    for (auto g = c.graphs.begin(); g != c.graphs.end(); ++g)
+      if (g->second.used < lru->second.used) lru = g;
+    (void)hipGraphExecDestroy(lru->second.exec);
+    c.graphs.erase(lru);
+  }
+  c.graphs[key] = Ctx::Graph{exec, fp, ++c.graph_clock};
+}
+
+// The graph kept under key, if its buffers are still the context's (a
+// buffer that moved since the capture makes every graph of the context
+// stale: they are dropped).
+static Ctx::Graph *find_graph(Ctx &c, const std::string &key, uint64_t fp) {
+  auto it = c.graphs.find(key);
+  if (it == c.graphs.end()) return nullptr;
+  if (it->second.buffers != fp) {
+    drop_graphs(c);
+    return nullptr;
+  }
+  it->second.used = ++c.graph_clock;
+  return &it->second;
+}
+
+// Assembly tasks of one family: every (frame, type) bucket in column tiles
+// as wide as the frame's LDS tile allows under its size class's tile
+// configuration, whatever the bucket's cluster count (the kernel reads the
+// bucket's cluster range from the device bounds, and a bucket without
+// clusters writes its tile of zeros).  The list depends only on the frame
+// sizes and the tile choice, so it is built before any device work.  Tasks
+// of adjacent classes with one configuration share a launch (each class is
+// launched alone while the autotuner times them); within a launch larger
+// frames come first (order: the frames by decreasing atom count).
+static void build_tasks(const Ctx &c, int w, const std::vector<DevFrame> &hf, const std::vector<int> &by_size,
+                        std::vector<Task> &tasks, std::vector<TaskGroup> &groups) {
+  const DevFamily &F = c.fam[w];
+  auto cfg_of = [&](int cls) { return w == 0 ? 0 : c.tile_cfg[w][cls]; };
+  tasks.clear();
+  groups.clear();
+  // frames whose force rows do not fit the largest LDS tile (or above the
+  // context's limit) accumulate in A itself: their tasks come last, in a
+  // launch of their own, 64 columns wide
+  auto global_acc = [&](int n) {
+    return (c.gacc_above > 0 && n > c.gacc_above) || tile_bytes(n, 1) > kTileCfg[0].lds;
+  };
+  for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) {
+    const TileCfg &cfg = kTileCfg[cfg_of(cls)];
+    const size_t first = tasks.size();
+    size_t lds = 0;
+    for (int fi : by_size) {
+      const int n = hf[fi].natoms;
+      if (tile_class(n) != cls || global_acc(n)) continue;
+      int wmax = cfg.wcap;
+      while (wmax > 0 && tile_bytes(n, wmax) > cfg.lds) --wmax;
+      if (wmax < 1) wmax = 1;  // (configurations with half tiles: the frame fits one column of the full tile)
+      for (int t = 0; t < F.ntypes; ++t) {
+        const int nc = F.h_ncols[t];
+        if (nc <= 0) continue;
+        const int ntile = (nc + wmax - 1) / wmax;
+        const int wt = (nc + ntile - 1) / ntile;
+        lds = std::max(lds, tile_bytes(n, wt));
+        for (int col = 0; col < nc; col += wt) tasks.push_back(Task{fi, t, col, std::min(wt, nc - col), 0, 0});
+      }
+    }
+    if (tasks.size() == first) continue;
+    if (!groups.empty() && !c.split_classes && groups.back().cls1 == cls - 1 &&
+        cfg_of(groups.back().cls0) == cfg_of(cls)) {
+      // merged launch: its tasks keep the order of their classes (classes
+      // of larger frames come later, so the heavy tasks of the merged
+      // launch are reordered first)
+      std::vector<Task> merged(tasks.begin() + first, tasks.end());
+      merged.insert(merged.end(), tasks.begin() + groups.back().begin, tasks.begin() + first);
+      std::copy(merged.begin(), merged.end(), tasks.begin() + groups.back().begin);
+      groups.back().end = tasks.size();
+      groups.back().cls1 = cls;
+      groups.back().lds = std::max(groups.back().lds, lds);
+    } else {
+      groups.push_back(TaskGroup{first, tasks.size(), cls, cls, lds});
+    }
+  }
+  const size_t first = tasks.size();
+  for (int fi : by_size) {
+    if (!global_acc(hf[fi].natoms)) continue;
+    for (int t = 0; t < F.ntypes; ++t) {
+      const int nc = F.h_ncols[t];
+      if (nc <= 0) continue;
+      const int ntile = (nc + 63) / 64;
+      const int wt = (nc + ntile - 1) / ntile;
+      for (int col = 0; col < nc; col += wt) tasks.push_back(Task{fi, t, col, std::min(wt, nc - col), 0, 0});
+    }
+  }
+  if (tasks.size() > first) {
+    TaskGroup g{first, tasks.size(), CG_TILE_CLASSES - 1, CG_TILE_CLASSES - 1, 0};
+    g.gacc = true;
+    groups.push_back(g);
+  }
+}
+
+// Shape of a capacity-mode assembly: everything its launches depend on
+// besides the contents of the uploaded arrays.
+static std::string asm_shape_key(const Ctx &c, const Batch &B, const Caps &caps, int prec) {
+  std::string k = "A|";
+  auto add = [&](long long v) { k += std::to_string(v); k += ','; };
+  add(prec); add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
+  for (int w = 0; w < 3; ++w)
+    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) add(c.tile_cfg[w][cls]);
+  for (const DevFrame &d : B.hf) {
+    add(d.natoms); add(d.small); add(d.cell); add(d.nbin[0]); add(d.nbin[1]); add(d.nbin[2]); add(d.nlayers);
+    add(d.stress_rows); add(d.ener_rows);
+  }
+  return k;
+}
+
+// Design-matrix assembly of a batch of frames into the context's A.
+//
+// The first assembly on a context reads the neighbour and cluster totals back
+// to size its lists exactly.  Later ones (without fit statistics) size them
+// from the per-atom rates seen so far, as force calls do, and read nothing
+// back before the end: the kernels write nothing past a capacity, and the
+// device totals come back with the end of the call (a call that did not fit
+// is repeated with exact sizes).  An assembly of the same shape as the
+// previous call (frame sizes and cells, capacities, tile choice) is captured
+// into a HIP graph and replayed by later calls of that shape, after the
+// frame arrays and the task lists have been uploaded to the same buffers.
+// Every entry of A is written by a kernel (tiles of zeros for empty
+// buckets), so A is cleared first only when the model has FITCOUL charge
+// columns.
+static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_host, bool exact) {
   const cg_model &m = c.m;
+  ++c.n_asm;
+  c.asm_replayed = false;
+  int natoms = 0;
+  for (int i = 0; i < nframes; ++i) natoms += std::max(frames[i].natoms, 0);
+  Caps caps;
+  const bool capmode = !exact && c.rate_nb > 0.0 && !c.stats.on;
+  if (capmode) caps = plan_caps(c, c.cap_asm, natoms);
   Batch B;
   CG_HIP(hipEventRecord(c.ev[0], c.stream));
-  prepare(c, frames, nframes, 0, B);
+  prepare_host(c, frames, nframes, B, true);
   std::vector<DevFrame> &hf = B.hf;
   const long long rows = B.rows;
-  DevFrame *d_frames = B.d_frames;
-  int *d_type = B.d_type;
-  FamOut *fo = B.fo;
   // ---- b (A_MAT::PRINT_FRAME b.txt order)
   c.h_b.assign(rows, 0.0);
   {
@@ -2551,99 +2817,153 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
     }
     if (b_host) std::memcpy(b_host, c.h_b.data(), rows * sizeof(double));
   }
-  // ---- A allocation and tasks
+  // ---- A and the task lists (uploaded before any capture)
   const size_t esz = prec == CG_FP32 ? sizeof(float) : sizeof(double);
   const long long ld = m.ncols;
   void *A = c.b_A.get((size_t)rows * ld * esz);
   if (!A) throw Error(CG_ERR_MEMORY, "A matrix does not fit in device memory");
-  CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
-  int maxn = 0;
-  for (auto &d : hf) maxn = std::max(maxn, d.natoms);
-  bool fam_ran[3] = {false, false, false};
-  std::vector<Task> atasks[3];  // host sources of the task uploads, alive until the final sync
-  std::vector<std::pair<int, int>> fam_groups[3];  // first and last size class of each launch
-  for (int w = 0; w < 3; ++w) c.ncl[w] = fo[w].ncl;
-  for (int w = 0; w < 3; ++w) {
-    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) c.ms_cls[w][cls] = 0.f;
-    const DevFamily &F = c.fam[w];
-    if (F.ntypes == 0 || F.ncols == 0 || fo[w].ncl == 0) continue;
-    // tasks per frame-size class, with that class's tile configuration;
-    // adjacent classes of one configuration share a launch (each class is
-    // launched alone while the autotuner times them)
-    auto cfg_of = [&](int cls) { return w == 0 ? 0 : c.tile_cfg[w][cls]; };
-    std::vector<Task> &tasks = atasks[w];
-    struct Group { size_t begin, end; int cls0, cls1; };
-    std::vector<Group> groups;
-    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) {
-      const TileCfg &cfg = kTileCfg[cfg_of(cls)];
-      const size_t first = tasks.size();
-      for (int fi = 0; fi < nframes; ++fi) {
-        if (tile_class(hf[fi].natoms) != cls) continue;
-        int wmax = cfg.wcap;
-        while (wmax > 0 && tile_bytes(hf[fi].natoms, wmax) > cfg.lds) --wmax;
-        if (wmax < 1) throw Error(CG_ERR_UNSUPPORTED, "frame too large for the LDS tile (" + std::to_string(hf[fi].natoms) + " atoms)");
-        for (int t = 0; t < F.ntypes; ++t) {
-          int c0 = fo[w].boff[fi * F.ntypes + t], c1 = fo[w].boff[fi * F.ntypes + t + 1];
-          if (c1 <= c0) continue;
-          int nc = F.h_ncols[t];
-          int ntile = (nc + wmax - 1) / wmax;
-          int wt = (nc + ntile - 1) / ntile;
-          for (int col = 0; col < nc; col += wt) tasks.push_back(Task{fi, t, col, std::min(wt, nc - col), c0, c1});
+  // columns written by the assembly kernels: the families' (every bucket)
+  // and the energy offsets (k_offsets, every row)
+  int covered = m.n_offset_cols;
+  for (int w = 0; w < 3; ++w)
+    if (c.fam[w].ntypes > 0) covered += c.fam[w].ncols;
+  bool clear_A = covered != m.ncols || m.n_charge_cols > 0;
+  // task lists: rebuilt and uploaded when the frame sizes or the tile choice
+  // differ from the last assembly's
+  {
+    std::string tkey;
+    for (int w = 0; w < 3; ++w)
+      for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) tkey += std::to_string(c.tile_cfg[w][cls]) + ',';
+    tkey += c.split_classes ? "s|" : "m|";
+    tkey += std::to_string(c.gacc_above) + '|';
+    for (const DevFrame &d : hf) tkey += std::to_string(d.natoms) + ',';
+    bool same = tkey == c.asm_task_key;
+    for (int w = 0; w < 3 && same; ++w)
+      same = c.asm_tasks[w].empty() || c.b_atasks[w].cap >= c.asm_tasks[w].size() * sizeof(Task);
+    if (!same) {
+      c.asm_task_key.clear();
+      std::vector<int> by_size(nframes);
+      for (int i = 0; i < nframes; ++i) by_size[i] = i;
+      std::stable_sort(by_size.begin(), by_size.end(), [&](int x, int y) { return hf[x].natoms > hf[y].natoms; });
+      for (int w = 0; w < 3; ++w) {
+        c.asm_tasks[w].clear();
+        c.asm_groups[w].clear();
+        const DevFamily &F = c.fam[w];
+        if (F.ntypes == 0 || F.ncols == 0) continue;
+        build_tasks(c, w, hf, by_size, c.asm_tasks[w], c.asm_groups[w]);
+        if (c.asm_tasks[w].empty()) continue;
+        Task *d = c.b_atasks[w].as<Task>(c.asm_tasks[w].size());
+        if (!d) throw Error(CG_ERR_MEMORY, "task list");
+        upload(c, d, c.asm_tasks[w].data(), c.asm_tasks[w].size() * sizeof(Task));
+      }
+      c.asm_task_key = tkey;
+    }
+  }
+  const std::vector<Task> *atasks = c.asm_tasks;
+  const std::vector<TaskGroup> *fam_groups = c.asm_groups;
+  // frames accumulated in A itself need it zeroed first
+  for (int w = 0; w < 3; ++w)
+    for (const TaskGroup &g : fam_groups[w]) clear_A = clear_A || g.gacc;
+  Task *d_tasks[3] = {nullptr, nullptr, nullptr};
+  for (int w = 0; w < 3; ++w)
+    if (!atasks[w].empty()) d_tasks[w] = c.b_atasks[w].as<Task>(atasks[w].size());
+  double *d_tot = c.b_atot.as<double>(8);
+  if (!d_tot) throw Error(CG_ERR_MEMORY, "assembly totals");
+  // ---- device work: lists, clusters, families, offsets, totals
+  auto body = [&]() {
+    if (clear_A) CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
+    prepare_device(c, 0, B, capmode ? &caps : nullptr);
+    for (int w = 0; w < 3; ++w) {
+      if (!d_tasks[w]) continue;
+      const std::vector<TaskGroup> &groups = fam_groups[w];
+      if (!c.capturing) CG_HIP(hipEventRecord(c.fev[w][0], c.stream));
+      // launches of different configurations run side by side (their tails
+      // overlap); the autotuner's timing runs keep them in sequence
+      if (groups.size() > CG_TILE_CLASSES + 1) throw Error(CG_ERR_DEVICE, "assembly launch groups");
+      const bool fork = groups.size() > 1 && !c.split_classes;
+      if (fork) CG_HIP(hipEventRecord(c.fork, c.stream));
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const TaskGroup &g = groups[gi];
+        hipStream_t st = fork && gi > 0 ? c.aux[gi - 1] : c.stream;
+        if (st != c.stream) CG_HIP(hipStreamWaitEvent(st, c.fork, 0));
+        const size_t lds = g.lds;
+        const int bs = kTileCfg[w == 0 ? 0 : c.tile_cfg[w][g.cls0]].bs;
+        const bool timed = !c.capturing && !g.gacc;
+        if (timed) CG_HIP(hipEventRecord(c.cev[w][g.cls0][0], st));
+        if (prec == CG_FP32)
+          launch_family<float>(c, w, g.end - g.begin, d_tasks[w] + g.begin, B.d_frames, B.fo[w].refs,
+                               B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc);
+        else
+          launch_family<double>(c, w, g.end - g.begin, d_tasks[w] + g.begin, B.d_frames, B.fo[w].refs,
+                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc);
+        if (timed) CG_HIP(hipEventRecord(c.cev[w][g.cls0][1], st));
+        if (st != c.stream) {
+          CG_HIP(hipEventRecord(c.join[gi - 1], st));
+          CG_HIP(hipStreamWaitEvent(c.stream, c.join[gi - 1], 0));
         }
       }
-      if (tasks.size() == first) continue;
-      if (!groups.empty() && !c.split_classes && groups.back().cls1 == cls - 1 &&
-          cfg_of(groups.back().cls0) == cfg_of(cls)) {
-        groups.back().end = tasks.size();
-        groups.back().cls1 = cls;
-      } else {
-        groups.push_back(Group{first, tasks.size(), cls, cls});
-      }
+      if (!c.capturing) CG_HIP(hipEventRecord(c.fev[w][1], c.stream));
     }
-    // heavy tasks first within a launch
-    for (const Group &g : groups)
-      std::stable_sort(tasks.begin() + g.begin, tasks.begin() + g.end, [](const Task &x, const Task &y) {
-        return (long long)(x.c1 - x.c0) * x.ncols > (long long)(y.c1 - y.c0) * y.ncols;
-      });
-    Task *d_tasks = c.b_tasks[w].as<Task>(tasks.size());
-    if (!d_tasks) throw Error(CG_ERR_MEMORY, "task list");
-    upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(Task));
-    CG_HIP(hipEventRecord(c.fev[w][0], c.stream));
-    // launches of different configurations run side by side (their tails
-    // overlap); the autotuner's timing runs keep them in sequence
-    const bool fork = groups.size() > 1 && !c.split_classes;
-    if (fork) CG_HIP(hipEventRecord(c.fork, c.stream));
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-      const Group &g = groups[gi];
-      hipStream_t st = fork && gi > 0 ? c.aux[gi - 1] : c.stream;
-      if (st != c.stream) CG_HIP(hipStreamWaitEvent(st, c.fork, 0));
-      size_t lds = 0;
-      for (size_t i = g.begin; i < g.end; ++i) lds = std::max(lds, tile_bytes(hf[tasks[i].frame].natoms, tasks[i].ncols));
-      const int bs = kTileCfg[cfg_of(g.cls0)].bs;
-      CG_HIP(hipEventRecord(c.cev[w][g.cls0][0], st));
+    if (m.n_offset_cols > 0) {
       if (prec == CG_FP32)
-        launch_family<float>(c, w, g.end - g.begin, d_tasks + g.begin, d_frames, fo[w].refs, lds, A, ld, bs, st);
+        hipLaunchKernelGGL(k_offsets<float>, dim3(nframes), dim3(256), 0, c.stream, B.d_frames, B.d_type, nframes,
+                           m.ntypes, m.ncols_short, (float *)A, ld);
       else
-        launch_family<double>(c, w, g.end - g.begin, d_tasks + g.begin, d_frames, fo[w].refs, lds, A, ld, bs, st);
-      CG_HIP(hipEventRecord(c.cev[w][g.cls0][1], st));
-      if (st != c.stream) {
-        CG_HIP(hipEventRecord(c.join[gi - 1], st));
-        CG_HIP(hipStreamWaitEvent(c.stream, c.join[gi - 1], 0));
+        hipLaunchKernelGGL(k_offsets<double>, dim3(nframes), dim3(256), 0, c.stream, B.d_frames, B.d_type, nframes,
+                           m.ntypes, m.ncols_short, (double *)A, ld);
+    }
+    hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, c.stream, B.tot, d_tot);
+    CG_HIP(hipMemcpyAsync(c.h_atot, d_tot, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  };
+  bool replayed = false;
+  // calls with Ewald columns (per-call k tables), a cleared A or statistics
+  // stay out of graphs
+  if (capmode && c.asm_graphs && !clear_A && !c.split_classes) {
+    const std::string key = asm_shape_key(c, B, caps, prec);
+    const uint64_t fp = buffers_fingerprint(c);
+    if (Ctx::Graph *g = find_graph(c, key, fp)) {
+      CG_HIP(hipGraphLaunch(g->exec, c.stream));
+      ++c.n_asm_graph;
+      replayed = true;
+    } else if (key == c.last_asm_key) {
+      hipGraphExec_t exec = nullptr;
+      if (capture_graph(c, body, &exec)) {
+        ++c.n_asm_capture;
+        keep_graph(c, key, exec, fp);
+        CG_HIP(hipGraphLaunch(exec, c.stream));
+        replayed = true;
       }
     }
-    CG_HIP(hipEventRecord(c.fev[w][1], c.stream));
-    for (const Group &g : groups) fam_groups[w].push_back({g.cls0, g.cls1});
-    fam_ran[w] = true;
+    c.last_asm_key = key;
   }
-  if (m.n_offset_cols > 0) {
-    if (prec == CG_FP32)
-      hipLaunchKernelGGL(k_offsets<float>, dim3(nframes), dim3(256), 0, c.stream, d_frames, d_type, nframes, m.ntypes, m.ncols_short, (float *)A, ld);
-    else
-      hipLaunchKernelGGL(k_offsets<double>, dim3(nframes), dim3(256), 0, c.stream, d_frames, d_type, nframes, m.ntypes, m.ncols_short, (double *)A, ld);
+  if (!replayed) body();
+  if (m.n_charge_cols > 0) ewald_columns(c, frames, hf, B.d_af, B.d_type, B.d_wrap, B.natoms, A, ld, prec);
+  CG_HIP(hipEventRecord(c.ev[2], c.stream));
+  CG_HIP(hipEventSynchronize(c.ev[2]));
+  CG_HIP(hipGetLastError());
+  // ---- totals: list sizes against the capacities, rates for later calls
+  const double *tot = c.h_atot;
+  const double na = std::max(natoms, 1);
+  if (capmode) {
+    bool fits = tot[0] <= caps.nnb;
+    for (int w = 0; w < 3; ++w) fits = fits && tot[1 + w] <= caps.ncl[w];
+    if (!fits) {
+      c.rate_nb = std::max(c.rate_nb, tot[0] / na);
+      for (int w = 0; w < 3; ++w) c.rate_cl[w] = std::max(c.rate_cl[w], tot[1 + w] / na);
+      ++c.n_asm_repeat;
+      --c.n_asm;
+      assemble_impl(c, frames, nframes, prec, b_host, true);
+      return;
+    }
   }
-  if (m.n_charge_cols > 0) ewald_columns(c, frames, hf, B.d_af, d_type, B.d_wrap, B.natoms, A, ld, prec);
+  c.rate_nb = std::max(c.rate_nb, tot[0] / na);
+  for (int w = 0; w < 3; ++w) {
+    c.rate_cl[w] = std::max(c.rate_cl[w], tot[1 + w] / na);
+    c.ncl[w] = tot[1 + w] > 0 ? (long long)tot[1 + w] : 0;
+  }
   if (c.stats.on) {
     LsqStats &S = c.stats;
+    FamOut *fo = B.fo;
     for (int w = 0; w < 3; ++w) {
       const DevFamily &F = c.fam[w];
       if (S.count[w].size() != (size_t)F.ntypes) S.count[w].assign(F.ntypes, 0);
@@ -2662,25 +2982,40 @@ void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_h
       std::copy(h.begin() + nframes, h.end(), S.flat4.begin());
     }
   }
-  CG_HIP(hipEventRecord(c.ev[2], c.stream));
-  CG_HIP(hipEventSynchronize(c.ev[2]));
-  CG_HIP(hipGetLastError());
-  CG_HIP(hipEventElapsedTime(&c.ms_enum, c.ev[0], c.ev[1]));
-  CG_HIP(hipEventElapsedTime(&c.ms_asm, c.ev[1], c.ev[2]));
-  c.ms_total = c.ms_enum + c.ms_asm;
+  // ---- timing: per family and class on eager calls; a replayed call has
+  // only its total (the graph's kernels carry no timing events)
+  c.asm_replayed = replayed;
   for (int w = 0; w < 3; ++w) {
     c.ms_fam[w] = 0.f;
-    if (fam_ran[w]) CG_HIP(hipEventElapsedTime(&c.ms_fam[w], c.fev[w][0], c.fev[w][1]));
-    for (const auto &g : fam_groups[w]) {
-      float ms = 0.f;
-      CG_HIP(hipEventElapsedTime(&ms, c.cev[w][g.first][0], c.cev[w][g.first][1]));
-      for (int cls = g.first; cls <= g.second; ++cls) c.ms_cls[w][cls] = ms;  // shared by a merged launch
+    for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) c.ms_cls[w][cls] = 0.f;
+  }
+  if (replayed) {
+    c.ms_enum = 0.f;
+    CG_HIP(hipEventElapsedTime(&c.ms_asm, c.ev[0], c.ev[2]));
+    for (int w = 0; w < 3; ++w) c.ms_fam[w] = -1.f;
+  } else {
+    CG_HIP(hipEventElapsedTime(&c.ms_enum, c.ev[0], c.ev[1]));
+    CG_HIP(hipEventElapsedTime(&c.ms_asm, c.ev[1], c.ev[2]));
+    for (int w = 0; w < 3; ++w) {
+      if (!d_tasks[w]) continue;
+      CG_HIP(hipEventElapsedTime(&c.ms_fam[w], c.fev[w][0], c.fev[w][1]));
+      for (const TaskGroup &g : fam_groups[w]) {
+        if (g.gacc) continue;
+        float ms = 0.f;
+        CG_HIP(hipEventElapsedTime(&ms, c.cev[w][g.cls0][0], c.cev[w][g.cls0][1]));
+        for (int cls = g.cls0; cls <= g.cls1; ++cls) c.ms_cls[w][cls] = ms;  // shared by a merged launch
+      }
     }
   }
+  c.ms_total = c.ms_enum + c.ms_asm;
   c.A = A;
   c.A_prec = prec;
   c.A_rows = rows;
   c.A_ld = (int)ld;
+}
+
+void assemble(Ctx &c, const cg_frame *frames, int nframes, int prec, double *b_host) {
+  assemble_impl(c, frames, nframes, prec, b_host, false);
 }
 
 
@@ -2819,62 +3154,6 @@ static std::string shape_key(const Ctx &c, const Batch &B, const Caps &caps) {
   return k;
 }
 
-constexpr size_t kMaxGraphs = 16;  // captured shapes kept per context
-
-// Every grow-only device buffer of a context.
-template <class Fn> static void for_each_dbuf(Ctx &c, Fn fn) {
-  for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
-                  &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1],
-                  &c.b_fam[2], &c.b_tasks[0], &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1],
-                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
-                  &c.b_coef[2], &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars})
-    fn(*b);
-}
-
-// Addresses of the context's buffers (and pinned result buffer), hashed: a
-// captured graph is replayed only while they are the ones it was captured
-// with.
-static uint64_t buffers_fingerprint(Ctx &c) {
-  uint64_t h = 1469598103934665603ull;
-  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
-  for_each_dbuf(c, [&](DBuf &b) { mix((uint64_t)(uintptr_t)b.p); mix(b.cap); });
-  mix((uint64_t)(uintptr_t)c.h_pin);
-  mix(c.h_pin_cap);
-  return h;
-}
-
-static void drop_graphs(Ctx &c) {
-  for (auto &g : c.graphs)
-    if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
-  c.graphs.clear();
-}
-
-// Capture prepare_device + force_device of this call into a graph; false
-// (nothing enqueued) when the capture fails, e.g. because a buffer would
-// have to grow.
-static bool capture_call(Ctx &c, Batch &B, const Caps &caps, double *d_F, size_t nout, hipGraphExec_t *exec) {
-  hipGraph_t graph = nullptr;
-  c.capturing = true;
-  t_no_grow = true;
-  bool ok = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-  if (ok) {
-    try {
-      prepare_device(c, 1, B, &caps);
-      force_device(c, B, d_F, nout);
-    } catch (const Error &) {
-      ok = false;
-    }
-    if (hipStreamEndCapture(c.stream, &graph) != hipSuccess) ok = false;
-  }
-  c.capturing = false;
-  t_no_grow = false;
-  (void)hipGetLastError();
-  if (ok && graph) ok = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
-  else ok = false;
-  if (graph) (void)hipGraphDestroy(graph);
-  return ok;
-}
-
 // Force / energy / virial of a batch of configurations, each standing alone
 // (no small-cell stickiness across them).  F: [3 * sum natoms], E: [nframes],
 // vir: [9 * nframes].
@@ -2899,26 +3178,7 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   for (int i = 0; i < nframes; ++i) natoms += frames[i].natoms;
   Caps caps;
   const bool capmode = !noalloc && !exact && c.rate_nb > 0.0;
-  if (capmode) {
-    // rounded up to 8 steps per octave, so that the capacities (part of a
-    // graph's shape) stay put while the counts drift from call to call
-    auto cap = [&](double rate) {
-      const double v = std::min(1.9e9, natoms * rate * 1.25 + 1024.0);
-      const double g = std::exp2(std::floor(std::log2(v)) - 3.0);
-      return (int)(std::ceil(v / g) * g);
-    };
-    caps.nnb = cap(c.rate_nb);
-    for (int w = 0; w < 3; ++w) caps.ncl[w] = cap(c.rate_cl[w]);
-    // a call with as many atoms as the last keeps the last capacities while
-    // they leave 5 % headroom over the largest rates seen
-    if (natoms == c.last_caps_natoms) {
-      bool keep = c.last_caps.nnb >= natoms * c.rate_nb * 1.05;
-      for (int w = 0; w < 3; ++w) keep = keep && c.last_caps.ncl[w] >= natoms * c.rate_cl[w] * 1.05;
-      if (keep) caps = c.last_caps;
-    }
-    c.last_caps = caps;
-    c.last_caps_natoms = natoms;
-  }
+  if (capmode) caps = plan_caps(c, c.cap_force, natoms);
   Batch B;
   prepare_host(c, frames, nframes, B, false);
   const size_t nout = 3 * (size_t)natoms + 10 * (size_t)nframes;
@@ -2939,29 +3199,19 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   if (capmode && m.n_charge_cols == 0) {
     const std::string key = shape_key(c, B, caps);
     const uint64_t fp = buffers_fingerprint(c);
-    auto it = c.graphs.find(key);
-    if (it != c.graphs.end() && it->second.buffers != fp) {
-      // a buffer moved since the capture: every graph of the context is stale
-      drop_graphs(c);
-      it = c.graphs.end();
-    }
-    if (it != c.graphs.end()) {
-      it->second.used = ++c.graph_clock;
-      CG_HIP(hipGraphLaunch(it->second.exec, c.stream));
+    if (Ctx::Graph *g = find_graph(c, key, fp)) {
+      CG_HIP(hipGraphLaunch(g->exec, c.stream));
       ++c.n_graph;
       done = true;
     } else if (key == c.last_key) {
       hipGraphExec_t exec = nullptr;
-      if (capture_call(c, B, caps, d_F, nout, &exec)) {
+      auto body = [&]() {
+        prepare_device(c, 1, B, &caps);
+        force_device(c, B, d_F, nout);
+      };
+      if (capture_graph(c, body, &exec)) {
         ++c.n_capture;
-        if (c.graphs.size() >= kMaxGraphs) {
-          auto lru = c.graphs.begin();
-          for (auto g = c.graphs.begin(); g != c.graphs.end(); ++g)
-            if (g->second.used < lru->second.used) lru = g;
-          (void)hipGraphExecDestroy(lru->second.exec);
-          c.graphs.erase(lru);
-        }
-        c.graphs[key] = Ctx::Graph{exec, fp, ++c.graph_clock};
+        keep_graph(c, key, exec, fp);
         CG_HIP(hipGraphLaunch(exec, c.stream));
         done = true;
       }
@@ -3091,6 +3341,7 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
   for (auto &e : c.ev) CG_HIP(hipEventCreate(&e));
   for (auto &fe : c.fev) for (auto &e : fe) CG_HIP(hipEventCreate(&e));
   for (auto &fc : c.cev) for (auto &fe : fc) for (auto &e : fe) CG_HIP(hipEventCreate(&e));
+  CG_HIP(hipHostMalloc((void **)&c.h_atot, 8 * sizeof(double), hipHostMallocDefault));
   {
     // tile choices made by cg_autotune for this model earlier in the process
     std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -3122,6 +3373,8 @@ void ctx_free(Ctx &c) {
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   c.h_pin = nullptr;
   c.h_pin_cap = 0;
+  if (c.h_atot) (void)hipHostFree(c.h_atot);
+  c.h_atot = nullptr;
   if (c.d_int_pair_map) (void)hipFree(c.d_int_pair_map);
   for (auto &e : c.ev) if (e) (void)hipEventDestroy(e);
   for (auto &fe : c.fev) for (auto &e : fe) if (e) (void)hipEventDestroy(e);
@@ -3182,6 +3435,13 @@ cg_status cg_autotune(cg_ctx *ctx, const cg_frame *frames, int nframes, int prec
   cg::Ctx &c = ctx->c;
   int keep[3][NC];
   std::memcpy(keep, c.tile_cfg, sizeof(keep));
+  // every tuning assembly runs eagerly (a graph replay has no per-launch times)
+  struct GraphsOff {
+    cg::Ctx &c;
+    bool was;
+    explicit GraphsOff(cg::Ctx &cc) : c(cc), was(cc.asm_graphs) { c.asm_graphs = false; }
+    ~GraphsOff() { c.asm_graphs = was; }
+  } graphs_off(c);
   try {
     CG_HIP(hipSetDevice(c.device));
     double ms[NK][3][NC] = {};
@@ -3479,6 +3739,29 @@ cg_status cg_synchronize(cg_ctx *ctx) {
     cg::set_error(e.what());
     return e.code;
   }
+}
+
+cg_status cg_assembly_graphs(cg_ctx *ctx, int on) {
+  if (!ctx) { cg::set_error("cg_assembly_graphs: bad argument"); return CG_ERR_ARG; }
+  ctx->c.asm_graphs = on != 0;
+  return CG_OK;
+}
+
+cg_status cg_assembly_global_above(cg_ctx *ctx, int natoms) {
+  if (!ctx || natoms < 0) { cg::set_error("cg_assembly_global_above: bad argument"); return CG_ERR_ARG; }
+  ctx->c.gacc_above = natoms;
+  return CG_OK;
+}
+
+cg_status cg_assembly_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats, int64_t *graph_replays,
+                            int64_t *captures) {
+  if (!ctx) { cg::set_error("cg_assembly_stats: bad argument"); return CG_ERR_ARG; }
+  const cg::Ctx &c = ctx->c;
+  if (calls) *calls = c.n_asm;
+  if (repeats) *repeats = c.n_asm_repeat;
+  if (graph_replays) *graph_replays = c.n_asm_graph;
+  if (captures) *captures = c.n_asm_capture;
+  return CG_OK;
 }
 
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_asm, double *ms_total) {

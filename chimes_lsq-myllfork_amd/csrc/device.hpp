@@ -143,6 +143,20 @@ static_assert(ref_round_trip(), "image reference encoding");
 // Sizes of a call without host reads: capacity of the neighbour list and of
 // each family's cluster list.  The kernels write nothing past them; the
 // device totals tell afterwards whether the call fitted.
+// A work item of the assembly (a column tile of one (frame, type) bucket)
+// or force kernels (a chunk of clusters of one bucket).
+struct Task {
+  int frame;
+  int type;
+  int col0;    // first column of the tile, relative to the type (assembly)
+  int ncols;   // tile width (assembly)
+  int c0, c1;  // cluster range in the bucketed arrays (force evaluation)
+};
+// Tasks [begin, end) of one assembly launch: frame-size classes cls0..cls1
+// under one tile configuration, with the largest LDS tile among them.
+// gacc: the tasks accumulate in A itself (frames beyond the LDS tile).
+struct TaskGroup { size_t begin, end; int cls0, cls1; size_t lds; bool gacc = false; };
+
 struct Caps {
   int nnb = 0;
   int ncl[3] = {0, 0, 0};
@@ -194,8 +208,10 @@ struct Ctx {
   hipStream_t stream = nullptr;
   // assembly launches of further frame-size classes run on these (default
   // priority), forked from and joined back to `stream`
-  hipStream_t aux[CG_TILE_CLASSES - 1] = {};
-  hipEvent_t fork = nullptr, join[CG_TILE_CLASSES - 1] = {};
+  // (one per launch after the first: the size classes and the
+  // global-accumulation launch)
+  hipStream_t aux[CG_TILE_CLASSES] = {};
+  hipEvent_t fork = nullptr, join[CG_TILE_CLASSES] = {};
   const cg_model *model = nullptr;
   cg_model m{};               // copy of the POD header
   DevFamily fam[3];           // 0: 2B pairs, 1: 3B, 2: 4B
@@ -236,8 +252,27 @@ struct Ctx {
   uint64_t graph_clock = 0;
   std::map<std::string, Graph> graphs;
   std::string last_key;
-  Caps last_caps;                         // list capacities of the last capacity-mode call
-  int last_caps_natoms = -1;
+  // list capacities of the last capacity-mode call (force calls and
+  // assemblies separately) and its atom count
+  struct CapPlan { Caps last; int natoms = -1; };
+  CapPlan cap_force, cap_asm;
+  // assemblies: repeated with exact sizes, replayed from a graph, captured;
+  // capture of repeated shapes on/off (cg_assembly_graphs)
+  int64_t n_asm = 0, n_asm_repeat = 0, n_asm_graph = 0, n_asm_capture = 0;
+  bool asm_graphs = true;
+  std::string last_asm_key;               // shape of the last capacity-mode assembly
+  bool asm_replayed = false;              // the last assembly was a graph replay
+  DBuf b_atot;                            // device totals of an assembly
+  // assembly task lists of the last shape (frame sizes, tile choice), kept on
+  // the host and in b_atasks while the shape repeats
+  DBuf b_atasks[3];
+  std::string asm_task_key;
+  std::vector<Task> asm_tasks[3];
+  std::vector<TaskGroup> asm_groups[3];
+  // frames with more atoms than this accumulate in A with global atomics
+  // (0: only frames whose force rows do not fit the LDS tile)
+  int gacc_above = 0;
+  double *h_atot = nullptr;               // pinned host copy of them (4 doubles)
   double *h_pin = nullptr;                // pinned host copy of results and totals
   size_t h_pin_cap = 0;
   bool capturing = false;

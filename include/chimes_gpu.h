@@ -277,13 +277,42 @@ cg_status cg_tile_choice(cg_ctx *ctx, int *choice);
  * 0..CG_TILE_CONFIGS-1, and 0 for the 2-body family. */
 cg_status cg_tile_set(cg_ctx *ctx, const int *choice);
 /* Kernel timing of the last cg_assemble (ms, measured with HIP events on the
- * context stream): neighbour+enumeration, assembly kernels, total. */
+ * context stream): neighbour+enumeration, assembly kernels, total.  After a
+ * call replayed from a captured graph (below) ms_enum is 0 and ms_assemble
+ * is the whole call. */
 cg_status cg_last_timing(cg_ctx *ctx, double *ms_enum, double *ms_assemble, double *ms_total);
 
 /* Per-family detail of the last cg_assemble: kernel time of the 2-body,
- * 3-body and 4-body assembly launches (ms, HIP events on the context stream)
- * and the number of pairs / triplets / quadruplets assembled. */
+ * 3-body and 4-body assembly launches (ms, HIP events on the context stream;
+ * -1 after a graph replay, whose kernels carry no timing events) and the
+ * number of pairs / triplets / quadruplets assembled. */
 cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3);
+
+/* Graph capture of repeated assemblies (on by default).  The first
+ * cg_assemble on a context reads its list sizes back; later ones (without
+ * statistics) size the neighbour and cluster lists from the per-atom counts
+ * seen so far and read nothing back before the end of the call, where the
+ * device totals are checked (a call that did not fit is repeated with exact
+ * sizes).  Such a call with the same shape as the previous call on ctx
+ * (frame sizes and cells, list capacities, tile choice, precision) is
+ * captured into a HIP graph, one launch for the whole assembly, and replayed
+ * by later calls of that shape.  Models with FITCOUL charge columns are not
+ * captured.  on = 0 keeps every call eager.  No counterpart in the reference
+ * (the frame loop of src/chimes_lsq.C:292-328 assembles one frame at a
+ * time). */
+cg_status cg_assembly_graphs(cg_ctx *ctx, int on);
+/* Frames whose force rows do not fit the LDS tile of the assembly kernels
+ * (above about 6300 atoms) are accumulated straight into A with global
+ * atomics, slower but without a size limit; A is then zeroed first and the
+ * call is not captured.  natoms > 0 sends every frame with more atoms than
+ * that down the same path (0, the default: only those that do not fit).
+ * The reference has no limit either (A_MAT is a per-frame host buffer,
+ * src/A_Matrix.C:26-49). */
+cg_status cg_assembly_global_above(cg_ctx *ctx, int natoms);
+/* Assembly calls on ctx, those repeated with exact list sizes, those
+ * replayed from a captured graph and the captures made. */
+cg_status cg_assembly_stats(cg_ctx *ctx, int64_t *calls, int64_t *repeats, int64_t *graph_replays,
+                            int64_t *captures);
 
 /* Statistics chimes_lsq prints at the end of a run, gathered by the
  * following cg_assemble calls on ctx (print_bond_stats,

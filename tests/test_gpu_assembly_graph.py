@@ -1,0 +1,101 @@
+"""Repeated assemblies on one context: list capacities from per-atom counts,
+graph capture and replay, exact repeat on overflow, and no stale entries of A
+(every entry is written, buckets without clusters as zeros).
+
+Bar: a replayed or capacity-sized assembly equals the assembly of a context
+that stays eager to 1e-12 column-scaled (the LDS sums run in a different
+order from run to run), and the oracle's to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import colscaled_error, synth_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _moved(ch, f, rng, sigma=0.01):
+    xyz = np.array(f.xyz, dtype=np.float64).reshape(-1, 3)
+    return ch.PyFrame(f.natoms, xyz + rng.normal(0.0, sigma, xyz.shape), np.array(f.types, dtype=np.int32),
+                      np.array(f.cell, dtype=np.float64), np.array(f.forces), np.array(f.stress), f.energy, f.index)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_assembly_graph_replay_matches_eager(ch, synth, oracle, tmp_path, prec):
+    """A mixed C/H/N/O batch assembled six times with moved atoms: the third
+    call is captured and the later ones replay it; each result equals an
+    eager context's (graphs off), with other work on the device between the
+    replays."""
+    print("prec", prec, flush=True)
+    P = ch.CG_FP64 if prec == "fp64" else ch.CG_FP32
+    gen = synth.chno(nframes=6, nmin=40, nmax=700, seed=61)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    rng = np.random.default_rng(61)
+    dev, ref = ch.Device(s), ch.Device(s)
+    ref.assembly_graphs(False)
+    x = rng.normal(0, 1e-3, s.model.ncols)
+    for step in range(6):
+        batch = [_moved(ch, f, rng) for f in fr]
+        A, b = dev.assemble(batch, prec=P)
+        Ar, br = ref.assemble(batch, prec=P)
+        assert np.array_equal(b, br)
+        err = colscaled_error(A, Ar)
+        print("step", step, "vs eager %.2e" % err, dev.assembly_stats(), flush=True)
+        assert err < (1e-12 if prec == "fp64" else 1e-6), step
+        if step == 0:
+            assert colscaled_error(A, oracle.assemble(s, batch)) < (1e-10 if prec == "fp64" else 5e-3)
+        dev.force(x, batch[0])  # other device work between the replays
+        dev.synchronize()
+    calls, repeats, replays, captures = dev.assembly_stats()
+    assert calls == 6 and captures >= 1 and replays >= 2, (calls, repeats, replays, captures)
+    assert ref.assembly_stats()[2] == 0
+    dev.close()
+    ref.close()
+
+
+def test_assembly_capacity_overflow_repeats_exactly(ch, synth, oracle, tmp_path):
+    """After a dilute batch the next assembly sizes its lists from the dilute
+    per-atom counts; a dense batch of the same atom count overflows them, is
+    repeated with exact sizes and equals the oracle."""
+    text = synth.fm_setup(["C", "H", "N", "O"], [12.011, 1.008, 14.007, 15.999], 12, 6, 3, 0.8, 5.0, 1.4,
+                          smax3=4.0, smax4=3.5)
+    p = tmp_path / "fm_setup.in"
+    open(p, "w").write(text)
+    s = ch.Setup(str(p))
+    rng = np.random.default_rng(62)
+    big, small = np.diag([30.0] * 3), np.diag([16.0] * 3)
+    dil = ch.PyFrame(300, synth.random_sequential(rng, 300, big, 0.9), rng.integers(0, 4, 300).astype(np.int32),
+                     big, rng.normal(size=900), np.zeros(9), 0.0, 0)
+    den = ch.PyFrame(300, synth.random_sequential(rng, 300, small, 0.9), rng.integers(0, 4, 300).astype(np.int32),
+                     small, rng.normal(size=900), np.zeros(9), 0.0, 0)
+    dev = ch.Device(s)
+    dev.assemble([dil])
+    A1, _ = dev.assemble([den])
+    calls, repeats, _, _ = dev.assembly_stats()
+    assert calls == 2 and repeats == 1, dev.assembly_stats()
+    A2, _ = dev.assemble([den])
+    assert dev.assembly_stats()[1] == 1
+    Ao = oracle.assemble(s, [den])
+    assert colscaled_error(A1, Ao) < 1e-10
+    assert colscaled_error(A2, Ao) < 1e-10
+    dev.close()
+
+
+def test_assembly_leaves_no_stale_entries(ch, synth, oracle, tmp_path):
+    """A large mixed batch, then a smaller one on the same context (same
+    buffers, not cleared): the second A equals the oracle's, including the
+    zeros of cluster types its frames do not contain."""
+    gen = synth.chno(nframes=4, nmin=200, nmax=600, seed=63)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    rng = np.random.default_rng(63)
+    cell = np.diag([14.0] * 3)
+    # carbon and hydrogen only: every cluster type with N or O is empty
+    ch_only = ch.PyFrame(150, synth.random_sequential(rng, 150, cell, 0.9), rng.integers(0, 2, 150).astype(np.int32),
+                         cell, rng.normal(size=450), np.zeros(9), 0.0, 0)
+    dev = ch.Device(s)
+    dev.assemble(fr)
+    A, _ = dev.assemble([ch_only])
+    Ao = oracle.assemble(s, [ch_only])
+    assert colscaled_error(A, Ao) < 1e-10
+    assert np.all(A[Ao == 0] == 0)
+    dev.close()
