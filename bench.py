@@ -318,6 +318,10 @@ def config_c4(ch, synth, nframes=256):
         out[tag] = assemble_rate(ch, dev, fr, rows, prec, steps=2)
         out[tag.replace("rows_per_s", "family_ms")] = family_ms(dev, fr, prec)
     out["clusters"] = dict(zip(["pairs", "triplets", "quadruplets"], dev.detail()[1]))
+    dev.assembly_deterministic(True)
+    out["fp64_deterministic_rows_per_s"] = assemble_rate(ch, dev, fr, rows, ch.CG_FP64, steps=2)
+    out["fp64_deterministic_family_ms"] = family_ms(dev, fr, ch.CG_FP64)
+    dev.assembly_deterministic(False)
     calls, repeats, replays, captures = dev.assembly_stats()
     out["assembly_calls"] = {"calls": calls, "exact_repeats": repeats, "graph_replays": replays,
                              "graph_captures": captures}
@@ -402,6 +406,16 @@ def main():
         asm.append(dev.timing())
     dev.synchronize()
     ms_step_eager = (time.perf_counter() - t0) / args.steps * 1e3
+    # bitwise-reproducible mode (waves add to the LDS tile in turn): its cost
+    dev.assembly_deterministic(True)
+    k4d = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(ch.CG_FP64)
+        k4d.append(dev.detail()[0][2])
+    dev.synchronize()
+    ms_step_det = (time.perf_counter() - t0) / args.steps * 1e3
+    dev.assembly_deterministic(False)
     dev.assembly_graphs(GRAPHS)
     ms4 = float(np.mean(k4))
     # algorithmic work of the 4-body kernel: 4 flops per pair per allowed power set
@@ -425,6 +439,8 @@ def main():
              "assembly_calls": {"graph_replays": replays, "graph_captures": captures, "exact_repeats": repeats,
                                 "calls": calls},
              "eager_ms_per_step": ms_step_eager, "eager_rows_per_s": rows / (ms_step_eager * 1e-3),
+             "deterministic": {"ms_per_step": ms_step_det, "rows_per_s": rows / (ms_step_det * 1e-3),
+                               "kernel_4b_ms": float(np.mean(k4d))},
              "family_ms": dict(zip(["2B", "3B", "4B"], dev.detail()[0])),
              "clusters": dict(zip(["pairs", "triplets", "quadruplets"], ncl))}
     if not args.no_fit:

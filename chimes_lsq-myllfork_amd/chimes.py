@@ -127,6 +127,7 @@ def lib(path=None):
         opt("cg_force_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
         opt("cg_assembly_graphs", [C.c_void_p, C.c_int])
         opt("cg_assembly_global_above", [C.c_void_p, C.c_int])
+        opt("cg_assembly_deterministic", [C.c_void_p, C.c_int])
         opt("cg_assembly_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
@@ -471,6 +472,10 @@ class Device:
     def assembly_graphs(self, on):
         """Capture repeated-shape assemblies into HIP graphs (on by default) or keep them eager."""
         check(self.L.cg_assembly_graphs(self.h, 1 if on else 0), self.h)
+
+    def assembly_deterministic(self, on):
+        """Bitwise-reproducible assembly (waves add to the LDS tile in a fixed order)."""
+        check(self.L.cg_assembly_deterministic(self.h, 1 if on else 0), self.h)
 
     def assembly_global_above(self, natoms):
         """Accumulate frames with more than natoms atoms in A with global atomics (0: only those beyond

@@ -789,6 +789,7 @@ struct AsmArgs {
   int transform, fix_type, fcut_type, stress_mode;
   double deriv_const, smooth_dist, fcut_offset;
   int nb_scale;             // index into perm_scale (2, 3, 4)
+  int det;                  // waves add to the LDS tile in a fixed order (bitwise-reproducible A)
 };
 
 template <class T, int N> using vecN = T __attribute__((ext_vector_type(N)));
@@ -1377,7 +1378,8 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     const int j = threadIdx.x * R + rnd;
     // a wave whose first lane is past the bucket's end has no cluster in this
     // round (or any later one): it leaves the loop (there is no barrier in it)
-    if ((int)(threadIdx.x & ~63u) * R + rnd >= ncl) break;
+    // (in deterministic mode every wave stays for the barriers of each column)
+    if (!a.det && (int)(threadIdx.x & ~63u) * R + rnd >= ncl) break;
     const bool valid = j < ncl;
     // lanes past the bucket's end build the tables of its last cluster (all
     // registers then hold defined values without a clearing pass) and drop
@@ -1476,28 +1478,29 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
         for (int k = 0; k < NP; ++k) g[k] += dv[k] * (pre[k] * suf[k + 1]);
       }
       }
-      if (valid) {
-        // force on each canonical atom: sum over its pairs of +-g v / r
-        TP F[NB][3];
+      // force on each canonical atom: sum over its pairs of +-g v / r (lanes
+      // without a cluster compute it too and drop it)
+      TP F[NB][3];
 #pragma unroll
-        for (int s = 0; s < NB; ++s) F[s][0] = F[s][1] = F[s][2] = (TP)0;
-        {
-          int k = 0;
+      for (int s = 0; s < NB; ++s) F[s][0] = F[s][1] = F[s][2] = (TP)0;
+      {
+        int k = 0;
 #pragma unroll
-          for (int x = 0; x < NB; ++x)
+        for (int x = 0; x < NB; ++x)
 #pragma unroll
-            for (int y = x + 1; y < NB; ++y) {
-              const TP gk = g[k] * sc[k];
+          for (int y = x + 1; y < NB; ++y) {
+            const TP gk = g[k] * sc[k];
 #pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                const TP v = x == 0 ? dpos[y][c] : dpos[y][c] - dpos[x][c];
-                F[x][c] += gk * v;
-                F[y][c] -= gk * v;
-              }
-              ++k;
+            for (int c = 0; c < 3; ++c) {
+              const TP v = x == 0 ? dpos[y][c] : dpos[y][c] - dpos[x][c];
+              F[x][c] += gk * v;
+              F[y][c] -= gk * v;
             }
-        }
-        if constexpr (GACC) {
+            ++k;
+          }
+      }
+      if constexpr (GACC) {
+        if (valid) {
           T *Ag = reinterpret_cast<T *>(a.A);
           const long long colg = a.col_abs[ty] + t.col0 + cc;
 #pragma unroll
@@ -1519,8 +1522,10 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           }
           for (int k = 0; k < ner; ++k)
             unsafeAtomicAdd(&Ag[(f.row_off + nf + nsr + k) * a.ld + colg], (T)(double)(ps * e));
-          continue;
         }
+        continue;
+      }
+      auto emit = [&]() {
 #pragma unroll
         for (int s = 0; s < NB; ++s) {
           double *dst = acc + ((size_t)atom[s] * Wp + cc) * 3;
@@ -1543,6 +1548,18 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           }
         }
         lds_add(&se_col[6], (double)(ps * e));
+      };
+      if (a.det) {
+        // deterministic order: the waves add their column in turn (the lanes
+        // of one LDS atomic are served in a fixed order), so the tile, and
+        // A, are the same bits on every run
+        const int wv = threadIdx.x >> 6;
+        for (int q = 0; q < BS / 64; ++q) {
+          if (wv == q && valid) emit();
+          __syncthreads();
+        }
+      } else if (valid) {
+        emit();
       }
     }
   }
@@ -2100,6 +2117,7 @@ static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks,
   a.tasks = d_tasks;
   a.bounds = bounds;
   a.ntypes = F.ntypes;
+  a.det = c.asm_det ? 1 : 0;
   a.frames = d_frames;
   a.wrap = c.b_wrap.as<double>(0);
   a.refs = refs;
@@ -2765,7 +2783,7 @@ static void build_tasks(const Ctx &c, int w, const std::vector<DevFrame> &hf, co
 static std::string asm_shape_key(const Ctx &c, const Batch &B, const Caps &caps, int prec) {
   std::string k = "A|";
   auto add = [&](long long v) { k += std::to_string(v); k += ','; };
-  add(prec); add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
+  add(prec); add(c.asm_det); add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
   for (int w = 0; w < 3; ++w)
     for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) add(c.tile_cfg[w][cls]);
   for (const DevFrame &d : B.hf) {
@@ -3744,6 +3762,12 @@ cg_status cg_synchronize(cg_ctx *ctx) {
 cg_status cg_assembly_graphs(cg_ctx *ctx, int on) {
   if (!ctx) { cg::set_error("cg_assembly_graphs: bad argument"); return CG_ERR_ARG; }
   ctx->c.asm_graphs = on != 0;
+  return CG_OK;
+}
+
+cg_status cg_assembly_deterministic(cg_ctx *ctx, int on) {
+  if (!ctx) { cg::set_error("cg_assembly_deterministic: bad argument"); return CG_ERR_ARG; }
+  ctx->c.asm_det = on != 0;
   return CG_OK;
 }
 

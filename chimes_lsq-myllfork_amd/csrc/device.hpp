@@ -272,6 +272,7 @@ struct Ctx {
   // frames with more atoms than this accumulate in A with global atomics
   // (0: only frames whose force rows do not fit the LDS tile)
   int gacc_above = 0;
+  bool asm_det = false;                   // bitwise-reproducible assembly (cg_assembly_deterministic)
   double *h_atot = nullptr;               // pinned host copy of them (4 doubles)
   double *h_pin = nullptr;                // pinned host copy of results and totals
   size_t h_pin_cap = 0;

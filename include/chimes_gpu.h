@@ -301,6 +301,14 @@ cg_status cg_last_detail(cg_ctx *ctx, double *ms_family3, int64_t *clusters3);
  * (the frame loop of src/chimes_lsq.C:292-328 assembles one frame at a
  * time). */
 cg_status cg_assembly_graphs(cg_ctx *ctx, int on);
+/* Bitwise-reproducible assembly (off by default).  The assembly kernels add
+ * the contributions of a workgroup's waves to a shared LDS tile with FP64
+ * atomics, so two runs of one batch agree to rounding (about 1e-16 relative),
+ * not bit for bit.  on != 0 makes the waves add each column in a fixed order
+ * (a barrier per wave and column), so that A comes out the same bits on
+ * every run, at a cost in time; frames accumulated in A itself (above) stay
+ * order-dependent.  The reference sums serially (src/A_Matrix.C). */
+cg_status cg_assembly_deterministic(cg_ctx *ctx, int on);
 /* Frames whose force rows do not fit the LDS tile of the assembly kernels
  * (above about 6300 atoms) are accumulated straight into A with global
  * atomics, slower but without a size limit; A is then zeroed first and the

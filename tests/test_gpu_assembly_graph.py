@@ -99,3 +99,28 @@ def test_assembly_leaves_no_stale_entries(ch, synth, oracle, tmp_path):
     assert colscaled_error(A, Ao) < 1e-10
     assert np.all(A[Ao == 0] == 0)
     dev.close()
+
+
+@pytest.mark.parametrize("case", ["carbon", "chno"])
+def test_deterministic_assembly_is_bitwise_stable(ch, synth, oracle, tmp_path, case):
+    """With cg_assembly_deterministic every run of a batch gives the same bits
+    (repeated calls on one context, eager and replayed, and a fresh context),
+    and the oracle's A to 1e-10."""
+    print("case", case, flush=True)
+    gen = (synth.carbon(nframes=2, natoms=200, seed=64) if case == "carbon"
+           else synth.chno(nframes=4, nmin=60, nmax=900, seed=65))
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    runs = []
+    dev = ch.Device(s)
+    dev.assembly_deterministic(True)
+    for _ in range(4):
+        runs.append(dev.assemble(fr)[0])
+    assert dev.assembly_stats()[2] >= 1  # the last call replays the captured graph
+    other = ch.Device(s)
+    other.assembly_deterministic(True)
+    runs.append(other.assemble(fr)[0])
+    for A in runs[1:]:
+        assert np.array_equal(A, runs[0])
+    assert colscaled_error(runs[0], oracle.assemble(s, fr)) < 1e-10
+    dev.close()
+    other.close()
