@@ -3354,6 +3354,17 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
   scratch_setup(device);
   CG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (auto &st : c.aux) CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  // buffers grow stream-ordered on the context stream; the device's pool
+  // keeps freed blocks for reuse instead of returning them at every sync
+  for_each_dbuf(c, [&](DBuf &b) { b.s = c.stream; });
+  {
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
+  }
   CG_HIP(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
   for (auto &e : c.join) CG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto &e : c.ev) CG_HIP(hipEventCreate(&e));
@@ -3377,7 +3388,9 @@ void ctx_init(Ctx &c, const cg_model *m, int device) {
 }
 
 void ctx_free(Ctx &c) {
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
   for_each_dbuf(c, [](DBuf &b) { b.release(); });
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
                     (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.recip, (void *)F.cct, (void *)F.csmax,

@@ -65,10 +65,15 @@ struct Arena {
 // Installed by the allocation-free entry point for the duration of its call.
 extern thread_local Arena *t_arena;
 
-// Grow-only device buffer.
+// Grow-only device buffer.  A buffer owned by a context grows through the
+// stream-ordered allocator on the context's stream (set at context creation):
+// moving it waits for nothing but that stream's earlier work, so a context
+// that grows does not stall the others sharing the GPU (hipFree waits for the
+// whole device).
 struct DBuf {
   void *p = nullptr;
   size_t cap = 0;
+  hipStream_t s = nullptr;  // stream of stream-ordered growth (null: hipMalloc / hipFree)
   void *get(size_t bytes) {
     if (t_arena) return t_arena->take(this, bytes);
     if (bytes > cap) {
@@ -76,19 +81,23 @@ struct DBuf {
       // geometric growth (at least twice the old capacity, at most twice the
       // request): a buffer that follows growing requests moves O(log) times
       const size_t old = cap;
-      if (p) (void)hipFree(p);
-      p = nullptr;
-      cap = 0;
+      free_now();
       ++t_grow_events;
       size_t n = bytes + bytes / 4 + 256;
       if (old > 0) n = std::max(n, std::min(2 * old, 2 * bytes + 256));
-      if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; cap = 0; return nullptr; }
+      const hipError_t e = s ? hipMallocAsync(&p, n, s) : hipMalloc(&p, n);
+      if (e != hipSuccess) { p = nullptr; cap = 0; return nullptr; }
       cap = n;
     }
     return p;
   }
   template <class T> T *as(size_t count) { return static_cast<T *>(get(count * sizeof(T))); }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  void free_now() {
+    if (p) (void)(s ? hipFreeAsync(p, s) : hipFree(p));
+    p = nullptr;
+    cap = 0;
+  }
+  void release() { free_now(); }
 };
 
 // Per-frame geometry handed to the kernels (computed on the host exactly as
