@@ -3204,7 +3204,9 @@ static void force_batch_impl(Ctx &c, const double *x_host, const cg_frame *frame
   if (!d_F) throw Error(CG_ERR_MEMORY, "force buffers");
   if (!noalloc && c.h_pin_cap < nout + 4) {
     const size_t old_cap = c.h_pin_cap;
-    if (c.h_pin) (void)hipHostFree(c.h_pin);
+    // the old block is freed with the context (hipHostFree would wait for
+    // the whole device, stalling other contexts' work)
+    if (c.h_pin) c.h_pin_retired.push_back(c.h_pin);
     c.h_pin = nullptr;
     c.h_pin_cap = 0;
     ++t_grow_events;
@@ -3402,6 +3404,8 @@ void ctx_free(Ctx &c) {
   if (c.stats.d_flat) (void)hipFree(c.stats.d_flat);
   c.stats = LsqStats{};
   if (c.h_pin) (void)hipHostFree(c.h_pin);
+  for (double *p : c.h_pin_retired) (void)hipHostFree(p);
+  c.h_pin_retired.clear();
   c.h_pin = nullptr;
   c.h_pin_cap = 0;
   if (c.h_atot) (void)hipHostFree(c.h_atot);

@@ -285,6 +285,7 @@ struct Ctx {
   double *h_atot = nullptr;               // pinned host copy of them (4 doubles)
   double *h_pin = nullptr;                // pinned host copy of results and totals
   size_t h_pin_cap = 0;
+  std::vector<double *> h_pin_retired;    // outgrown pinned blocks (freed with the context)
   bool capturing = false;
   DBuf b_coef[3];                         // per-family coefficient tensors (force evaluation)
   int coef_size[3] = {0, 0, 0};           // tensor entries per cluster type

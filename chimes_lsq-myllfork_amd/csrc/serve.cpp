@@ -81,6 +81,10 @@ struct Server {
   // captured into a new graph, calls repeated with exact list sizes, and
   // buffer moves (allocations)
   int64_t n_replay = 0, n_capture = 0, n_repeat = 0, n_grow = 0;
+  // largest neighbour / cluster counts per atom any worker has seen: every
+  // worker sizes its lists from them, so that a configuration one worker has
+  // met is not repeated with exact sizes on another
+  double rate_nb = 0.0, rate_cl[3] = {0.0, 0.0, 0.0};
   std::vector<std::thread> workers;
   std::string init_error;
   int ready = 0;
@@ -156,6 +160,8 @@ struct Server {
           queue.pop_front();
         }
         inflight += bytes;
+        ctx.rate_nb = std::max(ctx.rate_nb, rate_nb);
+        for (int w = 0; w < 3; ++w) ctx.rate_cl[w] = std::max(ctx.rate_cl[w], rate_cl[w]);
       }
       // one pass over the whole batch; if it fails, each request is
       // evaluated alone so that only the requests that fail by themselves
@@ -203,6 +209,8 @@ struct Server {
         n_capture += ctx.n_capture - c0;
         n_repeat += ctx.n_repeat - r0;
         n_grow += (int64_t)(cg::t_grow_events - a0);
+        rate_nb = std::max(rate_nb, ctx.rate_nb);
+        for (int w = 0; w < 3; ++w) rate_cl[w] = std::max(rate_cl[w], ctx.rate_cl[w]);
         inflight -= bytes;
         ++n_batches;
         n_requests += (int64_t)batch.size();
