@@ -37,7 +37,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int GT = 64;      // Gram output tile (GT x GT per workgroup)
-constexpr int GK = 16;      // rows staged per LDS step (FP64 path)
+constexpr int GK = 32;      // rows staged per LDS step (FP64 path)
 constexpr int GK32 = 32;    // rows staged per LDS step (BF16 path)
 
 // Upper-triangular tile pair index -> (ti, tj) with ti <= tj.
@@ -69,20 +69,37 @@ __global__ __launch_bounds__(256) void k_gram_f64(const T *__restrict__ A, long 
   d4 acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = (d4)0.0;
-  for (long long k = k0; k < k1; k += GK) {
-    // stage GK rows of the I and J column panels
-    for (int e = threadIdx.x; e < GK * GT; e += 256) {
-      int r = e / GT, c = e % GT;
-      long long row = k + r;
-      double vi = 0.0, vj = 0.0;
+  // each thread stages PER elements of each panel per slab of GK rows; the
+  // next slab is loaded into registers while the current one is multiplied
+  // (an MFMA slab takes longer than the loads' latency)
+  // (FP32 entries are widened when they go to LDS, so that the loads'
+  // results are not waited for at the load)
+  constexpr int PER = GK * GT / 256;
+  T ri[PER], rj[PER];
+  auto load = [&](long long k) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x + q * 256;
+      const int r = e / GT, c = e % GT;
+      const long long row = k + r;
+      ri[q] = (T)0;
+      rj[q] = (T)0;
       if (row < k1) {
-        if (i0 + c < n) vi = (double)A[row * ld + i0 + c];
-        if (j0 + c < n) vj = (double)A[row * ld + j0 + c];
+        if (i0 + c < n) ri[q] = A[row * ld + i0 + c];
+        if (j0 + c < n) rj[q] = A[row * ld + j0 + c];
       }
-      sI[r][c] = vi;
-      sJ[r][c] = vj;
+    }
+  };
+  if (k0 < k1) load(k0);
+  for (long long k = k0; k < k1; k += GK) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x + q * 256;
+      sI[e / GT][e % GT] = (double)ri[q];
+      sJ[e / GT][e % GT] = (double)rj[q];
     }
     __syncthreads();
+    if (k + GK < k1) load(k + GK);
 #pragma unroll
     for (int kk = 0; kk < GK; kk += 4) {
       const int kr = kk + lane / 16;
