@@ -226,8 +226,17 @@ __global__ __launch_bounds__(256) void k_residual(const T *__restrict__ A, long 
   int lane = threadIdx.x % 64;
   if (row >= m) return;
   const T *a = A + row * ld;
-  double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += (double)a[j] * x[j];
+  // four independent partial sums keep four row loads in flight per lane
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int j = lane;
+  for (; j + 192 < n; j += 256) {
+    s0 += (double)a[j] * x[j];
+    s1 += (double)a[j + 64] * x[j + 64];
+    s2 += (double)a[j + 128] * x[j + 128];
+    s3 += (double)a[j + 192] * x[j + 192];
+  }
+  for (; j < n; j += 64) s0 += (double)a[j] * x[j];
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if (lane == 0) r[row] = (b ? b[row] : 0.0) - s;
 }
@@ -239,9 +248,17 @@ __global__ __launch_bounds__(256) void k_atv(const T *__restrict__ A, long long 
   int j = blockIdx.x * 256 + threadIdx.x;
   long long r0 = (long long)blockIdx.y * slab, r1 = min(m, r0 + slab);
   if (j >= n) return;
-  double s = 0.0;
-  for (long long r = r0; r < r1; ++r) s += (double)A[r * ld + j] * v[r];
-  atomicAdd(&g[j], s);
+  // four rows per step with independent partial sums (four loads in flight)
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  long long r = r0;
+  for (; r + 3 < r1; r += 4) {
+    s0 += (double)A[r * ld + j] * v[r];
+    s1 += (double)A[(r + 1) * ld + j] * v[r + 1];
+    s2 += (double)A[(r + 2) * ld + j] * v[r + 2];
+    s3 += (double)A[(r + 3) * ld + j] * v[r + 3];
+  }
+  for (; r < r1; ++r) s0 += (double)A[r * ld + j] * v[r];
+  atomicAdd(&g[j], (s0 + s1) + (s2 + s3));
 }
 
 __global__ void k_scale(double *v, const double *w, int n) {
@@ -310,7 +327,8 @@ static FitState &fit_state(Ctx &c) {
 template <class T>
 static void atv(Ctx &c, const T *A, long long m, int n, const double *v, double *g) {
   CG_HIP(hipMemsetAsync(g, 0, n * sizeof(double), c.stream));
-  long long slab = 2048;
+  // slabs small enough for several workgroups per CU
+  long long slab = 256;
   long long nslab = (m + slab - 1) / slab;
   hipLaunchKernelGGL(k_atv<T>, dim3((n + 255) / 256, (unsigned)nslab), dim3(256), 0, c.stream, A, (long long)c.A_ld,
                      m, n, v, slab, g);
