@@ -914,13 +914,21 @@ __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double sm
   }
 }
 
-// 4-body FP64 tables for the hand-scheduled set loop: F and D of the six
-// pairs in four 32-dword register blocks held in fixed registers
-// v[128:255] (F of pairs 0-2, F of pairs 3-5, D of pairs 0-2, D of pairs
+// 4-body FP64 tables for the hand-scheduled set loop: F and Q = D / F of the
+// six pairs in four 32-dword register blocks held in fixed registers
+// v[128:255] (F of pairs 0-2, F of pairs 3-5, Q of pairs 0-2, Q of pairs
 // 3-5; pair slot of 10 dwords, entry of power p at 2p).  A wave-uniform
 // power selects an entry through GPR-index mode applied to src0 of the
 // arithmetic itself, so a table read costs no instruction of its own.
+//
+// With Q the set loop needs only the product P = prod_k F_k of a set:
+// E += P and G_k += Q_k P (= D_k prod_{j != k} F_j).  Multiplication and
+// division keep relative errors, so Q_k P carries a few roundings more than
+// the direct product however small F_k is; F entries below 1e-100 in
+// magnitude (T_n(x) = 0 at a Chebyshev node) are held at 1e-100, which keeps
+// Q finite and changes E by at most 1e-100 times the other factors.
 typedef vecN<uint32_t, 32> v32u;
+constexpr double kTabFmin = 1e-100;
 struct Tab4 {
   v32u FA, FB, DA, DB;
   __device__ __forceinline__ void zero() { FA = 0u; FB = 0u; DA = 0u; DB = 0u; }
@@ -929,106 +937,155 @@ struct Tab4Pair {
   Tab4 &t;
   int k;
   __device__ __forceinline__ void set(int i, double f, double d) {
-    const uint64_t fb = __builtin_bit_cast(uint64_t, f), db = __builtin_bit_cast(uint64_t, d);
+    const double fs = fabs(f) < kTabFmin ? kTabFmin : f;
+    const double q = d / fs;
+    const uint64_t fb = __builtin_bit_cast(uint64_t, fs), qb = __builtin_bit_cast(uint64_t, q);
     const int o = 10 * (k % 3) + 2 * i;
     if (k < 3) {
       t.FA[o] = (uint32_t)fb; t.FA[o + 1] = (uint32_t)(fb >> 32);
-      t.DA[o] = (uint32_t)db; t.DA[o + 1] = (uint32_t)(db >> 32);
+      t.DA[o] = (uint32_t)qb; t.DA[o + 1] = (uint32_t)(qb >> 32);
     } else {
       t.FB[o] = (uint32_t)fb; t.FB[o + 1] = (uint32_t)(fb >> 32);
-      t.DB[o] = (uint32_t)db; t.DB[o + 1] = (uint32_t)(db >> 32);
+      t.DB[o] = (uint32_t)qb; t.DB[o + 1] = (uint32_t)(qb >> 32);
     }
   }
 };
 
-// One allowed set (indices i0..i5 = 2 * power of each pair in SGPRs):
-// E += prod_k F_k, G_k += D_k prod_{j != k} F_j, as L = F0 F1 F2 and
-// R = F3 F4 F5 with their partial derivatives, each pair's index region
-// entered once.  F_k sits at v[128 + 10k] (k < 3) / v[160 + 10(k-3)], D_k
-// 64 registers higher.
+// One allowed set (indices i0..i5 = 2 * power of each pair in SGPRs, index
+// mode on): the product runs through the six pair regions, E and G_5 are
+// added in the last one, and G_4 .. G_0 walking the regions back, so a set
+// takes 11 index changes and 13 VALU.  F_k sits at v[128 + 10k] (k < 3) /
+// v[160 + 10(k-3)], Q_k 64 registers higher.
 #define CG_SET4(i0, i1, i2, i3, i4, i5)                      \
-  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
-  "v_mov_b64 %[X0], v[128:129]\n\t"                          \
-  "v_mov_b64 %[Y0], v[192:193]\n\t"                          \
-  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
-  "v_mov_b64 %[X5], v[180:181]\n\t"                          \
-  "v_mov_b64 %[Y5], v[244:245]\n\t"                          \
+  "s_set_gpr_idx_idx " #i0 "\n\t"                            \
+  "v_mov_b64 %[X], v[128:129]\n\t"                           \
   "s_set_gpr_idx_idx " #i1 "\n\t"                            \
-  "v_mul_f64 %[M], v[202:203], %[X0]\n\t"                    \
-  "v_mul_f64 %[N], v[138:139], %[Y0]\n\t"                    \
-  "v_mul_f64 %[L], v[138:139], %[X0]\n\t"                    \
-  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
-  "v_mul_f64 %[M4], v[234:235], %[X5]\n\t"                   \
-  "v_mul_f64 %[N4], v[170:171], %[Y5]\n\t"                   \
-  "v_mul_f64 %[R], v[170:171], %[X5]\n\t"                    \
+  "v_mul_f64 %[X], v[138:139], %[X]\n\t"                     \
   "s_set_gpr_idx_idx " #i2 "\n\t"                            \
-  "v_mul_f64 %[DL2], v[212:213], %[L]\n\t"                   \
-  "v_mul_f64 %[L], v[148:149], %[L]\n\t"                     \
-  "v_mul_f64 %[M], v[148:149], %[M]\n\t"                     \
-  "v_mul_f64 %[N], v[148:149], %[N]\n\t"                     \
+  "v_mul_f64 %[X], v[148:149], %[X]\n\t"                     \
   "s_set_gpr_idx_idx " #i3 "\n\t"                            \
-  "v_mul_f64 %[DR3], v[224:225], %[R]\n\t"                   \
-  "v_mul_f64 %[R], v[160:161], %[R]\n\t"                     \
-  "v_mul_f64 %[M4], v[160:161], %[M4]\n\t"                   \
-  "v_mul_f64 %[N4], v[160:161], %[N4]\n\t"                   \
-  "s_set_gpr_idx_off\n\t"                                    \
-  "v_fmac_f64 %[E], %[L], %[R]\n\t"                          \
-  "v_fmac_f64 %[G0], %[N], %[R]\n\t"                         \
-  "v_fmac_f64 %[G1], %[M], %[R]\n\t"                         \
-  "v_fmac_f64 %[G2], %[DL2], %[R]\n\t"                       \
-  "v_fmac_f64 %[G3], %[DR3], %[L]\n\t"                       \
-  "v_fmac_f64 %[G4], %[M4], %[L]\n\t"                        \
-  "v_fmac_f64 %[G5], %[N4], %[L]\n\t"
+  "v_mul_f64 %[X], v[160:161], %[X]\n\t"                     \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_mul_f64 %[X], v[170:171], %[X]\n\t"                     \
+  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
+  "v_mul_f64 %[P], v[180:181], %[X]\n\t"                     \
+  "v_fma_f64 %[E], 1.0, %[P], %[E]\n\t"                      \
+  "v_fmac_f64 %[G5], v[244:245], %[P]\n\t"                   \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_fmac_f64 %[G4], v[234:235], %[P]\n\t"                   \
+  "s_set_gpr_idx_idx " #i3 "\n\t"                            \
+  "v_fmac_f64 %[G3], v[224:225], %[P]\n\t"                   \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_fmac_f64 %[G2], v[212:213], %[P]\n\t"                   \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_fmac_f64 %[G1], v[202:203], %[P]\n\t"                   \
+  "s_set_gpr_idx_idx " #i0 "\n\t"                            \
+  "v_fmac_f64 %[G0], v[192:193], %[P]\n\t"
 
-// Sum over the allowed sets of one column: cp points at the column's entry
-// of colptr (first and end set), records are 32 bytes at sets8 = (lo, hi);
-// the two records after a column's last are read ahead and must exist.  Set
-// records are loaded one ahead into alternating SGPR blocks.
+// Two sets a, b interleaved region by region: independent product chains,
+// then the sums in an order that puts another instruction between the two
+// updates of each accumulator.
+#define CG_SET4X2(a0, a1, a2, a3, a4, a5, b0, b1, b2, b3, b4, b5)                   \
+  "s_set_gpr_idx_idx " #a0 "\n\t" "v_mov_b64 %[X], v[128:129]\n\t"                 \
+  "s_set_gpr_idx_idx " #b0 "\n\t" "v_mov_b64 %[XB], v[128:129]\n\t"                \
+  "s_set_gpr_idx_idx " #a1 "\n\t" "v_mul_f64 %[X], v[138:139], %[X]\n\t"           \
+  "s_set_gpr_idx_idx " #b1 "\n\t" "v_mul_f64 %[XB], v[138:139], %[XB]\n\t"         \
+  "s_set_gpr_idx_idx " #a2 "\n\t" "v_mul_f64 %[X], v[148:149], %[X]\n\t"           \
+  "s_set_gpr_idx_idx " #b2 "\n\t" "v_mul_f64 %[XB], v[148:149], %[XB]\n\t"         \
+  "s_set_gpr_idx_idx " #a3 "\n\t" "v_mul_f64 %[X], v[160:161], %[X]\n\t"           \
+  "s_set_gpr_idx_idx " #b3 "\n\t" "v_mul_f64 %[XB], v[160:161], %[XB]\n\t"         \
+  "s_set_gpr_idx_idx " #a4 "\n\t" "v_mul_f64 %[X], v[170:171], %[X]\n\t"           \
+  "s_set_gpr_idx_idx " #b4 "\n\t" "v_mul_f64 %[XB], v[170:171], %[XB]\n\t"         \
+  "s_set_gpr_idx_idx " #a5 "\n\t" "v_mul_f64 %[P], v[180:181], %[X]\n\t"           \
+  "s_set_gpr_idx_idx " #b5 "\n\t" "v_mul_f64 %[PB], v[180:181], %[XB]\n\t"         \
+  "v_fmac_f64 %[G5], v[244:245], %[PB]\n\t"                                 \
+  "v_fma_f64 %[E], 1.0, %[P], %[E]\n\t"                                     \
+  "s_set_gpr_idx_idx " #a4 "\n\t" "v_fmac_f64 %[G4], v[234:235], %[P]\n\t"  \
+  "s_set_gpr_idx_idx " #a5 "\n\t" "v_fmac_f64 %[G5], v[244:245], %[P]\n\t"  \
+  "v_fma_f64 %[E], 1.0, %[PB], %[E]\n\t"                                    \
+  "s_set_gpr_idx_idx " #b4 "\n\t" "v_fmac_f64 %[G4], v[234:235], %[PB]\n\t" \
+  "s_set_gpr_idx_idx " #b3 "\n\t" "v_fmac_f64 %[G3], v[224:225], %[PB]\n\t" \
+  "s_set_gpr_idx_idx " #a2 "\n\t" "v_fmac_f64 %[G2], v[212:213], %[P]\n\t"  \
+  "s_set_gpr_idx_idx " #a3 "\n\t" "v_fmac_f64 %[G3], v[224:225], %[P]\n\t"  \
+  "s_set_gpr_idx_idx " #b2 "\n\t" "v_fmac_f64 %[G2], v[212:213], %[PB]\n\t" \
+  "s_set_gpr_idx_idx " #b1 "\n\t" "v_fmac_f64 %[G1], v[202:203], %[PB]\n\t" \
+  "s_set_gpr_idx_idx " #a0 "\n\t" "v_fmac_f64 %[G0], v[192:193], %[P]\n\t"  \
+  "s_set_gpr_idx_idx " #a1 "\n\t" "v_fmac_f64 %[G1], v[202:203], %[P]\n\t"  \
+  "s_set_gpr_idx_idx " #b0 "\n\t" "v_fmac_f64 %[G0], v[192:193], %[PB]\n\t"
+
+// Record stream of the 4-body set loops (FP64 and FP32): cp points at the
+// column's entry of colptr (first and end set), records are 32 bytes at
+// sets8 = (lo, hi).  Two sets per step, their records loaded one pair ahead
+// into alternating SGPR blocks s[40:55] / s[60:75]; an odd last set alone.
+// The loop reads at most two records past a column's last, which must exist.
+// SET1 / SET2 name the one-set and two-set macros.
+#define CG_LOOP4(SET1, SET2, TAG)                                                        \
+  "s_mov_b32 s59, m0\n\t"                                                                \
+  "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"                                              \
+  "s_waitcnt lgkmcnt(0)\n\t"                                                             \
+  "s_sub_i32 s58, s41, s40\n\t"                                                          \
+  "s_cmp_lt_i32 s58, 1\n\t"                                                              \
+  "s_cbranch_scc1 L_" TAG "_done_%=\n\t"                                                 \
+  "s_mov_b32 s41, 0\n\t"                                                                 \
+  "s_lshl_b64 s[40:41], s[40:41], 5\n\t"                                                 \
+  "s_add_u32 s56, s40, %[lo]\n\t"                                                        \
+  "s_addc_u32 s57, s41, %[hi]\n\t"                                                       \
+  "s_load_dwordx16 s[40:55], s[56:57], 0x0\n\t"                                          \
+  "s_waitcnt lgkmcnt(0)\n\t"                                                             \
+  "s_set_gpr_idx_on s40, gpr_idx(SRC0)\n"                                                \
+  "L_" TAG "_loop_%=:\n\t"                                                               \
+  "s_cmp_lt_i32 s58, 2\n\t"                                                              \
+  "s_cbranch_scc1 L_" TAG "_tail_a_%=\n\t"                                               \
+  "s_load_dwordx16 s[60:75], s[56:57], 0x40\n\t"                                         \
+  SET2(s40, s41, s42, s43, s44, s45, s48, s49, s50, s51, s52, s53)                       \
+  "s_add_u32 s56, s56, 0x40\n\t"                                                         \
+  "s_addc_u32 s57, s57, 0\n\t"                                                           \
+  "s_add_i32 s58, s58, -2\n\t"                                                           \
+  "s_waitcnt lgkmcnt(0)\n\t"                                                             \
+  "s_cmp_lt_i32 s58, 2\n\t"                                                              \
+  "s_cbranch_scc1 L_" TAG "_tail_b_%=\n\t"                                               \
+  "s_load_dwordx16 s[40:55], s[56:57], 0x40\n\t"                                         \
+  SET2(s60, s61, s62, s63, s64, s65, s68, s69, s70, s71, s72, s73)                       \
+  "s_add_u32 s56, s56, 0x40\n\t"                                                         \
+  "s_addc_u32 s57, s57, 0\n\t"                                                           \
+  "s_add_i32 s58, s58, -2\n\t"                                                           \
+  "s_waitcnt lgkmcnt(0)\n\t"                                                             \
+  "s_branch L_" TAG "_loop_%=\n"                                                         \
+  "L_" TAG "_tail_b_%=:\n\t"                                                             \
+  "s_cmp_eq_u32 s58, 0\n\t"                                                              \
+  "s_cbranch_scc1 L_" TAG "_off_%=\n\t"                                                  \
+  SET1(s60, s61, s62, s63, s64, s65)                                                     \
+  "s_branch L_" TAG "_off_%=\n"                                                          \
+  "L_" TAG "_tail_a_%=:\n\t"                                                             \
+  "s_cmp_eq_u32 s58, 0\n\t"                                                              \
+  "s_cbranch_scc1 L_" TAG "_off_%=\n\t"                                                  \
+  SET1(s40, s41, s42, s43, s44, s45)                                                     \
+  "L_" TAG "_off_%=:\n\t"                                                                \
+  "s_set_gpr_idx_off\n"                                                                  \
+  "L_" TAG "_done_%=:\n\t"                                                               \
+  "s_waitcnt lgkmcnt(0)\n\t"                                                             \
+  "s_mov_b32 m0, s59\n\t"
+
+#define CG_LOOP4_CLOBBERS                                                                                      \
+  "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s55", \
+      "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70",    \
+      "s71", "s72", "s73", "s74", "s75", "scc"
+
+// Sum over the allowed sets of one column (FP64 tables and sums).
 __device__ __forceinline__ void set_loop4(const Tab4 &t, const int *cp, uint32_t lo, uint32_t hi, double &E,
                                           double *G) {
-  double X0, Y0, X5, Y5, L, M, N, DL2, R, M4, N4, DR3;
+  double X, P, XB, PB;
   double G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
-  asm volatile(
-      "s_mov_b32 s59, m0\n\t"
-      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_sub_i32 s58, s41, s40\n\t"
-      "s_cmp_lt_i32 s58, 1\n\t"
-      "s_cbranch_scc1 L_set4_done_%=\n\t"
-      "s_mov_b32 s41, 0\n\t"
-      "s_lshl_b64 s[40:41], s[40:41], 5\n\t"
-      "s_add_u32 s56, s40, %[lo]\n\t"
-      "s_addc_u32 s57, s41, %[hi]\n\t"
-      "s_load_dwordx8 s[40:47], s[56:57], 0x0\n\t"
-      "s_waitcnt lgkmcnt(0)\n"
-      "L_set4_%=:\n\t"
-      "s_load_dwordx8 s[48:55], s[56:57], 0x20\n\t"
-      CG_SET4(s40, s41, s42, s43, s44, s45)
-      "s_add_i32 s58, s58, -1\n\t"
-      "s_cmp_eq_u32 s58, 0\n\t"
-      "s_cbranch_scc1 L_set4_done_%=\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_load_dwordx8 s[40:47], s[56:57], 0x40\n\t"
-      CG_SET4(s48, s49, s50, s51, s52, s53)
-      "s_add_u32 s56, s56, 0x40\n\t"
-      "s_addc_u32 s57, s57, 0\n\t"
-      "s_add_i32 s58, s58, -1\n\t"
-      "s_cmp_eq_u32 s58, 0\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_cbranch_scc0 L_set4_%=\n"
-      "L_set4_done_%=:\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_mov_b32 m0, s59\n\t"
+  asm volatile(CG_LOOP4(CG_SET4, CG_SET4X2, "set4")
       : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
-        [X0] "=&v"(X0), [Y0] "=&v"(Y0), [X5] "=&v"(X5), [Y5] "=&v"(Y5), [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N),
-        [DL2] "=&v"(DL2), [R] "=&v"(R), [M4] "=&v"(M4), [N4] "=&v"(N4), [DR3] "=&v"(DR3)
+        [X] "=&v"(X), [P] "=&v"(P), [XB] "=&v"(XB), [PB] "=&v"(PB)
       : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[128:159]}"(t.FA), "{v[160:191]}"(t.FB), "{v[192:223]}"(t.DA),
         "{v[224:255]}"(t.DB)
-      : "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s55",
-        "s56", "s57", "s58", "s59", "scc");
+      : CG_LOOP4_CLOBBERS);
   G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
 }
 #undef CG_SET4
+#undef CG_SET4X2
 
 // 3-body FP64 tables for the hand-scheduled set loop (powers below 9): F
 // of pair k at v[128 + 18k], D at v[182 + 18k], entry of power p at 2p, in
@@ -1119,9 +1176,9 @@ __device__ __forceinline__ void set_loop3(const Tab3 &t, const int *cp, uint32_t
 #undef CG_SET3
 
 // FP32 tables of the FP32-A path (4-body powers below 5, 3-body below 9):
-// F and D of power p of a pair are one 64-bit entry (F low, D high) at offset
-// 2p of the pair's slot, so the FP64 set records (2 * power) select entries
-// unchanged and one 64-bit move reads both.  4-body: pair k at v[64 + 10k]
+// F and D (4-body: Q = D / F) of power p of a pair are one 64-bit entry (F
+// low) at offset 2p of the pair's slot, so the FP64 set records (2 * power)
+// select entries unchanged.  4-body: pair k at v[64 + 10k]
 // (k < 3) / v[96 + 10(k-3)]; 3-body: pair k at v[64 + 18k].  The tables sit
 // in v[64:127].
 struct Tab4f {
@@ -1131,12 +1188,15 @@ struct Tab4f {
 struct Tab4fPair {
   Tab4f &t;
   int k;
+  // F and Q = D / F as the FP64 tables (F held at >= 1e-30 in magnitude)
   __device__ __forceinline__ void set(int i, float f, float d) {
+    const float fs = fabsf(f) < 1e-30f ? 1e-30f : f;
+    const float q = d * __builtin_amdgcn_rcpf(fs);
     const int o = 10 * (k % 3) + 2 * i;
     if (k < 3) {
-      t.A[o] = __builtin_bit_cast(uint32_t, f); t.A[o + 1] = __builtin_bit_cast(uint32_t, d);
+      t.A[o] = __builtin_bit_cast(uint32_t, fs); t.A[o + 1] = __builtin_bit_cast(uint32_t, q);
     } else {
-      t.B[o] = __builtin_bit_cast(uint32_t, f); t.B[o + 1] = __builtin_bit_cast(uint32_t, d);
+      t.B[o] = __builtin_bit_cast(uint32_t, fs); t.B[o + 1] = __builtin_bit_cast(uint32_t, q);
     }
   }
 };
@@ -1157,85 +1217,78 @@ struct Tab3fPair {
   }
 };
 
-// One allowed 4-body set in FP32, the products of CG_SET4: (F0, D0) and
-// (F5, D5) are copied to v[60:63], the other pairs are read in place.
+// One allowed 4-body set in FP32, as the FP64 CG_SET4 (F_k at the even,
+// Q_k = D_k / F_k at the odd register of an entry).
 #define CG_SET4F(i0, i1, i2, i3, i4, i5)                     \
-  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
-  "v_mov_b64 v[60:61], v[64:65]\n\t"                         \
-  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
-  "v_mov_b64 v[62:63], v[116:117]\n\t"                       \
+  "s_set_gpr_idx_idx " #i0 "\n\t"                            \
+  "v_mov_b32 %[X], v64\n\t"                                  \
   "s_set_gpr_idx_idx " #i1 "\n\t"                            \
-  "v_mul_f32 %[M], v75, v60\n\t"                             \
-  "v_mul_f32 %[N], v74, v61\n\t"                             \
-  "v_mul_f32 %[L], v74, v60\n\t"                             \
-  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
-  "v_mul_f32 %[M4], v107, v62\n\t"                           \
-  "v_mul_f32 %[N4], v106, v63\n\t"                           \
-  "v_mul_f32 %[R], v106, v62\n\t"                            \
+  "v_mul_f32 %[X], v74, %[X]\n\t"                            \
   "s_set_gpr_idx_idx " #i2 "\n\t"                            \
-  "v_mul_f32 %[DL2], v85, %[L]\n\t"                          \
-  "v_mul_f32 %[L], v84, %[L]\n\t"                            \
-  "v_mul_f32 %[M], v84, %[M]\n\t"                            \
-  "v_mul_f32 %[N], v84, %[N]\n\t"                            \
+  "v_mul_f32 %[X], v84, %[X]\n\t"                            \
   "s_set_gpr_idx_idx " #i3 "\n\t"                            \
-  "v_mul_f32 %[DR3], v97, %[R]\n\t"                          \
-  "v_mul_f32 %[R], v96, %[R]\n\t"                            \
-  "v_mul_f32 %[M4], v96, %[M4]\n\t"                          \
-  "v_mul_f32 %[N4], v96, %[N4]\n\t"                          \
-  "s_set_gpr_idx_off\n\t"                                    \
-  "v_fmac_f32 %[E], %[L], %[R]\n\t"                          \
-  "v_fmac_f32 %[G0], %[N], %[R]\n\t"                         \
-  "v_fmac_f32 %[G1], %[M], %[R]\n\t"                         \
-  "v_fmac_f32 %[G2], %[DL2], %[R]\n\t"                       \
-  "v_fmac_f32 %[G3], %[DR3], %[L]\n\t"                       \
-  "v_fmac_f32 %[G4], %[M4], %[L]\n\t"                        \
-  "v_fmac_f32 %[G5], %[N4], %[L]\n\t"
+  "v_mul_f32 %[X], v96, %[X]\n\t"                            \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_mul_f32 %[X], v106, %[X]\n\t"                           \
+  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
+  "v_mul_f32 %[P], v116, %[X]\n\t"                           \
+  "v_fmac_f32 %[E], 1.0, %[P]\n\t"                           \
+  "v_fmac_f32 %[G5], v117, %[P]\n\t"                         \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_fmac_f32 %[G4], v107, %[P]\n\t"                         \
+  "s_set_gpr_idx_idx " #i3 "\n\t"                            \
+  "v_fmac_f32 %[G3], v97, %[P]\n\t"                          \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_fmac_f32 %[G2], v85, %[P]\n\t"                          \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_fmac_f32 %[G1], v75, %[P]\n\t"                          \
+  "s_set_gpr_idx_idx " #i0 "\n\t"                            \
+  "v_fmac_f32 %[G0], v65, %[P]\n\t"
+
+#define CG_SET4FX2(a0, a1, a2, a3, a4, a5, b0, b1, b2, b3, b4, b5)                  \
+  "s_set_gpr_idx_idx " #a0 "\n\t" "v_mov_b32 %[X], v64\n\t"                        \
+  "s_set_gpr_idx_idx " #b0 "\n\t" "v_mov_b32 %[XB], v64\n\t"                       \
+  "s_set_gpr_idx_idx " #a1 "\n\t" "v_mul_f32 %[X], v74, %[X]\n\t"                  \
+  "s_set_gpr_idx_idx " #b1 "\n\t" "v_mul_f32 %[XB], v74, %[XB]\n\t"                \
+  "s_set_gpr_idx_idx " #a2 "\n\t" "v_mul_f32 %[X], v84, %[X]\n\t"                  \
+  "s_set_gpr_idx_idx " #b2 "\n\t" "v_mul_f32 %[XB], v84, %[XB]\n\t"                \
+  "s_set_gpr_idx_idx " #a3 "\n\t" "v_mul_f32 %[X], v96, %[X]\n\t"                  \
+  "s_set_gpr_idx_idx " #b3 "\n\t" "v_mul_f32 %[XB], v96, %[XB]\n\t"                \
+  "s_set_gpr_idx_idx " #a4 "\n\t" "v_mul_f32 %[X], v106, %[X]\n\t"                 \
+  "s_set_gpr_idx_idx " #b4 "\n\t" "v_mul_f32 %[XB], v106, %[XB]\n\t"               \
+  "s_set_gpr_idx_idx " #a5 "\n\t" "v_mul_f32 %[P], v116, %[X]\n\t"                 \
+  "s_set_gpr_idx_idx " #b5 "\n\t" "v_mul_f32 %[PB], v116, %[XB]\n\t"               \
+  "v_fmac_f32 %[G5], v117, %[PB]\n\t"                                 \
+  "v_fmac_f32 %[E], 1.0, %[P]\n\t"                                    \
+  "s_set_gpr_idx_idx " #a4 "\n\t" "v_fmac_f32 %[G4], v107, %[P]\n\t"  \
+  "s_set_gpr_idx_idx " #a5 "\n\t" "v_fmac_f32 %[G5], v117, %[P]\n\t"  \
+  "v_fmac_f32 %[E], 1.0, %[PB]\n\t"                                   \
+  "s_set_gpr_idx_idx " #b4 "\n\t" "v_fmac_f32 %[G4], v107, %[PB]\n\t" \
+  "s_set_gpr_idx_idx " #b3 "\n\t" "v_fmac_f32 %[G3], v97, %[PB]\n\t"  \
+  "s_set_gpr_idx_idx " #a2 "\n\t" "v_fmac_f32 %[G2], v85, %[P]\n\t"   \
+  "s_set_gpr_idx_idx " #a3 "\n\t" "v_fmac_f32 %[G3], v97, %[P]\n\t"   \
+  "s_set_gpr_idx_idx " #b2 "\n\t" "v_fmac_f32 %[G2], v85, %[PB]\n\t"  \
+  "s_set_gpr_idx_idx " #b1 "\n\t" "v_fmac_f32 %[G1], v75, %[PB]\n\t"  \
+  "s_set_gpr_idx_idx " #a0 "\n\t" "v_fmac_f32 %[G0], v65, %[P]\n\t"   \
+  "s_set_gpr_idx_idx " #a1 "\n\t" "v_fmac_f32 %[G1], v75, %[P]\n\t"   \
+  "s_set_gpr_idx_idx " #b0 "\n\t" "v_fmac_f32 %[G0], v65, %[PB]\n\t"
 
 // set_loop4 with FP32 tables and sums (same record stream and read-ahead).
 __device__ __forceinline__ void set_loop4f(const Tab4f &t, const int *cp, uint32_t lo, uint32_t hi, float &E,
                                            float *G) {
-  float L, M, N, DL2, R, M4, N4, DR3;
+  float X, P, XB, PB;
   float G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
-  asm volatile(
-      "s_mov_b32 s59, m0\n\t"
-      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_sub_i32 s58, s41, s40\n\t"
-      "s_cmp_lt_i32 s58, 1\n\t"
-      "s_cbranch_scc1 L_set4f_done_%=\n\t"
-      "s_mov_b32 s41, 0\n\t"
-      "s_lshl_b64 s[40:41], s[40:41], 5\n\t"
-      "s_add_u32 s56, s40, %[lo]\n\t"
-      "s_addc_u32 s57, s41, %[hi]\n\t"
-      "s_load_dwordx8 s[40:47], s[56:57], 0x0\n\t"
-      "s_waitcnt lgkmcnt(0)\n"
-      "L_set4f_%=:\n\t"
-      "s_load_dwordx8 s[48:55], s[56:57], 0x20\n\t"
-      CG_SET4F(s40, s41, s42, s43, s44, s45)
-      "s_add_i32 s58, s58, -1\n\t"
-      "s_cmp_eq_u32 s58, 0\n\t"
-      "s_cbranch_scc1 L_set4f_done_%=\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_load_dwordx8 s[40:47], s[56:57], 0x40\n\t"
-      CG_SET4F(s48, s49, s50, s51, s52, s53)
-      "s_add_u32 s56, s56, 0x40\n\t"
-      "s_addc_u32 s57, s57, 0\n\t"
-      "s_add_i32 s58, s58, -1\n\t"
-      "s_cmp_eq_u32 s58, 0\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_cbranch_scc0 L_set4f_%=\n"
-      "L_set4f_done_%=:\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_mov_b32 m0, s59\n\t"
+  asm volatile(CG_LOOP4(CG_SET4F, CG_SET4FX2, "set4f")
       : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
-        [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N), [DL2] "=&v"(DL2), [R] "=&v"(R), [M4] "=&v"(M4), [N4] "=&v"(N4),
-        [DR3] "=&v"(DR3)
+        [X] "=&v"(X), [P] "=&v"(P), [XB] "=&v"(XB), [PB] "=&v"(PB)
       : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[64:95]}"(t.A), "{v[96:127]}"(t.B)
-      : "v60", "v61", "v62", "v63", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50",
-        "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "scc");
+      : CG_LOOP4_CLOBBERS);
   G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
 }
 #undef CG_SET4F
+#undef CG_SET4FX2
+#undef CG_LOOP4
+#undef CG_LOOP4_CLOBBERS
 
 // One allowed 3-body set in FP32 (products of CG_SET3).
 #define CG_SET3F(i0, i1, i2)                                 \
