@@ -790,6 +790,7 @@ struct AsmArgs {
   double deriv_const, smooth_dist, fcut_offset;
   int nb_scale;             // index into perm_scale (2, 3, 4)
   int det;                  // waves add to the LDS tile in a fixed order (bitwise-reproducible A)
+  int qform;                // 4-body set loops in quotient form (F, Q = D / F tables)
 };
 
 template <class T, int N> using vecN = T __attribute__((ext_vector_type(N)));
@@ -914,21 +915,25 @@ __device__ __forceinline__ void pair_table(const AsmArgs &a, double r, double sm
   }
 }
 
-// 4-body FP64 tables for the hand-scheduled set loop: F and Q = D / F of the
-// six pairs in four 32-dword register blocks held in fixed registers
-// v[128:255] (F of pairs 0-2, F of pairs 3-5, Q of pairs 0-2, Q of pairs
-// 3-5; pair slot of 10 dwords, entry of power p at 2p).  A wave-uniform
-// power selects an entry through GPR-index mode applied to src0 of the
-// arithmetic itself, so a table read costs no instruction of its own.
+// 4-body FP64 tables for the hand-scheduled set loops: F and D (product
+// form) or F and Q = D / F (quotient form) of the six pairs in four 32-dword
+// register blocks held in fixed registers v[128:255] (F of pairs 0-2, F of
+// pairs 3-5, D or Q of pairs 0-2, D or Q of pairs 3-5; pair slot of 10
+// dwords, entry of power p at 2p).  A wave-uniform power selects an entry
+// through GPR-index mode applied to src0 of the arithmetic itself, so a table
+// read costs no instruction of its own.
 //
 // With Q the set loop needs only the product P = prod_k F_k of a set:
 // E += P and G_k += Q_k P (= D_k prod_{j != k} F_j).  Multiplication and
 // division keep relative errors, so Q_k P carries a few roundings more than
-// the direct product however small F_k is; F entries below 1e-100 in
-// magnitude (T_n(x) = 0 at a Chebyshev node) are held at 1e-100, which keeps
-// Q finite and changes E by at most 1e-100 times the other factors.
+// the direct product however small F_k is; F entries below 1e-60 in
+// magnitude (T_n(x) = 0 at a Chebyshev node) are held at 1e-60, which keeps
+// Q finite and changes E by at most 1e-60 times the other factors.  The
+// quotients of a pair come from one division (the reciprocal of the product
+// of its F entries, unwound entry by entry), which the bound keeps clear of
+// underflow for up to five entries.
 typedef vecN<uint32_t, 32> v32u;
-constexpr double kTabFmin = 1e-100;
+constexpr double kTabFmin = 1e-60;
 struct Tab4 {
   v32u FA, FB, DA, DB;
   __device__ __forceinline__ void zero() { FA = 0u; FB = 0u; DA = 0u; DB = 0u; }
@@ -936,27 +941,55 @@ struct Tab4 {
 struct Tab4Pair {
   Tab4 &t;
   int k;
+  bool q;  // quotient form: F held at >= kTabFmin (tab4_quotient then turns D into Q)
   __device__ __forceinline__ void set(int i, double f, double d) {
-    const double fs = fabs(f) < kTabFmin ? kTabFmin : f;
-    const double q = d / fs;
-    const uint64_t fb = __builtin_bit_cast(uint64_t, fs), qb = __builtin_bit_cast(uint64_t, q);
+    const double fs = q && fabs(f) < kTabFmin ? kTabFmin : f;
+    const uint64_t fb = __builtin_bit_cast(uint64_t, fs), db = __builtin_bit_cast(uint64_t, d);
     const int o = 10 * (k % 3) + 2 * i;
     if (k < 3) {
       t.FA[o] = (uint32_t)fb; t.FA[o + 1] = (uint32_t)(fb >> 32);
-      t.DA[o] = (uint32_t)qb; t.DA[o + 1] = (uint32_t)(qb >> 32);
+      t.DA[o] = (uint32_t)db; t.DA[o + 1] = (uint32_t)(db >> 32);
     } else {
       t.FB[o] = (uint32_t)fb; t.FB[o + 1] = (uint32_t)(fb >> 32);
-      t.DB[o] = (uint32_t)qb; t.DB[o + 1] = (uint32_t)(qb >> 32);
+      t.DB[o] = (uint32_t)db; t.DB[o + 1] = (uint32_t)(db >> 32);
     }
   }
 };
+__device__ __forceinline__ double tab_get(const v32u &v, int o) {
+  return __builtin_bit_cast(double, ((uint64_t)v[o + 1] << 32) | (uint64_t)v[o]);
+}
+__device__ __forceinline__ void tab_put(v32u &v, int o, double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  v[o] = (uint32_t)b; v[o + 1] = (uint32_t)(b >> 32);
+}
+// D -> Q = D / F for the VW entries of pair k: c_i = F_0 ... F_i, one
+// division 1 / c_{VW-1}, then 1 / F_i = (1 / c_i) c_{i-1} and
+// 1 / c_{i-1} = (1 / c_i) F_i from the last entry down.
+template <int VW> __device__ __forceinline__ void tab4_quotient(Tab4 &t, int k) {
+  v32u &Fv = k < 3 ? t.FA : t.FB;
+  v32u &Dv = k < 3 ? t.DA : t.DB;
+  const int o = 10 * (k % 3);
+  double f[VW], c[VW];
+#pragma unroll
+  for (int i = 0; i < VW; ++i) f[i] = tab_get(Fv, o + 2 * i);
+  c[0] = f[0];
+#pragma unroll
+  for (int i = 1; i < VW; ++i) c[i] = c[i - 1] * f[i];
+  double inv = 1.0 / c[VW - 1];
+#pragma unroll
+  for (int i = VW - 1; i >= 1; --i) {
+    tab_put(Dv, o + 2 * i, tab_get(Dv, o + 2 * i) * (inv * c[i - 1]));
+    inv *= f[i];
+  }
+  tab_put(Dv, o, tab_get(Dv, o) * inv);
+}
 
 // One allowed set (indices i0..i5 = 2 * power of each pair in SGPRs, index
 // mode on): the product runs through the six pair regions, E and G_5 are
 // added in the last one, and G_4 .. G_0 walking the regions back, so a set
 // takes 11 index changes and 13 VALU.  F_k sits at v[128 + 10k] (k < 3) /
 // v[160 + 10(k-3)], Q_k 64 registers higher.
-#define CG_SET4(i0, i1, i2, i3, i4, i5)                      \
+#define CG_SET4Q(i0, i1, i2, i3, i4, i5)                      \
   "s_set_gpr_idx_idx " #i0 "\n\t"                            \
   "v_mov_b64 %[X], v[128:129]\n\t"                           \
   "s_set_gpr_idx_idx " #i1 "\n\t"                            \
@@ -985,7 +1018,7 @@ struct Tab4Pair {
 // Two sets a, b interleaved region by region: independent product chains,
 // then the sums in an order that puts another instruction between the two
 // updates of each accumulator.
-#define CG_SET4X2(a0, a1, a2, a3, a4, a5, b0, b1, b2, b3, b4, b5)                   \
+#define CG_SET4QX2(a0, a1, a2, a3, a4, a5, b0, b1, b2, b3, b4, b5)                   \
   "s_set_gpr_idx_idx " #a0 "\n\t" "v_mov_b64 %[X], v[128:129]\n\t"                 \
   "s_set_gpr_idx_idx " #b0 "\n\t" "v_mov_b64 %[XB], v[128:129]\n\t"                \
   "s_set_gpr_idx_idx " #a1 "\n\t" "v_mul_f64 %[X], v[138:139], %[X]\n\t"           \
@@ -1071,12 +1104,12 @@ struct Tab4Pair {
       "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70",    \
       "s71", "s72", "s73", "s74", "s75", "scc"
 
-// Sum over the allowed sets of one column (FP64 tables and sums).
-__device__ __forceinline__ void set_loop4(const Tab4 &t, const int *cp, uint32_t lo, uint32_t hi, double &E,
+// Sum over the allowed sets of one column, quotient form (FP64 F and Q tables and sums).
+__device__ __forceinline__ void set_loop4q(const Tab4 &t, const int *cp, uint32_t lo, uint32_t hi, double &E,
                                           double *G) {
   double X, P, XB, PB;
   double G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
-  asm volatile(CG_LOOP4(CG_SET4, CG_SET4X2, "set4")
+  asm volatile(CG_LOOP4(CG_SET4Q, CG_SET4QX2, "set4q")
       : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
         [X] "=&v"(X), [P] "=&v"(P), [XB] "=&v"(XB), [PB] "=&v"(PB)
       : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[128:159]}"(t.FA), "{v[160:191]}"(t.FB), "{v[192:223]}"(t.DA),
@@ -1084,8 +1117,99 @@ __device__ __forceinline__ void set_loop4(const Tab4 &t, const int *cp, uint32_t
       : CG_LOOP4_CLOBBERS);
   G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
 }
+#undef CG_SET4Q
+#undef CG_SET4QX2
+
+// Product form (F and D tables), for families with few sets per column
+// where the quotients would not repay their set-up.  One allowed set
+// (indices i0..i5 = 2 * power of each pair in SGPRs):
+// E += prod_k F_k, G_k += D_k prod_{j != k} F_j, as L = F0 F1 F2 and
+// R = F3 F4 F5 with their partial derivatives, each pair's index region
+// entered once.  F_k sits at v[128 + 10k] (k < 3) / v[160 + 10(k-3)], D_k
+// 64 registers higher.
+#define CG_SET4(i0, i1, i2, i3, i4, i5)                      \
+  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
+  "v_mov_b64 %[X0], v[128:129]\n\t"                          \
+  "v_mov_b64 %[Y0], v[192:193]\n\t"                          \
+  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
+  "v_mov_b64 %[X5], v[180:181]\n\t"                          \
+  "v_mov_b64 %[Y5], v[244:245]\n\t"                          \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_mul_f64 %[M], v[202:203], %[X0]\n\t"                    \
+  "v_mul_f64 %[N], v[138:139], %[Y0]\n\t"                    \
+  "v_mul_f64 %[L], v[138:139], %[X0]\n\t"                    \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_mul_f64 %[M4], v[234:235], %[X5]\n\t"                   \
+  "v_mul_f64 %[N4], v[170:171], %[Y5]\n\t"                   \
+  "v_mul_f64 %[R], v[170:171], %[X5]\n\t"                    \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_mul_f64 %[DL2], v[212:213], %[L]\n\t"                   \
+  "v_mul_f64 %[L], v[148:149], %[L]\n\t"                     \
+  "v_mul_f64 %[M], v[148:149], %[M]\n\t"                     \
+  "v_mul_f64 %[N], v[148:149], %[N]\n\t"                     \
+  "s_set_gpr_idx_idx " #i3 "\n\t"                            \
+  "v_mul_f64 %[DR3], v[224:225], %[R]\n\t"                   \
+  "v_mul_f64 %[R], v[160:161], %[R]\n\t"                     \
+  "v_mul_f64 %[M4], v[160:161], %[M4]\n\t"                   \
+  "v_mul_f64 %[N4], v[160:161], %[N4]\n\t"                   \
+  "s_set_gpr_idx_off\n\t"                                    \
+  "v_fmac_f64 %[E], %[L], %[R]\n\t"                          \
+  "v_fmac_f64 %[G0], %[N], %[R]\n\t"                         \
+  "v_fmac_f64 %[G1], %[M], %[R]\n\t"                         \
+  "v_fmac_f64 %[G2], %[DL2], %[R]\n\t"                       \
+  "v_fmac_f64 %[G3], %[DR3], %[L]\n\t"                       \
+  "v_fmac_f64 %[G4], %[M4], %[L]\n\t"                        \
+  "v_fmac_f64 %[G5], %[N4], %[L]\n\t"
+
+// Sum over the allowed sets of one column: cp points at the column's entry
+// of colptr (first and end set), records are 32 bytes at sets8 = (lo, hi);
+// the two records after a column's last are read ahead and must exist.  Set
+// records are loaded one ahead into alternating SGPR blocks.
+__device__ __forceinline__ void set_loop4(const Tab4 &t, const int *cp, uint32_t lo, uint32_t hi, double &E,
+                                          double *G) {
+  double X0, Y0, X5, Y5, L, M, N, DL2, R, M4, N4, DR3;
+  double G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
+  asm volatile(
+      "s_mov_b32 s59, m0\n\t"
+      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_sub_i32 s58, s41, s40\n\t"
+      "s_cmp_lt_i32 s58, 1\n\t"
+      "s_cbranch_scc1 L_set4_done_%=\n\t"
+      "s_mov_b32 s41, 0\n\t"
+      "s_lshl_b64 s[40:41], s[40:41], 5\n\t"
+      "s_add_u32 s56, s40, %[lo]\n\t"
+      "s_addc_u32 s57, s41, %[hi]\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_set4_%=:\n\t"
+      "s_load_dwordx8 s[48:55], s[56:57], 0x20\n\t"
+      CG_SET4(s40, s41, s42, s43, s44, s45)
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_cbranch_scc1 L_set4_done_%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x40\n\t"
+      CG_SET4(s48, s49, s50, s51, s52, s53)
+      "s_add_u32 s56, s56, 0x40\n\t"
+      "s_addc_u32 s57, s57, 0\n\t"
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cbranch_scc0 L_set4_%=\n"
+      "L_set4_done_%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, s59\n\t"
+      : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
+        [X0] "=&v"(X0), [Y0] "=&v"(Y0), [X5] "=&v"(X5), [Y5] "=&v"(Y5), [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N),
+        [DL2] "=&v"(DL2), [R] "=&v"(R), [M4] "=&v"(M4), [N4] "=&v"(N4), [DR3] "=&v"(DR3)
+      : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[128:159]}"(t.FA), "{v[160:191]}"(t.FB), "{v[192:223]}"(t.DA),
+        "{v[224:255]}"(t.DB)
+      : "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s55",
+        "s56", "s57", "s58", "s59", "scc");
+  G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
+}
 #undef CG_SET4
-#undef CG_SET4X2
 
 // 3-body FP64 tables for the hand-scheduled set loop (powers below 9): F
 // of pair k at v[128 + 18k], D at v[182 + 18k], entry of power p at 2p, in
@@ -1188,10 +1312,10 @@ struct Tab4f {
 struct Tab4fPair {
   Tab4f &t;
   int k;
-  // F and Q = D / F as the FP64 tables (F held at >= 1e-30 in magnitude)
+  bool q;  // quotient form: F and Q = D / F (F held at >= 1e-30 in magnitude)
   __device__ __forceinline__ void set(int i, float f, float d) {
-    const float fs = fabsf(f) < 1e-30f ? 1e-30f : f;
-    const float q = d * __builtin_amdgcn_rcpf(fs);
+    const float fs = q && fabsf(f) < 1e-30f ? 1e-30f : f;
+    const float q = this->q ? d * __builtin_amdgcn_rcpf(fs) : d;
     const int o = 10 * (k % 3) + 2 * i;
     if (k < 3) {
       t.A[o] = __builtin_bit_cast(uint32_t, fs); t.A[o + 1] = __builtin_bit_cast(uint32_t, q);
@@ -1217,9 +1341,9 @@ struct Tab3fPair {
   }
 };
 
-// One allowed 4-body set in FP32, as the FP64 CG_SET4 (F_k at the even,
+// One allowed 4-body set in FP32, as the FP64 CG_SET4Q (F_k at the even,
 // Q_k = D_k / F_k at the odd register of an entry).
-#define CG_SET4F(i0, i1, i2, i3, i4, i5)                     \
+#define CG_SET4QF(i0, i1, i2, i3, i4, i5)                     \
   "s_set_gpr_idx_idx " #i0 "\n\t"                            \
   "v_mov_b32 %[X], v64\n\t"                                  \
   "s_set_gpr_idx_idx " #i1 "\n\t"                            \
@@ -1245,7 +1369,7 @@ struct Tab3fPair {
   "s_set_gpr_idx_idx " #i0 "\n\t"                            \
   "v_fmac_f32 %[G0], v65, %[P]\n\t"
 
-#define CG_SET4FX2(a0, a1, a2, a3, a4, a5, b0, b1, b2, b3, b4, b5)                  \
+#define CG_SET4QFX2(a0, a1, a2, a3, a4, a5, b0, b1, b2, b3, b4, b5)                  \
   "s_set_gpr_idx_idx " #a0 "\n\t" "v_mov_b32 %[X], v64\n\t"                        \
   "s_set_gpr_idx_idx " #b0 "\n\t" "v_mov_b32 %[XB], v64\n\t"                       \
   "s_set_gpr_idx_idx " #a1 "\n\t" "v_mul_f32 %[X], v74, %[X]\n\t"                  \
@@ -1273,20 +1397,100 @@ struct Tab3fPair {
   "s_set_gpr_idx_idx " #a1 "\n\t" "v_fmac_f32 %[G1], v75, %[P]\n\t"   \
   "s_set_gpr_idx_idx " #b0 "\n\t" "v_fmac_f32 %[G0], v65, %[PB]\n\t"
 
-// set_loop4 with FP32 tables and sums (same record stream and read-ahead).
-__device__ __forceinline__ void set_loop4f(const Tab4f &t, const int *cp, uint32_t lo, uint32_t hi, float &E,
+// set_loop4q with FP32 tables and sums.
+__device__ __forceinline__ void set_loop4qf(const Tab4f &t, const int *cp, uint32_t lo, uint32_t hi, float &E,
                                            float *G) {
   float X, P, XB, PB;
   float G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
-  asm volatile(CG_LOOP4(CG_SET4F, CG_SET4FX2, "set4f")
+  asm volatile(CG_LOOP4(CG_SET4QF, CG_SET4QFX2, "set4qf")
       : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
         [X] "=&v"(X), [P] "=&v"(P), [XB] "=&v"(XB), [PB] "=&v"(PB)
       : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[64:95]}"(t.A), "{v[96:127]}"(t.B)
       : CG_LOOP4_CLOBBERS);
   G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
 }
+#undef CG_SET4QF
+#undef CG_SET4QFX2
+
+// One allowed 4-body set in FP32, the products of CG_SET4: (F0, D0) and
+// (F5, D5) are copied to v[60:63], the other pairs are read in place.
+#define CG_SET4F(i0, i1, i2, i3, i4, i5)                     \
+  "s_set_gpr_idx_on " #i0 ", gpr_idx(SRC0)\n\t"              \
+  "v_mov_b64 v[60:61], v[64:65]\n\t"                         \
+  "s_set_gpr_idx_idx " #i5 "\n\t"                            \
+  "v_mov_b64 v[62:63], v[116:117]\n\t"                       \
+  "s_set_gpr_idx_idx " #i1 "\n\t"                            \
+  "v_mul_f32 %[M], v75, v60\n\t"                             \
+  "v_mul_f32 %[N], v74, v61\n\t"                             \
+  "v_mul_f32 %[L], v74, v60\n\t"                             \
+  "s_set_gpr_idx_idx " #i4 "\n\t"                            \
+  "v_mul_f32 %[M4], v107, v62\n\t"                           \
+  "v_mul_f32 %[N4], v106, v63\n\t"                           \
+  "v_mul_f32 %[R], v106, v62\n\t"                            \
+  "s_set_gpr_idx_idx " #i2 "\n\t"                            \
+  "v_mul_f32 %[DL2], v85, %[L]\n\t"                          \
+  "v_mul_f32 %[L], v84, %[L]\n\t"                            \
+  "v_mul_f32 %[M], v84, %[M]\n\t"                            \
+  "v_mul_f32 %[N], v84, %[N]\n\t"                            \
+  "s_set_gpr_idx_idx " #i3 "\n\t"                            \
+  "v_mul_f32 %[DR3], v97, %[R]\n\t"                          \
+  "v_mul_f32 %[R], v96, %[R]\n\t"                            \
+  "v_mul_f32 %[M4], v96, %[M4]\n\t"                          \
+  "v_mul_f32 %[N4], v96, %[N4]\n\t"                          \
+  "s_set_gpr_idx_off\n\t"                                    \
+  "v_fmac_f32 %[E], %[L], %[R]\n\t"                          \
+  "v_fmac_f32 %[G0], %[N], %[R]\n\t"                         \
+  "v_fmac_f32 %[G1], %[M], %[R]\n\t"                         \
+  "v_fmac_f32 %[G2], %[DL2], %[R]\n\t"                       \
+  "v_fmac_f32 %[G3], %[DR3], %[L]\n\t"                       \
+  "v_fmac_f32 %[G4], %[M4], %[L]\n\t"                        \
+  "v_fmac_f32 %[G5], %[N4], %[L]\n\t"
+
+// set_loop4 with FP32 tables and sums (same record stream and read-ahead).
+__device__ __forceinline__ void set_loop4f(const Tab4f &t, const int *cp, uint32_t lo, uint32_t hi, float &E,
+                                           float *G) {
+  float L, M, N, DL2, R, M4, N4, DR3;
+  float G0 = G[0], G1 = G[1], G2 = G[2], G3 = G[3], G4 = G[4], G5 = G[5];
+  asm volatile(
+      "s_mov_b32 s59, m0\n\t"
+      "s_load_dwordx2 s[40:41], %[cp], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_sub_i32 s58, s41, s40\n\t"
+      "s_cmp_lt_i32 s58, 1\n\t"
+      "s_cbranch_scc1 L_set4f_done_%=\n\t"
+      "s_mov_b32 s41, 0\n\t"
+      "s_lshl_b64 s[40:41], s[40:41], 5\n\t"
+      "s_add_u32 s56, s40, %[lo]\n\t"
+      "s_addc_u32 s57, s41, %[hi]\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_set4f_%=:\n\t"
+      "s_load_dwordx8 s[48:55], s[56:57], 0x20\n\t"
+      CG_SET4F(s40, s41, s42, s43, s44, s45)
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_cbranch_scc1 L_set4f_done_%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_load_dwordx8 s[40:47], s[56:57], 0x40\n\t"
+      CG_SET4F(s48, s49, s50, s51, s52, s53)
+      "s_add_u32 s56, s56, 0x40\n\t"
+      "s_addc_u32 s57, s57, 0\n\t"
+      "s_add_i32 s58, s58, -1\n\t"
+      "s_cmp_eq_u32 s58, 0\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cbranch_scc0 L_set4f_%=\n"
+      "L_set4f_done_%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, s59\n\t"
+      : [E] "+v"(E), [G0] "+v"(G0), [G1] "+v"(G1), [G2] "+v"(G2), [G3] "+v"(G3), [G4] "+v"(G4), [G5] "+v"(G5),
+        [L] "=&v"(L), [M] "=&v"(M), [N] "=&v"(N), [DL2] "=&v"(DL2), [R] "=&v"(R), [M4] "=&v"(M4), [N4] "=&v"(N4),
+        [DR3] "=&v"(DR3)
+      : [cp] "s"(cp), [lo] "s"(lo), [hi] "s"(hi), "{v[64:95]}"(t.A), "{v[96:127]}"(t.B)
+      : "v60", "v61", "v62", "v63", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50",
+        "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "scc");
+  G[0] = G0; G[1] = G1; G[2] = G2; G[3] = G3; G[4] = G4; G[5] = G5;
+}
 #undef CG_SET4F
-#undef CG_SET4FX2
 #undef CG_LOOP4
 #undef CG_LOOP4_CLOBBERS
 
@@ -1472,14 +1676,15 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           sc[k] = (TP)((double)ps * recip_nr(r));
           const int q = ty * NP + k;
           if constexpr (ASM4 && F32A) {
-            Tab4fPair tp{tab4f, k};
+            Tab4fPair tp{tab4f, k, a.qform != 0};
             pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
           } else if constexpr (ASM3 && F32A) {
             Tab3fPair tp{tab3f, k};
             pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
           } else if constexpr (ASM4) {
-            Tab4Pair tp{tab4, k};
+            Tab4Pair tp{tab4, k, a.qform != 0};
             pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            if (a.qform) tab4_quotient<VW>(tab4, k);
           } else if constexpr (ASM3) {
             Tab3Pair tp{tab3, k};
             pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
@@ -1499,15 +1704,21 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
         float ef = 0.0f, gf[NP];
 #pragma unroll
         for (int k = 0; k < NP; ++k) gf[k] = 0.0f;
-        if constexpr (ASM4)
-          set_loop4f(tab4f, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), ef, gf);
-        else
+        if constexpr (ASM4) {
+          if (a.qform)
+            set_loop4qf(tab4f, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), ef, gf);
+          else
+            set_loop4f(tab4f, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), ef, gf);
+        } else
           set_loop3f(tab3f, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), ef, gf);
         e = ef;
 #pragma unroll
         for (int k = 0; k < NP; ++k) g[k] = gf[k];
       } else if constexpr (ASM4) {
-        set_loop4(tab4, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
+        if (a.qform)
+          set_loop4q(tab4, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
+        else
+          set_loop4(tab4, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
       } else if constexpr (ASM3) {
         set_loop3(tab3, a.colptr + col, (uint32_t)(uintptr_t)a.sets8, (uint32_t)((uintptr_t)a.sets8 >> 32), e, g);
       } else {
@@ -2104,6 +2315,10 @@ static void build_family(Ctx &c, int which) {
     F.sets8 = dalloc_copy(s4);
   }
   if (F.nb == 4 && F.max_pow <= 5) {
+    // The quotient form saves 12 VALU per set and costs one division and
+    // ~4 VALU per table entry for every (cluster, column tile): families
+    // averaging 6 or more sets per column take it.
+    F.qform = perm.size() >= 6 * (size_t)F.ncols ? 1 : 0;
     // records of the hand-scheduled 4-body set loop, plus the two it reads ahead
     std::vector<int> s8((perm.size() + 2) * 8, 0);
     for (size_t i = 0; i < perm.size(); ++i)
@@ -2186,6 +2401,7 @@ static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks,
   a.stress_mode = c.m.fit_stress;
   a.deriv_const = c.m.deriv_const; a.smooth_dist = c.m.smooth_dist; a.fcut_offset = c.m.fcut_offset;
   a.nb_scale = F.nb;
+  a.qform = F.qform;
   dim3 grid((unsigned)ntasks), block(512);
   (void)block;
   if (ntasks == 0) return;

@@ -178,6 +178,7 @@ struct DevFamily {
   int ntypes = 0;        // cluster (or pair) types
   int ncols = 0;         // columns of the family
   int max_pow = 0;       // table width needed (largest power index + 1)
+  int qform = 0;         // 4-body set loops in quotient form (6 or more sets per column on average)
   double cutoff = 0;     // list cutoff of the family
   std::vector<int> h_col_offset;   // absolute first column per type
   std::vector<int> h_ncols;        // columns per type
