@@ -60,6 +60,7 @@ def test_synthetic_cases(ch, synth, oracle, tmp_path, case):
     elif case == "water":
         gen = synth.water(nframes=2, seed=22)
     elif case == "chno":
+        # 2.8 allowed 4-body sets per column: the product-form set loops
         gen = synth.chno(nframes=3, nmin=40, nmax=220, seed=23)
     elif case == "high_orders":
         # 3-body order 10 and 4-body order 6: the generic table kernels (the
@@ -71,6 +72,11 @@ def test_synthetic_cases(ch, synth, oracle, tmp_path, case):
         text, frames, names = synth.carbon(nframes=1, natoms=8, density=1.6, seed=24, o2=8, o3=4, o4=3)
         gen = (text, frames, names)
     s, fr = synth_setup(ch, synth, tmp_path, gen)
+    c4 = s.model.c4
+    if case == "chno":
+        assert c4.total_allowed < 6 * c4.ncols
+    elif case == "carbon":  # orders 16/9/5: 17.5 sets per column, quotient form
+        assert c4.total_allowed >= 6 * c4.ncols
     dev = _dev(ch, s)
     A, b = dev.assemble(fr)
     Ao = oracle.assemble(s, fr)

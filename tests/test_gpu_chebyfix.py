@@ -28,6 +28,9 @@ def _setup(ch, synth, tmp_path, mode, dist):
 def test_assembly_below_smin(ch, synth, oracle, tmp_path, mode, dist):
     print("assembly", mode, dist, flush=True)
     s = _setup(ch, synth, tmp_path, mode, dist)
+    # 6 or more allowed 4-body sets per column: the quotient-form set loops
+    c4 = s.model.c4
+    assert c4.total_allowed >= 6 * c4.ncols
     fr = close_frames(ch, synth)
     for f in fr:  # the 3- and 4-body families see pairs below S_MINIM
         assert pairs_below_smin(f, ["C", "H"], SMIN, 3.5) >= 3
