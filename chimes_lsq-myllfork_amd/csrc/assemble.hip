@@ -1602,9 +1602,14 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     T *Aout = reinterpret_cast<T *>(a.A);
     const int nrow = nf + f.stress_rows + f.ener_rows;
     const int cabs = a.col_abs[ty] + t.col0;
-    for (int i = threadIdx.x; i < nrow * W; i += BS) {
-      const int r = i / W, cc = i - r * W;
+    // element i = r * W + cc of the tile, stepped by BS without a division
+    const int dr = BS / W, dc = BS - dr * W;
+    int r = threadIdx.x / W, cc = threadIdx.x - r * W;
+    while (r < nrow) {
       Aout[(f.row_off + r) * a.ld + cabs + cc] = (T)0;
+      r += dr;
+      cc += dc;
+      if (cc >= W) { cc -= W; ++r; }
     }
     return;
   }
@@ -1833,10 +1838,17 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   T *Aout = reinterpret_cast<T *>(a.A);
   const long long r0 = f.row_off;
   const int cabs = a.col_abs[ty] + t.col0;
-  for (int i = threadIdx.x; i < nf * W; i += BS) {
-    const int r = i / W, cc = i - r * W;
-    const int ia = r / 3;
-    Aout[(r0 + r) * a.ld + cabs + cc] = (T)acc[(ia * Wp + cc) * 3 + (r - 3 * ia)];
+  {
+    // element i = r * W + cc of the tile, stepped by BS without a division
+    const int dr = BS / W, dc = BS - dr * W;
+    int r = threadIdx.x / W, cc = threadIdx.x - r * W;
+    while (r < nf) {
+      const int ia = r / 3;
+      Aout[(r0 + r) * a.ld + cabs + cc] = (T)acc[(ia * Wp + cc) * 3 + (r - 3 * ia)];
+      r += dr;
+      cc += dc;
+      if (cc >= W) { cc -= W; ++r; }
+    }
   }
   const int nsr = f.stress_rows, ner = f.ener_rows;
   const int sel9[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
