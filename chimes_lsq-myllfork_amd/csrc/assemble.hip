@@ -1565,6 +1565,18 @@ __host__ __device__ constexpr size_t tile_bytes(int natoms, int w) {
 // waves per SIMD so that 256-thread workgroups can share a CU.
 template <int NB, int VW, class T> constexpr bool asm_heavy() { return 2 * (NB * (NB - 1) / 2) * VW * (int)sizeof(T) / 4 > 150; }
 
+// Workgroups go to the eight XCDs round-robin (workgroup b to XCD b % 8).
+// Runs of eight consecutive tasks are kept on one XCD instead (runs still go
+// round-robin, which keeps the XCDs' loads even): neighbouring column tiles
+// of a bucket write adjacent pieces (W x 8 bytes) of the same rows of A, and
+// side by side on one XCD its L2 can merge them into whole lines.
+__device__ __forceinline__ int xcd_task(int b, int n) {
+  constexpr int G = 8, SPAN = 8 * G;
+  const int base = b - b % SPAN;
+  if (n - base < SPAN) return b;  // the last, partial span keeps its order
+  return base + (b & 7) * G + (b >> 3) % G;
+}
+
 // GACC: frames too large for the LDS tile accumulate straight into A (zeroed
 // first) with global atomics: 12 force, the stress rows and the energy rows
 // per (cluster, column).
@@ -1585,7 +1597,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
   // path then rounds each A entry once, at the write-out.
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double *acc = reinterpret_cast<double *>(smem_raw);
-  const Task t = a.tasks[blockIdx.x];
+  const Task t = a.tasks[xcd_task((int)blockIdx.x, (int)gridDim.x)];
   const DevFrame &f = a.frames[t.frame];
   const int nf = 3 * f.natoms;
   const int W = t.ncols;
