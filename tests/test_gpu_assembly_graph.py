@@ -124,3 +124,40 @@ def test_deterministic_assembly_is_bitwise_stable(ch, synth, oracle, tmp_path, c
     assert colscaled_error(runs[0], oracle.assemble(s, fr)) < 1e-10
     dev.close()
     other.close()
+
+
+def test_same_sizes_new_species(ch, synth, oracle, tmp_path):
+    """Batches of the same frame sizes whose atoms change species (other
+    clusters, other list sizes) assembled on one context (graphs on, each
+    shape twice) equal a fresh eager context's result (1e-12 column-scaled)
+    and the oracle's (1e-11 of each column's largest magnitude: these
+    150-300-atom frames hold entries near 1e-3 of their column's largest whose
+    FP64 sums differ from the oracle's at ~1.3e-10 column-scaled)."""
+    gen = synth.chno(nframes=3, nmin=150, nmax=300, seed=72)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    rng = np.random.default_rng(72)
+    dev = ch.Device(s)
+    for trial in range(3):
+        batch = []
+        for f in fr:
+            t = np.array(f.types, dtype=np.int32)
+            if trial == 1:  # most atoms carbon
+                t = np.where(rng.random(t.size) < 0.8, 0, t).astype(np.int32)
+            elif trial == 2:  # species permuted
+                t = rng.permutation(t).astype(np.int32)
+            batch.append(ch.PyFrame(f.natoms, np.array(f.xyz), t, np.array(f.cell), np.array(f.forces),
+                                    np.array(f.stress), f.energy, f.index))
+        for rep in range(2):
+            A, b = dev.assemble(batch)
+        fresh = ch.Device(s)
+        fresh.assembly_graphs(False)
+        Af, _ = fresh.assemble(batch)
+        fresh.close()
+        e_fresh = colscaled_error(A, Af)
+        Ao = oracle.assemble(s, batch)
+        e_orc = float((np.abs(A - Ao) / np.maximum(np.abs(Ao).max(axis=0), 1e-300)).max())
+        print("trial", trial, "vs fresh %.2e vs oracle %.2e" % (e_fresh, e_orc), dev.assembly_stats(), flush=True)
+        assert e_fresh < 1e-12, trial
+        assert e_orc < 1e-11, trial
+    assert dev.assembly_stats()[3] >= 1  # graphs were captured along the way
+    dev.close()
