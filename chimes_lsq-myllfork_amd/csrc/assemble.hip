@@ -1315,12 +1315,12 @@ struct Tab4fPair {
   bool q;  // quotient form: F and Q = D / F (F held at >= 1e-30 in magnitude)
   __device__ __forceinline__ void set(int i, float f, float d) {
     const float fs = q && fabsf(f) < 1e-30f ? 1e-30f : f;
-    const float q = this->q ? d * __builtin_amdgcn_rcpf(fs) : d;
+    const float dq = q ? d * __builtin_amdgcn_rcpf(fs) : d;
     const int o = 10 * (k % 3) + 2 * i;
     if (k < 3) {
-      t.A[o] = __builtin_bit_cast(uint32_t, fs); t.A[o + 1] = __builtin_bit_cast(uint32_t, q);
+      t.A[o] = __builtin_bit_cast(uint32_t, fs); t.A[o + 1] = __builtin_bit_cast(uint32_t, dq);
     } else {
-      t.B[o] = __builtin_bit_cast(uint32_t, fs); t.B[o + 1] = __builtin_bit_cast(uint32_t, q);
+      t.B[o] = __builtin_bit_cast(uint32_t, fs); t.B[o + 1] = __builtin_bit_cast(uint32_t, dq);
     }
   }
 };
