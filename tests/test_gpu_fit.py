@@ -90,14 +90,16 @@ def test_gram_matches_numpy(ch, synth, tmp_path, prec):
 
 def test_bf16x3_fit_guarded(ch, synth, tmp_path):
     """The split-BF16x3 Gram resolves only the large singular values; the
-    refinement must never return an iterate worse than its starting point."""
+    refinement must never return an iterate worse than its starting point.
+    The two calls each form A^T b with slab partial sums added by atomics, so
+    their starting iterates agree to rounding (RMS within 1e-9 relative)."""
     s, fr = synth_setup(ch, synth, tmp_path, synth.carbon(nframes=3, natoms=128, seed=31))
     dev = ch.Device(s)
     dev.assemble(fr, prec=ch.CG_FP32, copy=False)
     _, b = dev.assemble(fr, prec=ch.CG_FP32, copy=False)
     x0, r0 = dev.fit(b, eps=1e-5, refine=0, gram=ch.CG_GRAM_BF16X3)
     x1, r1 = dev.fit(b, eps=1e-5, refine=6, gram=ch.CG_GRAM_BF16X3)
-    assert np.all(np.isfinite(x1)) and r1.rms <= r0.rms
+    assert np.all(np.isfinite(x1)) and r1.rms <= r0.rms * (1.0 + 1e-9)
     with pytest.raises(RuntimeError):
         dev.assemble(fr, prec=ch.CG_FP64, copy=False)
         dev.fit(b, gram=ch.CG_GRAM_BF16X3)
