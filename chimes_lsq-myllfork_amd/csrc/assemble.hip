@@ -2922,7 +2922,7 @@ template <class Fn> static void for_each_dbuf(Ctx &c, Fn fn) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1],
                   &c.b_fam[2], &c.b_tasks[0], &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1],
-                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
+                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_atvp, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
                   &c.b_coef[2], &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars, &c.b_atot, &c.b_atasks[0], &c.b_atasks[1], &c.b_atasks[2]})
     fn(*b);
 }

@@ -240,6 +240,7 @@ struct Ctx {
   DBuf b_tcount[3];                       // force tasks: chunks per bucket and their offsets
   DBuf b_A, b_scan;                       // design matrix; hipcub scratch
   DBuf b_x;                               // fit workspace
+  DBuf b_atvp;                            // fit: per-slab partial sums of A^T v
   DBuf b_fitkeep;                         // kept columns of a fit (device copy)
   DBuf b_qr;                              // QR fit: FP64 copy of A, factors, SVD
   DBuf b_fitw;                            // row weights of a fit (device copy)

@@ -241,10 +241,11 @@ __global__ __launch_bounds__(256) void k_residual(const T *__restrict__ A, long 
   if (lane == 0) r[row] = (b ? b[row] : 0.0) - s;
 }
 
-// g = A^T v over a slab of rows; one thread per column, slabs summed with atomics.
+// A^T v over a slab of rows; one thread per column, the slab's sum stored at
+// part[column * nslab + slab] (summed over the slabs by k_slab_sum).
 template <class T>
 __global__ __launch_bounds__(256) void k_atv(const T *__restrict__ A, long long ld, long long m, int n,
-                                             const double *__restrict__ v, long long slab, double *__restrict__ g) {
+                                             const double *__restrict__ v, long long slab, double *__restrict__ part) {
   int j = blockIdx.x * 256 + threadIdx.x;
   long long r0 = (long long)blockIdx.y * slab, r1 = min(m, r0 + slab);
   if (j >= n) return;
@@ -258,7 +259,24 @@ __global__ __launch_bounds__(256) void k_atv(const T *__restrict__ A, long long 
     s3 += (double)A[(r + 3) * ld + j] * v[r + 3];
   }
   for (; r < r1; ++r) s0 += (double)A[r * ld + j] * v[r];
-  atomicAdd(&g[j], (s0 + s1) + (s2 + s3));
+  part[(size_t)j * gridDim.y + blockIdx.y] = (s0 + s1) + (s2 + s3);
+}
+
+// g[j] = sum over the slabs of part[j * nslab + slab]: one workgroup per
+// column, each thread a fixed strided subset, then a fixed tree in LDS, so
+// A^T v comes out the same bits on every call.
+__global__ __launch_bounds__(256) void k_slab_sum(const double *__restrict__ part, int nslab, double *__restrict__ g) {
+  __shared__ double red[256];
+  const double *p = part + (size_t)blockIdx.x * nslab;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nslab; k += 256) s += p[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g[blockIdx.x] = red[0];
 }
 
 __global__ void k_scale(double *v, const double *w, int n) {
@@ -319,6 +337,10 @@ static FitState &fit_state(Ctx &c) {
   static thread_local FitState fs;
   if (!fs.h) {
     if (rocblas_create_handle(&fs.h) != rocblas_status_success) throw Error(CG_ERR_DEVICE, "rocblas_create_handle");
+    // no atomics in rocBLAS (and in the rocSOLVER routines that call it): a
+    // fit of the same A gives the same bits on every call
+    if (rocblas_set_atomics_mode(fs.h, rocblas_atomics_not_allowed) != rocblas_status_success)
+      throw Error(CG_ERR_DEVICE, "rocblas_set_atomics_mode");
   }
   rocblas_set_stream(fs.h, c.stream);
   return fs;
@@ -326,12 +348,19 @@ static FitState &fit_state(Ctx &c) {
 
 template <class T>
 static void atv(Ctx &c, const T *A, long long m, int n, const double *v, double *g) {
-  CG_HIP(hipMemsetAsync(g, 0, n * sizeof(double), c.stream));
-  // slabs small enough for several workgroups per CU
+  if (m <= 0 || n <= 0) {
+    if (n > 0) CG_HIP(hipMemsetAsync(g, 0, n * sizeof(double), c.stream));
+    return;
+  }
+  // slabs small enough for several workgroups per CU; their partial sums are
+  // added in slab order (no atomics: the same bits on every call)
   long long slab = 256;
   long long nslab = (m + slab - 1) / slab;
+  double *part = c.b_atvp.as<double>((size_t)nslab * n);
+  if (!part) throw Error(CG_ERR_MEMORY, "A^T v partial sums");
   hipLaunchKernelGGL(k_atv<T>, dim3((n + 255) / 256, (unsigned)nslab), dim3(256), 0, c.stream, A, (long long)c.A_ld,
-                     m, n, v, slab, g);
+                     m, n, v, slab, part);
+  hipLaunchKernelGGL(k_slab_sum, dim3((unsigned)n), dim3(256), 0, c.stream, part, (int)nslab, g);
 }
 
 template <class T>
