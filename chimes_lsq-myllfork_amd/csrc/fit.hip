@@ -352,9 +352,10 @@ static void atv(Ctx &c, const T *A, long long m, int n, const double *v, double 
     if (n > 0) CG_HIP(hipMemsetAsync(g, 0, n * sizeof(double), c.stream));
     return;
   }
-  // slabs small enough for several workgroups per CU; their partial sums are
-  // added in slab order (no atomics: the same bits on every call)
-  long long slab = 256;
+  // slabs small enough for several workgroups per CU (at most 4096 of them,
+  // which bounds the partial-sum buffer); their partial sums are added in slab
+  // order (no atomics: the same bits on every call)
+  long long slab = std::max(256LL, (m + 4095) / 4096);
   long long nslab = (m + slab - 1) / slab;
   double *part = c.b_atvp.as<double>((size_t)nslab * n);
   if (!part) throw Error(CG_ERR_MEMORY, "A^T v partial sums");
