@@ -128,6 +128,7 @@ def lib(path=None):
         opt("cg_assembly_graphs", [C.c_void_p, C.c_int])
         opt("cg_assembly_global_above", [C.c_void_p, C.c_int])
         opt("cg_assembly_deterministic", [C.c_void_p, C.c_int])
+        opt("cg_assembly_pair_tables", [C.c_void_p, C.c_int, C.POINTER(C.c_int)])
         opt("cg_assembly_stats", [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)])
         opt("cg_copy_probe", [C.c_void_p, C.c_size_t])
@@ -476,6 +477,13 @@ class Device:
     def assembly_deterministic(self, on):
         """Bitwise-reproducible assembly (waves add to the LDS tile in a fixed order)."""
         check(self.L.cg_assembly_deterministic(self.h, 1 if on else 0), self.h)
+
+    def assembly_pair_tables(self, on):
+        """4-body pair tables computed once per neighbour entry (on by default) or rebuilt per column
+        tile; returns whether FP64 assemblies of this model use them."""
+        act = C.c_int(0)
+        check(self.L.cg_assembly_pair_tables(self.h, 1 if on else 0, C.byref(act)), self.h)
+        return bool(act.value)
 
     def assembly_global_above(self, natoms):
         """Accumulate frames with more than natoms atoms in A with global atomics (0: only those beyond

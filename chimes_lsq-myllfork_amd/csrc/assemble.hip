@@ -791,6 +791,11 @@ struct AsmArgs {
   int nb_scale;             // index into perm_scale (2, 3, 4)
   int det;                  // waves add to the LDS tile in a fixed order (bitwise-reproducible A)
   int qform;                // 4-body set loops in quotient form (F, Q = D / F tables)
+  // FP64 4-body: pair tables per neighbour entry (F then Q or D, 2 * VW
+  // doubles each) and the entry of each cluster pair (-1: build it here);
+  // null: every cluster builds its tables
+  const double *ptab;
+  const int *pidx;
 };
 
 template <class T, int N> using vecN = T __attribute__((ext_vector_type(N)));
@@ -982,6 +987,80 @@ template <int VW> __device__ __forceinline__ void tab4_quotient(Tab4 &t, int k) 
     inv *= f[i];
   }
   tab_put(Dv, o, tab_get(Dv, o) * inv);
+}
+
+// 4-body pair tables once per neighbour entry of the family list (the
+// entries closer than the family cutoff): entry e holds F[0..VW) and then
+// Q[0..VW) (quotient form) or D[0..VW) at tab + 2 * VW * e, from the same
+// table code as a cluster's own build (pair_table into a Tab4 slot, then
+// tab4_quotient), with the parameters of the entry's 2-body pair type
+// (ptp: smin, smax, xavg, 1/lambda, 1/xdiff, 1/smax).  A column tile then
+// loads its clusters' tables instead of building them: a frame has far
+// fewer 4-body pairs than (cluster, pair) slots.  One thread per entry below
+// the device total.
+template <int VW, int BASE>
+__global__ void k_pair_tab(AsmArgs a, const double4 *svec, const int *sown, const int *styp, const int *type,
+                           const int *int_pair_map, int ntypes_atom, const double *ptp, const int *ntot, int cap,
+                           double *tab) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= min(*ntot, cap)) return;
+  const double *p = ptp + 6 * int_pair_map[type[sown[e]] * ntypes_atom + styp[e]];
+  Tab4 t;
+  t.zero();
+  Tab4Pair tp{t, 0, a.qform != 0};
+  pair_table<double, VW, BASE>(a, svec[e].w, p[0], p[1], p[2], p + 3, tp);
+  if (a.qform) tab4_quotient<VW>(t, 0);
+  double2 *o = reinterpret_cast<double2 *>(tab + (size_t)e * (2 * VW));
+  double v[2 * VW];
+#pragma unroll
+  for (int i = 0; i < VW; ++i) {
+    v[i] = tab_get(t.FA, 2 * i);
+    v[VW + i] = tab_get(t.DA, 2 * i);
+  }
+#pragma unroll
+  for (int i = 0; i < VW; ++i) o[i] = make_double2(v[2 * i], v[2 * i + 1]);
+}
+
+// Neighbour entry of each pair of a bucketed 4-body cluster, pairs in the
+// assembly's order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) of its refs: the
+// entry of the lower-indexed atom's list holding the other atom at their
+// relative image shift (lists hold j >= i, every image in small cells).  -1
+// where that list lacks it (a distance within rounding of the cutoff, or a
+// shift the refs cannot encode): the assembly builds that table itself.
+// Capacity slots past the device total carry the sentinel key.
+__global__ void k_pair_idx(const uint32_t *refs, const int *keys, int ncl, int nkeys_type, int nkeys,
+                           const DevFrame *frames, const int *soff, const uint32_t *slist, int cap, int *pidx) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const int key = keys[c];
+  int out[6] = {-1, -1, -1, -1, -1, -1};
+  if (key >= 0 && key < nkeys) {
+    const int aoff = frames[key / nkeys_type].atom_off;
+    uint32_t r[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) r[s] = refs[(size_t)4 * c + s];
+    int k = 0;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = x + 1; y < 4; ++y, ++k) {
+        uint32_t ra = r[x], rb = r[y];
+        if (ref_atom(rb) < ref_atom(ra)) { const uint32_t t = ra; ra = rb; rb = t; }
+        const int dx = ref_sx(rb) - ref_sx(ra), dy = ref_sy(rb) - ref_sy(ra), dz = ref_sz(rb) - ref_sz(ra);
+        const int lo = -kRefShiftBias, hi = 15 - kRefShiftBias;
+        if (dx < lo || dx > hi || dy < lo || dy > hi || dz < lo || dz > hi) continue;
+        const uint32_t want = make_ref(ref_atom(rb), dx, dy, dz);
+        const int g = aoff + ref_atom(ra);
+        const int b = min(soff[g], cap), en = min(soff[g + 1], cap);
+        for (int i = b; i < en; ++i)
+          if (slist[i] == want) {
+            out[k] = i;
+            break;
+          }
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) pidx[(size_t)6 * c + k] = out[k];
 }
 
 // One allowed set (indices i0..i5 = 2 * power of each pair in SGPRs, index
@@ -1670,6 +1749,22 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
     TP dpos[NB][3], sc[NP];
     int atom[NB];
+    // FP64 4-body with entry tables: the neighbour entry of each pair, read
+    // before the geometry so that the table loads can start early
+    int pent[ASM4 && !F32A ? NP : 1];
+#pragma unroll
+    for (int k = 0; k < (ASM4 && !F32A ? NP : 1); ++k) pent[k] = -1;
+    if constexpr (ASM4 && !F32A) {
+      if (a.pidx) {
+        const int2 *pp = reinterpret_cast<const int2 *>(a.pidx + (size_t)NP * ci);
+#pragma unroll
+        for (int k = 0; k < NP / 2; ++k) {
+          const int2 w = pp[k];
+          pent[2 * k] = w.x;
+          pent[2 * k + 1] = w.y;
+        }
+      }
+    }
     {
       double p[NB][3];
       dpos[0][0] = dpos[0][1] = dpos[0][2] = (TP)0;
@@ -1699,9 +1794,27 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
             Tab3fPair tp{tab3f, k};
             pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
           } else if constexpr (ASM4) {
-            Tab4Pair tp{tab4, k, a.qform != 0};
-            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
-            if (a.qform) tab4_quotient<VW>(tab4, k);
+            if (pent[k] >= 0) {
+              const double2 *src = reinterpret_cast<const double2 *>(a.ptab + (size_t)pent[k] * (2 * VW));
+              double v[2 * VW];
+#pragma unroll
+              for (int i = 0; i < VW; ++i) {
+                const double2 w = src[i];
+                v[2 * i] = w.x;
+                v[2 * i + 1] = w.y;
+              }
+              v32u &Fv = k < 3 ? tab4.FA : tab4.FB;
+              v32u &Dv = k < 3 ? tab4.DA : tab4.DB;
+#pragma unroll
+              for (int i = 0; i < VW; ++i) {
+                tab_put(Fv, 10 * (k % 3) + 2 * i, v[i]);
+                tab_put(Dv, 10 * (k % 3) + 2 * i, v[VW + i]);
+              }
+            } else {
+              Tab4Pair tp{tab4, k, a.qform != 0};
+              pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+              if (a.qform) tab4_quotient<VW>(tab4, k);
+            }
           } else if constexpr (ASM3) {
             Tab3Pair tp{tab3, k};
             pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
@@ -2362,6 +2475,29 @@ static void build_family(Ctx &c, int which) {
     recip[3 * q + 2] = 1.0 / smax[q];
   }
   F.recip = dalloc_copy(recip);
+  if (F.nb == 4 && F.max_pow <= 5) {
+    // Pair tables per neighbour entry need one parameter set per 2-body pair
+    // type: every (cluster type, pair slot) of a pair type must carry the
+    // same cutoffs (SPECIFIC 4B cutoffs may differ per cluster type, and then
+    // the tables stay per cluster).
+    const cg_cluster_set &cs = m.c4;
+    std::vector<double> ptp(6 * (size_t)std::max(m.npairs, 1), 0.0);
+    std::vector<char> seen(std::max(m.npairs, 1), 0);
+    bool same = true;
+    for (int q = 0; q < (int)smin.size() && same; ++q) {
+      const int pt = cs.pair_type[q];
+      if (pt < 0 || pt >= m.npairs) { same = false; break; }
+      const double v[6] = {smin[q], smax[q], xavg[q], recip[3 * q], recip[3 * q + 1], recip[3 * q + 2]};
+      if (!seen[pt]) {
+        seen[pt] = 1;
+        for (int i = 0; i < 6; ++i) ptp[6 * pt + i] = v[i];
+      } else {
+        for (int i = 0; i < 6; ++i) same = same && ptp[6 * pt + i] == v[i];
+      }
+    }
+    F.pcache = same ? 1 : 0;
+    if (same) F.ptp = dalloc_copy(ptp);
+  }
   std::vector<int> both(col_rel);
   both.insert(both.end(), col_abs.begin(), col_abs.end());
   F.col_offset = dalloc_copy(both);  // [0,ntypes): relative, [ntypes,2*ntypes): absolute
@@ -2403,7 +2539,8 @@ static std::string tile_key(const cg_model &m, int w, int cls) {
 template <class T>
 static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks, const DevFrame *d_frames,
                           const uint32_t *refs, const int *bounds, size_t lds_bytes, void *A, long long ld,
-                          int want_bs, hipStream_t stream, bool gacc = false) {
+                          int want_bs, hipStream_t stream, bool gacc = false, const double *ptab = nullptr,
+                          const int *pidx = nullptr) {
   const DevFamily &F = c.fam[which];
   AsmArgs a{};
   a.tasks = d_tasks;
@@ -2426,6 +2563,8 @@ static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks,
   a.deriv_const = c.m.deriv_const; a.smooth_dist = c.m.smooth_dist; a.fcut_offset = c.m.fcut_offset;
   a.nb_scale = F.nb;
   a.qform = F.qform;
+  a.ptab = ptab;
+  a.pidx = ptab ? pidx : nullptr;
   dim3 grid((unsigned)ntasks), block(512);
   (void)block;
   if (ntasks == 0) return;
@@ -2495,7 +2634,20 @@ struct FamOut {
   uint32_t *refs = nullptr;
   std::vector<int> boff;       // host bucket bounds (exact mode only)
   int *d_bounds = nullptr;     // device bucket bounds [nkeys + 1]
+  const int *keys = nullptr;   // device keys of the bucketed clusters (sentinel nkeys past the total)
   int nkeys = 0;
+};
+
+// The 4-body family's neighbour list (its sublist, or the full list when its
+// cutoff is the list cutoff): offsets per atom, refs, vectors, owner atoms,
+// target types, the device total and the capacity.
+struct FamList {
+  const int *off = nullptr;
+  const uint32_t *list = nullptr;
+  const double4 *vec = nullptr;
+  const int *own = nullptr, *typ = nullptr;
+  const int *tot = nullptr;
+  int cap = 0;
 };
 
 struct Batch {
@@ -2511,6 +2663,7 @@ struct Batch {
   int *d_type = nullptr;
   double *d_xyz = nullptr, *d_wrap = nullptr;
   FamOut fo[3];
+  FamList l4;
   TotalPtrs tot{};             // device totals: neighbour entries, clusters per family
 };
 
@@ -2767,6 +2920,13 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
                        (int *)nullptr, (const int *)soff, slist, svec, sown, styp, (int)nn1);
     fl_list[w] = slist; fl_vec[w] = svec; fl_own[w] = sown; fl_typ[w] = styp; fl_off[w] = soff;
   }
+  B.l4 = FamList{};
+  if (nnb > 0) {
+    const bool sub4 = c.fam[2].cutoff < mx;
+    B.l4.off = fl_off[2]; B.l4.list = fl_list[2]; B.l4.vec = fl_vec[2]; B.l4.own = fl_own[2]; B.l4.typ = fl_typ[2];
+    B.l4.tot = fl_off[2] + natoms;
+    B.l4.cap = sub4 ? (int)nn1 : nnb;
+  }
   // ---- per-family cluster counts: 2-body one thread per neighbour entry,
   // 3/4-body one wave per atom over its LDS-staged list
   FamOut *fo = B.fo;
@@ -2821,6 +2981,7 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
     const DevFamily &F = c.fam[w];
     fo[w].ncl = 0;
     fo[w].refs = nullptr;
+    fo[w].keys = nullptr;
     fo[w].boff.assign((size_t)nframes * std::max(F.ntypes, 1) + 1, 0);
     if (!active[w]) continue;
     const int ncl = tot[w];
@@ -2872,6 +3033,7 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
         keys = key2;
       }
     }
+    fo[w].keys = keys;
     hipLaunchKernelGGL(k_bucket_bounds, dim3((nkeys + 1 + TB - 1) / TB), dim3(TB), 0, c.stream, keys, ncl, nkeys,
                        d_bounds[w]);
     CG_HIP(hipGetLastError());
@@ -2883,6 +3045,41 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
   }
   if (!caps) CG_HIP(hipStreamSynchronize(c.stream));                         // host sync 3 (exact mode)
   if (!c.capturing) CG_HIP(hipEventRecord(c.ev[1], c.stream));
+}
+
+// 4-body pair tables once per neighbour entry, and the entry of each pair of
+// every bucketed cluster, for an FP64 assembly of a family whose tables
+// depend on the entry alone (DevFamily::pcache).  Runs over the list and
+// cluster capacities with device totals, so it needs no host read.
+static void pair_cache(Ctx &c, const Batch &B, const double *&ptab, const int *&pidx) {
+  const DevFamily &F = c.fam[2];
+  const FamOut &o = B.fo[2];
+  ptab = nullptr;
+  pidx = nullptr;
+  if (!c.asm_pcache || !F.pcache || !F.ptp || !o.refs || !o.keys || o.ncl <= 0 || !B.l4.off || B.l4.cap <= 0)
+    return;
+  const int VW = F.max_pow <= 4 ? 4 : 5, TB = 256;
+  double *tab = c.b_ptab.as<double>((size_t)B.l4.cap * 2 * VW);
+  int *idx = c.b_pidx.as<int>((size_t)o.ncl * 6);
+  if (!tab || !idx) throw Error(CG_ERR_MEMORY, "pair tables");
+  AsmArgs a{};
+  a.transform = c.m.transform; a.fix_type = c.m.fix_type; a.fcut_type = c.m.fcut_type;
+  a.deriv_const = c.m.deriv_const; a.smooth_dist = c.m.smooth_dist; a.fcut_offset = c.m.fcut_offset;
+  a.qform = F.qform;
+  const dim3 tg((unsigned)((B.l4.cap + TB - 1) / TB)), tb(TB);
+  if (VW == 4)
+    hipLaunchKernelGGL((k_pair_tab<4, 0>), tg, tb, 0, c.stream, a, B.l4.vec, B.l4.own, B.l4.typ,
+                       (const int *)B.d_type, (const int *)c.d_int_pair_map, c.m.ntypes, (const double *)F.ptp,
+                       B.l4.tot, B.l4.cap, tab);
+  else
+    hipLaunchKernelGGL((k_pair_tab<5, 0>), tg, tb, 0, c.stream, a, B.l4.vec, B.l4.own, B.l4.typ,
+                       (const int *)B.d_type, (const int *)c.d_int_pair_map, c.m.ntypes, (const double *)F.ptp,
+                       B.l4.tot, B.l4.cap, tab);
+  hipLaunchKernelGGL(k_pair_idx, dim3((unsigned)((o.ncl + TB - 1) / TB)), dim3(TB), 0, c.stream, o.refs, o.keys,
+                     o.ncl, F.ntypes, o.nkeys, (const DevFrame *)B.d_frames, B.l4.off, B.l4.list, B.l4.cap, idx);
+  CG_HIP(hipGetLastError());
+  ptab = tab;
+  pidx = idx;
 }
 
 static void prepare(Ctx &c, const cg_frame *frames, int nframes, int md, Batch &B, bool sticky = true,
@@ -2922,7 +3119,7 @@ template <class Fn> static void for_each_dbuf(Ctx &c, Fn fn) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1],
                   &c.b_fam[2], &c.b_tasks[0], &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1],
-                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_atvp, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
+                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_atvp, &c.b_ptab, &c.b_pidx, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
                   &c.b_coef[2], &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars, &c.b_atot, &c.b_atasks[0], &c.b_atasks[1], &c.b_atasks[2]})
     fn(*b);
 }
@@ -3076,7 +3273,7 @@ static void build_tasks(const Ctx &c, int w, const std::vector<DevFrame> &hf, co
 static std::string asm_shape_key(const Ctx &c, const Batch &B, const Caps &caps, int prec) {
   std::string k = "A|";
   auto add = [&](long long v) { k += std::to_string(v); k += ','; };
-  add(prec); add(c.asm_det); add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
+  add(prec); add(c.asm_det); add(c.asm_pcache); add(B.nframes); add(caps.nnb); add(caps.ncl[0]); add(caps.ncl[1]); add(caps.ncl[2]);
   for (int w = 0; w < 3; ++w)
     for (int cls = 0; cls < CG_TILE_CLASSES; ++cls) add(c.tile_cfg[w][cls]);
   for (const DevFrame &d : B.hf) {
@@ -3184,6 +3381,9 @@ static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec,
   auto body = [&]() {
     if (clear_A) CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
     prepare_device(c, 0, B, capmode ? &caps : nullptr);
+    const double *ptab = nullptr;
+    const int *pidx = nullptr;
+    if (prec != CG_FP32 && d_tasks[2]) pair_cache(c, B, ptab, pidx);
     for (int w = 0; w < 3; ++w) {
       if (!d_tasks[w]) continue;
       const std::vector<TaskGroup> &groups = fam_groups[w];
@@ -3206,7 +3406,8 @@ static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec,
                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc);
         else
           launch_family<double>(c, w, g.end - g.begin, d_tasks[w] + g.begin, B.d_frames, B.fo[w].refs,
-                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc);
+                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc, w == 2 ? ptab : nullptr,
+                                w == 2 ? pidx : nullptr);
         if (timed) CG_HIP(hipEventRecord(c.cev[w][g.cls0][1], st));
         if (st != c.stream) {
           CG_HIP(hipEventRecord(c.join[gi - 1], st));
@@ -3689,7 +3890,7 @@ void ctx_free(Ctx &c) {
   for (auto &F : c.fam) {
     for (void *p : {(void *)F.ncol_type, (void *)F.col_offset, (void *)F.colptr, (void *)F.perm, (void *)F.smin, (void *)F.smax,
                     (void *)F.xavg, (void *)F.xdiff, (void *)F.lambda, (void *)F.recip, (void *)F.cct, (void *)F.csmax,
-                    (void *)F.cperm, (void *)F.cslot, (void *)F.sets8})
+                    (void *)F.cperm, (void *)F.cslot, (void *)F.sets8, (void *)F.ptp})
       if (p) (void)hipFree(p);
   }
   drop_graphs(c);
@@ -4078,6 +4279,13 @@ cg_status cg_assembly_graphs(cg_ctx *ctx, int on) {
 cg_status cg_assembly_deterministic(cg_ctx *ctx, int on) {
   if (!ctx) { cg::set_error("cg_assembly_deterministic: bad argument"); return CG_ERR_ARG; }
   ctx->c.asm_det = on != 0;
+  return CG_OK;
+}
+
+cg_status cg_assembly_pair_tables(cg_ctx *ctx, int on, int *active) {
+  if (!ctx) { cg::set_error("cg_assembly_pair_tables: bad argument"); return CG_ERR_ARG; }
+  ctx->c.asm_pcache = on != 0;
+  if (active) *active = on && ctx->c.fam[2].pcache && ctx->c.fam[2].ncols > 0 ? 1 : 0;
   return CG_OK;
 }
 

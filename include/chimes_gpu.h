@@ -309,6 +309,16 @@ cg_status cg_assembly_graphs(cg_ctx *ctx, int on);
  * every run, at a cost in time; frames accumulated in A itself (above) stay
  * order-dependent.  The reference sums serially (src/A_Matrix.C). */
 cg_status cg_assembly_deterministic(cg_ctx *ctx, int on);
+/* The 4-body tile kernels of an FP64 assembly load each cluster's pair
+ * tables (Chebyshev values and derivatives of its six pair distances) from
+ * tables computed once per neighbour entry, instead of rebuilding them for
+ * every column tile (on by default).  on = 0 rebuilds them per tile; both
+ * agree to rounding.  *active (may be null) is set to 1 when FP64
+ * assemblies of this model will use them: on, 4-body order <= 5 (the
+ * hand-scheduled set loops), and every cluster slot of a pair type with that
+ * type's 4-body cutoffs.  The reference builds them per cluster
+ * (src/Cheby.C:1279-1729). */
+cg_status cg_assembly_pair_tables(cg_ctx *ctx, int on, int *active);
 /* Frames whose force rows do not fit the LDS tile of the assembly kernels
  * (above about 6300 atoms) are accumulated straight into A with global
  * atomics, slower but without a size limit; A is then zeroed first and the
