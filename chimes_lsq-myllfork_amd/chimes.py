@@ -479,11 +479,11 @@ class Device:
         check(self.L.cg_assembly_deterministic(self.h, 1 if on else 0), self.h)
 
     def assembly_pair_tables(self, on):
-        """4-body pair tables computed once per neighbour entry (on by default) or rebuilt per column
-        tile; returns whether FP64 assemblies of this model use them."""
+        """3/4-body pair tables computed once per neighbour entry (on by default) or rebuilt per column
+        tile; returns the families whose FP64 assemblies use them (bit 1: 3-body, bit 2: 4-body)."""
         act = C.c_int(0)
         check(self.L.cg_assembly_pair_tables(self.h, 1 if on else 0, C.byref(act)), self.h)
-        return bool(act.value)
+        return act.value
 
     def assembly_global_above(self, natoms):
         """Accumulate frames with more than natoms atoms in A with global atomics (0: only those beyond

@@ -791,7 +791,7 @@ struct AsmArgs {
   int nb_scale;             // index into perm_scale (2, 3, 4)
   int det;                  // waves add to the LDS tile in a fixed order (bitwise-reproducible A)
   int qform;                // 4-body set loops in quotient form (F, Q = D / F tables)
-  // FP64 4-body: pair tables per neighbour entry (F then Q or D, 2 * VW
+  // FP64 3/4-body: pair tables per neighbour entry (F then Q or D, 2 * VW
   // doubles each) and the entry of each cluster pair (-1: build it here);
   // null: every cluster builds its tables
   const double *ptab;
@@ -989,61 +989,80 @@ template <int VW> __device__ __forceinline__ void tab4_quotient(Tab4 &t, int k) 
   tab_put(Dv, o, tab_get(Dv, o) * inv);
 }
 
-// 4-body pair tables once per neighbour entry of the family list (the
+// Pair tables once per neighbour entry of a 3- or 4-body family list (the
 // entries closer than the family cutoff): entry e holds F[0..VW) and then
-// Q[0..VW) (quotient form) or D[0..VW) at tab + 2 * VW * e, from the same
-// table code as a cluster's own build (pair_table into a Tab4 slot, then
-// tab4_quotient), with the parameters of the entry's 2-body pair type
-// (ptp: smin, smax, xavg, 1/lambda, 1/xdiff, 1/smax).  A column tile then
-// loads its clusters' tables instead of building them: a frame has far
-// fewer 4-body pairs than (cluster, pair) slots.  One thread per entry below
-// the device total.
-template <int VW, int BASE>
+// D[0..VW) (or Q = D / F for the 4-body quotient form) at tab + 2 * VW * e,
+// from the same table code as a cluster's own build (pair_table, and for
+// 4-body the Tab4 slot with its F bound and tab4_quotient), with the
+// parameters of the entry's 2-body pair type (ptp: smin, smax, xavg,
+// 1/lambda, 1/xdiff, 1/smax).  A column tile then loads its clusters' tables
+// instead of building them: a frame has far fewer pairs within the family
+// cutoff than (cluster, pair) slots.  One thread per entry below the device
+// total.
+template <int VW> struct MemTab {
+  double f[VW], d[VW];
+  __device__ __forceinline__ void set(int i, double x, double y) { f[i] = x; d[i] = y; }
+};
+template <int NB, int VW, int BASE>
 __global__ void k_pair_tab(AsmArgs a, const double4 *svec, const int *sown, const int *styp, const int *type,
                            const int *int_pair_map, int ntypes_atom, const double *ptp, const int *ntot, int cap,
                            double *tab) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= min(*ntot, cap)) return;
   const double *p = ptp + 6 * int_pair_map[type[sown[e]] * ntypes_atom + styp[e]];
-  Tab4 t;
-  t.zero();
-  Tab4Pair tp{t, 0, a.qform != 0};
-  pair_table<double, VW, BASE>(a, svec[e].w, p[0], p[1], p[2], p + 3, tp);
-  if (a.qform) tab4_quotient<VW>(t, 0);
-  double2 *o = reinterpret_cast<double2 *>(tab + (size_t)e * (2 * VW));
   double v[2 * VW];
+  if constexpr (NB == 4) {
+    Tab4 t;
+    t.zero();
+    Tab4Pair tp{t, 0, a.qform != 0};
+    pair_table<double, VW, BASE>(a, svec[e].w, p[0], p[1], p[2], p + 3, tp);
+    if (a.qform) tab4_quotient<VW>(t, 0);
 #pragma unroll
-  for (int i = 0; i < VW; ++i) {
-    v[i] = tab_get(t.FA, 2 * i);
-    v[VW + i] = tab_get(t.DA, 2 * i);
+    for (int i = 0; i < VW; ++i) {
+      v[i] = tab_get(t.FA, 2 * i);
+      v[VW + i] = tab_get(t.DA, 2 * i);
+    }
+  } else {
+    MemTab<VW> t;
+    pair_table<double, VW, BASE>(a, svec[e].w, p[0], p[1], p[2], p + 3, t);
+#pragma unroll
+    for (int i = 0; i < VW; ++i) {
+      v[i] = t.f[i];
+      v[VW + i] = t.d[i];
+    }
   }
+  double2 *o = reinterpret_cast<double2 *>(tab + (size_t)e * (2 * VW));
 #pragma unroll
   for (int i = 0; i < VW; ++i) o[i] = make_double2(v[2 * i], v[2 * i + 1]);
 }
 
-// Neighbour entry of each pair of a bucketed 4-body cluster, pairs in the
-// assembly's order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) of its refs: the
+// Neighbour entry of each pair of a bucketed 3- or 4-body cluster, pairs in
+// the assembly's order (0,1) (0,2) [(0,3)] (1,2) [(1,3) (2,3)] of its refs: the
 // entry of the lower-indexed atom's list holding the other atom at their
 // relative image shift (lists hold j >= i, every image in small cells).  -1
 // where that list lacks it (a distance within rounding of the cutoff, or a
 // shift the refs cannot encode): the assembly builds that table itself.
 // Capacity slots past the device total carry the sentinel key.
+template <int NB>
 __global__ void k_pair_idx(const uint32_t *refs, const int *keys, int ncl, int nkeys_type, int nkeys,
                            const DevFrame *frames, const int *soff, const uint32_t *slist, int cap, int *pidx) {
+  constexpr int NP = NB * (NB - 1) / 2;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncl) return;
   const int key = keys[c];
-  int out[6] = {-1, -1, -1, -1, -1, -1};
+  int out[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) out[k] = -1;
   if (key >= 0 && key < nkeys) {
     const int aoff = frames[key / nkeys_type].atom_off;
-    uint32_t r[4];
+    uint32_t r[NB];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) r[s] = refs[(size_t)4 * c + s];
+    for (int s = 0; s < NB; ++s) r[s] = refs[(size_t)NB * c + s];
     int k = 0;
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < NB; ++x)
 #pragma unroll
-      for (int y = x + 1; y < 4; ++y, ++k) {
+      for (int y = x + 1; y < NB; ++y, ++k) {
         uint32_t ra = r[x], rb = r[y];
         if (ref_atom(rb) < ref_atom(ra)) { const uint32_t t = ra; ra = rb; rb = t; }
         const int dx = ref_sx(rb) - ref_sx(ra), dy = ref_sy(rb) - ref_sy(ra), dz = ref_sz(rb) - ref_sz(ra);
@@ -1060,7 +1079,7 @@ __global__ void k_pair_idx(const uint32_t *refs, const int *keys, int ncl, int n
       }
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) pidx[(size_t)6 * c + k] = out[k];
+  for (int k = 0; k < NP; ++k) pidx[(size_t)NP * c + k] = out[k];
 }
 
 // One allowed set (indices i0..i5 = 2 * power of each pair in SGPRs, index
@@ -1749,11 +1768,12 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
     TP dpos[NB][3], sc[NP];
     int atom[NB];
-    // FP64 4-body with entry tables: the neighbour entry of each pair, read
+    // FP64 3/4-body with entry tables: the neighbour entry of each pair, read
     // before the geometry so that the table loads can start early
-    int pent[ASM4 && !F32A ? NP : 1];
+    constexpr bool PENT = (ASM4 || ASM3) && !F32A;
+    int pent[PENT ? NP : 1];
 #pragma unroll
-    for (int k = 0; k < (ASM4 && !F32A ? NP : 1); ++k) pent[k] = -1;
+    for (int k = 0; k < (PENT ? NP : 1); ++k) pent[k] = -1;
     if constexpr (ASM4 && !F32A) {
       if (a.pidx) {
         const int2 *pp = reinterpret_cast<const int2 *>(a.pidx + (size_t)NP * ci);
@@ -1763,6 +1783,11 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           pent[2 * k] = w.x;
           pent[2 * k + 1] = w.y;
         }
+      }
+    } else if constexpr (PENT) {
+      if (a.pidx) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) pent[k] = a.pidx[(size_t)NP * ci + k];
       }
     }
     {
@@ -1817,7 +1842,20 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
             }
           } else if constexpr (ASM3) {
             Tab3Pair tp{tab3, k};
-            pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            if (pent[k] >= 0) {
+              const double2 *src = reinterpret_cast<const double2 *>(a.ptab + (size_t)pent[k] * (2 * VW));
+              double v[2 * VW];
+#pragma unroll
+              for (int i = 0; i < VW; ++i) {
+                const double2 w = src[i];
+                v[2 * i] = w.x;
+                v[2 * i + 1] = w.y;
+              }
+#pragma unroll
+              for (int i = 0; i < VW; ++i) tp.set(i, v[i], v[VW + i]);
+            } else {
+              pair_table<double, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            }
           } else {
             pair_table<T, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tab[k]);
           }
@@ -2475,12 +2513,12 @@ static void build_family(Ctx &c, int which) {
     recip[3 * q + 2] = 1.0 / smax[q];
   }
   F.recip = dalloc_copy(recip);
-  if (F.nb == 4 && F.max_pow <= 5) {
+  if ((F.nb == 4 && F.max_pow <= 5) || (F.nb == 3 && F.max_pow <= 9)) {
     // Pair tables per neighbour entry need one parameter set per 2-body pair
     // type: every (cluster type, pair slot) of a pair type must carry the
-    // same cutoffs (SPECIFIC 4B cutoffs may differ per cluster type, and then
-    // the tables stay per cluster).
-    const cg_cluster_set &cs = m.c4;
+    // same cutoffs (SPECIFIC 3B/4B cutoffs may differ per cluster type, and
+    // then the tables stay per cluster).
+    const cg_cluster_set &cs = F.nb == 4 ? m.c4 : m.c3;
     std::vector<double> ptp(6 * (size_t)std::max(m.npairs, 1), 0.0);
     std::vector<char> seen(std::max(m.npairs, 1), 0);
     bool same = true;
@@ -2495,7 +2533,11 @@ static void build_family(Ctx &c, int which) {
         for (int i = 0; i < 6; ++i) same = same && ptp[6 * pt + i] == v[i];
       }
     }
-    F.pcache = same ? 1 : 0;
+    // 3-body families with few sets per column leave too little work per
+    // (cluster, column tile) to hide the table loads (C/H/N/O batch, 1.7
+    // sets per column: 16.8 -> 18.6 ms with them, carbon, 4.5: 5.1 -> 4.8 ms)
+    const bool enough = F.nb == 4 || perm.size() >= 3 * (size_t)F.ncols;
+    F.pcache = same && enough ? 1 : 0;
     if (same) F.ptp = dalloc_copy(ptp);
   }
   std::vector<int> both(col_rel);
@@ -2638,9 +2680,9 @@ struct FamOut {
   int nkeys = 0;
 };
 
-// The 4-body family's neighbour list (its sublist, or the full list when its
-// cutoff is the list cutoff): offsets per atom, refs, vectors, owner atoms,
-// target types, the device total and the capacity.
+// A 3- or 4-body family's neighbour list (its sublist, or the full list when
+// its cutoff is the list cutoff): offsets per atom, refs, vectors, owner
+// atoms, target types, the device total and the capacity.
 struct FamList {
   const int *off = nullptr;
   const uint32_t *list = nullptr;
@@ -2663,7 +2705,7 @@ struct Batch {
   int *d_type = nullptr;
   double *d_xyz = nullptr, *d_wrap = nullptr;
   FamOut fo[3];
-  FamList l4;
+  FamList fl[3];               // 3- and 4-body family lists (entries 1, 2)
   TotalPtrs tot{};             // device totals: neighbour entries, clusters per family
 };
 
@@ -2920,12 +2962,13 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
                        (int *)nullptr, (const int *)soff, slist, svec, sown, styp, (int)nn1);
     fl_list[w] = slist; fl_vec[w] = svec; fl_own[w] = sown; fl_typ[w] = styp; fl_off[w] = soff;
   }
-  B.l4 = FamList{};
-  if (nnb > 0) {
-    const bool sub4 = c.fam[2].cutoff < mx;
-    B.l4.off = fl_off[2]; B.l4.list = fl_list[2]; B.l4.vec = fl_vec[2]; B.l4.own = fl_own[2]; B.l4.typ = fl_typ[2];
-    B.l4.tot = fl_off[2] + natoms;
-    B.l4.cap = sub4 ? (int)nn1 : nnb;
+  for (int w = 0; w < 3; ++w) {
+    FamList &L = B.fl[w];
+    L = FamList{};
+    if (nnb == 0 || w == 0) continue;
+    L.off = fl_off[w]; L.list = fl_list[w]; L.vec = fl_vec[w]; L.own = fl_own[w]; L.typ = fl_typ[w];
+    L.tot = fl_off[w] + natoms;
+    L.cap = nnb;
   }
   // ---- per-family cluster counts: 2-body one thread per neighbour entry,
   // 3/4-body one wave per atom over its LDS-staged list
@@ -3047,36 +3090,52 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
   if (!c.capturing) CG_HIP(hipEventRecord(c.ev[1], c.stream));
 }
 
-// 4-body pair tables once per neighbour entry, and the entry of each pair of
-// every bucketed cluster, for an FP64 assembly of a family whose tables
-// depend on the entry alone (DevFamily::pcache).  Runs over the list and
-// cluster capacities with device totals, so it needs no host read.
-static void pair_cache(Ctx &c, const Batch &B, const double *&ptab, const int *&pidx) {
-  const DevFamily &F = c.fam[2];
-  const FamOut &o = B.fo[2];
+// 3/4-body pair tables once per neighbour entry, and the entry of each pair
+// of every bucketed cluster, for an FP64 assembly of a family whose tables
+// depend on the entry alone (DevFamily::pcache) and whose tile kernel takes
+// the hand-scheduled set loops (4-body orders up to 5, 3-body up to 9).  Runs
+// over the list and cluster capacities with device totals, so it needs no
+// host read.
+static void pair_cache(Ctx &c, const Batch &B, int w, const double *&ptab, const int *&pidx) {
+  const DevFamily &F = c.fam[w];
+  const FamOut &o = B.fo[w];
+  const FamList &L = B.fl[w];
   ptab = nullptr;
   pidx = nullptr;
-  if (!c.asm_pcache || !F.pcache || !F.ptp || !o.refs || !o.keys || o.ncl <= 0 || !B.l4.off || B.l4.cap <= 0)
+  if (w < 1 || !c.asm_pcache || !F.pcache || !F.ptp || !o.refs || !o.keys || o.ncl <= 0 || !L.off || L.cap <= 0)
     return;
-  const int VW = F.max_pow <= 4 ? 4 : 5, TB = 256;
-  double *tab = c.b_ptab.as<double>((size_t)B.l4.cap * 2 * VW);
-  int *idx = c.b_pidx.as<int>((size_t)o.ncl * 6);
+  const int nb = F.nb, np = nb * (nb - 1) / 2, TB = 256;
+  // table width of the kernel instance launch_family picks for the order
+  const int VW = nb == 4 ? (F.max_pow <= 4 ? 4 : 5) : (F.max_pow <= 4 ? 4 : F.max_pow <= 8 ? 8 : 9);
+  double *tab = c.b_ptab[w - 1].as<double>((size_t)L.cap * 2 * VW);
+  int *idx = c.b_pidx[w - 1].as<int>((size_t)o.ncl * np);
   if (!tab || !idx) throw Error(CG_ERR_MEMORY, "pair tables");
   AsmArgs a{};
   a.transform = c.m.transform; a.fix_type = c.m.fix_type; a.fcut_type = c.m.fcut_type;
   a.deriv_const = c.m.deriv_const; a.smooth_dist = c.m.smooth_dist; a.fcut_offset = c.m.fcut_offset;
   a.qform = F.qform;
-  const dim3 tg((unsigned)((B.l4.cap + TB - 1) / TB)), tb(TB);
-  if (VW == 4)
-    hipLaunchKernelGGL((k_pair_tab<4, 0>), tg, tb, 0, c.stream, a, B.l4.vec, B.l4.own, B.l4.typ,
-                       (const int *)B.d_type, (const int *)c.d_int_pair_map, c.m.ntypes, (const double *)F.ptp,
-                       B.l4.tot, B.l4.cap, tab);
+  const dim3 tg((unsigned)((L.cap + TB - 1) / TB)), tb(TB);
+  const int *pm = c.d_int_pair_map;
+  const double *ptp = F.ptp;
+#define CG_PTAB(NBv, VWv)                                                                                   \
+  hipLaunchKernelGGL((k_pair_tab<NBv, VWv, 0>), tg, tb, 0, c.stream, a, L.vec, L.own, L.typ,                \
+                     (const int *)B.d_type, pm, c.m.ntypes, ptp, L.tot, L.cap, tab)
+  if (nb == 4) {
+    if (VW == 4) CG_PTAB(4, 4);
+    else CG_PTAB(4, 5);
+  } else {
+    if (VW == 4) CG_PTAB(3, 4);
+    else if (VW == 8) CG_PTAB(3, 8);
+    else CG_PTAB(3, 9);
+  }
+#undef CG_PTAB
+  const dim3 ig((unsigned)((o.ncl + TB - 1) / TB)), ib(TB);
+  if (nb == 4)
+    hipLaunchKernelGGL(k_pair_idx<4>, ig, ib, 0, c.stream, o.refs, o.keys, o.ncl, F.ntypes, o.nkeys,
+                       (const DevFrame *)B.d_frames, L.off, L.list, L.cap, idx);
   else
-    hipLaunchKernelGGL((k_pair_tab<5, 0>), tg, tb, 0, c.stream, a, B.l4.vec, B.l4.own, B.l4.typ,
-                       (const int *)B.d_type, (const int *)c.d_int_pair_map, c.m.ntypes, (const double *)F.ptp,
-                       B.l4.tot, B.l4.cap, tab);
-  hipLaunchKernelGGL(k_pair_idx, dim3((unsigned)((o.ncl + TB - 1) / TB)), dim3(TB), 0, c.stream, o.refs, o.keys,
-                     o.ncl, F.ntypes, o.nkeys, (const DevFrame *)B.d_frames, B.l4.off, B.l4.list, B.l4.cap, idx);
+    hipLaunchKernelGGL(k_pair_idx<3>, ig, ib, 0, c.stream, o.refs, o.keys, o.ncl, F.ntypes, o.nkeys,
+                       (const DevFrame *)B.d_frames, L.off, L.list, L.cap, idx);
   CG_HIP(hipGetLastError());
   ptab = tab;
   pidx = idx;
@@ -3119,7 +3178,7 @@ template <class Fn> static void for_each_dbuf(Ctx &c, Fn fn) {
   for (DBuf *b : {&c.b_frames, &c.b_xyz, &c.b_type, &c.b_wrap, &c.b_ncount, &c.b_noff, &c.b_nlist, &c.b_nvec,
                   &c.b_nown, &c.b_sub[0], &c.b_sub[1], &c.b_bins, &c.b_ccount, &c.b_coff, &c.b_fam[0], &c.b_fam[1],
                   &c.b_fam[2], &c.b_tasks[0], &c.b_tasks[1], &c.b_tasks[2], &c.b_tcount[0], &c.b_tcount[1],
-                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_atvp, &c.b_ptab, &c.b_pidx, &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
+                  &c.b_tcount[2], &c.b_A, &c.b_scan, &c.b_x, &c.b_atvp, &c.b_ptab[0], &c.b_ptab[1], &c.b_pidx[0], &c.b_pidx[1], &c.b_fitkeep, &c.b_force, &c.b_coef[0], &c.b_coef[1],
                   &c.b_coef[2], &c.b_tmp, &c.b_ewald, &c.b_qr, &c.b_fitw, &c.b_lars, &c.b_atot, &c.b_atasks[0], &c.b_atasks[1], &c.b_atasks[2]})
     fn(*b);
 }
@@ -3381,9 +3440,11 @@ static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec,
   auto body = [&]() {
     if (clear_A) CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
     prepare_device(c, 0, B, capmode ? &caps : nullptr);
-    const double *ptab = nullptr;
-    const int *pidx = nullptr;
-    if (prec != CG_FP32 && d_tasks[2]) pair_cache(c, B, ptab, pidx);
+    const double *ptab[3] = {nullptr, nullptr, nullptr};
+    const int *pidx[3] = {nullptr, nullptr, nullptr};
+    if (prec != CG_FP32)
+      for (int w = 1; w < 3; ++w)
+        if (d_tasks[w]) pair_cache(c, B, w, ptab[w], pidx[w]);
     for (int w = 0; w < 3; ++w) {
       if (!d_tasks[w]) continue;
       const std::vector<TaskGroup> &groups = fam_groups[w];
@@ -3406,8 +3467,7 @@ static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec,
                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc);
         else
           launch_family<double>(c, w, g.end - g.begin, d_tasks[w] + g.begin, B.d_frames, B.fo[w].refs,
-                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc, w == 2 ? ptab : nullptr,
-                                w == 2 ? pidx : nullptr);
+                                B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc, ptab[w], pidx[w]);
         if (timed) CG_HIP(hipEventRecord(c.cev[w][g.cls0][1], st));
         if (st != c.stream) {
           CG_HIP(hipEventRecord(c.join[gi - 1], st));
@@ -4285,7 +4345,11 @@ cg_status cg_assembly_deterministic(cg_ctx *ctx, int on) {
 cg_status cg_assembly_pair_tables(cg_ctx *ctx, int on, int *active) {
   if (!ctx) { cg::set_error("cg_assembly_pair_tables: bad argument"); return CG_ERR_ARG; }
   ctx->c.asm_pcache = on != 0;
-  if (active) *active = on && ctx->c.fam[2].pcache && ctx->c.fam[2].ncols > 0 ? 1 : 0;
+  if (active) {
+    *active = 0;
+    for (int w = 1; w < 3; ++w)
+      if (on && ctx->c.fam[w].pcache && ctx->c.fam[w].ncols > 0) *active |= w;
+  }
   return CG_OK;
 }
 

@@ -179,7 +179,7 @@ struct DevFamily {
   int ncols = 0;         // columns of the family
   int max_pow = 0;       // table width needed (largest power index + 1)
   int qform = 0;         // 4-body set loops in quotient form (6 or more sets per column on average)
-  // 4-body pair tables computed once per neighbour entry (FP64 assembly):
+  // 3/4-body pair tables computed once per neighbour entry (FP64 assembly):
   // set when every cluster slot of a pair type has that type's cutoffs and
   // scaling, so a table depends on the entry alone; ptp holds per 2-body pair
   // type smin, smax, xavg, 1/lambda, 1/xdiff, 1/smax
@@ -247,8 +247,8 @@ struct Ctx {
   DBuf b_A, b_scan;                       // design matrix; hipcub scratch
   DBuf b_x;                               // fit workspace
   DBuf b_atvp;                            // fit: per-slab partial sums of A^T v
-  DBuf b_ptab;                            // 4-body pair tables per neighbour entry (F, Q or D)
-  DBuf b_pidx;                            // 4-body: neighbour entry of each cluster pair (-1: none)
+  DBuf b_ptab[2];                         // 3/4-body pair tables per neighbour entry (F, then Q or D)
+  DBuf b_pidx[2];                         // 3/4-body: neighbour entry of each cluster pair (-1: none)
   DBuf b_fitkeep;                         // kept columns of a fit (device copy)
   DBuf b_qr;                              // QR fit: FP64 copy of A, factors, SVD
   DBuf b_fitw;                            // row weights of a fit (device copy)
@@ -292,7 +292,7 @@ struct Ctx {
   // (0: only frames whose force rows do not fit the LDS tile)
   int gacc_above = 0;
   bool asm_det = false;                   // bitwise-reproducible assembly (cg_assembly_deterministic)
-  bool asm_pcache = true;                 // 4-body pair tables per neighbour entry (cg_assembly_pair_tables)
+  bool asm_pcache = true;                 // 3/4-body pair tables per neighbour entry (cg_assembly_pair_tables)
   double *h_atot = nullptr;               // pinned host copy of them (4 doubles)
   double *h_pin = nullptr;                // pinned host copy of results and totals
   size_t h_pin_cap = 0;

@@ -1,6 +1,6 @@
-"""4-body pair tables computed once per neighbour entry (cg_assembly_pair_tables):
-an FP64 assembly that loads each cluster's six pair tables from them equals
-one that rebuilds the tables per column tile, and the oracle.
+"""3/4-body pair tables computed once per neighbour entry (cg_assembly_pair_tables):
+an FP64 assembly that loads each cluster's pair tables from them equals one
+that rebuilds the tables per column tile, and the oracle.
 
 Bar: tables on vs off within 1e-13 of each column's largest magnitude (the
 tables come from the list's pair distances instead of the cluster's image
@@ -21,6 +21,13 @@ pytestmark = pytest.mark.gpu
 def colmax_error(A, R):
     """max |A - R| / max_i |R[i, j]| over the columns j."""
     return float((np.abs(A - R) / np.maximum(np.abs(R).max(axis=0), 1e-300)).max()) if R.size else 0.0
+
+
+def _expected(s):
+    """Families that should load entry tables: 4-body always (orders <= 5
+    here), 3-body with order <= 9 and 3 or more allowed sets per column."""
+    c3 = s.model.c3
+    return 2 | (1 if c3.order <= 9 and c3.total_allowed >= 3 * c3.ncols else 0)
 
 
 def _both(ch, s, fr):
@@ -49,7 +56,12 @@ def test_pair_tables_match_rebuilt_tables(ch, synth, oracle, tmp_path, case):
         gen = synth.carbon(nframes=2, natoms=96, seed=84, o2=8, o3=4, o4=3)
     s, fr = synth_setup(ch, synth, tmp_path, gen)
     active, A1, A0 = _both(ch, s, fr)
-    assert active
+    print("active", active, flush=True)
+    assert active == _expected(s)  # bit 1: 3-body family, bit 2: 4-body family
+    if case == "carbon":
+        assert active == 3
+    elif case == "chno":  # 1.7 3-body sets per column: 3-body tables stay per tile
+        assert active == 2
     Ao = oracle.assemble(s, fr)
     err = colmax_error(A1, A0)
     print("on vs off %.2e (column-scaled %.2e), on vs oracle %.2e, off vs oracle %.2e"
@@ -60,12 +72,12 @@ def test_pair_tables_match_rebuilt_tables(ch, synth, oracle, tmp_path, case):
 
 def test_pair_tables_cell_list_frame(ch, synth, tmp_path):
     """1400 carbon atoms (cell-list search: neighbour shifts from the bin
-    stencil) with 4-body terms: tables on equal tables off."""
+    stencil) with 3- and 4-body terms: tables on equal tables off."""
     gen = synth.carbon(nframes=1, natoms=1400, seed=85, o2=8, o3=4, o4=3)
     s, fr = synth_setup(ch, synth, tmp_path, gen)
     assert np.floor(np.asarray(fr[0].cell)[0] / 4.5) >= 5
     active, A1, A0 = _both(ch, s, fr)
-    assert active
+    assert active == _expected(s)
     err = colmax_error(A1, A0)
     print("on vs off %.2e (column-scaled %.2e)" % (err, colscaled_error(A1, A0)), flush=True)
     assert err < 1e-13
@@ -79,7 +91,7 @@ def test_pair_tables_capacity_mode_and_graphs(ch, synth, tmp_path):
     s, fr = synth_setup(ch, synth, tmp_path, gen)
     rng = np.random.default_rng(86)
     dev, ref = ch.Device(s), ch.Device(s)
-    assert dev.assembly_pair_tables(True)
+    assert dev.assembly_pair_tables(True) == 3  # carbon orders 16/9/5
     ref.assembly_pair_tables(False)
     ref.assembly_graphs(False)
     for step in range(4):
