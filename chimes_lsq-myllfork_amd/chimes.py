@@ -480,7 +480,7 @@ class Device:
 
     def assembly_pair_tables(self, on):
         """3/4-body pair tables computed once per neighbour entry (on by default) or rebuilt per column
-        tile; returns the families whose FP64 assemblies use them (bit 1: 3-body, bit 2: 4-body)."""
+        tile; returns the families whose assemblies use them (bit 1: 3-body, bit 2: 4-body)."""
         act = C.c_int(0)
         check(self.L.cg_assembly_pair_tables(self.h, 1 if on else 0, C.byref(act)), self.h)
         return act.value

@@ -791,10 +791,11 @@ struct AsmArgs {
   int nb_scale;             // index into perm_scale (2, 3, 4)
   int det;                  // waves add to the LDS tile in a fixed order (bitwise-reproducible A)
   int qform;                // 4-body set loops in quotient form (F, Q = D / F tables)
-  // FP64 3/4-body: pair tables per neighbour entry (F then Q or D, 2 * VW
-  // doubles each) and the entry of each cluster pair (-1: build it here);
-  // null: every cluster builds its tables
-  const double *ptab;
+  // 3/4-body with the hand-scheduled set loops: pair tables per neighbour
+  // entry (FP64 A: F[VW] then Q or D [VW], doubles; FP32 A: (F, Q or D)
+  // float pairs in the register order) and the entry of each cluster pair
+  // (-1: build it here); null: every cluster builds its tables
+  const void *ptab;
   const int *pidx;
 };
 
@@ -1006,7 +1007,8 @@ template <int VW> struct MemTab {
 template <int NB, int VW, int BASE>
 __global__ void k_pair_tab(AsmArgs a, const double4 *svec, const int *sown, const int *styp, const int *type,
                            const int *int_pair_map, int ntypes_atom, const double *ptp, const int *ntot, int cap,
-                           double *tab) {
+                           void *tab_out) {
+  double *tab = static_cast<double *>(tab_out);
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= min(*ntot, cap)) return;
   const double *p = ptp + 6 * int_pair_map[type[sown[e]] * ntypes_atom + styp[e]];
@@ -1407,13 +1409,19 @@ struct Tab4f {
   v32u A, B;
   __device__ __forceinline__ void zero() { A = 0u; B = 0u; }
 };
+// One FP32 4-body table entry: F and D, or in quotient form F held at >= 1e-30
+// in magnitude and Q = D / F.
+__device__ __forceinline__ void tab4f_entry(bool q, float f, float d, float &fs, float &dq) {
+  fs = q && fabsf(f) < 1e-30f ? 1e-30f : f;
+  dq = q ? d * __builtin_amdgcn_rcpf(fs) : d;
+}
 struct Tab4fPair {
   Tab4f &t;
   int k;
   bool q;  // quotient form: F and Q = D / F (F held at >= 1e-30 in magnitude)
   __device__ __forceinline__ void set(int i, float f, float d) {
-    const float fs = q && fabsf(f) < 1e-30f ? 1e-30f : f;
-    const float dq = q ? d * __builtin_amdgcn_rcpf(fs) : d;
+    float fs, dq;
+    tab4f_entry(q, f, d, fs, dq);
     const int o = 10 * (k % 3) + 2 * i;
     if (k < 3) {
       t.A[o] = __builtin_bit_cast(uint32_t, fs); t.A[o + 1] = __builtin_bit_cast(uint32_t, dq);
@@ -1438,6 +1446,33 @@ struct Tab3fPair {
     put(18 * k + 2 * i + 1, d);
   }
 };
+
+// FP32 pair tables once per neighbour entry (FP32 A): as k_pair_tab, with
+// the FP32 table code (pair_table in FP32 and, for 4-body, tab4f_entry's bound
+// and quotient), entry e holding the float pairs (F_i, Q_i or D_i) at
+// tab + 2 * VW * e, in the order of the registers they are loaded into.
+template <int VW> struct MemTabF {
+  bool q;
+  float v[2 * VW];
+  __device__ __forceinline__ void set(int i, float f, float d) {
+    if (q) tab4f_entry(true, f, d, v[2 * i], v[2 * i + 1]);
+    else { v[2 * i] = f; v[2 * i + 1] = d; }
+  }
+};
+template <int NB, int VW, int BASE>
+__global__ void k_pair_tab_f(AsmArgs a, const double4 *svec, const int *sown, const int *styp, const int *type,
+                             const int *int_pair_map, int ntypes_atom, const double *ptp, const int *ntot, int cap,
+                             void *tab_out) {
+  float *tab = static_cast<float *>(tab_out);
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= min(*ntot, cap)) return;
+  const double *p = ptp + 6 * int_pair_map[type[sown[e]] * ntypes_atom + styp[e]];
+  MemTabF<VW> t{NB == 4 && a.qform != 0};
+  pair_table<float, VW, BASE>(a, svec[e].w, p[0], p[1], p[2], p + 3, t);
+  float2 *o = reinterpret_cast<float2 *>(tab + (size_t)e * (2 * VW));
+#pragma unroll
+  for (int i = 0; i < VW; ++i) o[i] = make_float2(t.v[2 * i], t.v[2 * i + 1]);
+}
 
 // One allowed 4-body set in FP32, as the FP64 CG_SET4Q (F_k at the even,
 // Q_k = D_k / F_k at the odd register of an entry).
@@ -1768,13 +1803,13 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
     // +g v / r, and the stress of the cluster is sum_s F_s (x) d_s
     TP dpos[NB][3], sc[NP];
     int atom[NB];
-    // FP64 3/4-body with entry tables: the neighbour entry of each pair, read
+    // 3/4-body with entry tables: the neighbour entry of each pair, read
     // before the geometry so that the table loads can start early
-    constexpr bool PENT = (ASM4 || ASM3) && !F32A;
+    constexpr bool PENT = ASM4 || ASM3;
     int pent[PENT ? NP : 1];
 #pragma unroll
     for (int k = 0; k < (PENT ? NP : 1); ++k) pent[k] = -1;
-    if constexpr (ASM4 && !F32A) {
+    if constexpr (ASM4) {
       if (a.pidx) {
         const int2 *pp = reinterpret_cast<const int2 *>(a.pidx + (size_t)NP * ci);
 #pragma unroll
@@ -1813,14 +1848,35 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           sc[k] = (TP)((double)ps * recip_nr(r));
           const int q = ty * NP + k;
           if constexpr (ASM4 && F32A) {
-            Tab4fPair tp{tab4f, k, a.qform != 0};
-            pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            if (pent[k] >= 0) {
+              const float2 *src = static_cast<const float2 *>(a.ptab) + (size_t)pent[k] * VW;
+              v32u &Fv = k < 3 ? tab4f.A : tab4f.B;
+#pragma unroll
+              for (int i = 0; i < VW; ++i) {
+                const float2 w = src[i];
+                Fv[10 * (k % 3) + 2 * i] = __builtin_bit_cast(uint32_t, w.x);
+                Fv[10 * (k % 3) + 2 * i + 1] = __builtin_bit_cast(uint32_t, w.y);
+              }
+            } else {
+              Tab4fPair tp{tab4f, k, a.qform != 0};
+              pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            }
           } else if constexpr (ASM3 && F32A) {
             Tab3fPair tp{tab3f, k};
-            pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            if (pent[k] >= 0) {
+              const float2 *src = static_cast<const float2 *>(a.ptab) + (size_t)pent[k] * VW;
+#pragma unroll
+              for (int i = 0; i < VW; ++i) {
+                const float2 w = src[i];
+                tp.put(18 * k + 2 * i, w.x);
+                tp.put(18 * k + 2 * i + 1, w.y);
+              }
+            } else {
+              pair_table<float, VW, BASE>(a, r, a.smin[q], a.smax[q], a.xavg[q], a.recip + 3 * q, tp);
+            }
           } else if constexpr (ASM4) {
             if (pent[k] >= 0) {
-              const double2 *src = reinterpret_cast<const double2 *>(a.ptab + (size_t)pent[k] * (2 * VW));
+              const double2 *src = static_cast<const double2 *>(a.ptab) + (size_t)pent[k] * VW;
               double v[2 * VW];
 #pragma unroll
               for (int i = 0; i < VW; ++i) {
@@ -1843,7 +1899,7 @@ __global__ __launch_bounds__(BS, (asm_heavy<NB, VW, T>() ? 1 : 2)) void k_assemb
           } else if constexpr (ASM3) {
             Tab3Pair tp{tab3, k};
             if (pent[k] >= 0) {
-              const double2 *src = reinterpret_cast<const double2 *>(a.ptab + (size_t)pent[k] * (2 * VW));
+              const double2 *src = static_cast<const double2 *>(a.ptab) + (size_t)pent[k] * VW;
               double v[2 * VW];
 #pragma unroll
               for (int i = 0; i < VW; ++i) {
@@ -2581,7 +2637,7 @@ static std::string tile_key(const cg_model &m, int w, int cls) {
 template <class T>
 static void launch_family(Ctx &c, int which, size_t ntasks, const Task *d_tasks, const DevFrame *d_frames,
                           const uint32_t *refs, const int *bounds, size_t lds_bytes, void *A, long long ld,
-                          int want_bs, hipStream_t stream, bool gacc = false, const double *ptab = nullptr,
+                          int want_bs, hipStream_t stream, bool gacc = false, const void *ptab = nullptr,
                           const int *pidx = nullptr) {
   const DevFamily &F = c.fam[which];
   AsmArgs a{};
@@ -3090,13 +3146,13 @@ static void prepare_device(Ctx &c, int md, Batch &B, const Caps *caps) {
   if (!c.capturing) CG_HIP(hipEventRecord(c.ev[1], c.stream));
 }
 
-// 3/4-body pair tables once per neighbour entry, and the entry of each pair
-// of every bucketed cluster, for an FP64 assembly of a family whose tables
-// depend on the entry alone (DevFamily::pcache) and whose tile kernel takes
-// the hand-scheduled set loops (4-body orders up to 5, 3-body up to 9).  Runs
-// over the list and cluster capacities with device totals, so it needs no
-// host read.
-static void pair_cache(Ctx &c, const Batch &B, int w, const double *&ptab, const int *&pidx) {
+// 3/4-body pair tables once per neighbour entry (FP64 or FP32, as the
+// assembly), and the entry of each pair of every bucketed cluster, for a
+// family whose tables depend on the entry alone (DevFamily::pcache) and whose
+// tile kernel takes the hand-scheduled set loops (4-body orders up to 5,
+// 3-body up to 9).  Runs over the list and cluster capacities with device
+// totals, so it needs no host read.
+static void pair_cache(Ctx &c, const Batch &B, int w, bool fp32, const void *&ptab, const int *&pidx) {
   const DevFamily &F = c.fam[w];
   const FamOut &o = B.fo[w];
   const FamList &L = B.fl[w];
@@ -3107,7 +3163,7 @@ static void pair_cache(Ctx &c, const Batch &B, int w, const double *&ptab, const
   const int nb = F.nb, np = nb * (nb - 1) / 2, TB = 256;
   // table width of the kernel instance launch_family picks for the order
   const int VW = nb == 4 ? (F.max_pow <= 4 ? 4 : 5) : (F.max_pow <= 4 ? 4 : F.max_pow <= 8 ? 8 : 9);
-  double *tab = c.b_ptab[w - 1].as<double>((size_t)L.cap * 2 * VW);
+  void *tab = c.b_ptab[w - 1].get((size_t)L.cap * 2 * VW * (fp32 ? sizeof(float) : sizeof(double)));
   int *idx = c.b_pidx[w - 1].as<int>((size_t)o.ncl * np);
   if (!tab || !idx) throw Error(CG_ERR_MEMORY, "pair tables");
   AsmArgs a{};
@@ -3118,8 +3174,14 @@ static void pair_cache(Ctx &c, const Batch &B, int w, const double *&ptab, const
   const int *pm = c.d_int_pair_map;
   const double *ptp = F.ptp;
 #define CG_PTAB(NBv, VWv)                                                                                   \
-  hipLaunchKernelGGL((k_pair_tab<NBv, VWv, 0>), tg, tb, 0, c.stream, a, L.vec, L.own, L.typ,                \
-                     (const int *)B.d_type, pm, c.m.ntypes, ptp, L.tot, L.cap, tab)
+  do {                                                                                                      \
+    if (fp32)                                                                                               \
+      hipLaunchKernelGGL((k_pair_tab_f<NBv, VWv, 0>), tg, tb, 0, c.stream, a, L.vec, L.own, L.typ,          \
+                         (const int *)B.d_type, pm, c.m.ntypes, ptp, L.tot, L.cap, tab);                    \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_pair_tab<NBv, VWv, 0>), tg, tb, 0, c.stream, a, L.vec, L.own, L.typ,            \
+                         (const int *)B.d_type, pm, c.m.ntypes, ptp, L.tot, L.cap, tab);                    \
+  } while (0)
   if (nb == 4) {
     if (VW == 4) CG_PTAB(4, 4);
     else CG_PTAB(4, 5);
@@ -3440,11 +3502,10 @@ static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec,
   auto body = [&]() {
     if (clear_A) CG_HIP(hipMemsetAsync(A, 0, (size_t)rows * ld * esz, c.stream));
     prepare_device(c, 0, B, capmode ? &caps : nullptr);
-    const double *ptab[3] = {nullptr, nullptr, nullptr};
+    const void *ptab[3] = {nullptr, nullptr, nullptr};
     const int *pidx[3] = {nullptr, nullptr, nullptr};
-    if (prec != CG_FP32)
-      for (int w = 1; w < 3; ++w)
-        if (d_tasks[w]) pair_cache(c, B, w, ptab[w], pidx[w]);
+    for (int w = 1; w < 3; ++w)
+      if (d_tasks[w]) pair_cache(c, B, w, prec == CG_FP32, ptab[w], pidx[w]);
     for (int w = 0; w < 3; ++w) {
       if (!d_tasks[w]) continue;
       const std::vector<TaskGroup> &groups = fam_groups[w];
@@ -3464,7 +3525,7 @@ static void assemble_impl(Ctx &c, const cg_frame *frames, int nframes, int prec,
         if (timed) CG_HIP(hipEventRecord(c.cev[w][g.cls0][0], st));
         if (prec == CG_FP32)
           launch_family<float>(c, w, g.end - g.begin, d_tasks[w] + g.begin, B.d_frames, B.fo[w].refs,
-                               B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc);
+                               B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc, ptab[w], pidx[w]);
         else
           launch_family<double>(c, w, g.end - g.begin, d_tasks[w] + g.begin, B.d_frames, B.fo[w].refs,
                                 B.fo[w].d_bounds, lds, A, ld, bs, st, g.gacc, ptab[w], pidx[w]);

@@ -179,7 +179,7 @@ struct DevFamily {
   int ncols = 0;         // columns of the family
   int max_pow = 0;       // table width needed (largest power index + 1)
   int qform = 0;         // 4-body set loops in quotient form (6 or more sets per column on average)
-  // 3/4-body pair tables computed once per neighbour entry (FP64 assembly):
+  // 3/4-body pair tables computed once per neighbour entry (FP64 or FP32):
   // set when every cluster slot of a pair type has that type's cutoffs and
   // scaling, so a table depends on the entry alone; ptp holds per 2-body pair
   // type smin, smax, xavg, 1/lambda, 1/xdiff, 1/smax

@@ -309,12 +309,13 @@ cg_status cg_assembly_graphs(cg_ctx *ctx, int on);
  * every run, at a cost in time; frames accumulated in A itself (above) stay
  * order-dependent.  The reference sums serially (src/A_Matrix.C). */
 cg_status cg_assembly_deterministic(cg_ctx *ctx, int on);
-/* The 3- and 4-body tile kernels of an FP64 assembly load each cluster's
- * pair tables (Chebyshev values and derivatives of its pair distances) from
- * tables computed once per neighbour entry, instead of rebuilding them for
- * every column tile (on by default).  on = 0 rebuilds them per tile; both
- * agree to rounding.  *active (may be null) gets bit 1 (3-body) and bit 2
- * (4-body) for the families whose FP64 assemblies will use them: on, 3-body
+/* The 3- and 4-body tile kernels of an assembly (FP64 or FP32 A) load each
+ * cluster's pair tables (Chebyshev values and derivatives of its pair
+ * distances) from tables computed once per neighbour entry, instead of
+ * rebuilding them for every column tile (on by default).  on = 0 rebuilds
+ * them per tile; both agree to rounding.  *active (may be null) gets bit 1
+ * (3-body) and bit 2 (4-body) for the families whose assemblies will use
+ * them: on, 3-body
  * order <= 9 with 3 or more allowed sets per column on average, or 4-body
  * order <= 5 (the hand-scheduled set loops), and every cluster slot of a
  * pair type with that type's cutoffs.  The reference

@@ -1,6 +1,6 @@
 """3/4-body pair tables computed once per neighbour entry (cg_assembly_pair_tables):
-an FP64 assembly that loads each cluster's pair tables from them equals one
-that rebuilds the tables per column tile, and the oracle.
+an assembly that loads each cluster's pair tables from them equals one that
+rebuilds the tables per column tile, and the oracle (FP64 and FP32 A).
 
 Bar: tables on vs off within 1e-13 of each column's largest magnitude (the
 tables come from the list's pair distances instead of the cluster's image
@@ -30,12 +30,13 @@ def _expected(s):
     return 2 | (1 if c3.order <= 9 and c3.total_allowed >= 3 * c3.ncols else 0)
 
 
-def _both(ch, s, fr):
+def _both(ch, s, fr, prec=None):
     on, off = ch.Device(s), ch.Device(s)
     active = on.assembly_pair_tables(True)
     assert not off.assembly_pair_tables(False)
-    A1, b1 = on.assemble(fr)
-    A0, b0 = off.assemble(fr)
+    P = ch.CG_FP64 if prec is None else prec
+    A1, b1 = on.assemble(fr, prec=P)
+    A0, b0 = off.assemble(fr, prec=P)
     assert np.array_equal(b1, b0)
     on.close()
     off.close()
@@ -68,6 +69,31 @@ def test_pair_tables_match_rebuilt_tables(ch, synth, oracle, tmp_path, case):
           % (err, colscaled_error(A1, A0), colscaled_error(A1, Ao), colscaled_error(A0, Ao)), flush=True)
     assert err < 1e-13
     assert colscaled_error(A1, Ao) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["carbon", "chno", "order3"])
+def test_pair_tables_fp32(ch, synth, oracle, tmp_path, case):
+    """FP32 A: the entry tables hold the FP32 tables (the same FP32 table code),
+    so on and off agree to a few FP32 roundings of the column's largest entry;
+    both meet the FP32 assembly bar against the oracle (5e-3 column-scaled,
+    3e-5 of the column's largest magnitude)."""
+    print("case", case, flush=True)
+    if case == "carbon":
+        gen = synth.carbon(nframes=2, natoms=128, seed=87)
+    elif case == "chno":
+        gen = synth.chno(nframes=3, nmin=40, nmax=220, seed=88)
+    else:
+        gen = synth.carbon(nframes=2, natoms=96, seed=89, o2=8, o3=4, o4=3)
+    s, fr = synth_setup(ch, synth, tmp_path, gen)
+    active, A1, A0 = _both(ch, s, fr, prec=ch.CG_FP32)
+    assert active == _expected(s)
+    Ao = oracle.assemble(s, fr)
+    err = colmax_error(A1, A0)
+    e1, e0 = colmax_error(A1, Ao), colmax_error(A0, Ao)
+    print("on vs off %.2e, on vs oracle %.2e (column-scaled %.2e), off vs oracle %.2e"
+          % (err, e1, colscaled_error(A1, Ao), e0), flush=True)
+    assert err < 2e-6
+    assert e1 < 3e-5 and colscaled_error(A1, Ao) < 5e-3
 
 
 def test_pair_tables_cell_list_frame(ch, synth, tmp_path):
